@@ -1,0 +1,191 @@
+/*
+ * og_engine.h -- C ABI of the MI355X (gfx950) force/integration engine that
+ * replaces the CUDA backend of celeste/omegagene.
+ *
+ * The reference exposes a flat extern "C" ABI from src/celeste/cuda/cuda_setup.cu
+ * and cuda_common.cu, declared privately at the top of its callers
+ * (src/celeste/core/SubBox.cpp:5-78, MiniCell.cpp:7-35), with device state in
+ * file-scope globals (cuda_setup.h:77-133).  This header keeps the call points
+ * and their data semantics (grid-order gradient "work" = dE/dx, energies
+ * {vdw, ele}) but:
+ *   - state lives behind an og_system handle, so many systems can coexist;
+ *   - the grid, the cell-pair list, the pair kernel, the reductions and the
+ *     integrator all run on the device; no per-step PCIe copies;
+ *   - every call returns an og_status instead of calling exit();
+ *   - batched calls run many independent systems (walkers) per launch.
+ *
+ * Threading: one handle is used from one host thread at a time; distinct
+ * handles (and batches) are independent.  Ownership: the library owns device
+ * memory; callers own every host array and pass it with its count.
+ * Units: Angstrom, fs, kcal/mol, amu, e (reference units, CelesteObject.cpp:27-40).
+ */
+#ifndef OG_ENGINE_H
+#define OG_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference aborts instead: cuda_common.h:7-22) ---- */
+typedef enum {
+    OG_OK = 0,
+    OG_E_ARG = 1,       /* invalid argument */
+    OG_E_CAPACITY = 2,  /* a device list (cells, cell pairs, column sort) overflowed */
+    OG_E_NAN = 3,       /* non-finite coordinate or force */
+    OG_E_NOCONV = 4,    /* SHAKE did not converge within --const-max-loops */
+    OG_E_DEVICE = 5,    /* HIP runtime error */
+    OG_E_STATE = 6      /* call out of order */
+} og_status;
+
+typedef struct og_system og_system;
+typedef struct og_batch og_batch;
+
+/* integrator / constraint / electrostatics selectors (Config.cpp:71-131) */
+enum { OG_INTEGRATOR_LEAPFROG_PRESTO = 0, OG_INTEGRATOR_LANGEVIN = 1 };
+enum { OG_CONSTRAINT_NONE = 0, OG_CONSTRAINT_SHAKE = 1, OG_CONSTRAINT_SHAKE_SETTLE = 2 };
+
+/* energy terms returned by og_get_energies (array of OG_NTERMS doubles) */
+enum {
+    OG_TERM_VDW = 0,    /* LJ pair energy                     (MmSystem::pote_vdw) */
+    OG_TERM_ELE,        /* ZD pair energy, no excess/self      */
+    OG_TERM_EXCESS,     /* ZD excluded-pair energy             (SubBox.cpp:1347-1367) */
+    OG_TERM_SELF,       /* ZD self energy                      (MmSystem::energy_self_sum) */
+    OG_TERM_BOND,       /* harmonic bonds                      (MmSystem::pote_bond) */
+    OG_TERM_KINETIC,    /* kinetic energy                      (MmSystem::kinetic_e) */
+    OG_TERM_VIRIAL,     /* sum over pairs of g*r^2 = r_ij . dE/dr_i (new; the reference has none) */
+    OG_TERM_EXTEND,     /* VcMD bias energy                    (MmSystem::pote_extend) */
+    OG_NTERMS
+};
+
+/* One system as read from a .cls launch set (Read.cpp:50-794) plus the
+ * derived tables MmSystem builds (nb15off MmSystem.cpp:478-525, excess pairs
+ * MmSystem.cpp:700-736).  All pointers are host memory owned by the caller and
+ * only read during og_system_create. */
+typedef struct {
+    int           n_atoms;
+    const double *box;          /* [12] {Lx,0,0, 0,Ly,0, 0,0,Lz, cx,cy,cz} */
+    const double *crd;          /* [n*3] */
+    const double *vel;          /* [n*3] velocities as read (vel_just), A/fs */
+    const double *charge;       /* [n] */
+    const double *mass;         /* [n] */
+    const int    *atom_type;    /* [n] 0-based */
+    int           n_lj_types;
+    const double *lj6;          /* [n_types*n_types] A6 = 4 eps sigma^6 */
+    const double *lj12;         /* [n_types*n_types] A12 = 4 eps sigma^12 */
+    const int    *nb15off;      /* [n*32] partner lists, -1 terminated */
+    int           n_excess;     /* excluded pairs i<j, reference order */
+    const int    *excess;       /* [n_excess*2] */
+    int           n_bonds;
+    const int    *bond_ids;     /* [n_bonds*2] */
+    const double *bond_k;       /* E = k (r - r0)^2 */
+    const double *bond_r0;
+    int           n_shake_pairs;
+    const int    *shake_pair_ids;   /* [n*2] */
+    const double *shake_pair_d2;    /* squared distances */
+    int           n_shake_trios;
+    const int    *shake_trio_ids;   /* [n*3] */
+    const double *shake_trio_d2;    /* [n*3] squared d12, d23, d31 */
+    int           n_settle;
+    const int    *settle_ids;       /* [n*3] O, H1, H2 */
+    double        settle_dOH, settle_dHH;
+    int           n_com_groups;     /* COM-cancel groups (disjoint) */
+    const int    *com_group_off;    /* [n_com_groups+1] */
+    const int    *com_group_atoms;
+    int           d_free;           /* degrees of freedom for the temperature */
+} og_system_desc;
+
+/* Run parameters: the .cfg keys of the path (Config.cpp:37-367). */
+typedef struct {
+    double   cutoff;               /* --cutoff */
+    double   nsgrid_cutoff;        /* --nsgrid-cutoff */
+    int      nsgrid_update_intvl;  /* --nsgrid-update-intvl */
+    double   ele_alpha;            /* --ele-alpha */
+    double   time_step;            /* --time-step (fs) */
+    int      integrator;           /* OG_INTEGRATOR_* */
+    int      constraint;           /* OG_CONSTRAINT_* */
+    double   const_tolerance;      /* --const-tolerance */
+    int      const_max_loops;      /* --const-max-loops */
+    double   temperature;          /* --temperature (K) */
+    double   langevin_gamma;       /* --langevin-gamma already in fs^-1 (Config.cpp:345-348) */
+    uint64_t random_seed;          /* --random-seed */
+    int      com_cancel;           /* --com-motion cancel */
+    int      gap_bug;              /* 1: reproduce the uniform-z scan stop at the first empty bin (MiniCell.cpp:1063-1077) */
+    int      excess_bug;           /* 1: alpha>0 excess energy uses -erf(ar)*r as ZeroMultipoleSum.cpp:214 */
+    int      log_stride;           /* energies kept every log_stride steps */
+    int      log_cap;              /* entries kept in the device energy log */
+    double   capacity_factor;      /* cell-pair list capacity multiplier (1.0 default) */
+} og_params;
+
+void og_params_default(og_params *p);
+
+/* ---- device / lifetime (cuda_set_device cuda_common.cu:3-6;
+ *      cuda_alloc_atom_info cuda_setup.cu:42-77; cuda_alloc_set_lj_params :205-244;
+ *      cuda_set_pbc :138-146; cuda_set_constant :193-203; cuda_zerodipole_constant :148-153;
+ *      cuda_memcpy_htod_atom_info :115-119; cuda_free_atom_info :79-102) ---- */
+og_status og_set_device(int device_id);
+og_status og_system_create(const og_system_desc *desc, const og_params *params, og_system **out);
+og_status og_system_destroy(og_system *sys);
+
+/* Grid rebuild on the device: atom->column binning, column sort, cells, cell
+ * pairs and exclusion masks.  Replaces SubBox::nsgrid_update's host MiniCell
+ * work plus cuda_set_cell_constant :157-189, cuda_memcpy_htod_atomids :104-110,
+ * cuda_set_atominfo :404-425 and cuda_enumerate_cell_pairs :1095-1216. */
+og_status og_nsgrid_update(og_system *sys);
+
+/* Pair + excluded-pair + bond forces and energies for the current
+ * coordinates.  Replaces cuda_pairwise_ljzd :777-792 + cuda_pair_sync :797-801 +
+ * cuda_memcpy_dtoh_work :803-823 (+ the host bonded/excess terms,
+ * SubBox.cpp:923-929).  work_atom: [n*3] dE/dx in atom order (may be NULL);
+ * energies: [OG_NTERMS] (may be NULL). */
+og_status og_calc_energy_force(og_system *sys, int rebuild, double *work_atom, double *energies);
+
+/* n_steps MD steps on the device (DynamicsModePresto/Langevin::calc_in_each_step,
+ * DynamicsMode.cpp:468-560 / :798-899), grid rebuilt every nsgrid_update_intvl
+ * steps; launched as a captured hipGraph. */
+og_status og_md_run(og_system *sys, int n_steps);
+
+/* State exchange (host arrays [n*3], atom order). */
+og_status og_get_state(og_system *sys, double *crd, double *vel_next, double *vel_just);
+og_status og_set_state(og_system *sys, const double *crd, const double *vel_next);
+og_status og_get_energies(og_system *sys, double *energies /* [OG_NTERMS] */);
+/* energy log of the last run: rows of OG_NTERMS doubles, one per log_stride steps */
+og_status og_get_energy_log(og_system *sys, double *out, int max_rows, int *n_rows);
+og_status og_get_step(og_system *sys, long long *step);
+
+/* Grid / cell-pair inspection in the reference layouts (parity tests):
+ * atomids [cap >= n_atom_array] grid slot -> atom (-1 dummy);
+ * idx_xy_head_cell [n_columns+1]; cell pairs as CellPair
+ * {cell_id1, cell_id2, image, pair_mask[2]} (MiniCell.h:17-26) and
+ * idx_head_cell_pairs [n_cells+1]. */
+og_status og_get_grid_info(og_system *sys, int *n_cells, int *n_columns, int *n_cell_pairs, int *n_atom_array);
+og_status og_get_grid(og_system *sys, int *atomids, double *grid_crd, int *idx_xy_head_cell);
+og_status og_get_cell_pairs(og_system *sys, int32_t *cell_pairs5, int *idx_head_cell_pairs);
+/* fixed-point gradient accumulators would be zero after a force call; this
+ * returns the last gradient in grid order (reference cuda_memcpy_dtoh_work layout) */
+og_status og_get_work_grid(og_system *sys, double *work_grid);
+
+/* ---- batched walkers (the reference runs them as separate processes:
+ *      samples/cg_q8_vcmd/1/c2_exe.bash:11-19) ---- */
+og_status og_batch_create(og_batch **out);
+og_status og_batch_add(og_batch *b, og_system *sys);
+og_status og_batch_remove(og_batch *b, og_system *sys);
+og_status og_batch_run(og_batch *b, int n_steps);
+og_status og_batch_destroy(og_batch *b);
+og_status og_batch_size(og_batch *b, int *n);
+
+/* diagnostics */
+const char *og_status_string(og_status s);
+og_status   og_last_error(char *buf, int len);
+/* device time of the pair kernel over the last og_md_run / og_batch_run (ms) and its launch count;
+ * recorded with HIP events around each pair launch on the run's stream when timing is on */
+og_status   og_set_pair_timing(og_system *sys, int on);
+og_status   og_pair_kernel_time(og_system *sys, double *ms, int *n_launches);
+/* bytes of device memory a system with this description would hold */
+og_status   og_system_footprint(const og_system_desc *desc, const og_params *params, uint64_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,971 @@
+// og_engine.cpp -- host side of the MI355X engine: system/batch lifetime,
+// device memory layout, block maps, the per-step launch plan, hipGraph capture
+// and the C ABI declared in include/og_engine.h.
+#include "og_internal.h"
+#include "../../include/og_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+
+namespace {
+
+thread_local std::string g_last_error;
+
+void set_err(const char *fmt, ...) {
+    char    buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+#define OG_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess) {                                                                      \
+            set_err("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__);      \
+            return OG_E_DEVICE;                                                                      \
+        }                                                                                            \
+    } while (0)
+
+const double CHARGE_COEFF = 332.06378;  // CelesteObject.cpp:36
+const double EPS_REF      = 1e-10;      // CelesteObject.cpp:31
+const double PI_REF       = 3.14159265; // CelesteObject.cpp:26
+
+// Grid geometry of MiniCell::set_grid_parameters/set_box_info/set_grid_xy
+// (MiniCell.cpp:1279-1303, :1327-1362, :221-290), float semantics.
+struct GridGeom {
+    float L[3], lb[3], ub[3];
+    float cutoff_pair, cutoff_pair_half, cutoff_pair2;
+    int   nx, ny, nnx, nny, n_columns, n_uni_z;
+    float L_cell[2], L_z_uni;
+};
+
+#pragma clang fp contract(off)
+int grid_geom(GridGeom &g, int n_atoms, const double *box, double nscut) {
+    g.L[0] = (float)box[0];
+    g.L[1] = (float)box[4];
+    g.L[2] = (float)box[8];
+    for (int i = 0; i < 3; i++) {
+        const float Lh = (float)(g.L[i] * 0.5);
+        g.lb[i]        = (float)(box[9 + i] - (double)Lh);
+        g.ub[i]        = g.lb[i] + g.L[i];
+    }
+    g.cutoff_pair      = (float)nscut;
+    g.cutoff_pair_half = (float)(g.cutoff_pair * 0.5);
+    g.cutoff_pair2     = g.cutoff_pair * g.cutoff_pair;
+    g.n_uni_z          = (int)(g.L[2] / g.cutoff_pair_half);
+    g.L_z_uni          = g.L[2] / (float)g.n_uni_z;
+    const float volume = g.L[0] * g.L[1] * g.L[2];
+    float       w      = powf((float)((float)OG_NAC / ((float)n_atoms / volume)), (float)(1.0 / 3.0));
+    if (w > g.cutoff_pair_half) w = g.cutoff_pair_half;
+    g.nx        = (int)floorf(g.L[0] / w);
+    g.ny        = (int)floorf(g.L[1] / w);
+    g.L_cell[0] = g.L[0] / (float)g.nx;
+    g.L_cell[1] = g.L[1] / (float)g.ny;
+    g.nnx       = (int)ceilf(g.cutoff_pair / g.L_cell[0]);
+    g.nny       = (int)ceilf(g.cutoff_pair / g.L_cell[1]);
+    g.n_columns = g.nx * g.ny;
+    if (g.nnx > g.nx || g.nny > g.ny || g.n_uni_z < 2) return -1;
+    return 0;
+}
+
+// ZeroMultipoleSum::set_zms_params, dipole members (ZeroMultipoleSum.cpp:108-190), float members.
+struct ZdConst {
+    float cutoff, alpha, fcoeff, bcoeff, zcore, piewald;
+};
+ZdConst zd_const(double cutoff, double alpha) {
+    ZdConst z;
+    z.cutoff       = (float)cutoff;
+    z.alpha        = (float)alpha;
+    float c2       = z.cutoff * z.cutoff;
+    float c3       = c2 * z.cutoff;
+    float tmp      = z.cutoff * z.alpha;
+    float errorfc  = erfcf(tmp);
+    float expterm  = expf(-tmp * tmp);
+    z.piewald      = 0.0f;
+    if (z.alpha < EPS_REF) {
+        z.fcoeff = (float)(1.0 / c3);
+        z.bcoeff = (float)(0.5 * z.fcoeff);
+    } else {
+        float root_pi = (float)sqrt(PI_REF);
+        z.piewald     = (float)(2.0 / root_pi * z.alpha);
+        z.fcoeff      = errorfc / c3 + z.piewald * expterm / c2;
+        z.bcoeff      = (float)(0.5 * z.fcoeff);
+    }
+    z.zcore = errorfc / z.cutoff + z.bcoeff * c2;
+    return z;
+}
+
+// ZeroMultipoleSum::cal_self_energy (ZeroMultipoleSum.cpp:43-106): per-atom FP64
+// terms summed into a float member (d_self_mon).
+double self_energy(const ZdConst &z, int n, const float *q, int n_ex, const int *ex) {
+    std::vector<double> dwork(n, 0.0);
+    for (int i = 0; i < n_ex; i++) {
+        dwork[ex[2 * i]] += q[ex[2 * i + 1]];
+        dwork[ex[2 * i + 1]] += q[ex[2 * i]];
+    }
+    const double zmcore = z.zcore;
+    float        mon    = 0.0f;
+    for (int i = 0; i < n; i++) {
+        const float  pq = -z.piewald * q[i] * q[i];
+        const double e  = ((double)pq - zmcore * q[i] * (q[i] + dwork[i])) * 0.5 * CHARGE_COEFF;
+        mon             = (float)((double)mon + e);
+    }
+    return (double)mon;
+}
+#pragma clang fp contract(on)
+
+// exclusion-connectivity components (union-find over nb15off)
+std::vector<int> components(int n, const int *nb15off) {
+    std::vector<int> p(n);
+    std::iota(p.begin(), p.end(), 0);
+    auto find = [&](int x) {
+        while (p[x] != x) x = p[x] = p[p[x]];
+        return x;
+    };
+    for (int a = 0; a < n; a++)
+        for (int k = 0; k < OG_MAX_NB15; k++) {
+            const int b = nb15off[a * OG_MAX_NB15 + k];
+            if (b < 0) break;
+            int ra = find(a), rb = find(b);
+            if (ra != rb) p[std::max(ra, rb)] = std::min(ra, rb);
+        }
+    for (int a = 0; a < n; a++) p[a] = find(a);
+    return p;
+}
+
+template <class T> hipError_t dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+    if (e == hipSuccess) e = hipMemset(*p, 0, sizeof(T) * n);
+    return e;
+}
+
+int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace
+
+// ----------------------------------------------------------------- types --
+enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_N };
+
+struct og_system {
+    og_dev_sys          h{};           // host copy of the descriptor (device pointers)
+    og_dev_sys         *d_self = nullptr;
+    og_dev_counters    *d_cnt  = nullptr;
+    og_params           prm{};
+    GridGeom            geom{};
+    std::vector<void *> allocs;
+    int                 alpha_mode = 0;
+    int                 d_free     = 0;
+    long long           host_step  = 0;  // steps completed (host mirror of cnt->step)
+    bool                grid_valid = false;
+    double              pair_ms    = 0.0;
+    int                 pair_launches = 0;
+    bool                time_pair     = false;
+    struct og_exec     *ctx = nullptr;   // private batch-of-one
+};
+
+struct og_exec {
+    std::vector<og_system *> members;
+    og_dev_sys              *d_sys = nullptr;
+    int2                    *d_map[KIND_N]{};
+    int                      n_blocks[KIND_N]{};
+    hipStream_t              stream = nullptr;
+    bool                     dirty  = true;
+    std::map<long long, hipGraphExec_t> graphs;
+    std::vector<hipEvent_t>  ev;          // pair-kernel timing events
+    int                      alpha_mode = 0;
+    bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
+    int                      intvl = 1;
+};
+
+struct og_batch {
+    og_exec ctx;
+};
+
+namespace {
+
+void free_graphs(og_exec &x) {
+    for (auto &kv : x.graphs) hipGraphExecDestroy(kv.second);
+    x.graphs.clear();
+}
+
+og_status exec_release(og_exec &x) {
+    free_graphs(x);
+    for (auto &e : x.ev) hipEventDestroy(e);
+    x.ev.clear();
+    for (int k = 0; k < KIND_N; k++)
+        if (x.d_map[k]) hipFree(x.d_map[k]), x.d_map[k] = nullptr;
+    if (x.d_sys) hipFree(x.d_sys), x.d_sys = nullptr;
+    if (x.stream) hipStreamDestroy(x.stream), x.stream = nullptr;
+    return OG_OK;
+}
+
+// (re)build the descriptor array and block maps of an execution context
+og_status exec_prepare(og_exec &x) {
+    if (!x.dirty) return OG_OK;
+    free_graphs(x);
+    if (!x.stream) OG_HIP(hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking));
+    if (x.d_sys) hipFree(x.d_sys), x.d_sys = nullptr;
+    for (int k = 0; k < KIND_N; k++)
+        if (x.d_map[k]) hipFree(x.d_map[k]), x.d_map[k] = nullptr;
+    const int ns = (int)x.members.size();
+    if (ns == 0) {
+        x.dirty = false;
+        return OG_OK;
+    }
+    std::vector<og_dev_sys> hs(ns);
+    std::vector<int2>       maps[KIND_N];
+    x.alpha_mode   = x.members[0]->alpha_mode;
+    x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = false;
+    x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
+    for (int si = 0; si < ns; si++) {
+        og_system *s = x.members[si];
+        if (s->alpha_mode != x.alpha_mode) {
+            set_err("systems in one batch must share the electrostatics form (alpha = 0 or > 0)");
+            return OG_E_ARG;
+        }
+        if (s->prm.nsgrid_update_intvl != x.intvl) {
+            set_err("systems in one batch must share --nsgrid-update-intvl");
+            return OG_E_ARG;
+        }
+        hs[si] = s->h;
+        const og_dev_sys &d = s->h;
+        x.any_langevin |= d.integrator == 1;
+        x.any_leapfrog |= d.integrator == 0;
+        x.any_com |= d.com_cancel && d.n_com_groups > 0;
+        x.any_constraint |= d.constraint != 0;
+        const int n_bins = (d.n_columns + 1) * d.n_uni_z;
+        int       cnt[KIND_N];
+        cnt[KIND_ATOM]  = cdiv(d.n_atoms, OG_THREADS);
+        cnt[KIND_SLOT]  = cdiv(std::max(d.n_columns, n_bins), OG_THREADS);
+        cnt[KIND_CELLW] = cdiv(d.cap_cells, OG_THREADS / 64);
+        cnt[KIND_COL]   = d.n_columns;
+        cnt[KIND_SYS]   = 1;
+        cnt[KIND_MISC]  = d.n_misc_blocks;
+        cnt[KIND_TRIO]  = cdiv(std::max(d.n_trios, d.n_settle), OG_TRIO_THREADS);
+        cnt[KIND_COMA] = 0;  // filled below
+        for (int k = 0; k < KIND_N; k++)
+            for (int b = 0; b < cnt[k]; b++) maps[k].push_back(make_int2(si, b));
+    }
+    // COM-apply map from per-system totals (kept on the system)
+    maps[KIND_COMA].clear();
+    for (int si = 0; si < ns; si++) {
+        og_system *s = x.members[si];
+        const int  n = (s->h.com_cancel && s->h.n_com_groups) ? s->h.n_atom_blocks : 0;  // com atoms <= n_atoms
+        for (int b = 0; b < n; b++) maps[KIND_COMA].push_back(make_int2(si, b));
+    }
+    OG_HIP(hipMalloc((void **)&x.d_sys, sizeof(og_dev_sys) * ns));
+    OG_HIP(hipMemcpy(x.d_sys, hs.data(), sizeof(og_dev_sys) * ns, hipMemcpyHostToDevice));
+    for (int k = 0; k < KIND_N; k++) {
+        x.n_blocks[k] = (int)maps[k].size();
+        if (x.n_blocks[k]) {
+            OG_HIP(hipMalloc((void **)&x.d_map[k], sizeof(int2) * maps[k].size()));
+            OG_HIP(hipMemcpy(x.d_map[k], maps[k].data(), sizeof(int2) * maps[k].size(), hipMemcpyHostToDevice));
+        }
+    }
+    x.dirty = false;
+    return OG_OK;
+}
+
+og_launch L(const og_exec &x, int kind) { return og_launch{x.d_sys, x.d_map[kind], x.n_blocks[kind], x.stream}; }
+
+void enqueue_rebuild(og_exec &x) {
+    og_launch_grid_clear(L(x, KIND_SLOT), L(x, KIND_SLOT));
+    og_launch_grid_bin(L(x, KIND_ATOM));
+    og_launch_grid_scan(L(x, KIND_SYS));
+    og_launch_grid_scatter(L(x, KIND_ATOM));
+    og_launch_grid_sort(L(x, KIND_COL));
+    og_launch_enum_count(L(x, KIND_CELLW));
+    og_launch_enum_scan(L(x, KIND_SYS));
+    og_launch_enum_write(L(x, KIND_CELLW));
+    og_launch_masks(L(x, KIND_CELLW));
+}
+
+void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1) {
+    if (e0) hipEventRecord(e0, x.stream);
+    og_launch_pair(L(x, KIND_CELLW), x.alpha_mode);
+    if (e1) hipEventRecord(e1, x.stream);
+    og_launch_misc_forces(L(x, KIND_MISC));
+    og_launch_forces_to_work(L(x, KIND_ATOM));
+    og_launch_energy_reduce(L(x, KIND_SYS));
+}
+
+void enqueue_integrate(og_exec &x) {
+    if (x.any_leapfrog) {
+        og_launch_leapfrog_vel(L(x, KIND_ATOM));
+        if (x.any_com) {
+            og_launch_com_sum(L(x, KIND_SYS));
+            og_launch_com_apply(L(x, KIND_COMA));
+        }
+        og_launch_positions(L(x, KIND_ATOM));
+    }
+    if (x.any_langevin) og_launch_langevin_positions(L(x, KIND_ATOM));
+    if (x.any_constraint) og_launch_constraints(L(x, KIND_TRIO), L(x, KIND_MISC));
+    og_launch_finish(L(x, KIND_ATOM));
+    og_launch_step_end(L(x, KIND_SYS));
+}
+
+// Capture K steps (rebuild at steps whose index within the chunk is a
+// multiple of intvl; chunks start aligned) into one graph.
+og_status get_graph(og_exec &x, int K, bool timing, hipGraphExec_t *out) {
+    const long long key = (long long)K * 2 + (timing ? 1 : 0);
+    auto            it  = x.graphs.find(key);
+    if (it != x.graphs.end()) {
+        *out = it->second;
+        return OG_OK;
+    }
+    if (timing && (int)x.ev.size() < 2 * K) {
+        for (int k = (int)x.ev.size(); k < 2 * K; k++) {
+            hipEvent_t e;
+            OG_HIP(hipEventCreate(&e));
+            x.ev.push_back(e);
+        }
+    }
+    hipGraph_t g;
+    OG_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K; k++) {
+        if (k % x.intvl == 0) enqueue_rebuild(x);
+        enqueue_forces(x, timing ? x.ev[2 * k] : nullptr, timing ? x.ev[2 * k + 1] : nullptr);
+        enqueue_integrate(x);
+    }
+    OG_HIP(hipStreamEndCapture(x.stream, &g));
+    hipGraphExec_t ge;
+    OG_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    hipGraphDestroy(g);
+    x.graphs[key] = ge;
+    *out          = ge;
+    return OG_OK;
+}
+
+og_status check_flags(og_exec &x) {
+    for (og_system *s : x.members) {
+        og_dev_counters c;
+        OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+        s->host_step = c.step;
+        if (c.flags & (OG_F_CP_OVERFLOW | OG_F_CELL_OVERFLOW | OG_F_COL_OVERFLOW)) {
+            set_err("device list overflow (flags 0x%x): cells %d, cell pairs cap %d", c.flags, c.n_cells,
+                    s->h.cap_cell_pairs);
+            return OG_E_CAPACITY;
+        }
+        if (c.flags & OG_F_NAN) {
+            set_err("non-finite coordinate or force (flags 0x%x)", c.flags);
+            return OG_E_NAN;
+        }
+        if (c.shake_fail) {
+            set_err("SHAKE did not converge for %d constraint groups", c.shake_fail);
+            OG_HIP(hipMemset(&s->d_cnt->shake_fail, 0, sizeof(int)));
+            return OG_E_NOCONV;
+        }
+    }
+    return OG_OK;
+}
+
+// Run n steps for every member; chunks of K steps as graph replays.
+og_status exec_run(og_exec &x, int n_steps, bool timing) {
+    og_status st = exec_prepare(x);
+    if (st != OG_OK) return st;
+    if (x.members.empty() || n_steps <= 0) return OG_OK;
+    long long step = x.members[0]->host_step;
+    for (og_system *s : x.members)
+        if (s->host_step % x.intvl != step % x.intvl) {
+            set_err("batch members are at different rebuild phases");
+            return OG_E_STATE;
+        }
+    for (og_system *s : x.members) {
+        s->pair_ms       = 0.0;
+        s->pair_launches = 0;
+    }
+    int done = 0;
+    // unaligned head: direct launches
+    while (done < n_steps && (step % x.intvl) != 0) {
+        enqueue_forces(x, nullptr, nullptr);
+        enqueue_integrate(x);
+        done++;
+        step++;
+    }
+    const int K = std::max(x.intvl, std::min(100 / x.intvl, 10) * x.intvl);
+    while (n_steps - done >= K) {
+        hipGraphExec_t ge;
+        st = get_graph(x, K, timing, &ge);
+        if (st != OG_OK) return st;
+        OG_HIP(hipGraphLaunch(ge, x.stream));
+        if (timing) {
+            OG_HIP(hipStreamSynchronize(x.stream));
+            double ms = 0.0;
+            for (int k = 0; k < K; k++) {
+                float t = 0.0f;
+                OG_HIP(hipEventElapsedTime(&t, x.ev[2 * k], x.ev[2 * k + 1]));
+                ms += t;
+            }
+            for (og_system *s : x.members) {
+                s->pair_ms += ms;
+                s->pair_launches += K;
+            }
+        }
+        done += K;
+        step += K;
+    }
+    while (done < n_steps) {
+        if (step % x.intvl == 0) enqueue_rebuild(x);
+        enqueue_forces(x, nullptr, nullptr);
+        enqueue_integrate(x);
+        done++;
+        step++;
+    }
+    OG_HIP(hipStreamSynchronize(x.stream));
+    for (og_system *s : x.members) s->grid_valid = true;
+    return check_flags(x);
+}
+
+// ------------------------------------------------------------ allocation --
+og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
+    og_dev_sys &h = s->h;
+    auto        A = [&](auto **ptr, size_t n) -> bool {
+        hipError_t e = dalloc(ptr, n);
+        if (e != hipSuccess) {
+            set_err("hipMalloc of %zu elements failed: %s", n, hipGetErrorString(e));
+            return false;
+        }
+        s->allocs.push_back((void *)*ptr);
+        return true;
+    };
+    const int n = d->n_atoms;
+    h.n_atoms   = n;
+    const GridGeom &g = s->geom;
+    h.n_columns = g.n_columns;
+    h.nx = g.nx, h.ny = g.ny, h.nnx = g.nnx, h.nny = g.nny, h.n_uni_z = g.n_uni_z;
+    h.cap_atom_array = n + g.n_columns * OG_NAC;
+    h.cap_cells      = h.cap_atom_array / OG_NAC;
+    // cell-pair capacity from the candidate geometry: cells per column-height h
+    const double per_col  = (double)n / g.n_columns;
+    const double hcell    = (double)g.L[2] * OG_NAC / std::max(per_col, 1.0);
+    double       cand     = 0.0;
+    for (int dx = -g.nnx; dx <= g.nnx; dx++)
+        for (int dy = -g.nny; dy <= g.nny; dy++) {
+            const double ex = std::max(0, std::abs(dx) - 1) * (double)g.L_cell[0];
+            const double ey = std::max(0, std::abs(dy) - 1) * (double)g.L_cell[1];
+            const double r2 = (double)g.cutoff_pair2 - ex * ex - ey * ey;
+            if (r2 < 0) continue;
+            cand += (2.0 * std::sqrt(r2) + 2.0 * hcell) / hcell + 1.0;
+        }
+    const double est   = 0.5 * cand * (double)h.cap_cells;
+    const double fac   = p->capacity_factor > 0 ? p->capacity_factor : 1.0;
+    h.cap_cell_pairs   = (int)std::min(2.0e9, est * 1.25 * fac + 4096.0);
+    h.n_types          = d->n_lj_types;
+    h.n_excess         = d->n_excess;
+    h.n_bonds          = d->n_bonds;
+    h.n_pairs_c        = p->constraint ? d->n_shake_pairs : 0;
+    h.n_trios          = p->constraint ? d->n_shake_trios : 0;
+    h.n_settle         = p->constraint == OG_CONSTRAINT_SHAKE_SETTLE ? d->n_settle : 0;
+    h.n_com_groups     = p->com_cancel ? d->n_com_groups : 0;
+    h.integrator       = p->integrator;
+    h.constraint       = p->constraint;
+    h.com_cancel       = p->com_cancel && d->n_com_groups > 0;
+    h.gap_bug          = p->gap_bug;
+    h.excess_bug       = p->excess_bug;
+    h.rebuild_intvl    = p->nsgrid_update_intvl;
+    h.max_loops        = p->const_max_loops;
+    h.log_stride       = std::max(1, p->log_stride);
+    h.log_cap          = std::max(1, p->log_cap);
+    for (int k = 0; k < 3; k++) h.L[k] = g.L[k], h.lb[k] = g.lb[k], h.ub[k] = g.ub[k];
+    h.L_cell[0] = g.L_cell[0], h.L_cell[1] = g.L_cell[1];
+    h.L_z_uni      = g.L_z_uni;
+    h.cutoff_pair2 = g.cutoff_pair2;
+    const ZdConst z = zd_const(p->cutoff, p->ele_alpha);
+    h.cutoff  = z.cutoff;
+    h.cutoff2 = z.cutoff * z.cutoff;
+    h.zcore = z.zcore, h.bcoeff = z.bcoeff, h.fcoeff = z.fcoeff, h.piewald = z.piewald, h.alpha = z.alpha;
+    s->alpha_mode = z.alpha > 0.0f ? 1 : 0;
+    h.dt          = p->time_step;
+    h.tol         = p->const_tolerance;
+    h.temperature = p->temperature;
+    h.gamma       = p->langevin_gamma;
+    h.settle_dOH  = d->settle_dOH;
+    h.settle_dHH  = d->settle_dHH;
+    h.seed_lo     = (uint32_t)p->random_seed;
+    h.seed_hi     = (uint32_t)(p->random_seed >> 32) ^ 0x9E3779B9u;
+    h.vc_n_states = 0;
+
+    std::vector<float> q(n), m(n), minv(n);
+    for (int i = 0; i < n; i++) {
+        q[i]    = (float)d->charge[i];
+        m[i]    = (float)d->mass[i];
+        minv[i] = (float)(1.0 / (double)m[i]);  // SubBox.cpp:691 (real_pw mass_inv = 1.0 / mass)
+    }
+    h.e_self = self_energy(z, n, q.data(), d->n_excess, d->excess);
+
+    const int cap = h.cap_atom_array;
+    bool      ok  = A(&h.crd, 3 * n) && A(&h.crd_prev, 3 * n) && A(&h.crd_prev2, 3 * n) && A(&h.vel, 3 * n) &&
+              A(&h.vel_next, 3 * n) && A(&h.work, 3 * n) && A(&h.mass, n) && A(&h.mass_inv, n) && A(&h.charge, n) &&
+              A(&h.atom_type, n) && A(&h.nb15off, (size_t)n * OG_MAX_NB15) && A(&h.comp, n) &&
+              A(&h.lj6, (size_t)h.n_types * h.n_types) && A(&h.lj12, (size_t)h.n_types * h.n_types) &&
+              A(&h.excess, 2 * (size_t)h.n_excess) && A(&h.bond_ids, 2 * (size_t)h.n_bonds) &&
+              A(&h.bond_k, h.n_bonds) && A(&h.bond_r0, h.n_bonds) && A(&h.trio_ids, 3 * (size_t)h.n_trios) &&
+              A(&h.trio_d2, 3 * (size_t)h.n_trios) && A(&h.pairc_ids, 2 * (size_t)h.n_pairs_c) &&
+              A(&h.pairc_d2, h.n_pairs_c) && A(&h.settle_ids, 3 * (size_t)h.n_settle) &&
+              A(&h.com_off, h.n_com_groups + 1) && A(&h.com_atoms, h.n_com_groups ? d->com_group_off[h.n_com_groups] : 1) &&
+              A(&h.com_minv, h.n_com_groups + 1) && A(&h.col_of_atom, n) && A(&h.zkey, n) &&
+              A(&h.n_atoms_col, h.n_columns) && A(&h.col_fill, h.n_columns) && A(&h.head_cell, h.n_columns + 1) &&
+              A(&h.head_atom, h.n_columns + 1) && A(&h.n_cells_z, h.n_columns) && A(&h.tmp_ids, cap) &&
+              A(&h.atomids, cap) && A(&h.rev, n) && A(&h.gcrd, 3 * (size_t)cap) && A(&h.gxyzq, cap) &&
+              A(&h.gtype, cap) && A(&h.gcomp, cap) && A(&h.cell_col, h.cap_cells) && A(&h.cell_z, h.cap_cells) &&
+              A(&h.cell_zmin, h.cap_cells) && A(&h.cell_zmax, h.cap_cells) && A(&h.cell_u0, h.cap_cells) &&
+              A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
+              A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) &&
+              A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
+              A(&h.fx, 3 * (size_t)cap) && A(&h.e_cell, 3 * (size_t)h.cap_cells);
+    if (!ok) return OG_E_DEVICE;
+    h.n_misc_blocks = cdiv(std::max(h.n_excess, std::max(h.n_bonds, h.n_pairs_c)), OG_THREADS);
+    h.n_atom_blocks = cdiv(n, OG_THREADS);
+    ok = A(&h.e_misc, 2 * (size_t)std::max(h.n_misc_blocks, 1)) && A(&h.e_atom, std::max(h.n_atom_blocks, 1)) &&
+         A(&h.com_mom, 3 * (size_t)(h.n_com_groups + 1)) && A(&h.energy, OG_E_NTERMS) &&
+         A(&h.elog, (size_t)h.log_cap * OG_E_NTERMS) && A(&s->d_cnt, 1) && A(&s->d_self, 1);
+    if (!ok) return OG_E_DEVICE;
+    h.cnt = s->d_cnt;
+
+    // uploads
+    std::vector<float> lj6f((size_t)h.n_types * h.n_types), lj12f(lj6f.size());
+    for (size_t k = 0; k < lj6f.size(); k++) lj6f[k] = (float)d->lj6[k], lj12f[k] = (float)d->lj12[k];
+    std::vector<int> comp = components(n, d->nb15off);
+    // MmSystem::revise_crd_inbox (MmSystem.cpp:808-816): wrap the input into [lb, ub)
+    std::vector<double> crd0(d->crd, d->crd + 3 * (size_t)n);
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            double &x = crd0[3 * (size_t)i + k];
+            if (!std::isfinite(x)) {
+                set_err("non-finite input coordinate (atom %d)", i);
+                return OG_E_NAN;
+            }
+            if (x < (double)g.lb[k] || x >= (double)g.ub[k]) x -= std::floor((x - g.lb[k]) / g.L[k]) * g.L[k];
+            while (x < (double)g.lb[k]) x += g.L[k];
+            while (x >= (double)g.ub[k]) x -= g.L[k];
+        }
+    OG_HIP(hipMemcpy(h.crd, crd0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.crd_prev, crd0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.crd_prev2, crd0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.vel_next, d->vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.vel, d->vel, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.mass, m.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.mass_inv, minv.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.charge, q.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.atom_type, d->atom_type, sizeof(int) * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.nb15off, d->nb15off, sizeof(int) * (size_t)n * OG_MAX_NB15, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.comp, comp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.lj6, lj6f.data(), sizeof(float) * lj6f.size(), hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.lj12, lj12f.data(), sizeof(float) * lj12f.size(), hipMemcpyHostToDevice));
+    if (h.n_excess) OG_HIP(hipMemcpy(h.excess, d->excess, sizeof(int) * 2 * h.n_excess, hipMemcpyHostToDevice));
+    if (h.n_bonds) {
+        OG_HIP(hipMemcpy(h.bond_ids, d->bond_ids, sizeof(int) * 2 * h.n_bonds, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.bond_k, d->bond_k, sizeof(double) * h.n_bonds, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.bond_r0, d->bond_r0, sizeof(double) * h.n_bonds, hipMemcpyHostToDevice));
+    }
+    if (h.n_trios) {
+        OG_HIP(hipMemcpy(h.trio_ids, d->shake_trio_ids, sizeof(int) * 3 * h.n_trios, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.trio_d2, d->shake_trio_d2, sizeof(double) * 3 * h.n_trios, hipMemcpyHostToDevice));
+    }
+    if (h.n_pairs_c) {
+        OG_HIP(hipMemcpy(h.pairc_ids, d->shake_pair_ids, sizeof(int) * 2 * h.n_pairs_c, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.pairc_d2, d->shake_pair_d2, sizeof(double) * h.n_pairs_c, hipMemcpyHostToDevice));
+    }
+    if (h.n_settle) OG_HIP(hipMemcpy(h.settle_ids, d->settle_ids, sizeof(int) * 3 * h.n_settle, hipMemcpyHostToDevice));
+    if (h.n_com_groups) {
+        const int ng = h.n_com_groups;
+        OG_HIP(hipMemcpy(h.com_off, d->com_group_off, sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.com_atoms, d->com_group_atoms, sizeof(int) * d->com_group_off[ng], hipMemcpyHostToDevice));
+        std::vector<float> gminv(ng);
+        for (int gi = 0; gi < ng; gi++) {
+            float sum = 0.0f;  // MmSystem.cpp:572-582 float accumulation
+            for (int k = d->com_group_off[gi]; k < d->com_group_off[gi + 1]; k++) sum += m[d->com_group_atoms[k]];
+            gminv[gi] = (float)(1.0 / (double)sum);
+        }
+        OG_HIP(hipMemcpy(h.com_minv, gminv.data(), sizeof(float) * ng, hipMemcpyHostToDevice));
+    }
+    OG_HIP(hipMemcpy(s->d_self, &h, sizeof h, hipMemcpyHostToDevice));
+    return OG_OK;
+}
+
+og_status ensure_ctx(og_system *s) {
+    if (!s->ctx) {
+        s->ctx = new og_exec();
+        s->ctx->members.push_back(s);
+    }
+    return exec_prepare(*s->ctx);
+}
+
+}  // namespace
+
+// ================================================================ C ABI ==
+extern "C" {
+
+void og_params_default(og_params *p) {
+    std::memset(p, 0, sizeof *p);
+    p->cutoff              = 12.0;
+    p->nsgrid_cutoff       = 13.0;   // Config.h:76 (cutoff + 1)
+    p->nsgrid_update_intvl = 1;      // Config.h:77
+    p->ele_alpha           = 0.0;
+    p->time_step           = 0.0005; // Config.h:31
+    p->integrator          = OG_INTEGRATOR_LEAPFROG_PRESTO;
+    p->constraint          = OG_CONSTRAINT_NONE;
+    p->const_tolerance     = 0.000001;
+    p->const_max_loops     = 1000;
+    p->temperature         = 300.0;
+    p->langevin_gamma      = 0.0;
+    p->random_seed         = 0;
+    p->com_cancel          = 0;
+    p->gap_bug             = 1;
+    p->excess_bug          = 1;
+    p->log_stride          = 1;
+    p->log_cap             = 1024;
+    p->capacity_factor     = 1.0;
+}
+
+const char *og_status_string(og_status s) {
+    switch (s) {
+        case OG_OK: return "ok";
+        case OG_E_ARG: return "invalid argument";
+        case OG_E_CAPACITY: return "device list capacity exceeded";
+        case OG_E_NAN: return "non-finite coordinate or force";
+        case OG_E_NOCONV: return "constraint iteration did not converge";
+        case OG_E_DEVICE: return "HIP runtime error";
+        case OG_E_STATE: return "call out of order";
+    }
+    return "unknown";
+}
+
+og_status og_last_error(char *buf, int len) {
+    if (buf && len > 0) {
+        std::strncpy(buf, g_last_error.c_str(), len - 1);
+        buf[len - 1] = 0;
+    }
+    return OG_OK;
+}
+
+og_status og_set_device(int device_id) {
+    OG_HIP(hipSetDevice(device_id));
+    return OG_OK;
+}
+
+og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint64_t *bytes) {
+    if (!d || !p || !bytes) return OG_E_ARG;
+    GridGeom g;
+    if (grid_geom(g, d->n_atoms, d->box, p->nsgrid_cutoff) != 0) return OG_E_ARG;
+    const double n = d->n_atoms, cap = n + g.n_columns * OG_NAC, cells = cap / OG_NAC;
+    const double cand  = (2 * g.nnx + 1.0) * (2 * g.nny + 1.0) * (2.0 * g.cutoff_pair / (g.L[2] * OG_NAC * g.n_columns / n) + 3.0);
+    const double pairs = 0.5 * cand * cells * 1.25 * (p->capacity_factor > 0 ? p->capacity_factor : 1.0);
+    *bytes = (uint64_t)(n * (6 * 24 + 3 * 4 + 4 * 4 + 128) + cap * (4 * 3 + 24 + 16 + 8 + 24) + cells * 48 + pairs * 12 +
+                        (double)d->n_lj_types * d->n_lj_types * 8 + 4096);
+    return OG_OK;
+}
+
+og_status og_system_create(const og_system_desc *d, const og_params *p, og_system **out) {
+    if (!d || !p || !out || d->n_atoms <= 0 || !d->crd || !d->box || !d->nb15off) {
+        set_err("og_system_create: missing arguments");
+        return OG_E_ARG;
+    }
+    auto s = std::make_unique<og_system>();
+    s->prm = *p;
+    if (s->prm.nsgrid_update_intvl < 1) s->prm.nsgrid_update_intvl = 1;
+    if (grid_geom(s->geom, d->n_atoms, d->box, p->nsgrid_cutoff) != 0) {
+        set_err("box too small for the pair-list radius (needs at least 2*n_neighbors+1 columns and 2 z bins)");
+        return OG_E_ARG;
+    }
+    {
+        // every index the kernels dereference must be in range
+        const int n = d->n_atoms;
+        auto bad_ids = [n](const int *v, long long cnt) {
+            for (long long k = 0; k < cnt; k++)
+                if (v[k] < 0 || v[k] >= n) return true;
+            return false;
+        };
+        if (d->n_lj_types <= 0 || !d->atom_type || !d->lj6 || !d->lj12 || !d->charge || !d->mass || !d->vel) {
+            set_err("og_system_create: incomplete topology");
+            return OG_E_ARG;
+        }
+        for (int i = 0; i < n; i++)
+            if (d->atom_type[i] < 0 || d->atom_type[i] >= d->n_lj_types || !(d->mass[i] > 0.0)) {
+                set_err("atom %d: type out of range or non-positive mass", i);
+                return OG_E_ARG;
+            }
+        for (long long k = 0; k < (long long)n * OG_MAX_NB15; k++)
+            if (d->nb15off[k] < -1 || d->nb15off[k] >= n) {
+                set_err("nb15off entry %lld out of range", k);
+                return OG_E_ARG;
+            }
+        if ((d->n_excess && bad_ids(d->excess, 2LL * d->n_excess)) ||
+            (d->n_bonds && bad_ids(d->bond_ids, 2LL * d->n_bonds)) ||
+            (p->constraint && d->n_shake_pairs && bad_ids(d->shake_pair_ids, 2LL * d->n_shake_pairs)) ||
+            (p->constraint && d->n_shake_trios && bad_ids(d->shake_trio_ids, 3LL * d->n_shake_trios)) ||
+            (p->constraint == OG_CONSTRAINT_SHAKE_SETTLE && d->n_settle && bad_ids(d->settle_ids, 3LL * d->n_settle))) {
+            set_err("an atom index in the excess/bond/constraint lists is out of range");
+            return OG_E_ARG;
+        }
+        if (p->constraint == OG_CONSTRAINT_SHAKE_SETTLE && d->n_settle &&
+            !(d->settle_dOH > 0.0 && d->settle_dHH > 0.0 && d->settle_dHH < 2.0 * d->settle_dOH)) {
+            set_err("SETTLE needs 0 < dHH < 2 dOH");
+            return OG_E_ARG;
+        }
+    }
+    if (p->com_cancel && d->n_com_groups > 0) {
+        // groups must be disjoint for the parallel removal
+        std::vector<char> seen(d->n_atoms, 0);
+        for (int k = 0; k < d->com_group_off[d->n_com_groups]; k++) {
+            const int a = d->com_group_atoms[k];
+            if (a < 0 || a >= d->n_atoms || seen[a]) {
+                set_err("COM-cancel groups must be disjoint atom sets");
+                return OG_E_ARG;
+            }
+            seen[a] = 1;
+        }
+    }
+    s->d_free = d->d_free;
+    og_status st = sys_alloc(s.get(), d, p);
+    if (st != OG_OK) {
+        for (void *ptr : s->allocs) hipFree(ptr);
+        return st;
+    }
+    *out = s.release();
+    return OG_OK;
+}
+
+og_status og_system_destroy(og_system *s) {
+    if (!s) return OG_E_ARG;
+    if (s->ctx) {
+        hipStreamSynchronize(s->ctx->stream);
+        exec_release(*s->ctx);
+        delete s->ctx;
+    }
+    for (void *p : s->allocs) hipFree(p);
+    delete s;
+    return OG_OK;
+}
+
+og_status og_nsgrid_update(og_system *s) {
+    if (!s) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    if (st != OG_OK) return st;
+    enqueue_rebuild(*s->ctx);
+    OG_HIP(hipStreamSynchronize(s->ctx->stream));
+    s->grid_valid = true;
+    return check_flags(*s->ctx);
+}
+
+og_status og_calc_energy_force(og_system *s, int rebuild, double *work_atom, double *energies) {
+    if (!s) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    if (st != OG_OK) return st;
+    og_exec &x = *s->ctx;
+    if (rebuild || !s->grid_valid) enqueue_rebuild(x);
+    enqueue_forces(x, nullptr, nullptr);
+    OG_HIP(hipStreamSynchronize(x.stream));
+    s->grid_valid = true;
+    st            = check_flags(x);
+    if (st != OG_OK) return st;
+    if (work_atom) OG_HIP(hipMemcpy(work_atom, s->h.work, sizeof(double) * 3 * s->h.n_atoms, hipMemcpyDeviceToHost));
+    if (energies) {
+        double e[OG_E_NTERMS];
+        OG_HIP(hipMemcpy(e, s->h.energy, sizeof e, hipMemcpyDeviceToHost));
+        for (int k = 0; k < OG_NTERMS; k++) energies[k] = e[k];
+    }
+    return OG_OK;
+}
+
+og_status og_md_run(og_system *s, int n_steps) {
+    if (!s || n_steps < 0) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    if (st != OG_OK) return st;
+    return exec_run(*s->ctx, n_steps, s->time_pair);
+}
+
+og_status og_get_state(og_system *s, double *crd, double *vel_next, double *vel_just) {
+    if (!s) return OG_E_ARG;
+    const size_t n = 3 * (size_t)s->h.n_atoms;
+    if (crd) OG_HIP(hipMemcpy(crd, s->h.crd, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (vel_next) OG_HIP(hipMemcpy(vel_next, s->h.vel_next, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (vel_just) {
+        std::vector<double> a(n), b(n);
+        OG_HIP(hipMemcpy(a.data(), s->h.vel, sizeof(double) * n, hipMemcpyDeviceToHost));
+        OG_HIP(hipMemcpy(b.data(), s->h.vel_next, sizeof(double) * n, hipMemcpyDeviceToHost));
+        if (s->h.integrator == 1)
+            for (size_t k = 0; k < n; k++) vel_just[k] = a[k];
+        else
+            for (size_t k = 0; k < n; k++) vel_just[k] = (a[k] + b[k]) * 0.5;
+    }
+    return OG_OK;
+}
+
+og_status og_set_state(og_system *s, const double *crd, const double *vel_next) {
+    if (!s) return OG_E_ARG;
+    const size_t n = 3 * (size_t)s->h.n_atoms;
+    if (crd) {
+        OG_HIP(hipMemcpy(s->h.crd, crd, sizeof(double) * n, hipMemcpyHostToDevice));
+        s->grid_valid = false;
+    }
+    if (vel_next) OG_HIP(hipMemcpy(s->h.vel_next, vel_next, sizeof(double) * n, hipMemcpyHostToDevice));
+    return OG_OK;
+}
+
+og_status og_get_energies(og_system *s, double *energies) {
+    if (!s || !energies) return OG_E_ARG;
+    double e[OG_E_NTERMS];
+    OG_HIP(hipMemcpy(e, s->h.energy, sizeof e, hipMemcpyDeviceToHost));
+    for (int k = 0; k < OG_NTERMS; k++) energies[k] = e[k];
+    return OG_OK;
+}
+
+og_status og_get_energy_log(og_system *s, double *out, int max_rows, int *n_rows) {
+    if (!s || !out) return OG_E_ARG;
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    const long long nlog = (c.step + s->h.log_stride - 1) / s->h.log_stride;
+    const int       n    = (int)std::min<long long>(std::min<long long>(nlog, s->h.log_cap), max_rows);
+    std::vector<double> all((size_t)s->h.log_cap * OG_E_NTERMS);
+    OG_HIP(hipMemcpy(all.data(), s->h.elog, sizeof(double) * all.size(), hipMemcpyDeviceToHost));
+    // oldest kept row first
+    const long long first = nlog - n;
+    for (int r = 0; r < n; r++) {
+        const long long idx = (first + r) % s->h.log_cap;
+        std::memcpy(out + (size_t)r * OG_NTERMS, all.data() + idx * OG_E_NTERMS, sizeof(double) * OG_NTERMS);
+    }
+    if (n_rows) *n_rows = n;
+    return OG_OK;
+}
+
+og_status og_get_step(og_system *s, long long *step) {
+    if (!s || !step) return OG_E_ARG;
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    *step = c.step;
+    return OG_OK;
+}
+
+og_status og_get_grid_info(og_system *s, int *n_cells, int *n_columns, int *n_cell_pairs, int *n_atom_array) {
+    if (!s) return OG_E_ARG;
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    if (n_cells) *n_cells = c.n_cells;
+    if (n_columns) *n_columns = s->h.n_columns;
+    if (n_cell_pairs) *n_cell_pairs = c.n_cell_pairs;
+    if (n_atom_array) *n_atom_array = c.n_cells * OG_NAC;
+    return OG_OK;
+}
+
+og_status og_get_grid(og_system *s, int *atomids, double *grid_crd, int *idx_xy_head_cell) {
+    if (!s) return OG_E_ARG;
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    const size_t na = (size_t)c.n_cells * OG_NAC;
+    if (atomids) OG_HIP(hipMemcpy(atomids, s->h.atomids, sizeof(int) * na, hipMemcpyDeviceToHost));
+    if (grid_crd) OG_HIP(hipMemcpy(grid_crd, s->h.gcrd, sizeof(double) * 3 * na, hipMemcpyDeviceToHost));
+    if (idx_xy_head_cell)
+        OG_HIP(hipMemcpy(idx_xy_head_cell, s->h.head_cell, sizeof(int) * (s->h.n_columns + 1), hipMemcpyDeviceToHost));
+    return OG_OK;
+}
+
+og_status og_get_cell_pairs(og_system *s, int32_t *cp5, int *idx_head) {
+    if (!s) return OG_E_ARG;
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    std::vector<int>      head(c.n_cells + 1);
+    std::vector<uint32_t> c2(c.n_cell_pairs);
+    std::vector<uint64_t> mk(c.n_cell_pairs);
+    OG_HIP(hipMemcpy(head.data(), s->h.cp_head, sizeof(int) * (c.n_cells + 1), hipMemcpyDeviceToHost));
+    if (c.n_cell_pairs) {
+        OG_HIP(hipMemcpy(c2.data(), s->h.cp_c2, sizeof(int) * c.n_cell_pairs, hipMemcpyDeviceToHost));
+        OG_HIP(hipMemcpy(mk.data(), s->h.cp_mask, sizeof(uint64_t) * c.n_cell_pairs, hipMemcpyDeviceToHost));
+    }
+    if (idx_head) std::memcpy(idx_head, head.data(), sizeof(int) * (c.n_cells + 1));
+    if (cp5)
+        for (int c1 = 0; c1 < c.n_cells; c1++)
+            for (int p = head[c1]; p < head[c1 + 1]; p++) {
+                cp5[5 * p]     = c1;
+                cp5[5 * p + 1] = (int32_t)(c2[p] & 0xffffffu);
+                cp5[5 * p + 2] = (int32_t)(c2[p] >> 24);
+                cp5[5 * p + 3] = (int32_t)(uint32_t)(mk[p] & 0xffffffffu);
+                cp5[5 * p + 4] = (int32_t)(uint32_t)(mk[p] >> 32);
+            }
+    return OG_OK;
+}
+
+og_status og_get_work_grid(og_system *s, double *work_grid) {
+    if (!s || !work_grid) return OG_E_ARG;
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    const int na = c.n_cells * OG_NAC;
+    std::vector<int>    ids(na);
+    std::vector<double> w(3 * (size_t)s->h.n_atoms);
+    OG_HIP(hipMemcpy(ids.data(), s->h.atomids, sizeof(int) * na, hipMemcpyDeviceToHost));
+    OG_HIP(hipMemcpy(w.data(), s->h.work, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
+    for (int g = 0; g < na; g++)
+        for (int k = 0; k < 3; k++) work_grid[3 * g + k] = ids[g] >= 0 ? w[3 * (size_t)ids[g] + k] : 0.0;
+    return OG_OK;
+}
+
+og_status og_pair_kernel_time(og_system *s, double *ms, int *n) {
+    if (!s) return OG_E_ARG;
+    if (ms) *ms = s->pair_ms;
+    if (n) *n = s->pair_launches;
+    return OG_OK;
+}
+
+// timing switch for the pair kernel events inside captured graphs
+og_status og_set_pair_timing(og_system *s, int on) {
+    if (!s) return OG_E_ARG;
+    s->time_pair = on != 0;
+    return OG_OK;
+}
+
+// ---- batches ----
+og_status og_batch_create(og_batch **out) {
+    if (!out) return OG_E_ARG;
+    *out = new og_batch();
+    return OG_OK;
+}
+og_status og_batch_add(og_batch *b, og_system *s) {
+    if (!b || !s) return OG_E_ARG;
+    if (std::find(b->ctx.members.begin(), b->ctx.members.end(), s) != b->ctx.members.end()) return OG_E_ARG;
+    b->ctx.members.push_back(s);
+    b->ctx.dirty = true;
+    return OG_OK;
+}
+og_status og_batch_remove(og_batch *b, og_system *s) {
+    if (!b || !s) return OG_E_ARG;
+    auto it = std::find(b->ctx.members.begin(), b->ctx.members.end(), s);
+    if (it == b->ctx.members.end()) return OG_E_ARG;
+    if (b->ctx.stream) OG_HIP(hipStreamSynchronize(b->ctx.stream));
+    b->ctx.members.erase(it);
+    b->ctx.dirty = true;
+    return OG_OK;
+}
+og_status og_batch_run(og_batch *b, int n_steps) {
+    if (!b) return OG_E_ARG;
+    bool timing = false;
+    for (og_system *s : b->ctx.members) timing |= s->time_pair;
+    return exec_run(b->ctx, n_steps, timing);
+}
+og_status og_batch_size(og_batch *b, int *n) {
+    if (!b || !n) return OG_E_ARG;
+    *n = (int)b->ctx.members.size();
+    return OG_OK;
+}
+og_status og_batch_destroy(og_batch *b) {
+    if (!b) return OG_E_ARG;
+    if (b->ctx.stream) hipStreamSynchronize(b->ctx.stream);
+    exec_release(b->ctx);
+    delete b;
+    return OG_OK;
+}
+
+}  // extern "C"

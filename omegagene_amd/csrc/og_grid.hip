@@ -1,0 +1,513 @@
+// og_grid.hip -- device build of the 8-atom-cell grid and the cell-pair list.
+//
+// Bit-exact integer work: column binning, column order, in-column z order,
+// dummy padding, uniform-z bins, cell-pair enumeration order and the 64-bit
+// exclusion masks reproduce MiniCell (src/celeste/core/MiniCell.cpp) on the
+// same FP64 coordinates.  This file is compiled with -ffp-contract=off and
+// correctly rounded float division so every float expression rounds where
+// the reference host build rounds it.
+#include "og_internal.h"
+
+// ---------------------------------------------------------------- clear --
+// Zero per-column counters and the uniform-z bin table of each system.
+__global__ void __launch_bounds__(OG_THREADS) k_grid_clear(const og_dev_sys *__restrict__ S,
+                                                           const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         t  = bm.y * OG_THREADS + threadIdx.x;
+    if (t < s.n_columns) {
+        s.n_atoms_col[t] = 0;
+        s.col_fill[t]    = 0;
+    }
+    const int n_bins = (s.n_columns + 1) * s.n_uni_z;
+    if (t < n_bins) {
+        s.u2c_min[t] = OG_EMPTY_BIN;
+        s.u2c_max[t] = -1;
+    }
+    if (t == 0) s.cnt->flags &= ~(OG_F_CP_OVERFLOW | OG_F_COL_OVERFLOW | OG_F_CELL_OVERFLOW);
+}
+
+// ------------------------------------------------------------------ bin --
+// MiniCell::set_atoms_into_grid_xy binning (MiniCell.cpp:316-359):
+//   float x = crd - lb (double subtraction rounded to float), periodic wrap
+//   in float, column = floor(x / L_cell) with IEEE float division.
+__global__ void __launch_bounds__(OG_THREADS) k_grid_bin(const og_dev_sys *__restrict__ S,
+                                                         const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms) return;
+    int  c[2];
+    bool bad = false;
+    for (int d = 0; d < 2; d++) {
+        const double cd = s.crd[3 * i + d];
+        if (!(cd == cd) || cd > 1e30 || cd < -1e30) {
+            bad = true;
+            c[d] = 0;
+            continue;
+        }
+        if (cd > (double)s.ub[d] || cd < (double)s.lb[d]) atomicOr(&s.cnt->flags, OG_F_OUTSIDE_BOX);
+        float x = (float)(cd - (double)s.lb[d]);
+        if (x < -4.0f * s.L[d] || x >= 5.0f * s.L[d]) {  // far outside: flag instead of looping
+            atomicOr(&s.cnt->flags, OG_F_OUTSIDE_BOX);
+            x = 0.0f;
+        }
+        while (x < 0.0f) x += s.L[d];
+        while (x >= s.L[d]) x -= s.L[d];
+        int k = (int)floorf(x / s.L_cell[d]);
+        const int n = d == 0 ? s.nx : s.ny;
+        while (k < 0) k += n;
+        while (k >= n) k -= n;
+        c[d] = k;
+    }
+    const double z = s.crd[3 * i + 2];
+    if (!(z == z) || z > 1e30 || z < -1e30) bad = true;
+    if (bad) atomicOr(&s.cnt->flags, OG_F_NAN);
+    const int col    = c[0] + c[1] * s.nx;
+    s.col_of_atom[i] = col;
+    s.zkey[i]        = og_float_key(bad ? 0.0f : (float)z);
+    atomicAdd(&s.n_atoms_col[col], 1);
+}
+
+// ----------------------------------------------------------------- scan --
+// MiniCell::set_idx_xy_head (MiniCell.cpp:785-861): cells per column,
+// exclusive scan to head cell/atom, -1 heads for empty columns.
+__global__ void __launch_bounds__(1024) k_grid_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s  = S[map[blockIdx.x].x];
+    const int         nc = s.n_columns;
+    const int         T  = blockDim.x;
+    const int         per = (nc + T - 1) / T;
+    const int         k0  = threadIdx.x * per;
+    const int         k1  = min(k0 + per, nc);
+    __shared__ int    sh[1024];
+    __shared__ int    big;
+    if (threadIdx.x == 0) big = 0;
+    int local = 0, mx = 0;
+    for (int k = k0; k < k1; k++) {
+        const int n = s.n_atoms_col[k];
+        local += (n + OG_NAC - 1) / OG_NAC;
+        mx = max(mx, n);
+    }
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    if (mx > OG_SORT_CAP) atomicOr(&big, 1);
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (int off = 1; off < T; off <<= 1) {
+        const int v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int cell = sh[threadIdx.x] - local;
+    for (int k = k0; k < k1; k++) {
+        const int n  = s.n_atoms_col[k];
+        const int nz = (n + OG_NAC - 1) / OG_NAC;
+        s.n_cells_z[k] = nz;
+        s.head_cell[k] = n ? cell : -1;
+        s.head_atom[k] = n ? cell * OG_NAC : -1;
+        s.col_fill[k]  = 0;
+        cell += nz;
+    }
+    if (threadIdx.x == T - 1) {
+        int total = sh[T - 1];
+        s.head_cell[nc] = total;
+        s.head_atom[nc] = total * OG_NAC;
+        int fl = 0;
+        if (total > s.cap_cells) fl |= OG_F_CELL_OVERFLOW;
+        if (big) fl |= OG_F_COL_OVERFLOW;
+        if (fl) {
+            atomicOr(&s.cnt->flags, fl);
+            total = 0;  // nothing downstream runs on a grid that does not fit
+        }
+        s.cnt->n_cells = total;
+    }
+}
+
+// -------------------------------------------------------------- scatter --
+__global__ void __launch_bounds__(OG_THREADS) k_grid_scatter(const og_dev_sys *__restrict__ S,
+                                                             const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms || s.cnt->n_cells == 0) return;
+    const int col  = s.col_of_atom[i];
+    const int slot = atomicAdd(&s.col_fill[col], 1);
+    s.tmp_ids[s.head_atom[col] + slot] = i;
+}
+
+// ----------------------------------------------------------------- sort --
+// One block per column.  Ascending float z with the atom id as tie-break
+// (the reference Hoare quicksort, MiniCell.cpp:636-691, orders distinct z
+// identically; exact float ties inside a column are the only difference),
+// then padding to a multiple of 8 with copies of the last real atom
+// (MiniCell.cpp:574-580), grid-order coordinates rounded through float
+// (MiniCell.cpp:414-415), cell z extents and uniform-z bins
+// (MiniCell.cpp:884-936).
+__global__ void __launch_bounds__(OG_THREADS) k_grid_sort(const og_dev_sys *__restrict__ S,
+                                                          const int2 *__restrict__ map) {
+    const int2        bm  = map[blockIdx.x];
+    const og_dev_sys &s   = S[bm.x];
+    const int         col = bm.y;
+    __shared__ uint64_t keys[OG_SORT_CAP];
+    if (s.cnt->n_cells == 0) return;
+    const int n = s.n_atoms_col[col];
+    if (n == 0) return;
+    const int base = s.head_atom[col];
+    int       P    = 1;
+    while (P < n) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        if (t < n) {
+            const int id = s.tmp_ids[base + t];
+            keys[t]      = ((uint64_t)s.zkey[id] << 32) | (uint32_t)id;
+        } else {
+            keys[t] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < P; t += blockDim.x) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const uint64_t a = keys[t], b = keys[u];
+                    const bool     up = (t & k) == 0;
+                    if ((a > b) == up) {
+                        keys[t] = b;
+                        keys[u] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int nz    = s.n_cells_z[col];
+    const int last  = (int)(keys[n - 1] & 0xffffffffu);
+    const int cap   = s.cap_atom_array;
+    for (int t = threadIdx.x; t < nz * OG_NAC; t += blockDim.x) {
+        const int slot = base + t;
+        const int id   = t < n ? (int)(keys[t] & 0xffffffffu) : -1;
+        const int src  = t < n ? id : last;
+        const float x = (float)s.crd[3 * src], y = (float)s.crd[3 * src + 1], z = (float)s.crd[3 * src + 2];
+        s.atomids[slot]      = id;
+        s.gcrd[3 * slot]     = (double)x;
+        s.gcrd[3 * slot + 1] = (double)y;
+        s.gcrd[3 * slot + 2] = (double)z;
+        s.gxyzq[slot]        = make_float4(x, y, z, id >= 0 ? s.charge[src] : 0.0f);
+        s.gtype[slot]        = id >= 0 ? s.atom_type[src] : 0;
+        s.gcomp[slot]        = id >= 0 ? s.comp[src] : -1;
+        if (id >= 0) s.rev[id] = slot;
+        (void)cap;
+    }
+    __syncthreads();
+    const int hc = s.head_cell[col];
+    for (int c = threadIdx.x; c < nz; c += blockDim.x) {
+        const int   cell = hc + c;
+        const float zmin = (float)s.gcrd[3 * (cell * OG_NAC) + 2];
+        const float zmax = (float)s.gcrd[3 * (cell * OG_NAC + OG_NAC - 1) + 2];
+        s.cell_col[cell]  = col;
+        s.cell_z[cell]    = c;
+        s.cell_zmin[cell] = zmin;
+        s.cell_zmax[cell] = zmax;
+        const float a = zmin - s.lb[2];
+        const float b = zmax - s.lb[2];
+        // (int) truncation as MiniCell.cpp:912-913; the clamp only guards
+        // coordinates outside the box (the reference indexes out of bounds there)
+        const int u0 = min(max((int)(a / s.L_z_uni), 0), s.n_uni_z);
+        const int u1 = min(max((int)(b / s.L_z_uni), u0), s.n_uni_z);
+        s.cell_u0[cell] = u0;
+        s.cell_u1[cell] = u1;
+        for (int u = u0; u <= u1; u++) {
+            // uni id = x*n_uni_z + y*n_uni_z*nx + z  (MiniCell.cpp:1596-1598); a top that
+            // rounds onto the upper face spills into the next column's first bin, as there.
+            const int bin = col * s.n_uni_z + u;
+            atomicMin(&s.u2c_min[bin], c);
+            atomicMax(&s.u2c_max[bin], c);
+        }
+    }
+}
+
+// ---------------------------------------------------------- enumeration --
+// MiniCell::enumerate_cell_pairs (MiniCell.cpp:938-1149) for one cell1 per
+// wave: lane k takes neighbour column k in the reference loop order
+// (dx outer, dy inner), walks the three z images and the candidate cells in
+// ascending id, so the per-lane prefix sum reproduces the CPU list order.
+__device__ __forceinline__ bool og_valid_pair(int c1, int c2) {
+    // MiniCell::check_valid_pair (MiniCell.cpp:1150-1160)
+    const bool o1 = (c1 & 1) != 0, o2 = (c2 & 1) != 0;
+    if (o1) return !((c2 < c1 && !o2) || (c2 > c1 && o2));
+    return !((c2 < c1 && o2) || (c2 > c1 && !o2));
+}
+
+template <bool WRITE>
+__device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
+    const int lane  = threadIdx.x & 63;
+    const int col1  = s.cell_col[c1];
+    const int x1    = col1 % s.nx, y1 = col1 / s.nx;
+    const float z1b = s.cell_zmin[c1], z1t = s.cell_zmax[c1];
+    const int   u0  = s.cell_u0[c1], u1 = s.cell_u1[c1];
+    const int   wy  = 2 * s.nny + 1;
+    const int   ncand = (2 * s.nnx + 1) * wy;
+    const int   nuz = s.n_uni_z;
+    // z-bin windows per image (MiniCell.cpp:1014-1038)
+    int fz[3], lz[3];
+    int tf = u0 - 2;
+    if (tf < 0) {
+        fz[0] = nuz + tf;
+        lz[0] = nuz - 1;
+        tf    = 0;
+    } else {
+        fz[0] = lz[0] = -1;
+    }
+    fz[1]  = tf;
+    int tl = u1 + 2;
+    if (tl >= nuz) {
+        fz[2] = 0;
+        lz[2] = tl - nuz;
+        lz[1] = nuz - 1;
+    } else {
+        fz[2] = lz[2] = -1;
+        lz[1]         = tl;
+    }
+    int carry = WRITE ? s.cp_head[c1] : 0;
+    int total = 0;
+    for (int r0 = 0; r0 < ncand; r0 += 64) {
+        const int k     = r0 + lane;
+        int       mine  = 0;
+        int       c2lo[3], c2hi[3], img2 = 0;
+        float     dd01  = 0.0f;
+        int       imgxy = 0;
+        for (int ii = 0; ii < 3; ii++) c2lo[ii] = 0, c2hi[ii] = -1;
+        if (k < ncand) {
+            const int dx = k / wy - s.nnx, dy = k % wy - s.nny;
+            int       rx = x1 + dx, ry = y1 + dy, cx = rx, cy = ry;
+            if (rx < 0) {
+                cx = s.nx + rx;
+                imgxy |= 1;
+            } else if (rx >= s.nx) {
+                cx = rx - s.nx;
+                imgxy |= 2;
+            }
+            if (ry < 0) {
+                cy = s.ny + ry;
+                imgxy |= 4;
+            } else if (ry >= s.ny) {
+                cy = ry - s.ny;
+                imgxy |= 8;
+            }
+            float d0 = dx > 0 ? (float)(dx - 1) * s.L_cell[0] : dx < 0 ? (float)(dx + 1) * s.L_cell[0] : 0.0f;
+            float d1 = dy > 0 ? (float)(dy - 1) * s.L_cell[1] : dy < 0 ? (float)(dy + 1) * s.L_cell[1] : 0.0f;
+            d0        = d0 * d0;
+            d1        = d1 * d1;
+            dd01      = d0 + d1;
+            const int col2  = cx + cy * s.nx;
+            const int head2 = s.head_cell[col2];
+            for (int ii = 0; ii < 3; ii++) {
+                if (fz[ii] < 0 || head2 < 0) continue;
+                int fu = col2 * nuz + fz[ii], lu = col2 * nuz + lz[ii];
+                int f = -1, l = -1;
+                if (s.gap_bug) {
+                    // stop at the first empty bin after a non-empty one (MiniCell.cpp:1063-1077)
+                    int stg = 0;
+                    for (int u = fu; u <= lu; u++) {
+                        const bool full = s.u2c_min[u] != OG_EMPTY_BIN;
+                        if (stg == 0 && full) {
+                            stg = 1;
+                            f   = u;
+                            l   = lu;
+                        } else if (stg == 1 && !full) {
+                            l = u - 1;
+                            break;
+                        }
+                    }
+                } else {
+                    for (int u = fu; u <= lu; u++)
+                        if (s.u2c_min[u] != OG_EMPTY_BIN) {
+                            if (f < 0) f = u;
+                            l = u;
+                        }
+                }
+                if (f < 0) continue;
+                // a cell top rounding onto the upper face spills into the next
+                // column's first bin (see k_grid_sort); keep the range in this column
+                const int zc0 = s.u2c_min[f], zc1 = min(s.u2c_max[l], s.n_cells_z[col2] - 1);
+                if (zc0 > zc1) continue;
+                c2lo[ii] = head2 + zc0;
+                c2hi[ii] = head2 + zc1;
+            }
+            for (int ii = 0; ii < 3; ii++) {
+                const float sh = (float)(ii - 1) * s.L[2];
+                for (int c2 = c2lo[ii]; c2 <= c2hi[ii]; c2++) {
+                    const float z2t = s.cell_zmax[c2] + sh;
+                    const float z2b = s.cell_zmin[c2] + sh;
+                    float       d2;
+                    if (z2t < z1b) {
+                        d2 = z1b - z2t;
+                        d2 = d2 * d2;
+                    } else if (z2b > z1t) {
+                        d2 = z2b - z1t;
+                        d2 = d2 * d2;
+                    } else {
+                        d2 = 0.0f;
+                    }
+                    if (dd01 + d2 > s.cutoff_pair2) continue;
+                    if (!og_valid_pair(c1, c2)) continue;
+                    mine++;
+                }
+            }
+        }
+        // exclusive prefix of per-lane counts in lane order
+        int incl = mine;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += v;
+        }
+        const int round_total = __shfl(incl, 63, 64);
+        if (WRITE && mine) {
+            int p = carry + incl - mine;
+            for (int ii = 0; ii < 3; ii++) {
+                const float sh = (float)(ii - 1) * s.L[2];
+                img2           = ii == 0 ? 16 : ii == 2 ? 32 : 0;
+                for (int c2 = c2lo[ii]; c2 <= c2hi[ii]; c2++) {
+                    const float z2t = s.cell_zmax[c2] + sh;
+                    const float z2b = s.cell_zmin[c2] + sh;
+                    float       d2;
+                    if (z2t < z1b) {
+                        d2 = z1b - z2t;
+                        d2 = d2 * d2;
+                    } else if (z2b > z1t) {
+                        d2 = z2b - z1t;
+                        d2 = d2 * d2;
+                    } else {
+                        d2 = 0.0f;
+                    }
+                    if (dd01 + d2 > s.cutoff_pair2) continue;
+                    if (!og_valid_pair(c1, c2)) continue;
+                    if (p < s.cap_cell_pairs) s.cp_c2[p] = c2 | ((imgxy | img2) << 24);
+                    p++;
+                }
+            }
+        }
+        carry += round_total;
+        total += round_total;
+    }
+    if (!WRITE && lane == 0) *out_count = total;
+}
+
+__global__ void __launch_bounds__(OG_THREADS) k_enum_count(const og_dev_sys *__restrict__ S,
+                                                           const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
+    if (c1 >= s.cnt->n_cells) return;
+    og_enum_cell<false>(s, c1, &s.cp_count[c1]);
+}
+
+__global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__restrict__ S,
+                                                           const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
+    if (c1 >= s.cnt->n_cells) return;
+    og_enum_cell<true>(s, c1, nullptr);
+}
+
+// Exclusive scan of per-cell pair counts -> cp_head (one 1024-thread block per system).
+__global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s   = S[map[blockIdx.x].x];
+    const int         n   = s.cnt->n_cells;
+    const int         T   = blockDim.x;
+    const int         per = (n + T - 1) / T;
+    const int         k0  = threadIdx.x * per, k1 = min(k0 + per, n);
+    __shared__ long long sh[1024];
+    long long local = 0;
+    for (int k = k0; k < k1; k++) local += s.cp_count[k];
+    sh[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < T; off <<= 1) {
+        const long long v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += v;
+        __syncthreads();
+    }
+    long long p = sh[threadIdx.x] - local;
+    for (int k = k0; k < k1; k++) {
+        s.cp_head[k] = (int)min(p, (long long)s.cap_cell_pairs);
+        p += s.cp_count[k];
+    }
+    if (threadIdx.x == T - 1) {
+        const long long total = sh[T - 1];
+        s.cp_head[n]          = (int)min(total, (long long)s.cap_cell_pairs);
+        if (total > s.cap_cell_pairs) {
+            atomicOr(&s.cnt->flags, OG_F_CP_OVERFLOW);
+            s.cnt->n_cell_pairs = 0;
+        } else {
+            s.cnt->n_cell_pairs = (int)total;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- masks --
+// MiniCell::set_cell_pair_bitmask (MiniCell.cpp:1198-1237): one wave per cell
+// pair, lane = bit a2*8+a1; bit set (= skip) for a dummy atom, the lower
+// triangle of a self pair, or an nb15off partner.  nb15off is scanned only
+// when both atoms lie in the same exclusion component.
+__global__ void __launch_bounds__(OG_THREADS) k_masks(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const int2        bm   = map[blockIdx.x];
+    const og_dev_sys &s    = S[bm.x];
+    const int         lane = threadIdx.x & 63;
+    const int         c1   = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
+    if (c1 >= s.cnt->n_cells || s.cnt->n_cell_pairs == 0) return;
+    const int i1 = lane & 7, i2 = lane >> 3;
+    const int p0 = s.cp_head[c1], p1 = s.cp_head[c1 + 1];
+    const int a1   = c1 * OG_NAC + i1;
+    const int id1  = s.atomids[a1];
+    const int cmp1 = s.gcomp[a1];
+    for (int p = p0; p < p1; p++) {
+        const int c2  = s.cp_c2[p] & 0xffffff;
+        const int a2  = c2 * OG_NAC + i2;
+        const int id2 = s.atomids[a2];
+        bool skip     = id1 < 0 || id2 < 0 || (c1 == c2 && a1 >= a2);
+        if (!skip && cmp1 == s.gcomp[a2]) {
+            const int *l = s.nb15off + (size_t)id1 * OG_MAX_NB15;
+            for (int k = 0; k < OG_MAX_NB15; k++) {
+                const int v = l[k];
+                if (v == -1) break;
+                if (v == id2) {
+                    skip = true;
+                    break;
+                }
+            }
+        }
+        const uint64_t m = __ballot(skip);
+        if (lane == 0) s.cp_mask[p] = m;
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+void og_launch_grid_clear(const og_launch &cols, const og_launch &) {
+    hipLaunchKernelGGL(k_grid_clear, dim3(cols.n_blocks), dim3(OG_THREADS), 0, cols.stream, cols.sys, cols.map);
+}
+void og_launch_grid_bin(const og_launch &a) {
+    hipLaunchKernelGGL(k_grid_bin, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_grid_scan(const og_launch &a) {
+    hipLaunchKernelGGL(k_grid_scan, dim3(a.n_blocks), dim3(1024), 0, a.stream, a.sys, a.map);
+}
+void og_launch_grid_scatter(const og_launch &a) {
+    hipLaunchKernelGGL(k_grid_scatter, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_grid_sort(const og_launch &a) {
+    hipLaunchKernelGGL(k_grid_sort, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_enum_count(const og_launch &a) {
+    hipLaunchKernelGGL(k_enum_count, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_enum_scan(const og_launch &a) {
+    hipLaunchKernelGGL(k_enum_scan, dim3(a.n_blocks), dim3(1024), 0, a.stream, a.sys, a.map);
+}
+void og_launch_enum_write(const og_launch &a) {
+    hipLaunchKernelGGL(k_enum_write, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_masks(const og_launch &a) {
+    hipLaunchKernelGGL(k_masks, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}

@@ -1,0 +1,449 @@
+// og_integrate.hip -- FP64 integration on the device: leapfrog (presto),
+// position Langevin, SHAKE pair/trio, analytic SETTLE, COM-motion removal,
+// grid-coordinate refresh, PBC wrap and kinetic-energy reduction.
+//
+// Compiled with -ffp-contract=off: each expression rounds like the reference
+// host code it follows, so these kernels match the CPU oracle bit for bit on
+// identical inputs (reductions aside, which use a fixed tree order).
+#include "og_internal.h"
+
+#define OG_FORCE_VEL 4.184e-4                          // CelesteObject.cpp:37
+#define OG_GAS_CONST 8.31451                           // CelesteObject.cpp:38
+#define OG_KINETIC_COEFF ((1e7 / (4.184 * 1e3)) * 0.5) // CelesteObject.cpp:40
+
+// ------------------------------------------------------------- Philox --
+// Philox4x32-10 counter-based generator (J. K. Salmon, M. A. Moraes,
+// R. O. Dror, D. E. Shaw, "Parallel random numbers: as easy as 1, 2, 3",
+// SC'11): ten rounds of two 32x32->64 multiplies with Weyl-sequence key bumps.
+__device__ __forceinline__ uint4 og_philox(uint4 c, uint2 k) {
+    for (int r = 0; r < 10; r++) {
+        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+// Box-Muller in FP64 on two 32-bit uniforms in (0,1)
+__device__ __forceinline__ double2 og_gauss2(uint32_t a, uint32_t b) {
+    const double u1 = ((double)a + 0.5) * 2.3283064365386963e-10;
+    const double u2 = ((double)b + 0.5) * 2.3283064365386963e-10;
+    const double r  = sqrt(-2.0 * log(u1));
+    double       sn, cs;
+    sincos(6.283185307179586 * u2, &sn, &cs);
+    return make_double2(r * cs, r * sn);
+}
+
+// PBC::diff_crd_minim_image, double overload (PBC.cpp:62-82)
+__device__ __forceinline__ double og_mimg(double v, float L) {
+    const float Lh  = L * 0.5f;
+    const float Lhi = 1.0f / Lh;
+    v               = v - (double)(L * (float)(int)(v * (double)Lhi));
+    v               = v - (double)(L * (float)(int)(v * (double)Lhi));
+    return v;
+}
+
+// --------------------------------------------------- leapfrog velocities --
+// SubBox::cpy_vel_prev + update_velocities (SubBox.cpp:1427-1454)
+__global__ void __launch_bounds__(OG_THREADS) k_leapfrog_vel(const og_dev_sys *__restrict__ S,
+                                                             const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms || s.integrator != 0) return;
+    const double minv = (double)s.mass_inv[i];
+    for (int d = 0; d < 3; d++) {
+        const double v    = s.vel_next[3 * i + d];
+        s.vel[3 * i + d]  = v;
+        s.vel_next[3 * i + d] = v + (((s.dt * -OG_FORCE_VEL) * s.work[3 * i + d]) * minv);
+    }
+}
+
+// ------------------------------------------------------- COM cancel --
+// COMMotion::cancel_translation (COMMotion.cpp:30-55) with disjoint groups:
+// one block per system sums the momentum of each group in a fixed tree order.
+__global__ void __launch_bounds__(OG_THREADS) k_com_sum(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s = S[map[blockIdx.x].x];
+    __shared__ double sh[3 * OG_THREADS];
+    if (!s.com_cancel || s.integrator != 0) return;
+    for (int g = 0; g < s.n_com_groups; g++) {
+        double m[3] = {0.0, 0.0, 0.0};
+        for (int k = s.com_off[g] + threadIdx.x; k < s.com_off[g + 1]; k += blockDim.x) {
+            const int    a  = s.com_atoms[k];
+            const double ms = (double)s.mass[a];
+            for (int d = 0; d < 3; d++) m[d] += s.vel_next[3 * a + d] * ms;
+        }
+        for (int d = 0; d < 3; d++) sh[3 * threadIdx.x + d] = m[d];
+        __syncthreads();
+        for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off)
+                for (int d = 0; d < 3; d++) sh[3 * threadIdx.x + d] += sh[3 * (threadIdx.x + off) + d];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            for (int d = 0; d < 3; d++) s.com_mom[3 * g + d] = sh[d] * (double)s.com_minv[g];
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(OG_THREADS) k_com_apply(const og_dev_sys *__restrict__ S,
+                                                          const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         t  = bm.y * OG_THREADS + threadIdx.x;
+    if (!s.com_cancel || s.integrator != 0 || s.n_com_groups == 0 || t >= s.com_off[s.n_com_groups]) return;
+    int g = 0;
+    while (t >= s.com_off[g + 1]) g++;
+    const int a = s.com_atoms[t];
+    for (int d = 0; d < 3; d++) s.vel_next[3 * a + d] -= s.com_mom[3 * g + d];
+}
+
+// ---------------------------------------------------------- positions --
+// SubBox::cpy_crd_prev + update_coordinates(dt, crd, vel_next) (SubBox.cpp:1409-1412,1562-1580)
+__global__ void __launch_bounds__(OG_THREADS) k_positions(const og_dev_sys *__restrict__ S,
+                                                          const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms || s.integrator != 0) return;
+    for (int d = 0; d < 3; d++) {
+        const double x        = s.crd[3 * i + d];
+        s.crd_prev[3 * i + d] = x;
+        const double diff     = s.vel_next[3 * i + d] * s.dt;
+        s.crd[3 * i + d]      = x + diff;
+    }
+}
+
+// ----------------------------------------------------- Langevin positions --
+// DynamicsModeLangevin::calc_in_each_step (DynamicsMode.cpp:798-899):
+// crd_prev2 <- crd_prev <- crd; first step x1 = x0 + dt v0; later steps
+// SubBox::update_coordinates_langevin (SubBox.cpp:2077-2114) with N(0,1)
+// noise from Philox keyed by (seed, step, atom).
+__global__ void __launch_bounds__(OG_THREADS) k_langevin_positions(const og_dev_sys *__restrict__ S,
+                                                                   const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms || s.integrator != 1) return;
+    const long long step = s.cnt->step;
+    double          x[3];
+    for (int d = 0; d < 3; d++) {
+        x[d]                   = s.crd[3 * i + d];
+        s.crd_prev2[3 * i + d] = s.crd_prev[3 * i + d];
+        s.crd_prev[3 * i + d]  = x[d];
+    }
+    if (step == 0) {
+        for (int d = 0; d < 3; d++) {
+            const double v     = s.vel_next[3 * i + d];
+            s.vel[3 * i + d]   = v;
+            s.crd[3 * i + d]   = x[d] + v * s.dt;
+        }
+        return;
+    }
+    const uint4   r  = og_philox(make_uint4((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)i, 0x4c414e47u),
+                                 make_uint2(s.seed_lo, s.seed_hi));
+    const double2 z0 = og_gauss2(r.x, r.y);
+    const double2 z1 = og_gauss2(r.z, r.w);
+    const double  zeta[3] = {z0.x, z0.y, z1.x};
+    const double  dth     = s.dt * 0.5;
+    const double  gm      = s.gamma;
+    const double  mass    = (double)s.mass[i];
+    const double  minv    = (double)s.mass_inv[i];
+    const double  sig     = sqrt(OG_GAS_CONST * s.temperature * gm * mass / dth * 1e-7);
+    for (int d = 0; d < 3; d++) {
+        const double det_f = -OG_FORCE_VEL * s.work[3 * i + d];
+        const double stc_f = zeta[d] * sig;
+        const double p1    = x[d];
+        double       p2    = s.crd_prev2[3 * i + d];
+        p2 += nearbyint((p1 - p2) / (double)s.L[d]) * (double)s.L[d];
+        s.crd[3 * i + d] = 1.0 / (1.0 + gm * dth) * (2 * p1 - p2 + gm * dth * p2 + 4 * dth * dth * minv * (det_f + stc_f));
+    }
+}
+
+// --------------------------------------------------------- constraints --
+// ConstraintShake::shake_pair (ConstraintShake.cpp:18-49), one thread per constraint.
+__global__ void __launch_bounds__(OG_THREADS) k_shake_pair(const og_dev_sys *__restrict__ S,
+                                                           const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         t  = bm.y * OG_THREADS + threadIdx.x;
+    if (s.constraint == 0 || t >= s.n_pairs_c) return;
+    const int a = s.pairc_ids[2 * t], b = s.pairc_ids[2 * t + 1];
+    double    d[3], dp[3];
+    for (int k = 0; k < 3; k++) {
+        d[k]  = og_mimg(s.crd[3 * b + k] - s.crd[3 * a + k], s.L[k]);
+        dp[k] = og_mimg(s.crd_prev[3 * b + k] - s.crd_prev[3 * a + k], s.L[k]);
+    }
+    const double m1 = s.mass_inv[a], m2 = s.mass_inv[b], m12 = m1 + m2;
+    const double dsq  = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double dsqp = dp[0] * dp[0] + dp[1] * dp[1] + dp[2] * dp[2];
+    const double dip  = d[0] * dp[0] + d[1] * dp[1] + d[2] * dp[2];
+    const double va = m12 * m12 * dsqp, vb = m12 * dip, vc = dsq - s.pairc_d2[t];
+    const double disc = vb * vb - vc * va;
+    const double vg   = (vb - sqrt(disc > 0.0 ? disc : 0.0)) / va;
+    for (int k = 0; k < 3; k++) {
+        s.crd[3 * a + k] += m1 * vg * dp[k];
+        s.crd[3 * b + k] -= m2 * vg * dp[k];
+    }
+}
+
+// ConstraintShake::shake_trio (ConstraintShake.cpp:51-149), one thread per trio.
+__device__ void og_shake_trio(const og_dev_sys &s, int a1, int a2, int a3, const double *D) {
+    double d12[3], d23[3], d31[3], p12[3], p23[3], p31[3];
+    double g[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        d12[k] = og_mimg(s.crd[3 * a1 + k] - s.crd[3 * a2 + k], s.L[k]);
+        p12[k] = og_mimg(s.crd_prev[3 * a1 + k] - s.crd_prev[3 * a2 + k], s.L[k]);
+        d23[k] = og_mimg(s.crd[3 * a2 + k] - s.crd[3 * a3 + k], s.L[k]);
+        p23[k] = og_mimg(s.crd_prev[3 * a2 + k] - s.crd_prev[3 * a3 + k], s.L[k]);
+        d31[k] = og_mimg(s.crd[3 * a3 + k] - s.crd[3 * a1 + k], s.L[k]);
+        p31[k] = og_mimg(s.crd_prev[3 * a3 + k] - s.crd_prev[3 * a1 + k], s.L[k]);
+    }
+    const double m1 = s.mass_inv[a1], m2 = s.mass_inv[a2], m3 = s.mass_inv[a3];
+    const double c11 = m1 + m2, c12 = -m2, c13 = -m1;
+    const double c21 = -m2, c22 = m2 + m3, c23 = -m3;
+    const double c31 = -m1, c32 = -m3, c33 = m3 + m1;
+    bool conv = false;
+    for (int it = 0; it < s.max_loops; it++) {
+        for (int k = 0; k < 3; k++) {
+            d12[k] += (c11 * p12[k] * g[0] + c12 * p23[k] * g[1] + c13 * p31[k] * g[2]);
+            d23[k] += (c21 * p12[k] * g[0] + c22 * p23[k] * g[1] + c23 * p31[k] * g[2]);
+            d31[k] += (c31 * p12[k] * g[0] + c32 * p23[k] * g[1] + c33 * p31[k] * g[2]);
+        }
+#define OG_DOT(u, v) (u[0] * v[0] + u[1] * v[1] + u[2] * v[2])
+        const double f11 = 2.0 * c11 * OG_DOT(d12, p12), f12 = 2.0 * c12 * OG_DOT(d12, p23),
+                     f13 = 2.0 * c13 * OG_DOT(d12, p31);
+        const double f21 = 2.0 * c21 * OG_DOT(d23, p12), f22 = 2.0 * c22 * OG_DOT(d23, p23),
+                     f23 = 2.0 * c23 * OG_DOT(d23, p31);
+        const double f31 = 2.0 * c31 * OG_DOT(d31, p12), f32 = 2.0 * c32 * OG_DOT(d31, p23),
+                     f33 = 2.0 * c33 * OG_DOT(d31, p31);
+        const double df0 = D[0] - OG_DOT(d12, d12), df1 = D[1] - OG_DOT(d23, d23), df2 = D[2] - OG_DOT(d31, d31);
+#undef OG_DOT
+        if (fabs(df0 / D[0]) < s.tol && fabs(df1 / D[1]) < s.tol && fabs(df2 / D[2]) < s.tol) {
+            conv = true;
+            break;
+        }
+        const double det = 1.0 / (f11 * f22 * f33 + f12 * f23 * f31 + f13 * f21 * f32 - f11 * f23 * f32
+                                  - f12 * f21 * f33 - f13 * f22 * f31);
+        const double b11 = f22 * f33 - f23 * f32, b12 = f32 * f13 - f33 * f12, b13 = f12 * f23 - f13 * f22;
+        const double b21 = f23 * f31 - f21 * f33, b22 = f33 * f11 - f31 * f13, b23 = f13 * f21 - f11 * f23;
+        const double b31 = f21 * f32 - f22 * f31, b32 = f31 * f12 - f32 * f11, b33 = f11 * f22 - f12 * f21;
+        g[0] = (b11 * df0 + b12 * df1 + b13 * df2) * det;
+        g[1] = (b21 * df0 + b22 * df1 + b23 * df2) * det;
+        g[2] = (b31 * df0 + b32 * df1 + b33 * df2) * det;
+        g[3] += g[0];
+        g[4] += g[1];
+        g[5] += g[2];
+    }
+    if (!conv) atomicAdd(&s.cnt->shake_fail, 1);
+    for (int k = 0; k < 3; k++) {
+        s.crd[3 * a1 + k] += m1 * (g[3] * p12[k] - g[5] * p31[k]);
+        s.crd[3 * a2 + k] += m2 * (-g[3] * p12[k] + g[4] * p23[k]);
+        s.crd[3 * a3 + k] += m3 * (-g[4] * p23[k] + g[5] * p31[k]);
+    }
+}
+
+// Analytic SETTLE (S. Miyamoto, P. A. Kollman, J. Comput. Chem. 13 (1992) 952-962)
+// for a rigid 3-site water (O, H1, H2); same arithmetic as the oracle's orc_settle.
+__device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double dOH, double dHH) {
+    const double mO = s.mass[iO], mH = s.mass[iH1];
+    const double wt = mO + 2.0 * mH;
+    const double rc = 0.5 * dHH;
+    const double h  = sqrt(dOH * dOH - rc * rc);
+    const double ra = 2.0 * mH * h / wt;
+    const double rb = h - ra;
+    double b0[3], c0[3], xb[3], xc[3], com[3], a1[3], b1[3], c1[3];
+    for (int d = 0; d < 3; d++) {
+        b0[d] = og_mimg(s.crd_prev[3 * iH1 + d] - s.crd_prev[3 * iO + d], s.L[d]);
+        c0[d] = og_mimg(s.crd_prev[3 * iH2 + d] - s.crd_prev[3 * iO + d], s.L[d]);
+        xb[d] = og_mimg(s.crd[3 * iH1 + d] - s.crd[3 * iO + d], s.L[d]);
+        xc[d] = og_mimg(s.crd[3 * iH2 + d] - s.crd[3 * iO + d], s.L[d]);
+    }
+    for (int d = 0; d < 3; d++) {
+        com[d] = (mH * xb[d] + mH * xc[d]) / wt;
+        a1[d]  = -com[d];
+        b1[d]  = xb[d] - com[d];
+        c1[d]  = xc[d] - com[d];
+    }
+    double Z[3] = {b0[1] * c0[2] - b0[2] * c0[1], b0[2] * c0[0] - b0[0] * c0[2], b0[0] * c0[1] - b0[1] * c0[0]};
+    double X[3] = {a1[1] * Z[2] - a1[2] * Z[1], a1[2] * Z[0] - a1[0] * Z[2], a1[0] * Z[1] - a1[1] * Z[0]};
+    double Y[3] = {Z[1] * X[2] - Z[2] * X[1], Z[2] * X[0] - Z[0] * X[2], Z[0] * X[1] - Z[1] * X[0]};
+    const double nx = 1.0 / sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+    const double ny = 1.0 / sqrt(Y[0] * Y[0] + Y[1] * Y[1] + Y[2] * Y[2]);
+    const double nz = 1.0 / sqrt(Z[0] * Z[0] + Z[1] * Z[1] + Z[2] * Z[2]);
+    for (int d = 0; d < 3; d++) {
+        X[d] *= nx;
+        Y[d] *= ny;
+        Z[d] *= nz;
+    }
+#define OG_P(A, v) (A[0] * v[0] + A[1] * v[1] + A[2] * v[2])
+    const double xb0 = OG_P(X, b0), yb0 = OG_P(Y, b0), xc0 = OG_P(X, c0), yc0 = OG_P(Y, c0);
+    const double za1 = OG_P(Z, a1);
+    const double xb1 = OG_P(X, b1), yb1 = OG_P(Y, b1), zb1 = OG_P(Z, b1);
+    const double xc1 = OG_P(X, c1), yc1 = OG_P(Y, c1), zc1 = OG_P(Z, c1);
+#undef OG_P
+    const double sinphi = za1 / ra;
+    const double cosphi = sqrt(1.0 - sinphi * sinphi);
+    const double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi);
+    const double cospsi = sqrt(1.0 - sinpsi * sinpsi);
+    const double ya2 = ra * cosphi;
+    const double xb2 = -rc * cospsi;
+    const double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
+    const double yb2 = t1 - t2, yc2 = t1 + t2;
+    const double alpa   = xb2 * (xb0 - xc0) + yb0 * yb2 + yc0 * yc2;
+    const double beta   = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
+    const double gama   = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
+    const double al2be2 = alpa * alpa + beta * beta;
+    const double sinthe = (alpa * gama - beta * sqrt(al2be2 - gama * gama)) / al2be2;
+    const double costhe = sqrt(1.0 - sinthe * sinthe);
+    const double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
+    const double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
+    const double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};
+    double Onew[3];
+    for (int d = 0; d < 3; d++) Onew[d] = s.crd[3 * iO + d];
+    for (int d = 0; d < 3; d++) {
+        const double base   = Onew[d] + com[d];
+        s.crd[3 * iO + d]  = base + X[d] * a3[0] + Y[d] * a3[1] + Z[d] * a3[2];
+        s.crd[3 * iH1 + d] = base + X[d] * b3[0] + Y[d] * b3[1] + Z[d] * b3[2];
+        s.crd[3 * iH2 + d] = base + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
+    }
+}
+
+__global__ void __launch_bounds__(OG_TRIO_THREADS) k_trio(const og_dev_sys *__restrict__ S,
+                                                          const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         t  = bm.y * OG_TRIO_THREADS + threadIdx.x;
+    if (s.constraint == 0) return;
+    if (t < s.n_trios) og_shake_trio(s, s.trio_ids[3 * t], s.trio_ids[3 * t + 1], s.trio_ids[3 * t + 2],
+                                     s.trio_d2 + 3 * t);
+    if (t < s.n_settle) og_settle(s, s.settle_ids[3 * t], s.settle_ids[3 * t + 1], s.settle_ids[3 * t + 2],
+                                  s.settle_dOH, s.settle_dHH);
+}
+
+// ------------------------------------------------------------- finish --
+// After constraints: velocity from positions (SubBox::set_velocity_from_crd,
+// SubBox.cpp:1462-1481) or the Langevin central difference
+// (SubBox::set_velocities_just_langevin, SubBox.cpp:2054-2066); grid
+// coordinates advanced by the FP64 displacement (MiniCell::move_atom,
+// MiniCell.cpp:1312-1325) and republished as float; PBC wrap
+// (SubBox::revise_coordinates_pbc, SubBox.cpp:1482-1502); kinetic-energy
+// partials (DynamicsMode::cal_kinetic_energy, DynamicsMode.cpp:358-372).
+__global__ void __launch_bounds__(OG_THREADS) k_finish(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    double            ke = 0.0;
+    if (i < s.n_atoms) {
+        const long long step = s.cnt->step;
+        const int       g    = s.rev[i];
+        double          x[3], xp[3];
+        for (int d = 0; d < 3; d++) {
+            x[d]  = s.crd[3 * i + d];
+            xp[d] = s.crd_prev[3 * i + d];
+        }
+        if (s.integrator == 0) {
+            if (s.constraint) {
+                const double dt_inv = 1.0 / s.dt;
+                for (int d = 0; d < 3; d++) s.vel_next[3 * i + d] = (x[d] - xp[d]) * dt_inv;
+            }
+            double ka = 0.0;
+            for (int d = 0; d < 3; d++) {
+                const double vj = (s.vel[3 * i + d] + s.vel_next[3 * i + d]) * 0.5;
+                ka += vj * vj;
+            }
+            ke = ka * (double)s.mass[i];
+        } else {
+            double ka = 0.0;
+            if (step > 0) {
+                const double dt2inv = 1.0 / (2 * s.dt);
+                for (int d = 0; d < 3; d++) {
+                    double p2 = s.crd_prev2[3 * i + d];
+                    p2 += nearbyint((x[d] - p2) / (double)s.L[d]) * (double)s.L[d];
+                    const double v   = (x[d] - p2) * dt2inv;
+                    s.vel[3 * i + d] = v;
+                    ka += v * v;
+                }
+            } else {
+                for (int d = 0; d < 3; d++) ka += s.vel_next[3 * i + d] * s.vel_next[3 * i + d];
+            }
+            ke = ka * (double)s.mass[i];
+        }
+        float f[3];
+        for (int d = 0; d < 3; d++) {
+            const double diff = x[d] - xp[d];
+            const double gx   = s.gcrd[3 * g + d] + diff;
+            s.gcrd[3 * g + d] = gx;
+            f[d]              = (float)gx;
+            double xw         = x[d];
+            if (xw >= (double)s.ub[d]) xw -= (double)s.L[d];
+            else if (xw < (double)s.lb[d]) xw += (double)s.L[d];
+            s.crd[3 * i + d] = xw;
+        }
+        const float4 old = s.gxyzq[g];
+        s.gxyzq[g]       = make_float4(f[0], f[1], f[2], old.w);
+    }
+    // block partial of the kinetic sum, fixed order
+    __shared__ double sh[OG_THREADS / 64];
+    for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ke;
+    __syncthreads();
+    if (threadIdx.x == 0 && bm.y < s.n_atom_blocks) {
+        double t = 0.0;
+        for (int k = 0; k < OG_THREADS / 64; k++) t += sh[k];
+        s.e_atom[bm.y] = t;
+    }
+}
+
+// Kinetic energy, energy log entry, step counter (one thread block per system).
+__global__ void __launch_bounds__(OG_THREADS) k_step_end(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s = S[map[blockIdx.x].x];
+    __shared__ double sh[OG_THREADS];
+    double            t = 0.0;
+    for (int k = threadIdx.x; k < s.n_atom_blocks; k += blockDim.x) t += s.e_atom[k];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s.energy[OG_E_KIN]  = sh[0] * OG_KINETIC_COEFF;
+        const long long stp = s.cnt->step;
+        if (s.log_cap > 0 && (stp % s.log_stride) == 0) {
+            double *e = s.elog + (size_t)((stp / s.log_stride) % s.log_cap) * OG_E_NTERMS;
+            for (int k = 0; k < OG_E_NTERMS; k++) e[k] = s.energy[k];
+        }
+        s.cnt->step = stp + 1;
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+void og_launch_leapfrog_vel(const og_launch &a) {
+    hipLaunchKernelGGL(k_leapfrog_vel, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_com_sum(const og_launch &a) {
+    hipLaunchKernelGGL(k_com_sum, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_com_apply(const og_launch &a) {
+    hipLaunchKernelGGL(k_com_apply, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_positions(const og_launch &a) {
+    hipLaunchKernelGGL(k_positions, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_langevin_positions(const og_launch &a) {
+    hipLaunchKernelGGL(k_langevin_positions, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_constraints(const og_launch &trio, const og_launch &misc) {
+    if (misc.n_blocks)
+        hipLaunchKernelGGL(k_shake_pair, dim3(misc.n_blocks), dim3(OG_THREADS), 0, misc.stream, misc.sys, misc.map);
+    if (trio.n_blocks)
+        hipLaunchKernelGGL(k_trio, dim3(trio.n_blocks), dim3(OG_TRIO_THREADS), 0, trio.stream, trio.sys, trio.map);
+}
+void og_launch_finish(const og_launch &a) {
+    hipLaunchKernelGGL(k_finish, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_step_end(const og_launch &a) {
+    hipLaunchKernelGGL(k_step_end, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}

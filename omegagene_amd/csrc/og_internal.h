@@ -1,0 +1,239 @@
+// og_internal.h -- device-side data layout of the MI355X engine.
+//
+// One og_dev_sys describes one simulated system (a walker).  Every kernel is
+// batched: it receives the device array of descriptors of a batch plus a
+// block map (int2: system index, block index within that system), so a
+// single system and 64 uneven walkers run through the same kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OG_NAC 8                 // atoms per cell (MiniCell.h:10 N_ATOM_CELL)
+#define OG_MAX_NB15 32           // exclusion partners per atom (CelesteObject.h:26)
+#define OG_SORT_CAP 2048         // max atoms per column handled by the in-LDS sort
+#define OG_EMPTY_BIN 0x7fffffff  // uniform-z bin with no cell
+#define OG_FX_SCALE 4294967296.0 // fixed-point force scale, 2^32 units per kcal/mol/A
+#define OG_FX_INV (1.0 / 4294967296.0)
+
+// energy terms kept per system and per step
+enum {
+    OG_E_VDW = 0,    // LJ pair energy
+    OG_E_ELE,        // ZD pair energy (no excess, no self)
+    OG_E_EXCESS,     // ZD excluded-pair ("excess") energy
+    OG_E_SELF,       // ZD self energy (constant)
+    OG_E_BOND,       // harmonic bonds
+    OG_E_KIN,        // kinetic energy
+    OG_E_VIRIAL,     // scalar pair virial  sum r_ij . f_ij
+    OG_E_EXTEND,     // VcMD bias energy
+    OG_E_NTERMS
+};
+
+// flags in og_dev_counters::flags
+enum {
+    OG_F_CP_OVERFLOW   = 1,
+    OG_F_COL_OVERFLOW  = 2,
+    OG_F_CELL_OVERFLOW = 4,
+    OG_F_NAN           = 8,
+    OG_F_SHAKE         = 16,
+    OG_F_OUTSIDE_BOX   = 32,
+};
+
+struct og_dev_counters {
+    int       n_cells;
+    int       n_cell_pairs;
+    int       flags;
+    int       shake_fail;
+    long long step;   // steps completed since system creation
+    int       vs;     // current VcMD virtual state
+    int       pad;
+};
+
+struct og_dev_sys {
+    // ---- sizes ----
+    int n_atoms;
+    int n_columns, nx, ny, nnx, nny, n_uni_z;
+    int cap_atom_array, cap_cells, cap_cell_pairs;
+    int n_types;
+    int n_excess, n_bonds, n_trios, n_pairs_c, n_settle;
+    int n_com_groups;
+    int integrator;        // 0 leapfrog-presto, 1 langevin
+    int constraint;        // 0 none, 1 shake, 2 shake+settle
+    int com_cancel;
+    int gap_bug;
+    int excess_bug;
+    int rebuild_intvl;
+    int max_loops;
+    int log_stride;        // energies stored every log_stride steps
+    int log_cap;           // entries in the energy log
+    // ---- geometry, reference float semantics (PBC.h:21-27, MiniCell.h:42-54) ----
+    float L[3], lb[3], ub[3];
+    float L_cell[2];
+    float L_z_uni;
+    float cutoff_pair2;    // float(nsgrid_cutoff)^2
+    float cutoff;          // pair cutoff (float)
+    float cutoff2;
+    // ---- zero-dipole constants (ZeroMultipoleSum.cpp:108-190) ----
+    float zcore, bcoeff, fcoeff, piewald, alpha;
+    // ---- integration (FP64) ----
+    double dt, tol, temperature, gamma;
+    double settle_dOH, settle_dHH;
+    double e_self;
+    uint32_t seed_lo, seed_hi;
+    // ---- VcMD 1-D bias on the distance between two group mass centres ----
+    int    vc_n_states, vc_grp_a0, vc_grp_a1, vc_grp_b0, vc_grp_b1;   // index ranges into com_atoms
+    double vc_sigma, vc_coef, vc_kT;
+    double *vc_lo, *vc_hi;   // [n_states]
+    double *vc_hist;         // visit counts per state
+    double *vc_lambda;       // lambda per step (log)
+    // ---- atom-order arrays ----
+    double *crd, *crd_prev, *crd_prev2, *vel, *vel_next, *work;
+    float  *mass, *mass_inv, *charge;
+    int    *atom_type;
+    int    *nb15off;        // [n_atoms*32], -1 terminated
+    int    *comp;           // exclusion-connectivity component of each atom
+    float  *lj6, *lj12;     // [n_types*n_types]
+    int    *excess;         // [n_excess*2]
+    int    *bond_ids;       // [n_bonds*2]
+    double *bond_k, *bond_r0;
+    int    *trio_ids;       // [n_trios*3]
+    double *trio_d2;        // [n_trios*3]
+    int    *pairc_ids;      // [n_pairs_c*2]
+    double *pairc_d2;
+    int    *settle_ids;     // [n_settle*3]
+    int    *com_off, *com_atoms;   // COM-cancel groups
+    float  *com_minv;
+    // ---- grid ----
+    int      *col_of_atom;  // [n_atoms]
+    uint32_t *zkey;         // [n_atoms] order-preserving float-z bits
+    int      *n_atoms_col;  // [n_columns]
+    int      *col_fill;     // [n_columns]
+    int      *head_cell;    // [n_columns+1], -1 = empty column
+    int      *head_atom;    // [n_columns+1], -1 = empty column
+    int      *n_cells_z;    // [n_columns]
+    int      *tmp_ids;      // [cap_atom_array] atoms scattered by column
+    int      *atomids;      // [cap_atom_array] grid slot -> atom, -1 dummy
+    int      *rev;          // [n_atoms] atom -> grid slot
+    double   *gcrd;         // [cap_atom_array*3] grid-order coordinates (float-exact at rebuild)
+    float4   *gxyzq;        // [cap_atom_array] float x, y, z, charge
+    int      *gtype;        // [cap_atom_array]
+    int      *gcomp;        // [cap_atom_array] exclusion component (-1 dummy)
+    int      *cell_col;     // [cap_cells]
+    int      *cell_z;       // [cap_cells]
+    float    *cell_zmin, *cell_zmax;
+    int      *cell_u0, *cell_u1;
+    int      *u2c_min, *u2c_max;   // [(n_columns+1)*n_uni_z]
+    // ---- cell pairs ----
+    int      *cp_count;     // [cap_cells]
+    int      *cp_head;      // [cap_cells+1]
+    int      *cp_c2;        // [cap_cell_pairs] cell2 | image << 24
+    uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
+    // ---- forces / energies ----
+    long long *fx;          // [3*cap_atom_array] fixed-point gradient, SoA x|y|z
+    double    *e_cell;      // [cap_cells*3] per-cell1 vdw, ele, virial
+    double    *e_misc;      // [n_misc_blocks*4] excess, bond, ... partials
+    double    *e_atom;      // [n_atom_blocks] kinetic partials
+    double    *com_mom;     // [n_com_groups*3]
+    double    *energy;      // [OG_E_NTERMS] last step
+    double    *elog;        // [log_cap*OG_E_NTERMS]
+    int        n_misc_blocks;
+    int        n_atom_blocks;
+    og_dev_counters *cnt;
+};
+
+// ---- block maps ----
+enum og_map_kind {
+    OG_MAP_ATOM = 0,   // 256 threads per block over atoms
+    OG_MAP_SLOT,       // 256 threads per block over grid slots (capacity)
+    OG_MAP_CELLW,      // 4 waves (256 thr) per block, one wave per cell (capacity)
+    OG_MAP_COL,        // one block per column
+    OG_MAP_SYS,        // one block per system
+    OG_MAP_MISC,       // 256 threads per block over max(n_excess, n_bonds, n_pairs_c)
+    OG_MAP_TRIO,       // 128 threads per block over trios / settle waters
+    OG_MAP_CP,         // 4 waves per block, grid-stride over cell pairs (fixed block count per system)
+    OG_MAP_BIN,        // 256 threads per block over uniform-z bins
+    OG_MAP_NKINDS
+};
+
+#define OG_THREADS 256
+#define OG_TRIO_THREADS 128
+#define OG_CP_BLOCKS_PER_SYS_MIN 64
+
+// ---- small device helpers ----
+__device__ __forceinline__ int og_lane() { return __lane_id(); }
+
+__device__ __forceinline__ float og_dpp_sum8(float x) {
+    // sum over the 8 lanes sharing lane>>3 (quad swaps, then half-row mirror)
+    x += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, true));
+    x += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x4E, 0xF, 0xF, true));
+    x += __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x141, 0xF, 0xF, true));
+    return x;
+}
+
+__device__ __forceinline__ long long og_to_fixed(double v) { return __double2ll_rn(v * OG_FX_SCALE); }
+
+// Round-to-nearest fixed point without a conversion sequence: adding
+// 1.5*2^52 puts round(d) in the low mantissa bits (valid for |d| < 2^51,
+// i.e. |gradient| < 2^19 kcal/mol/A at the 2^32 scale).
+#define OG_FX_MAGIC 6755399441055744.0
+#define OG_FX_MAGIC_BITS 0x4338000000000000LL
+#define OG_FX_LIMIT 262144.0
+__device__ __forceinline__ long long og_fx_fast(double v) {
+    const double t = v * OG_FX_SCALE + OG_FX_MAGIC;
+    return __double_as_longlong(t) - OG_FX_MAGIC_BITS;
+}
+
+#define OG_GLOBAL(T) __attribute__((address_space(1))) T
+
+__device__ __forceinline__ void og_atomic_add_fx(long long *p, long long v) {
+    if (v != 0) atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+__device__ __forceinline__ void og_atomic_add_fx_g(OG_GLOBAL(long long) *p, long long v) {
+    __hip_atomic_fetch_add((OG_GLOBAL(unsigned long long) *)p, (unsigned long long)v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// float -> orderable uint32 (ascending float order == ascending uint order); -0 -> +0
+__device__ __forceinline__ uint32_t og_float_key(float f) {
+    if (f == 0.0f) f = 0.0f;
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float og_key_float(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+// kernel launch declarations (defined in og_grid.hip, og_pair.hip, og_integrate.hip)
+struct og_launch {
+    const og_dev_sys *sys;   // device array
+    const int2       *map;   // device block map
+    int               n_blocks;
+    hipStream_t       stream;
+};
+
+// grid build
+void og_launch_grid_clear(const og_launch &cols, const og_launch &bins);
+void og_launch_grid_bin(const og_launch &atoms);
+void og_launch_grid_scan(const og_launch &sys);
+void og_launch_grid_scatter(const og_launch &atoms);
+void og_launch_grid_sort(const og_launch &cols);
+void og_launch_enum_count(const og_launch &cellw);
+void og_launch_enum_scan(const og_launch &sys);
+void og_launch_enum_write(const og_launch &cellw);
+void og_launch_masks(const og_launch &cp);
+// forces
+void og_launch_pair(const og_launch &cellw, int alpha_mode);
+void og_launch_misc_forces(const og_launch &misc);
+void og_launch_forces_to_work(const og_launch &atoms);
+void og_launch_energy_reduce(const og_launch &sys);
+// integration
+void og_launch_leapfrog_vel(const og_launch &atoms);
+void og_launch_com_sum(const og_launch &sys);
+void og_launch_com_apply(const og_launch &atoms);
+void og_launch_positions(const og_launch &atoms);
+void og_launch_langevin_positions(const og_launch &atoms);
+void og_launch_constraints(const og_launch &trio, const og_launch &misc);
+void og_launch_finish(const og_launch &atoms);
+void og_launch_step_end(const og_launch &sys);
+void og_launch_vcmd(const og_launch &sys, const og_launch &atoms);
+void og_launch_refresh_grid(const og_launch &slots);

@@ -1,0 +1,349 @@
+// og_pair.hip -- LJ + zero-dipole cell-pair kernel, excluded-pair (excess)
+// and bond terms, fixed-point gradient reduction and energy reductions.
+//
+// Pair kernel (reference: kernel_pairwise_ljzd + cal_pair, cuda_setup.cu:487-675;
+// CPU: SubBox::calc_energy_pairwise, SubBox.cpp:956-1125): one wave64 per
+// cell1; every cell pair is one 8x8 tile with lane = a1 + 8*a2, so one pass
+// over 64 lanes covers the tile (the reference needs two half passes of a
+// warp32).  FP32 pair math, j-gradients reduced over the 8 i-lanes with DPP
+// and added as 64-bit fixed point, i-gradients kept in registers across the
+// cell1's pairs, energies accumulated per lane in FP64 and reduced per cell
+// in a fixed order (deterministic, no floating-point atomics).
+#include "og_internal.h"
+
+#define OG_CHARGE_COEFF 332.06378f   // CelesteObject.cpp:36
+
+typedef float og_f4 __attribute__((ext_vector_type(4)));
+
+// erfc(x) for x >= 0 from Abramowitz & Stegun 7.1.26 (|abs error| <= 1.5e-7),
+// sharing exp(-x^2) with the force term.
+__device__ __forceinline__ float og_erfc_as(float x, float expmx2) {
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, x, 1.0f));
+    float       p = fmaf(t, 1.061405429f, -1.453152027f);
+    p             = fmaf(t, p, 1.421413741f);
+    p             = fmaf(t, p, -0.284496736f);
+    p             = fmaf(t, p, 0.254829592f);
+    return p * t * expmx2;
+}
+
+// Adds one tile's j-gradient (already summed over the 8 i-lanes) for the
+// 8 lanes with a1 == 0.  The fixed-point conversion is branch free; the
+// atomics are skipped when the whole column of the tile contributed nothing.
+__device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, int a2, float rx, float ry, float rz,
+                                           int *flag) {
+    const long long vx = og_fx_fast(-(double)rx), vy = og_fx_fast(-(double)ry), vz = og_fx_fast(-(double)rz);
+    if ((vx | vy | vz) != 0) {
+        og_atomic_add_fx_g(FX + a2, vx);
+        og_atomic_add_fx_g(FX + cap + a2, vy);
+        og_atomic_add_fx_g(FX + 2 * cap + a2, vz);
+    }
+    if (fabsf(rx) + fabsf(ry) + fabsf(rz) > (float)OG_FX_LIMIT) atomicOr(flag, OG_F_NAN);
+}
+
+template <int ALPHA>
+__global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
+    if (c1 >= s.cnt->n_cells || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
+    const int lane = threadIdx.x & 63;
+    const int i    = lane & 7;
+    const int jl   = lane >> 3;
+
+    const OG_GLOBAL(og_f4) *__restrict__ X      = (const OG_GLOBAL(og_f4) *)s.gxyzq;
+    const OG_GLOBAL(int) *__restrict__ TY       = (const OG_GLOBAL(int) *)s.gtype;
+    const OG_GLOBAL(float) *__restrict__ LJ6    = (const OG_GLOBAL(float) *)s.lj6;
+    const OG_GLOBAL(float) *__restrict__ LJ12   = (const OG_GLOBAL(float) *)s.lj12;
+    const OG_GLOBAL(unsigned) *__restrict__ CP  = (const OG_GLOBAL(unsigned) *)s.cp_c2;
+    const OG_GLOBAL(uint64_t) *__restrict__ CM  = (const OG_GLOBAL(uint64_t) *)s.cp_mask;
+    OG_GLOBAL(long long) *__restrict__ FX       = (OG_GLOBAL(long long) *)s.fx;
+    const int   cap    = s.cap_atom_array;
+    const int   nt     = s.n_types;
+    const float rc2    = s.cutoff2;
+    const float zcore  = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
+    const float alpha  = s.alpha, piewald = s.piewald;
+    const float L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+    const int   p0 = __builtin_amdgcn_readfirstlane(s.cp_head[c1]);
+    const int   p1 = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
+
+    const og_f4  pi   = X[c1 * OG_NAC + i];
+    const int    trow = TY[c1 * OG_NAC + i] * nt;
+    const float  qi   = pi.w * OG_CHARGE_COEFF;
+
+    double dfx = 0.0, dfy = 0.0, dfz = 0.0, ev = 0.0, ee = 0.0, vir = 0.0;
+
+    for (int base = p0; base < p1; base += 64) {
+        const int nb = min(64, p1 - base);
+        // 64 cell-pair records per vector load; broadcast with readlane
+        const unsigned cpv = lane < nb ? CP[base + lane] : 0u;
+        const uint64_t mkv = lane < nb ? CM[base + lane] : ~0ull;
+        const unsigned mlo = (unsigned)mkv, mhi = (unsigned)(mkv >> 32);
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f, fe = 0.0f, fq = 0.0f, fv = 0.0f;
+        unsigned c2n = __builtin_amdgcn_readlane(cpv, 0);
+        int      a2n = (int)(c2n & 0xffffffu) * OG_NAC + jl;
+        og_f4    pjn = X[a2n];
+        int      tjn = TY[a2n];
+        for (int k = 0; k < nb; k++) {
+            const unsigned c2i  = c2n;
+            const int      a2   = a2n;
+            const og_f4    pj   = pjn;
+            const int      tj   = tjn;
+            const float a6  = LJ6[trow + tj];
+            const float a12 = LJ12[trow + tj];
+            if (k + 1 < nb) {  // prefetch the next tile's j atoms
+                c2n = __builtin_amdgcn_readlane(cpv, k + 1);
+                a2n = (int)(c2n & 0xffffffu) * OG_NAC + jl;
+                pjn = X[a2n];
+                tjn = TY[a2n];
+            }
+            const uint64_t mask =
+                ((uint64_t)__builtin_amdgcn_readlane(mhi, k) << 32) | (uint64_t)__builtin_amdgcn_readlane(mlo, k);
+            const unsigned img = c2i >> 24;
+            const float    sx  = (img & 1u) ? -L0 : ((img & 2u) ? L0 : 0.0f);
+            const float    sy  = (img & 4u) ? -L1 : ((img & 8u) ? L1 : 0.0f);
+            const float    sz  = (img & 16u) ? -L2 : ((img & 32u) ? L2 : 0.0f);
+            const float    dx  = pi.x - (pj.x + sx);
+            const float    dy  = pi.y - (pj.y + sy);
+            const float    dz  = pi.z - (pj.z + sz);
+            const float    r2  = dx * dx + dy * dy + dz * dz;
+            const bool     excl = __builtin_amdgcn_inverse_ballot_w64(mask);
+            const bool     ok   = !excl && (r2 < rc2);
+            const float    r2s  = ok ? r2 : 1.0f;
+            const float    rinv  = __builtin_amdgcn_rsqf(r2s);
+            const float    r2inv = rinv * rinv;
+            const float    r6inv = r2inv * r2inv * r2inv;
+            const float    t6    = a6 * r6inv;
+            const float    t12   = a12 * r6inv * r6inv;
+            const float    cc    = qi * pj.w;
+            float          g, eel;
+            if (ALPHA == 0) {
+                eel = cc * (rinv - zcore + bcoeff * r2s);
+                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv - fcoeff);
+            } else {
+                const float r   = r2s * rinv;
+                const float ar  = alpha * r;
+                const float ex  = __expf(-ar * ar);
+                const float erc = og_erfc_as(ar, ex);
+                eel = cc * (rinv * erc - zcore + bcoeff * r2s);
+                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (erc * rinv * r2inv + piewald * ex * r2inv - fcoeff);
+            }
+            g              = ok ? g : 0.0f;
+            const float wx = g * dx, wy = g * dy, wz = g * dz;
+            fx += wx;
+            fy += wy;
+            fz += wz;
+            fe += ok ? (t12 - t6) : 0.0f;
+            fq += ok ? eel : 0.0f;
+            fv += g * r2s;
+            const float rjx = og_dpp_sum8(wx);
+            const float rjy = og_dpp_sum8(wy);
+            const float rjz = og_dpp_sum8(wz);
+            if (i == 0) og_flush_j(FX, cap, a2, rjx, rjy, rjz, &s.cnt->flags);
+        }
+        dfx += (double)fx;
+        dfy += (double)fy;
+        dfz += (double)fz;
+        ev += (double)fe;
+        ee += (double)fq;
+        vir += (double)fv;
+    }
+    for (int off = 8; off < 64; off <<= 1) {
+        dfx += __shfl_xor(dfx, off, 64);
+        dfy += __shfl_xor(dfy, off, 64);
+        dfz += __shfl_xor(dfz, off, 64);
+    }
+    if (jl == 0) {
+        const int a1 = c1 * OG_NAC + i;
+        if (!(fabs(dfx) + fabs(dfy) + fabs(dfz) <= OG_FX_LIMIT)) atomicOr(&s.cnt->flags, OG_F_NAN);
+        const long long vx = og_fx_fast(dfx), vy = og_fx_fast(dfy), vz = og_fx_fast(dfz);
+        if ((vx | vy | vz) != 0) {
+            og_atomic_add_fx_g(FX + a1, vx);
+            og_atomic_add_fx_g(FX + cap + a1, vy);
+            og_atomic_add_fx_g(FX + 2 * cap + a1, vz);
+        }
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        ev += __shfl_xor(ev, off, 64);
+        ee += __shfl_xor(ee, off, 64);
+        vir += __shfl_xor(vir, off, 64);
+    }
+    if (lane == 0) {
+        s.e_cell[3 * c1]     = ev;
+        s.e_cell[3 * c1 + 1] = ee;
+        s.e_cell[3 * c1 + 2] = vir;
+    }
+}
+
+// ------------------------------------------------------------ misc terms --
+// Excluded-pair ZD terms (SubBox::calc_energy_ele_excess SubBox.cpp:1347-1367,
+// ForceField::calc_zms_excess ForceField.cpp:420-437,
+// calc_zero02pole_excess(_alpha0) ZeroMultipoleSum.cpp:191-217) in the
+// reference float arithmetic, and harmonic bonds E = k (r - r0)^2 in FP64
+// (ForceField::calc_bond ForceField.cpp:108-136).  Both add into the
+// fixed-point gradient at the atoms' grid slots.
+__device__ __forceinline__ void og_block_sum2(double &a, double &b, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_down(a, off, 64);
+        b += __shfl_down(b, off, 64);
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) {
+        sh[2 * w]     = a;
+        sh[2 * w + 1] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a = 0.0;
+        b = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+            a += sh[2 * k];
+            b += sh[2 * k + 1];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(OG_THREADS) k_misc(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+#pragma clang fp contract(off)
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         t  = bm.y * OG_THREADS + threadIdx.x;
+    const int         cap = s.cap_atom_array;
+    double            e_ex = 0.0, e_b = 0.0;
+    if (t < s.n_excess) {
+        const int a = s.excess[2 * t], b = s.excess[2 * t + 1];
+        float     d[3];
+        for (int k = 0; k < 3; k++) {
+            const float Lh  = s.L[k] * 0.5f;
+            const float Lhi = 1.0f / Lh;
+            float       v   = (float)s.crd[3 * a + k] - (float)s.crd[3 * b + k];
+            v               = v - s.L[k] * (float)(int)(v * Lhi);
+            v               = v - s.L[k] * (float)(int)(v * Lhi);
+            d[k]            = v;
+        }
+        const float r2     = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        const float r      = sqrtf(r2);
+        const float rinv   = 1.0f / r;
+        const float r2inv  = rinv * rinv;
+        const float r3inv  = r2inv * rinv;
+        const float cc     = (float)((double)(s.charge[a] * s.charge[b]) * 332.06378);
+        float       ene, gc;
+        if (s.alpha == 0.0f) {
+            ene = cc * s.bcoeff * r2;
+            gc  = cc * s.fcoeff;
+        } else {
+            const float x      = s.alpha * r;
+            const float erfcv  = erfcf(x);
+            const float erfv   = (float)(1.0 - (double)erfcv);
+            ene = s.excess_bug ? cc * (-erfv * r + s.bcoeff * r2) : cc * (-erfv * rinv + s.bcoeff * r2);
+            gc  = -cc * (erfcv * r3inv + s.piewald * expf(-x * x) * r2inv - s.fcoeff - r3inv);
+        }
+        e_ex       = (double)ene;
+        const int ga = s.rev[a], gb = s.rev[b];
+        for (int k = 0; k < 3; k++) {
+            const double w = (double)(gc * d[k]);
+            og_atomic_add_fx(&s.fx[k * cap + ga], og_to_fixed(w));
+            og_atomic_add_fx(&s.fx[k * cap + gb], og_to_fixed(-w));
+        }
+    }
+    if (t < s.n_bonds) {
+        const int a = s.bond_ids[2 * t], b = s.bond_ids[2 * t + 1];
+        double    d[3];
+        for (int k = 0; k < 3; k++) {
+            const float Lh  = s.L[k] * 0.5f;
+            const float Lhi = 1.0f / Lh;
+            double      v   = s.crd[3 * a + k] - s.crd[3 * b + k];
+            v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
+            v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
+            d[k]            = v;
+        }
+        const double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        const double dif  = r - s.bond_r0[t];
+        const double coef = s.bond_k[t] * dif;
+        e_b               = coef * dif;
+        const double gcf  = 2.0 * coef / r;
+        const int    ga = s.rev[a], gb = s.rev[b];
+        for (int k = 0; k < 3; k++) {
+            const double w = d[k] * gcf;
+            og_atomic_add_fx(&s.fx[k * cap + ga], og_to_fixed(w));
+            og_atomic_add_fx(&s.fx[k * cap + gb], og_to_fixed(-w));
+        }
+    }
+    __shared__ double sh[2 * (OG_THREADS / 64)];
+    og_block_sum2(e_ex, e_b, sh);
+    if (threadIdx.x == 0 && bm.y < s.n_misc_blocks) {
+        s.e_misc[2 * bm.y]     = e_ex;
+        s.e_misc[2 * bm.y + 1] = e_b;
+    }
+}
+
+// fixed point (grid order) -> FP64 gradient "work" in atom order (SubBox::add_work_from_minicell,
+// SubBox.cpp:1398-1407); clears the accumulators for the next evaluation.
+__global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys *__restrict__ S,
+                                                               const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms) return;
+    const int g   = s.rev[i];
+    const int cap = s.cap_atom_array;
+    for (int k = 0; k < 3; k++) {
+        const long long v   = s.fx[k * cap + g];
+        s.fx[k * cap + g]   = 0;
+        s.work[3 * i + k]   = (double)v * OG_FX_INV;
+    }
+}
+
+// Deterministic per-system energy reduction (fixed thread/tree order).
+__global__ void __launch_bounds__(OG_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
+                                                              const int2 *__restrict__ map) {
+    const og_dev_sys &s = S[map[blockIdx.x].x];
+    const int         n = s.cnt->n_cells;
+    double            a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = 0.0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        a += s.e_cell[3 * k];
+        b += s.e_cell[3 * k + 1];
+        c += s.e_cell[3 * k + 2];
+    }
+    for (int k = threadIdx.x; k < s.n_misc_blocks; k += blockDim.x) {
+        d += s.e_misc[2 * k];
+        e += s.e_misc[2 * k + 1];
+    }
+    __shared__ double sh[5 * OG_THREADS];
+    sh[5 * threadIdx.x]     = a;
+    sh[5 * threadIdx.x + 1] = b;
+    sh[5 * threadIdx.x + 2] = c;
+    sh[5 * threadIdx.x + 3] = d;
+    sh[5 * threadIdx.x + 4] = e;
+    __syncthreads();
+    for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 5; k++) sh[5 * threadIdx.x + k] += sh[5 * (threadIdx.x + off) + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        s.energy[OG_E_VDW]    = sh[0];
+        s.energy[OG_E_ELE]    = sh[1];
+        s.energy[OG_E_VIRIAL] = sh[2];
+        s.energy[OG_E_EXCESS] = sh[3];
+        s.energy[OG_E_BOND]   = sh[4];
+        s.energy[OG_E_SELF]   = s.e_self;
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+void og_launch_pair(const og_launch &a, int alpha_mode) {
+    if (alpha_mode)
+        hipLaunchKernelGGL(k_pair<1>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+    else
+        hipLaunchKernelGGL(k_pair<0>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_misc_forces(const og_launch &a) {
+    if (a.n_blocks == 0) return;
+    hipLaunchKernelGGL(k_misc, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_forces_to_work(const og_launch &a) {
+    hipLaunchKernelGGL(k_forces_to_work, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_energy_reduce(const og_launch &a) {
+    hipLaunchKernelGGL(k_energy_reduce, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}

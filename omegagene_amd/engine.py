@@ -1,0 +1,304 @@
+"""Python handle on libog_engine.so (the C ABI of include/og_engine.h).
+
+This is host plumbing for tests, the bench and the serving loop: every
+computation runs in the HIP library.  Loading fails loudly when the library
+is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .cls import LaunchSet
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libog_engine.so")
+_LIB = None
+
+OG_OK, OG_E_ARG, OG_E_CAPACITY, OG_E_NAN, OG_E_NOCONV, OG_E_DEVICE, OG_E_STATE = range(7)
+TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend"]
+NTERMS = len(TERMS)
+INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
+CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("og_engine status %d: %s" % (code, msg))
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [("n_atoms", C.c_int), ("box", C.POINTER(C.c_double)), ("crd", C.POINTER(C.c_double)),
+                ("vel", C.POINTER(C.c_double)), ("charge", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
+                ("atom_type", C.POINTER(C.c_int)), ("n_lj_types", C.c_int), ("lj6", C.POINTER(C.c_double)),
+                ("lj12", C.POINTER(C.c_double)), ("nb15off", C.POINTER(C.c_int)), ("n_excess", C.c_int),
+                ("excess", C.POINTER(C.c_int)), ("n_bonds", C.c_int), ("bond_ids", C.POINTER(C.c_int)),
+                ("bond_k", C.POINTER(C.c_double)), ("bond_r0", C.POINTER(C.c_double)),
+                ("n_shake_pairs", C.c_int), ("shake_pair_ids", C.POINTER(C.c_int)),
+                ("shake_pair_d2", C.POINTER(C.c_double)), ("n_shake_trios", C.c_int),
+                ("shake_trio_ids", C.POINTER(C.c_int)), ("shake_trio_d2", C.POINTER(C.c_double)),
+                ("n_settle", C.c_int), ("settle_ids", C.POINTER(C.c_int)), ("settle_dOH", C.c_double),
+                ("settle_dHH", C.c_double), ("n_com_groups", C.c_int), ("com_group_off", C.POINTER(C.c_int)),
+                ("com_group_atoms", C.POINTER(C.c_int)), ("d_free", C.c_int)]
+
+
+class Params(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("nsgrid_cutoff", C.c_double), ("nsgrid_update_intvl", C.c_int),
+                ("ele_alpha", C.c_double), ("time_step", C.c_double), ("integrator", C.c_int),
+                ("constraint", C.c_int), ("const_tolerance", C.c_double), ("const_max_loops", C.c_int),
+                ("temperature", C.c_double), ("langevin_gamma", C.c_double), ("random_seed", C.c_uint64),
+                ("com_cancel", C.c_int), ("gap_bug", C.c_int), ("excess_bug", C.c_int), ("log_stride", C.c_int),
+                ("log_cap", C.c_int), ("capacity_factor", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libog_engine.so is not built (run python -m omegagene_amd.build); "
+                               "the engine has no CPU fallback")
+        _LIB = C.CDLL(LIB_PATH)
+        _LIB.og_status_string.restype = C.c_char_p
+    return _LIB
+
+
+def _check(code):
+    if code != OG_OK:
+        buf = C.create_string_buffer(512)
+        lib().og_last_error(buf, 512)
+        raise EngineError(code, buf.value.decode() or lib().og_status_string(code).decode())
+
+
+def params_from_cfg(cfg: dict) -> Params:
+    """Map the reference .cfg keys (Config.cpp:37-367) onto og_params."""
+    p = Params()
+    lib().og_params_default(C.byref(p))
+    g = lambda k, d=None: cfg.get(k, d)  # noqa: E731
+    p.cutoff = float(g("cutoff", 12.0))
+    p.nsgrid_cutoff = float(g("nsgrid-cutoff", p.cutoff + 1.0))
+    p.nsgrid_update_intvl = int(g("nsgrid-update-intvl", 1))
+    p.ele_alpha = float(g("ele-alpha", 0.0))
+    p.time_step = float(g("time-step", 0.0005))
+    p.integrator = INTEGRATORS[g("integrator", "leapfrog-presto")]
+    p.constraint = CONSTRAINTS[g("constraint", "none")]
+    p.const_tolerance = float(g("const-tolerance", 1e-6))
+    p.const_max_loops = int(g("const-max-loops", 1000))
+    p.temperature = float(g("temperature", 300.0))
+    p.langevin_gamma = float(g("langevin-gamma", 0.0)) / 1000.0  # ps^-1 -> fs^-1 (Config.cpp:345-348)
+    p.random_seed = int(g("random-seed", 0)) & 0xFFFFFFFFFFFFFFFF
+    p.com_cancel = 1 if g("com-motion", "none") == "cancel" else 0
+    for k, attr in (("gap-bug", "gap_bug"), ("excess-bug", "excess_bug"), ("log-stride", "log_stride"),
+                    ("log-cap", "log_cap")):
+        if k in cfg:
+            setattr(p, attr, int(cfg[k]))
+    if "capacity-factor" in cfg:
+        p.capacity_factor = float(cfg["capacity-factor"])
+    return p
+
+
+class _Keep:
+    """Holds the numpy arrays a Desc points into."""
+
+
+def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None):
+    k = _Keep()
+
+    def d(a, dt=np.float64):
+        a = np.ascontiguousarray(a, dt)
+        k.__dict__["a%d" % len(k.__dict__)] = a
+        return a.ctypes.data_as(C.POINTER(C.c_double if dt == np.float64 else C.c_int))
+
+    n = ls.n_atoms
+    x = Desc()
+    x.n_atoms = n
+    x.box = d(ls.box)
+    x.crd = d(ls.crd)
+    x.vel = d(ls.vel)
+    x.charge = d(ls.charge)
+    x.mass = d(ls.mass)
+    x.atom_type = d(ls.atom_type, np.int32)
+    x.n_lj_types = ls.n_lj_types
+    x.lj6 = d(ls.lj6)
+    x.lj12 = d(ls.lj12)
+    x.nb15off = d(ls.nb15off_table(), np.int32)
+    ex = ls.excess_pairs()
+    x.n_excess = len(ex)
+    x.excess = d(ex, np.int32)
+    x.n_bonds = len(ls.bond_ids)
+    x.bond_ids = d(ls.bond_ids, np.int32)
+    x.bond_k = d(ls.bond_k)
+    x.bond_r0 = d(ls.bond_r0)
+    x.n_shake_pairs = len(ls.shake_pair_ids)
+    x.shake_pair_ids = d(ls.shake_pair_ids, np.int32)
+    x.shake_pair_d2 = d(np.asarray(ls.shake_pair_d) ** 2)
+    x.n_shake_trios = len(ls.shake_trio_ids)
+    x.shake_trio_ids = d(ls.shake_trio_ids, np.int32)
+    x.shake_trio_d2 = d(np.asarray(ls.shake_trio_d) ** 2)
+    x.n_settle = len(ls.settle_ids)
+    x.settle_ids = d(ls.settle_ids, np.int32)
+    if len(ls.settle_ids):
+        x.settle_dOH = float(ls.settle_d[0][0])
+        x.settle_dHH = float(ls.settle_d[0][1])
+    groups = com_groups if com_groups is not None else [g for _, g in ls.groups]
+    off = np.zeros(len(groups) + 1, np.int32)
+    for i, g_ in enumerate(groups):
+        off[i + 1] = off[i] + len(g_)
+    x.n_com_groups = len(groups)
+    x.com_group_off = d(off, np.int32)
+    x.com_group_atoms = d(np.concatenate([np.asarray(g_, np.int32) for g_ in groups]) if groups else
+                          np.zeros(1, np.int32), np.int32)
+    x.d_free = ls.d_free(constraint != 0)
+    return x, k
+
+
+class System:
+    """One simulated system resident on the GPU (og_system)."""
+
+    def __init__(self, ls: LaunchSet, cfg: dict, device: int = 0, com_groups=None):
+        L = lib()
+        _check(L.og_set_device(C.c_int(device)))
+        self.params = params_from_cfg(cfg)
+        self.desc, self._keep = desc_from_launchset(ls, self.params.constraint, com_groups)
+        self.n_atoms = ls.n_atoms
+        self.d_free = self.desc.d_free
+        self.h = C.c_void_p()
+        _check(L.og_system_create(C.byref(self.desc), C.byref(self.params), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().og_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def nsgrid_update(self):
+        _check(lib().og_nsgrid_update(self.h))
+
+    def energy_force(self, rebuild=True):
+        w = np.zeros((self.n_atoms, 3))
+        e = np.zeros(NTERMS)
+        _check(lib().og_calc_energy_force(self.h, C.c_int(int(rebuild)), w.ctypes.data_as(C.c_void_p),
+                                          e.ctypes.data_as(C.c_void_p)))
+        return w, dict(zip(TERMS, e))
+
+    def run(self, n_steps: int):
+        _check(lib().og_md_run(self.h, C.c_int(n_steps)))
+
+    def state(self):
+        crd = np.zeros((self.n_atoms, 3))
+        vn = np.zeros((self.n_atoms, 3))
+        vj = np.zeros((self.n_atoms, 3))
+        _check(lib().og_get_state(self.h, crd.ctypes.data_as(C.c_void_p), vn.ctypes.data_as(C.c_void_p),
+                                  vj.ctypes.data_as(C.c_void_p)))
+        return crd, vn, vj
+
+    def set_state(self, crd=None, vel_next=None):
+        c = np.ascontiguousarray(crd, np.float64) if crd is not None else None
+        v = np.ascontiguousarray(vel_next, np.float64) if vel_next is not None else None
+        _check(lib().og_set_state(self.h, c.ctypes.data_as(C.c_void_p) if c is not None else None,
+                                  v.ctypes.data_as(C.c_void_p) if v is not None else None))
+
+    def energies(self):
+        e = np.zeros(NTERMS)
+        _check(lib().og_get_energies(self.h, e.ctypes.data_as(C.c_void_p)))
+        return dict(zip(TERMS, e))
+
+    def energy_log(self, max_rows=100000):
+        out = np.zeros((max_rows, NTERMS))
+        n = C.c_int()
+        _check(lib().og_get_energy_log(self.h, out.ctypes.data_as(C.c_void_p), C.c_int(max_rows), C.byref(n)))
+        return out[:n.value]
+
+    def step(self):
+        s = C.c_longlong()
+        _check(lib().og_get_step(self.h, C.byref(s)))
+        return s.value
+
+    def grid_info(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().og_get_grid_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return dict(n_cells=a.value, n_columns=b.value, n_cell_pairs=c.value, n_atom_array=d.value)
+
+    def grid(self):
+        gi = self.grid_info()
+        ids = np.zeros(gi["n_atom_array"], np.int32)
+        gcrd = np.zeros((gi["n_atom_array"], 3))
+        head = np.zeros(gi["n_columns"] + 1, np.int32)
+        _check(lib().og_get_grid(self.h, ids.ctypes.data_as(C.c_void_p), gcrd.ctypes.data_as(C.c_void_p),
+                                 head.ctypes.data_as(C.c_void_p)))
+        return ids, gcrd, head
+
+    def cell_pairs(self):
+        gi = self.grid_info()
+        cp = np.zeros((gi["n_cell_pairs"], 5), np.int32)
+        head = np.zeros(gi["n_cells"] + 1, np.int32)
+        _check(lib().og_get_cell_pairs(self.h, cp.ctypes.data_as(C.c_void_p), head.ctypes.data_as(C.c_void_p)))
+        return cp, head
+
+    def work_grid(self):
+        gi = self.grid_info()
+        w = np.zeros((gi["n_atom_array"], 3))
+        _check(lib().og_get_work_grid(self.h, w.ctypes.data_as(C.c_void_p)))
+        return w
+
+    def set_pair_timing(self, on=True):
+        _check(lib().og_set_pair_timing(self.h, C.c_int(int(on))))
+
+    def pair_time(self):
+        ms, n = C.c_double(), C.c_int()
+        _check(lib().og_pair_kernel_time(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+class Batch:
+    """Many systems advanced together by the same launches (og_batch)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        _check(lib().og_batch_create(C.byref(self.h)))
+        self.members = []
+
+    def add(self, s: System):
+        _check(lib().og_batch_add(self.h, s.h))
+        self.members.append(s)
+
+    def remove(self, s: System):
+        _check(lib().og_batch_remove(self.h, s.h))
+        self.members.remove(s)
+
+    def run(self, n_steps: int):
+        _check(lib().og_batch_run(self.h, C.c_int(n_steps)))
+
+    def close(self):
+        if self.h:
+            lib().og_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def footprint(ls: LaunchSet, cfg: dict) -> int:
+    p = params_from_cfg(cfg)
+    d, keep = desc_from_launchset(ls, p.constraint)
+    b = C.c_uint64()
+    _check(lib().og_system_footprint(C.byref(d), C.byref(p), C.byref(b)))
+    return int(b.value)
+
+
+def exports():
+    """Names of the C-ABI symbols declared in include/og_engine.h."""
+    import re
+    hdr = open(os.path.join(os.path.dirname(_HERE), "include", "og_engine.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:og_status|void|const char \*)\s*(og_\w+)\s*\(", hdr, re.M)))

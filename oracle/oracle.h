@@ -1,0 +1,188 @@
+/*
+ * oracle.h -- CPU restatement of the celeste (omegagene) hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path (libog_engine.so) never links, loads or calls it.
+ *
+ * Every function states the reference file:line it restates (paths relative
+ * to the reference source tree src/celeste/).  Numeric semantics follow the
+ * reference host build (g++ -std=c++14 -O3, x86-64 SSE, no FP contraction):
+ * `real` = double, `real_pw` = float, double literals promote.
+ *
+ * Parity pins (what checks each function) are listed per function in
+ * DESIGN.md and in the oracle test-suite; functions with no pin are marked
+ * "parity unpinned" here.
+ */
+#ifndef OG_ORACLE_H
+#define OG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_N_ATOM_CELL 8
+#define ORC_MAX_N_NB15OFF 32
+
+/* Grid geometry (core/MiniCell.cpp:221-290, :1279-1303, :1327-1362; PBC.cpp:18-41). */
+typedef struct {
+    float L[3];          /* PBC::L (float) */
+    float lb[3];         /* PBC::lower_bound (float) */
+    float cutoff_pair;   /* --nsgrid-cutoff (float) */
+    float cutoff_pair_half;
+    float cutoff_pair2;
+    int   n_cells_xyz[3];
+    float L_cell_xyz[3];
+    int   n_neighbors_xyz[3];
+    int   n_columns;
+    int   n_uni_z;
+    float L_z_uni;
+    int   max_n_cells;       /* capacity heuristic (MiniCell.cpp:264-270) */
+    int   max_n_cell_pairs;  /* capacity heuristic (MiniCell.cpp:272-280) */
+    int   max_n_atoms_column;
+} orc_grid_t;
+
+/* PBC from the .cls box block: val[12] = {Lx,0,0, 0,Ly,0, 0,0,Lz, cx,cy,cz} (PBC.cpp:18-41). */
+void orc_pbc_from_box(const double val[12], float L[3], float lb[3], float ub[3]);
+
+/* set_grid_parameters + set_box_info + set_max_n_atoms_region + set_grid_xy. */
+int orc_grid_setup(orc_grid_t *g, int n_atoms, const double box[12], double nsgrid_cutoff);
+
+/* Result of one grid rebuild (MiniCell::set_atoms_into_grid_xy and callees). */
+typedef struct {
+    int     n_cells;
+    int     n_atom_array;      /* n_cells * 8 */
+    int    *atomids;           /* [cap_atom_array] grid slot -> atom id, -1 = dummy */
+    double *crd;               /* [cap_atom_array*3] grid-order coordinates (float-exact) */
+    int    *cell_crd;          /* [cap_cells*3] (column x, column y, z index) */
+    int    *idx_xy_head_cell;  /* [n_columns+1]; -1 for an empty column */
+    int    *idx_xy_head_atom;  /* [n_columns+1]; -1 for an empty column */
+    int    *n_atoms_col;       /* [n_columns] */
+    int    *n_cells_z;         /* [n_columns] */
+    int    *column_of_atom;    /* [n_atoms] column id of each atom (atom order) */
+    int     cap_atom_array;
+    int     cap_cells;
+} orc_cells_t;
+
+/* Allocate / free the buffers of an orc_cells_t for n_atoms atoms. */
+int  orc_cells_alloc(orc_cells_t *c, const orc_grid_t *g, int n_atoms);
+void orc_cells_free(orc_cells_t *c);
+
+/* Full rebuild: binning, column scan, cycle-chasing reorder, dummy padding,
+ * in-column z quicksort (MiniCell.cpp:292-402, 404-501, 503-613, 618-691, 785-869).
+ * crd: atom-order double coordinates, assumed inside [lb, lb+L). */
+int orc_cells_build(orc_cells_t *c, const orc_grid_t *g, int n_atoms, const double *crd);
+
+/* Cell-pair record as the reference lays it out (MiniCell.h:17-26). */
+typedef struct {
+    int32_t cell_id1;
+    int32_t cell_id2;
+    int32_t image;
+    int32_t pair_mask[2];
+} orc_cellpair_t;
+
+/* Enumerate cell pairs in the reference CPU order and set their exclusion
+ * bitmasks (MiniCell.cpp:896-936 set_uniform_grid, :938-1149 enumerate_cell_pairs,
+ * :1150-1160 check_valid_pair, :1161-1196 add_cell_pair, :1198-1237 set_cell_pair_bitmask).
+ * nb15off: [n_atoms*32] exclusion partner lists, -1 terminated (MmSystem.cpp:478-525).
+ * gap_bug != 0 reproduces the reference truncation of the uniform-z scan at
+ * the first empty bin (MiniCell.cpp:1063-1077); 0 scans every bin in range.
+ * Returns the number of pairs, or -1 if cap is exceeded. */
+int orc_enumerate_cell_pairs(const orc_grid_t *g, const orc_cells_t *c, const int *nb15off,
+                             int gap_bug, orc_cellpair_t *out, int cap, int *idx_head /* [n_cells+1] */);
+
+/* Zero-dipole constants (ZeroMultipoleSum.cpp:108-190; float members). */
+typedef struct {
+    float cutoff, alpha;
+    float fcoeff, bcoeff, zcore, piewald;
+} orc_zd_t;
+void orc_zd_setup(orc_zd_t *zd, double cutoff, double alpha);
+
+/* Self energy (ZeroMultipoleSum.cpp:43-106). Returns the sum of per-atom terms
+ * (d_self_mon, stored in a float member) and writes the per-atom FP64 terms. */
+double orc_self_energy(const orc_zd_t *zd, int n_atoms, const float *charge,
+                       int n_excess, const int *excess_pairs /* [n_excess*2] */, double *e_self);
+
+/* Excluded-pair list i<j from nb15off (MmSystem.cpp:700-736). Returns count. */
+int orc_excess_pairs(int n_atoms, const int *nb15off, int *out_pairs /* cap*2 */, int cap);
+
+/* Reference-faithful FP32 pair term (ForceField.cpp:333-418 +
+ * ZeroMultipoleSum.cpp:309-346).  d12 = crd1 - crd2 already imaged.
+ * Returns 1 if r < cutoff (pair counted), 0 otherwise. */
+int orc_pair_fp32(const orc_zd_t *zd, const float d12[3], float a6, float a12, float q1, float q2,
+                  float *e_vdw, float *e_ele, float work[3]);
+
+/* Pair loop over a cell-pair list (SubBox.cpp:956-1125): FP32 pair terms,
+ * FP64 accumulation in list order.  work_grid[n_atom_array*3] (grid order,
+ * zeroed here), energy[2] = {vdw, ele}.  Returns in-cutoff pair count. */
+int64_t orc_pairs_celllist(const orc_zd_t *zd, const orc_grid_t *g, const orc_cells_t *c,
+                           const orc_cellpair_t *cps, int n_cp, const int *atom_type, int n_types,
+                           const float *lj6, const float *lj12, const float *charge,
+                           double *work_grid, double energy[2]);
+
+/* All-pairs loop with float minimum image (SubBox.cpp:1127-1220, the
+ * omegagene_wons build).  work[n_atoms*3] atom order (zeroed here). */
+int64_t orc_pairs_allpairs_fp32(const orc_zd_t *zd, const float L[3], int n_atoms, const double *crd,
+                                const int *atom_type, int n_types, const float *lj6, const float *lj12,
+                                const float *charge, const int *nb15off, double *work, double energy[2]);
+
+/* FP64 reference of the same physics (not a restatement of reference
+ * rounding): double coordinates, exact minimum image, erfc in double.
+ * Used to state force RMS relative error.  Uses the cell list for O(N). */
+int64_t orc_pairs_celllist_fp64(const orc_zd_t *zd, const orc_grid_t *g, const orc_cells_t *c,
+                                const orc_cellpair_t *cps, int n_cp, const double *crd_atom,
+                                const int *atom_type, int n_types, const float *lj6, const float *lj12,
+                                const float *charge, double *work_atom, double energy[2]);
+
+/* ZD excess over excluded pairs (SubBox.cpp:1347-1367, ForceField.cpp:420-437,
+ * ZeroMultipoleSum.cpp:191-217).  bug_compat != 0 keeps the alpha>0 energy
+ * term -erf(ar)*r of ZeroMultipoleSum.cpp:214; 0 uses -erf(ar)/r.
+ * Adds into work[n_atoms*3]; returns the energy. */
+double orc_excess(const orc_zd_t *zd, const float L[3], int n_excess, const int *pairs, const double *crd,
+                  const float *charge, int bug_compat, double *work);
+
+/* Harmonic bonds E = k (r - r0)^2 (ForceField.cpp:108-136, SubBox.cpp:1229-1249). */
+double orc_bonds(const float L[3], int n_bonds, const int *pairs, const double *k, const double *r0,
+                 const double *crd, double *work);
+
+/* ---- integration (all FP64) ---- */
+
+/* v' = v + dt*(-FORCE_VEL)*w*minv  (SubBox.cpp:1445-1454). */
+void orc_update_velocities(int n, double dt, const double *vel, const double *work, const float *mass_inv,
+                           double *vel_next);
+/* x' = x + v*dt (SubBox.cpp:1567-1580). */
+void orc_update_coordinates(int n, double dt, const double *crd_prev, const double *vel, double *crd);
+/* SHAKE pair (ConstraintShake.cpp:18-49). dist2 = squared distances. */
+void orc_shake_pair(const float L[3], int n, const int *ids, const double *dist2, double *crd,
+                    const double *crd_prev, const float *mass_inv);
+/* SHAKE trio (ConstraintShake.cpp:51-149). Returns number of non-converged trios. */
+int orc_shake_trio(const float L[3], int n, const int *ids, const double *dist2 /* n*3 */, double *crd,
+                   const double *crd_prev, const float *mass_inv, double tol, int max_loops);
+/* Analytic SETTLE (Miyamoto & Kollman 1992) for rigid 3-site waters; new code
+ * with no reference counterpart (the reference SETTLE is a stub,
+ * ConstraintShake.cpp:284-286).  ids = (O,H1,H2); dOH, dHH distances. */
+void orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH, double *crd,
+                const double *crd_prev, const float *mass);
+/* v' = (x' - x)/dt (SubBox.cpp:1462-1481). */
+void orc_velocity_from_crd(int n, double dt_inv, const double *crd, const double *crd_prev, double *vel_next);
+/* Wrap once into [lb, lb+L) (SubBox.cpp:1482-1502). */
+void orc_revise_pbc(int n, const float L[3], const float lb[3], double *crd);
+/* Remove group COM velocity (COMMotion.cpp:30-55). */
+void orc_cancel_com(int n_groups, const int *group_off /* n_groups+1 */, const int *group_atoms,
+                    const float *mass_inv_group, const float *mass, double *vel);
+/* KE = KINETIC_COEFF * sum m ((v+v')/2)^2 (DynamicsMode.cpp:358-372, SubBox.cpp:1504-1516). */
+double orc_kinetic(int n, const float *mass, const double *vel, const double *vel_next);
+/* Position Langevin (SubBox.cpp:2077-2114) with caller-supplied N(0,1) noise
+ * zeta[n*3] (the reference draws it from mt19937 + std::normal_distribution). */
+void orc_langevin_positions(int n, double dt_half, double gamma, double temperature, const float L[3],
+                            const double *crd_prev, const double *crd_prev2, const double *work,
+                            const float *mass, const float *mass_inv, const double *zeta, double *crd);
+/* Central-difference velocity (SubBox.cpp:2054-2066). */
+void orc_langevin_velocity(int n, double dt, const float L[3], const double *crd, const double *crd_prev2,
+                           double *vel);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,346 @@
+/*
+ * oracle_pair.c -- CPU restatement of the reference LJ + zero-dipole pair
+ * path, its constants, self energy, excluded-pair ("excess") terms and bonds.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker; never linked by the product).
+ * Compiled with -ffp-contract=off so every float/double rounding happens
+ * where the reference host build rounds it.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NAC ORC_N_ATOM_CELL
+static const double CHARGE_COEFF = 332.06378; /* CelesteObject.cpp:36 */
+static const double PI_REF       = 3.14159265; /* CelesteObject.cpp:26 */
+static const double EPS_REF      = 1e-10;      /* CelesteObject.cpp:31 */
+
+/* ZeroMultipoleSum::set_zms_params, dipole members only (ZeroMultipoleSum.cpp:108-190).
+ * All members are real_pw (float). */
+void orc_zd_setup(orc_zd_t *zd, double cutoff, double alpha) {
+    zd->cutoff      = (float)cutoff;
+    zd->alpha       = (float)alpha;
+    float cutoff_2  = zd->cutoff * zd->cutoff;
+    float cutoff_3  = cutoff_2 * zd->cutoff;
+    float tmp       = zd->cutoff * zd->alpha;
+    float errorfc   = erfcf(tmp);
+    float expterm   = expf(-tmp * tmp);
+    zd->piewald     = 0.0f;
+    if (zd->alpha < EPS_REF) {
+        zd->fcoeff = (float)(1.0 / cutoff_3);
+        zd->bcoeff = (float)(0.5 * zd->fcoeff);
+    } else {
+        float root_pi = (float)sqrt(PI_REF);
+        zd->piewald   = (float)(2.0 / root_pi * zd->alpha);
+        zd->fcoeff    = errorfc / cutoff_3 + zd->piewald * expterm / cutoff_2;
+        zd->bcoeff    = (float)(0.5 * zd->fcoeff);
+    }
+    zd->zcore = errorfc / zd->cutoff + zd->bcoeff * cutoff_2;
+}
+
+/* ZeroMultipoleSum::cal_self_energy (ZeroMultipoleSum.cpp:43-106), dipole case. */
+double orc_self_energy(const orc_zd_t *zd, int n, const float *q, int n_excess, const int *ex, double *e_self) {
+    double *dwork = (double *)calloc((size_t)n, sizeof(double));
+    for (int i = 0; i < n_excess; i++) {
+        dwork[ex[2 * i]] += q[ex[2 * i + 1]];
+        dwork[ex[2 * i + 1]] += q[ex[2 * i]];
+    }
+    double zmcore = zd->zcore;
+    float  mon    = 0.0f; /* d_self_mon is a real_pw member */
+    for (int i = 0; i < n; i++) {
+        float  pq = -zd->piewald * q[i] * q[i];
+        double e  = ((double)pq - zmcore * q[i] * (q[i] + dwork[i])) * 0.5 * CHARGE_COEFF;
+        if (e_self) e_self[i] = e;
+        mon = (float)((double)mon + e);
+    }
+    free(dwork);
+    return (double)mon;
+}
+
+/* MmSystem::set_excess_pairs (MmSystem.cpp:700-736): i<j from nb15off order. */
+int orc_excess_pairs(int n, const int *nb15off, int *out, int cap) {
+    int k = 0;
+    for (int a = 0; a < n; a++)
+        for (int j = 0; j < ORC_MAX_N_NB15OFF; j++) {
+            int b = nb15off[a * ORC_MAX_N_NB15OFF + j];
+            if (b == -1) break;
+            if (a >= b) continue;
+            if (k >= cap) return -1;
+            out[2 * k]     = a;
+            out[2 * k + 1] = b;
+            k++;
+        }
+    return k;
+}
+
+/* ForceField::calc_pairwise (ForceField.cpp:333-418) with the ZD kernels
+ * calc_zero02pole_alpha0 (ZeroMultipoleSum.cpp:309-332) / calc_zero02pole (:333-346).
+ * float locals, double literals promote then round on assignment. */
+int orc_pair_fp32(const orc_zd_t *zd, const float d12[3], float a6, float a12, float q1, float q2,
+                  float *e_vdw, float *e_ele, float work[3]) {
+    float r12_2 = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
+    float r12   = sqrtf(r12_2);
+    *e_vdw = *e_ele = 0.0f;
+    work[0] = work[1] = work[2] = 0.0f;
+    if (r12 >= zd->cutoff) return 0;
+    float r12_inv    = (float)(1.0 / (double)r12);
+    float r12_2_inv  = (float)(1.0 / (double)r12_2);
+    float r12_3_inv  = r12_inv * r12_2_inv;
+    float r12_6_inv  = r12_3_inv * r12_3_inv;
+    float r12_12_inv = r12_6_inv * r12_6_inv;
+    float term6      = a6 * r12_6_inv;
+    float term12     = a12 * r12_12_inv;
+    *e_vdw           = -term6 + term12;
+    float work_coef  = (float)((double)r12_2_inv * (-12.0 * (double)term12 + 6.0 * (double)term6));
+    float cc         = (float)((double)(q1 * q2) * CHARGE_COEFF);
+    float g_ele;
+    if (zd->alpha < EPS_REF) {
+        *e_ele = cc * (r12_inv - zd->zcore + zd->bcoeff * r12_2);
+        g_ele  = -cc * (r12_3_inv - zd->fcoeff);
+    } else {
+        float tmp     = r12 * zd->alpha;
+        float errorfc = erfcf(tmp);
+        *e_ele        = cc * (r12_inv * errorfc - zd->zcore + zd->bcoeff * r12_2);
+        g_ele = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff);
+    }
+    for (int d = 0; d < 3; d++) {
+        float wv = work_coef * d12[d];
+        float we = g_ele * d12[d];
+        work[d]  = wv + we;
+    }
+    return 1;
+}
+
+/* PBC::fix_pbc_image on a float coordinate (PBC.cpp:106-121). */
+static void fix_image(float *x, int image, const float L[3]) {
+    if (image == 0) return;
+    if ((image & 1) == 1) x[0] -= L[0];
+    else if ((image & 2) == 2) x[0] += L[0];
+    if ((image & 4) == 4) x[1] -= L[1];
+    else if ((image & 8) == 8) x[1] += L[1];
+    if ((image & 16) == 16) x[2] -= L[2];
+    else if ((image & 32) == 32) x[2] += L[2];
+}
+
+/* SubBox::calc_energy_pairwise (SubBox.cpp:956-1125) */
+int64_t orc_pairs_celllist(const orc_zd_t *zd, const orc_grid_t *g, const orc_cells_t *c,
+                           const orc_cellpair_t *cps, int n_cp, const int *atom_type, int n_types,
+                           const float *lj6, const float *lj12, const float *charge, double *work,
+                           double energy[2]) {
+    int64_t n_in = 0;
+    memset(work, 0, sizeof(double) * (size_t)c->n_atom_array * 3);
+    energy[0] = energy[1] = 0.0;
+    for (int cp = 0; cp < n_cp; cp++) {
+        int c1 = cps[cp].cell_id1, c2 = cps[cp].cell_id2;
+        for (int i2 = 0; i2 < NAC; i2++) {
+            int   g2 = c2 * NAC + i2;
+            int   id2 = c->atomids[g2];
+            float x2[3] = {(float)c->crd[g2 * 3], (float)c->crd[g2 * 3 + 1], (float)c->crd[g2 * 3 + 2]};
+            fix_image(x2, cps[cp].image, g->L);
+            for (int i1 = 0; i1 < NAC; i1++) {
+                int bit = i2 * NAC + i1;
+                if (((uint32_t)cps[cp].pair_mask[bit / 32] >> (bit % 32)) & 1u) continue;
+                int   g1 = c1 * NAC + i1;
+                int   id1 = c->atomids[g1];
+                float x1[3] = {(float)c->crd[g1 * 3], (float)c->crd[g1 * 3 + 1], (float)c->crd[g1 * 3 + 2]};
+                float d12[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
+                int   t = atom_type[id1] * n_types + atom_type[id2];
+                float ev, ee, w[3];
+                n_in += orc_pair_fp32(zd, d12, lj6[t], lj12[t], charge[id1], charge[id2], &ev, &ee, w);
+                energy[0] += ev;
+                energy[1] += ee;
+                for (int d = 0; d < 3; d++) {
+                    work[g1 * 3 + d] += w[d];
+                    work[g2 * 3 + d] += -(double)w[d];
+                }
+            }
+        }
+    }
+    return n_in;
+}
+
+/* SubBox::calc_energy_pairwise_wo_neighborsearch (SubBox.cpp:1127-1220) */
+int64_t orc_pairs_allpairs_fp32(const orc_zd_t *zd, const float L[3], int n, const double *crd,
+                                const int *atom_type, int n_types, const float *lj6, const float *lj12,
+                                const float *charge, const int *nb15off, double *work, double energy[2]) {
+    int64_t n_in = 0;
+    float   L_half[3];
+    for (int d = 0; d < 3; d++) L_half[d] = (float)(L[d] * 0.5);
+    memset(work, 0, sizeof(double) * (size_t)n * 3);
+    energy[0] = energy[1] = 0.0;
+    for (int a1 = 0; a1 < n; a1++) {
+        float x1[3] = {(float)crd[a1 * 3], (float)crd[a1 * 3 + 1], (float)crd[a1 * 3 + 2]};
+        for (int a2 = 0; a2 < a1; a2++) {
+            int excl = 0;
+            for (int k = 0; k < ORC_MAX_N_NB15OFF; k++)
+                if (nb15off[a1 * ORC_MAX_N_NB15OFF + k] == a2) excl = 1;
+            if (excl) continue;
+            float x2[3] = {(float)crd[a2 * 3], (float)crd[a2 * 3 + 1], (float)crd[a2 * 3 + 2]};
+            for (int d = 0; d < 3; d++) {
+                if (x2[d] - x1[d] >= L_half[d]) x2[d] -= L[d];
+                else if (x2[d] - x1[d] <= -L_half[d]) x2[d] += L[d];
+            }
+            float d12[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
+            int   t = atom_type[a1] * n_types + atom_type[a2];
+            float ev, ee, w[3];
+            n_in += orc_pair_fp32(zd, d12, lj6[t], lj12[t], charge[a1], charge[a2], &ev, &ee, w);
+            energy[0] += ev;
+            energy[1] += ee;
+            for (int d = 0; d < 3; d++) {
+                work[a1 * 3 + d] += w[d];
+                work[a2 * 3 + d] -= w[d];
+            }
+        }
+    }
+    return n_in;
+}
+
+/* FP64 version of the same physics for error statements (double math,
+ * exact minimum image on double coordinates, erfc in double). */
+int64_t orc_pairs_celllist_fp64(const orc_zd_t *zd, const orc_grid_t *g, const orc_cells_t *c,
+                                const orc_cellpair_t *cps, int n_cp, const double *crd_atom,
+                                const int *atom_type, int n_types, const float *lj6, const float *lj12,
+                                const float *charge, double *work, double energy[2]) {
+    int64_t n_in = 0;
+    int     n_atoms = 0;
+    for (int i = 0; i < c->n_atom_array; i++)
+        if (c->atomids[i] + 1 > n_atoms) n_atoms = c->atomids[i] + 1;
+    memset(work, 0, sizeof(double) * (size_t)n_atoms * 3);
+    energy[0] = energy[1] = 0.0;
+    const double rc = zd->cutoff, alpha = zd->alpha;
+    /* constants in double from the same closed forms */
+    double fcoeff, bcoeff, zcore, piewald = 0.0;
+    if (alpha < EPS_REF) {
+        fcoeff = 1.0 / (rc * rc * rc);
+        bcoeff = 0.5 * fcoeff;
+        zcore  = 1.0 / rc + bcoeff * rc * rc;
+    } else {
+        piewald = 2.0 / sqrt(M_PI) * alpha;
+        fcoeff  = erfc(alpha * rc) / (rc * rc * rc) + piewald * exp(-alpha * alpha * rc * rc) / (rc * rc);
+        bcoeff  = 0.5 * fcoeff;
+        zcore   = erfc(alpha * rc) / rc + bcoeff * rc * rc;
+    }
+    for (int cp = 0; cp < n_cp; cp++) {
+        int c1 = cps[cp].cell_id1, c2 = cps[cp].cell_id2;
+        for (int i2 = 0; i2 < NAC; i2++) {
+            int id2 = c->atomids[c2 * NAC + i2];
+            for (int i1 = 0; i1 < NAC; i1++) {
+                int bit = i2 * NAC + i1;
+                if (((uint32_t)cps[cp].pair_mask[bit / 32] >> (bit % 32)) & 1u) continue;
+                int    id1 = c->atomids[c1 * NAC + i1];
+                double d[3];
+                for (int k = 0; k < 3; k++) {
+                    d[k] = crd_atom[id1 * 3 + k] - crd_atom[id2 * 3 + k];
+                    d[k] -= (double)g->L[k] * nearbyint(d[k] / (double)g->L[k]);
+                }
+                double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                double r  = sqrt(r2);
+                if (r >= rc) continue;
+                n_in++;
+                int    t = atom_type[id1] * n_types + atom_type[id2];
+                double ri = 1.0 / r, r2i = ri * ri, r6i = r2i * r2i * r2i;
+                double t6 = (double)lj6[t] * r6i, t12 = (double)lj12[t] * r6i * r6i;
+                double cc = (double)charge[id1] * (double)charge[id2] * CHARGE_COEFF;
+                double ev = t12 - t6, ee, gcoef = r2i * (-12.0 * t12 + 6.0 * t6);
+                if (alpha < EPS_REF) {
+                    ee = cc * (ri - zcore + bcoeff * r2);
+                    gcoef += -cc * (ri * r2i - fcoeff);
+                } else {
+                    double ef = erfc(alpha * r);
+                    ee        = cc * (ri * ef - zcore + bcoeff * r2);
+                    gcoef += -cc * (ef * ri * r2i + piewald * exp(-alpha * alpha * r2) * r2i - fcoeff);
+                }
+                energy[0] += ev;
+                energy[1] += ee;
+                for (int k = 0; k < 3; k++) {
+                    work[id1 * 3 + k] += gcoef * d[k];
+                    work[id2 * 3 + k] -= gcoef * d[k];
+                }
+            }
+        }
+    }
+    return n_in;
+}
+
+/* PBC::diff_crd_minim_image, float overload (PBC.cpp:44-51). */
+static void minim_image_f(float d[3], const float *x1, const float *x2, const float L[3]) {
+    for (int i = 0; i < 3; i++) {
+        float L_half     = (float)(L[i] * 0.5);
+        float L_half_inv = (float)(1.0 / L_half);
+        d[i]             = x1[i] - x2[i];
+        d[i]             = d[i] - L[i] * (float)(int)(d[i] * L_half_inv);
+        d[i]             = d[i] - L[i] * (float)(int)(d[i] * L_half_inv);
+    }
+}
+/* double overload (PBC.cpp:62-82) */
+static void minim_image_d(double d[3], const double *x1, const double *x2, const float L[3]) {
+    for (int i = 0; i < 3; i++) {
+        float L_half     = (float)(L[i] * 0.5);
+        float L_half_inv = (float)(1.0 / L_half);
+        d[i]             = x1[i] - x2[i];
+        d[i]             = d[i] - (double)L[i] * (int)(d[i] * (double)L_half_inv);
+        d[i]             = d[i] - (double)L[i] * (int)(d[i] * (double)L_half_inv);
+    }
+}
+
+/* SubBox::calc_energy_ele_excess (SubBox.cpp:1347-1367) + ForceField::calc_zms_excess
+ * (ForceField.cpp:420-437) + calc_zero02pole_excess(_alpha0) (ZeroMultipoleSum.cpp:191-217). */
+double orc_excess(const orc_zd_t *zd, const float L[3], int n_excess, const int *pairs, const double *crd,
+                  const float *q, int bug_compat, double *work) {
+    double e_sum = 0.0;
+    for (int i = 0; i < n_excess; i++) {
+        int   a = pairs[2 * i], b = pairs[2 * i + 1];
+        float c1[3] = {(float)crd[a * 3], (float)crd[a * 3 + 1], (float)crd[a * 3 + 2]};
+        float c2[3] = {(float)crd[b * 3], (float)crd[b * 3 + 1], (float)crd[b * 3 + 2]};
+        float d12[3];
+        minim_image_f(d12, c1, c2, L);
+        float r12_2     = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
+        float r12       = sqrtf(r12_2);
+        float r12_inv   = (float)(1.0 / (double)r12);
+        float r12_2_inv = r12_inv * r12_inv;
+        float r12_3_inv = r12_2_inv * r12_inv;
+        float cc        = (float)((double)(q[a] * q[b]) * CHARGE_COEFF);
+        float ene, gc;
+        if (zd->alpha < EPS_REF) {
+            ene = cc * zd->bcoeff * r12_2;
+            gc  = cc * zd->fcoeff;
+        } else {
+            float tmp     = zd->alpha * r12;
+            float errorfc = erfcf(tmp);
+            float errorfn = (float)(1.0 - (double)errorfc);
+            if (bug_compat) ene = cc * (-errorfn * r12 + zd->bcoeff * r12_2);
+            else ene = cc * (-errorfn * r12_inv + zd->bcoeff * r12_2);
+            gc = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff - r12_3_inv);
+        }
+        e_sum += ene;
+        for (int d = 0; d < 3; d++) {
+            float w = gc * d12[d];
+            work[a * 3 + d] += w;
+            work[b * 3 + d] -= w;
+        }
+    }
+    return e_sum;
+}
+
+/* SubBox::calc_energy_bonds (SubBox.cpp:1229-1249) + ForceField::calc_bond (ForceField.cpp:108-136). */
+double orc_bonds(const float L[3], int n_bonds, const int *pairs, const double *k, const double *r0,
+                 const double *crd, double *work) {
+    double e = 0.0;
+    for (int i = 0; i < n_bonds; i++) {
+        int    a = pairs[2 * i], b = pairs[2 * i + 1];
+        double d[3];
+        minim_image_d(d, crd + a * 3, crd + b * 3, L);
+        double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        double dif  = r - r0[i];
+        double coef = k[i] * dif;
+        e += coef * dif;
+        double gcoef = 2.0 * coef / r;
+        for (int q = 0; q < 3; q++) {
+            double w = d[q] * gcoef;
+            work[a * 3 + q] += w;
+            work[b * 3 + q] -= w;
+        }
+    }
+    return e;
+}

@@ -1,0 +1,100 @@
+"""Write tests/golden/wons_*.npz from runs of the reference all-pairs build
+(omegagene_wons, compiled from the reference sources by oracle/Makefile).
+
+  wons_grad_a0.npz / wons_grad_a1.npz -- one leapfrog step without constraints
+      on a water + 20-bead-chain system (ZD alpha 0 and 0.1).  The per-atom
+      gradient follows from the restart velocities:
+      work = -(v_half - v_in) / (dt * 4.184e-4 * mass_inv_f32)
+      (SubBox.cpp:1445-1454; restart writer WriteTrr.cpp:258-313).
+  wons_traj.npz -- 20 leapfrog steps of 1,000 rigid waters with SHAKE; the
+      per-step energy terms of the log and the final restart.
+
+Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
+Run where the reference tree is mounted:
+    make -C oracle ref && python tests/golden/make_wons_fixtures.py
+"""
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from omegagene_amd import gen  # noqa: E402
+from omegagene_amd.cls import write_cls  # noqa: E402
+
+EXE = os.path.join(ROOT, "oracle", "_ref", "omegagene_wons")
+
+
+def read_restart(path):
+    b = open(path, "rb").read()
+    o = 4 + 80 + 4
+    n = np.frombuffer(b, np.int32, 2, o + 4)[0]
+    o += 4 + 8 + 4
+    step = np.frombuffer(b, np.int32, 1, o + 4)[0]
+    t, etot, ekin, epot = np.frombuffer(b, np.float64, 4, o + 8)
+    o += 4 + 36 + 4
+    crd = np.frombuffer(b, np.float64, 3 * n, o + 4).reshape(n, 3).copy()
+    o += 4 + 24 * n + 4
+    vel = np.frombuffer(b, np.float64, 3 * n, o + 4).reshape(n, 3).copy()
+    return dict(n=int(n), step=int(step), time=t, etot=etot, ekin=ekin, epot=epot, crd=crd, vel=vel)
+
+
+def parse_log(text):
+    rows = []
+    for blk in re.split(r"(?m)^Step:", text)[1:]:
+        def g(lbl):
+            m = re.search(lbl + r"\s*([-+0-9.eE]+)", blk)
+            return float(m.group(1)) if m else np.nan
+        rows.append([float(blk.split()[0]), g(r"(?m)^Potential:"), g("Kinetic:"), g(r"(?m)^Bond:"), g(r"(?m)^VDW:"),
+                     g(r"(?m)^VDW:\s*\S+\s+Ele:"), g(r"(?m)^Temperature:")])
+    return np.array(rows)
+
+
+def run(ls, cfg, workdir):
+    clsf = os.path.join(workdir, "in.cls")
+    cfgf = os.path.join(workdir, "in.cfg")
+    data = write_cls(ls, clsf)
+    with open(cfgf, "w") as f:
+        f.write(gen.cfg_text(cfg))
+    r = subprocess.run([EXE, "--cfg", cfgf, "--inp", clsf], cwd=workdir, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True)
+    # the reference may fail in its destructors after finishing the run; the
+    # run itself is complete once terminal_process has been printed
+    if "terminal_process" not in r.stdout:
+        raise RuntimeError(r.stdout[-2000:])
+    return data, r.stdout, read_restart(os.path.join(workdir, "out.restart"))
+
+
+BASE = {"mode": "md", "integrator": "leapfrog-presto", "cutoff": 12.0, "nsgrid-cutoff": 13.0,
+        "nsgrid-update-intvl": 1, "time-step": 2.0, "electrostatic": "zero-dipole", "thermostat": "none",
+        "com-motion": "none", "print-interval-log": 1, "fn-o-restart": "out.restart", "random-seed": 1}
+
+
+def main():
+    tmp = tempfile.mkdtemp()
+    for tag, alpha in (("a0", 0.0), ("a1", 0.1)):
+        ls = gen.build_system(700, seed=77, n_beads=20)
+        cfg = dict(BASE, **{"n-steps": 1, "ele-alpha": alpha, "constraint": "none"})
+        data, log, rs = run(ls, cfg, tmp)
+        minv = (1.0 / ls.mass.astype(np.float32).astype(np.float64)).astype(np.float32).astype(np.float64)
+        work = -(rs["vel"] - ls.vel) / (2.0 * 4.184e-4 * minv[:, None])
+        np.savez_compressed(os.path.join(HERE, "wons_grad_%s.npz" % tag), cls=np.frombuffer(data, np.uint8),
+                            alpha=np.float64(alpha), work=work, log=parse_log(log), crd1=rs["crd"], vel1=rs["vel"])
+        print(tag, "ok", parse_log(log)[0])
+    ls = gen.build_system(1000, seed=88)
+    cfg = dict(BASE, **{"n-steps": 20, "ele-alpha": 0.0, "constraint": "shake", "const-tolerance": 1e-10,
+                        "const-max-loops": 1000})
+    data, log, rs = run(ls, cfg, tmp)
+    lg = parse_log(log)
+    np.savez_compressed(os.path.join(HERE, "wons_traj.npz"), cls=np.frombuffer(data, np.uint8), log=lg,
+                        crd=rs["crd"], vel=rs["vel"], epot=rs["epot"], ekin=rs["ekin"], step=rs["step"])
+    print("traj ok", lg.shape, lg[0], lg[-1])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,218 @@
+"""HIP path (libog_engine.so through the C ABI) against the oracle and the
+reference fixtures.  Integer work is compared bit for bit; FP32 pair math
+within the stated tolerances: gradient RMS relative error <= 2e-6 against the
+FP32-faithful oracle and <= 1e-5 against the FP64 oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import cls_from_fixture, load_npz, oracle_forces, rms_rel
+from omegagene_amd import gen
+from omegagene_amd.cls import LaunchSet
+from omegagene_amd.engine import EngineError, System, Batch
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+BASE = {"cutoff": 12.0, "nsgrid-cutoff": 13.0, "nsgrid-update-intvl": 1, "time-step": 2.0,
+        "integrator": "leapfrog-presto", "constraint": "shake", "const-tolerance": 1e-10, "ele-alpha": 0.0}
+
+
+def grid_launchset(d):
+    tab = d["nb15off"]
+    n = len(d["crd"])
+    a = np.repeat(np.arange(n), 32).reshape(n, 32)
+    m = (tab >= 0) & (tab > a)
+    pairs = np.stack([a[m], tab[m]], 1).astype(np.int32)
+    return LaunchSet(box=d["box"], crd=d["crd"], vel=np.zeros((n, 3)), charge=np.zeros(n), mass=np.ones(n),
+                     atom_type=np.zeros(n, np.int32), n_lj_types=1, lj6=np.zeros((1, 1)), lj12=np.zeros((1, 1)),
+                     nb15off=pairs)
+
+
+@pytest.mark.parametrize("name", ["w1500", "cfg0", "chain", "chain200", "rc11"])
+def test_grid_and_cell_pairs_bitexact_vs_reference_minicell(name):
+    d = load_npz("minicell_%s.npz" % name)
+    say("grid", name, len(d["crd"]))
+    ls = grid_launchset(d)
+    assert np.array_equal(ls.nb15off_table() >= 0, d["nb15off"] >= 0)
+    s = System(ls, dict(BASE, **{"nsgrid-cutoff": float(d["rcl"]), "constraint": "none"}))
+    s.nsgrid_update()
+    ids, gcrd, head = s.grid()
+    assert np.array_equal(head, d["idx_xy_head_cell"])
+    assert np.array_equal(ids, d["atomids"])
+    assert np.array_equal(gcrd, d["grid_crd"])
+    cp, cph = s.cell_pairs()
+    assert np.array_equal(cph, d["idx_head_cell_pairs"])
+    assert np.array_equal(cp, d["cell_pairs"])
+    s.close()
+
+
+def _systems():
+    ls0, _ = gen.config0()
+    yield "cfg0", ls0, dict(BASE)
+    yield "chain", gen.build_system(1400, seed=7, n_beads=30), dict(BASE)
+    yield "alpha", gen.build_system(2500, seed=21), dict(BASE, **{"cutoff": 10.0, "nsgrid-cutoff": 11.0,
+                                                                    "ele-alpha": 0.1})
+
+
+@pytest.mark.parametrize("case", ["cfg0", "chain", "alpha"])
+def test_forces_energies_vs_oracle(case):
+    name, ls, cfg = [x for x in _systems() if x[0] == case][0]
+    say("forces", name, ls.n_atoms)
+    s = System(ls, cfg)
+    w, e = s.energy_force(rebuild=True)
+    wo, eo, (o, c, cps, head) = oracle_forces(ls, cutoff=cfg["cutoff"], alpha=cfg["ele-alpha"],
+                                              nscut=cfg["nsgrid-cutoff"])
+    err32 = rms_rel(w, wo)
+    say("  rms rel vs fp32 oracle %.3e" % err32)
+    assert err32 < 2e-6
+    for k in ("vdw", "ele"):
+        assert e[k] == pytest.approx(eo[k], rel=2e-6, abs=1e-4), k
+    assert e["excess"] == pytest.approx(eo["excess"], rel=1e-9, abs=1e-9)
+    assert e["bond"] == pytest.approx(eo["bond"], rel=1e-9, abs=1e-12)
+    assert e["self"] == eo["self"]
+    # FP64 reference of the same physics
+    zd = oracle.zd_setup(cfg["cutoff"], cfg["ele-alpha"])
+    lj6, lj12 = ls.lj_tables_f32()
+    w64, e64, _ = o.pairs_fp64(zd, cps, ls.crd, ls.atom_type, lj6, lj12, ls.charge.astype(np.float32))
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    oracle.excess(zd, L, ls.excess_pairs(), ls.crd, ls.charge.astype(np.float32), w64)
+    if len(ls.bond_ids):
+        oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, ls.crd, w64)
+    err64 = rms_rel(w, w64)
+    say("  rms rel vs fp64 oracle %.3e; vdw %.6e/%.6e ele %.6e/%.6e" % (err64, e["vdw"], e64[0], e["ele"], e64[1]))
+    assert err64 < 1e-5
+    assert e["vdw"] + e["ele"] == pytest.approx(e64[0] + e64[1], rel=1e-5)
+    # Newton's third law on the fixed-point sums
+    assert np.max(np.abs(w.sum(0))) < 1e-5 * np.abs(w).max()
+    s.close()
+
+
+def test_integration_step_bitexact_vs_oracle():
+    ls = gen.build_system(1000, seed=88)
+    say("integrator", ls.n_atoms)
+    s = System(ls, dict(BASE))
+    crd0, vn0, _ = s.state()
+    w, _ = s.energy_force(rebuild=True)
+    s.run(1)
+    crd1, vn1, _ = s.state()
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    m32 = ls.mass.astype(np.float32)
+    minv = (1.0 / m32.astype(np.float64)).astype(np.float32)
+    v = oracle.update_velocities(2.0, vn0, w, minv)
+    x = oracle.update_coordinates(2.0, crd0, v)
+    x, nf = oracle.shake_trio(L, ls.shake_trio_ids, ls.shake_trio_d ** 2, x, crd0, minv, 1e-10, 1000)
+    v = oracle.velocity_from_crd(2.0, x, crd0)
+    x = oracle.revise_pbc(L, lb, x)
+    dx, dv = np.max(np.abs(x - crd1)), np.max(np.abs(v - vn1))
+    say("  max |dx| %.3e  max |dv| %.3e  exact=%s" % (dx, dv, dx == 0 and dv == 0))
+    assert dx <= 1e-12 and dv <= 1e-14
+    s.close()
+
+
+def test_reference_trajectory_20_steps():
+    d = load_npz("wons_traj.npz")
+    ls = cls_from_fixture(d)
+    say("reference trajectory", ls.n_atoms)
+    s = System(ls, dict(BASE, **{"log-cap": 64}))
+    s.run(20)
+    log = s.energy_log()
+    ref = d["log"]
+    assert len(log) == 20
+    pot = log[:, 0] + log[:, 1] + log[:, 2] + log[:, 3] + log[:, 4]
+    for k in range(20):
+        assert pot[k] == pytest.approx(ref[k, 1], rel=1e-6, abs=2e-3), k
+        assert log[k, 5] == pytest.approx(ref[k, 2], rel=1e-6), k
+    crd, vn, _ = s.state()
+    say("  max |dx| vs reference %.3e" % np.max(np.abs(crd - d["crd"])))
+    assert np.max(np.abs(crd - d["crd"])) < 1e-5
+    s.close()
+
+
+def test_batch_equals_single_bitwise():
+    sizes = [3000, 5200, 8100]
+    cfg = dict(BASE, **{"constraint": "shake-settle", "nsgrid-update-intvl": 5})
+    systems = [gen.walker_system(n, seed=500 + i, vcmd=False) for i, n in enumerate(sizes)]
+    for ls in systems:  # waters through SETTLE
+        ls.settle_ids, ls.settle_d = ls.shake_trio_ids, ls.shake_trio_d
+        ls.shake_trio_ids = np.zeros((0, 3), np.int32)
+        ls.shake_trio_d = np.zeros((0, 3))
+    say("batch", sizes)
+    alone = []
+    for ls in systems:
+        s = System(ls, cfg)
+        s.run(20)
+        alone.append(s.state()[0])
+        s.close()
+    b = Batch()
+    ss = [System(ls, cfg) for ls in systems]
+    for s in ss:
+        b.add(s)
+    b.run(20)
+    for s, ref in zip(ss, alone):
+        assert np.array_equal(s.state()[0], ref)
+    b.close()
+    for s in ss:
+        s.close()
+
+
+def test_rerun_is_deterministic():
+    ls = gen.build_system(1500, seed=5)
+    cfg = dict(BASE, **{"nsgrid-update-intvl": 10})
+    out = []
+    for _ in range(2):
+        s = System(ls, cfg)
+        s.run(30)
+        out.append(s.state()[0])
+        s.close()
+    assert np.array_equal(out[0], out[1])
+
+
+def test_nve_energy_conservation_and_settle():
+    ls, cfg = gen.config0()
+    cfg = dict(cfg, **{"constraint": "shake", "const-tolerance": 1e-10, "log-cap": 512})
+    say("nve", ls.n_atoms)
+    s = System(ls, cfg)
+    s.run(200)
+    log = s.energy_log()
+    tot = log[:, 0] + log[:, 1] + log[:, 2] + log[:, 3] + log[:, 4] + log[:, 5]
+    drift = (tot[-1] - tot[20]) / abs(log[20:, 5].mean())
+    say("  total %.4f -> %.4f, KE %.1f, drift/KE %.2e" % (tot[20], tot[-1], log[-1, 5], drift))
+    assert abs(drift) < 0.05
+    crd, _, _ = s.state()
+    w = ls.shake_trio_ids
+    r = np.linalg.norm(crd[w[:, 0]] - crd[w[:, 1]] - ls.box[0] * np.round((crd[w[:, 0]] - crd[w[:, 1]]) / ls.box[0]),
+                       axis=1)
+    assert np.max(np.abs(r - gen.D_OH)) < 1e-6
+    s.close()
+
+
+def test_langevin_reaches_target_temperature():
+    ls = gen.build_system(2000, seed=31, n_beads=200)
+    ls.settle_ids, ls.settle_d = ls.shake_trio_ids, ls.shake_trio_d
+    ls.shake_trio_ids = np.zeros((0, 3), np.int32)
+    ls.shake_trio_d = np.zeros((0, 3))
+    cfg = dict(BASE, **{"integrator": "langevin", "constraint": "shake-settle", "langevin-gamma": 20.0,
+                        "temperature": 300.0, "nsgrid-update-intvl": 10, "random-seed": 7, "log-cap": 1024})
+    say("langevin", ls.n_atoms)
+    s = System(ls, cfg)
+    s.run(1000)
+    log = s.energy_log()
+    ke = log[-300:, 5].mean()
+    T = ke * 2 * 4184.0 / (8.31451 * s.d_free)
+    say("  <T> over last 300 steps %.1f K" % T)
+    assert 240.0 < T < 380.0
+    s.close()
+
+
+def test_box_too_small_is_an_error():
+    ls = gen.build_system(60, seed=1, L=12.0)
+    with pytest.raises(EngineError):
+        System(ls, dict(BASE))

@@ -1,0 +1,264 @@
+"""The oracle (oracle/*.c) pinned against everything the reference holds for
+this path: the ala3 sample log (samples/ala3/log03_md.txt:99-135), MiniCell
+grids and cell-pair lists from the reference sources, reference all-pairs
+runs (one-step gradients, a 20-step SHAKE trajectory), and closed forms."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import ala3_launchset, cls_from_fixture, load_npz, oracle_forces, rms_rel, wrap_inbox
+from omegagene_amd import gen
+
+
+def fmt(x):
+    return "%.10e" % x
+
+
+def test_ala3_step0_vdw_ele_match_log():
+    ls, exp = ala3_launchset()
+    zd = oracle.zd_setup(12.0, 0.0)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    tab = ls.nb15off_table()
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    work, en, npair = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, tab)
+    ex = ls.excess_pairs()
+    assert len(ex) == 6435  # log03_md.txt:79 "excess set 6435"
+    eex = oracle.excess(zd, L, ex, crd, q, np.zeros_like(work))
+    eself, _ = oracle.self_energy(zd, q, ex)
+    assert fmt(en[0]) == fmt(exp["vdw"])                     # VDW 4.1648848106e+03
+    assert fmt(en[1] + eex + eself) == fmt(exp["ele"])      # Ele -2.9051201002e+04
+
+
+def test_zd_constants_and_self_energy_kat():
+    ls, exp = ala3_launchset()
+    zd = oracle.zd_setup(12.0, 0.0)
+    assert "%.15g" % zd.fcoeff == "%.15g" % exp["fcoeff"]
+    assert "%.15g" % zd.bcoeff == "%.15g" % exp["bcoeff"]
+    assert zd.zcore == exp["zmcore"] == 0.125
+    eself, _ = oracle.self_energy(zd, ls.charge.astype(np.float32), ls.excess_pairs())
+    assert "%.15g" % eself == "%.15g" % exp["self_sum"]
+
+
+@pytest.mark.parametrize("name", ["w1500", "cfg0", "chain", "chain200", "rc11"])
+def test_oracle_grid_matches_reference_minicell(name):
+    d = load_npz("minicell_%s.npz" % name)
+    n = len(d["crd"])
+    o = oracle.Oracle(n, d["box"], float(d["rcl"]))
+    c = o.build(d["crd"])
+    assert np.array_equal(c.atomids, d["atomids"])
+    assert np.array_equal(c.crd, d["grid_crd"])
+    assert np.array_equal(c.idx_xy_head_cell, d["idx_xy_head_cell"])
+    cps, head = o.enumerate(d["nb15off"], gap_bug=True)
+    assert np.array_equal(head, d["idx_head_cell_pairs"])
+    assert np.array_equal(cps.view(np.int32).reshape(-1, 5), d["cell_pairs"])
+
+
+@pytest.mark.parametrize("tag,alpha", [("a0", 0.0), ("a1", 0.1)])
+def test_oracle_gradient_matches_reference_one_step(tag, alpha):
+    d = load_npz("wons_grad_%s.npz" % tag)
+    ls = cls_from_fixture(d)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    zd = oracle.zd_setup(12.0, alpha)
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    work, en, _ = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, ls.nb15off_table())
+    ex = ls.excess_pairs()
+    eb = oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, crd, work)
+    eex = oracle.excess(zd, L, ex, crd, q, work, bug_compat=True)
+    eself, _ = oracle.self_energy(zd, q, ex)
+    ref = d["work"]
+    assert rms_rel(work, ref) < 1e-11
+    assert np.max(np.abs(work - ref)) < 1e-8 * np.max(np.abs(ref))
+    log0 = d["log"][0]  # step, pot, kin, bond, vdw, ele, T
+    assert fmt(en[0]) == fmt(log0[4])
+    assert fmt(en[1] + eex + eself) == fmt(log0[5])
+    assert abs(eb - log0[3]) <= 1e-9 * max(1.0, abs(log0[3]))
+
+
+def _oracle_leapfrog_shake(ls, n_steps, dt=2.0, tol=1e-10):
+    """DynamicsModePresto::calc_in_each_step (DynamicsMode.cpp:468-560) with the
+    oracle's all-pairs forces, SHAKE, no thermostat, no COM cancel."""
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    zd = oracle.zd_setup(12.0, 0.0)
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    tab = ls.nb15off_table()
+    ex = ls.excess_pairs()
+    minv = (1.0 / ls.mass.astype(np.float32).astype(np.float64)).astype(np.float32)
+    m32 = ls.mass.astype(np.float32)
+    d2 = ls.shake_trio_d ** 2
+    eself, _ = oracle.self_energy(zd, q, ex)
+    vel_next = ls.vel.copy()
+    rows = []
+    for step in range(n_steps):
+        work, en, _ = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, tab)
+        eex = oracle.excess(zd, L, ex, crd, q, work)
+        vel = vel_next
+        vel_next = oracle.update_velocities(dt, vel, work, minv)
+        crd_prev = crd
+        crd = oracle.update_coordinates(dt, crd_prev, vel_next)
+        crd, nf = oracle.shake_trio(L, ls.shake_trio_ids, d2, crd, crd_prev, minv, tol, 1000)
+        assert nf == 0
+        vel_next = oracle.velocity_from_crd(dt, crd, crd_prev)
+        crd = oracle.revise_pbc(L, lb, crd)
+        ke = oracle.kinetic(m32, vel, vel_next)
+        rows.append([step, en[0] + en[1] + eex + eself, ke, en[0], en[1] + eex + eself])
+    return crd, vel_next, np.array(rows)
+
+
+def test_oracle_md_matches_reference_trajectory():
+    d = load_npz("wons_traj.npz")
+    ls = cls_from_fixture(d)
+    crd, vel, rows = _oracle_leapfrog_shake(ls, 20)
+    log = d["log"]  # step, pot, kin, bond, vdw, ele, T
+    for r, lg in zip(rows, log[:20]):
+        assert abs(r[1] - lg[1]) <= 2e-9 * abs(lg[5]) + 1e-8, (r, lg)   # potential
+        assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]), (r, lg)          # kinetic
+        assert fmt(r[3]) == fmt(lg[4])                                    # VDW at print precision
+    assert np.max(np.abs(crd - d["crd"])) < 1e-9
+    assert np.max(np.abs(vel - d["vel"])) < 1e-11
+
+
+# ---------------------------------------------------------------- closed forms
+
+def test_zd_energy_and_force_vanish_at_cutoff():
+    zd = oracle.zd_setup(12.0, 0.0)
+    for r in (12.0 * (1 - 1e-6), 11.9999):
+        n, ev, ee, w = oracle.pair_fp32(zd, np.array([r, 0, 0]), 0.0, 0.0, 1.0, -1.0)
+        assert n == 1
+        assert abs(ee) < 1e-3 * 332.06378 / 12.0 * 1e-1
+        assert abs(w[0]) < 1e-4 * 332.06378 / 144.0
+    n, *_ = oracle.pair_fp32(zd, np.array([12.0, 0, 0]), 1.0, 1.0, 1.0, 1.0)
+    assert n == 0  # r >= cutoff is skipped (ForceField.cpp:368)
+
+
+def test_pair_two_particle_closed_form():
+    zd = oracle.zd_setup(12.0, 0.0)
+    r, a6, a12, q1, q2 = 3.3, 600.0, 6e5, -0.8, 0.4
+    n, ev, ee, w = oracle.pair_fp32(zd, np.array([0.0, 0.0, r]), a6, a12, q1, q2)
+    cc = q1 * q2 * 332.06378
+    rc = 12.0
+    assert ev == pytest.approx(a12 / r ** 12 - a6 / r ** 6, rel=2e-6)
+    assert ee == pytest.approx(cc * (1 / r - 1 / rc - 0.5 / rc ** 3 * rc ** 2 + 0.5 / rc ** 3 * r ** 2), rel=2e-6)
+    # dE/dr * (x/r): derivative of the same closed form
+    dEdr = -12 * a12 / r ** 13 + 6 * a6 / r ** 7 + cc * (-1 / r ** 2 + r / rc ** 3)
+    assert w[2] == pytest.approx(dEdr, rel=2e-6)
+    assert w[0] == 0 and w[1] == 0
+
+
+def test_fp64_pair_gradient_finite_difference():
+    ls = gen.build_system(120, seed=3, L=16.0)  # tiny dense box, rc 7 / list 8
+    ls.crd[:] = np.mod(ls.crd, 16.0)
+    zd = oracle.zd_setup(7.0, 0.1)
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+
+    def energy_and_grad(crd):
+        o = oracle.Oracle(ls.n_atoms, ls.box, 8.0)
+        o.build(crd)
+        cps, _ = o.enumerate(ls.nb15off_table(), gap_bug=False)
+        w, en, _ = o.pairs_fp64(zd, cps, crd, ls.atom_type, lj6, lj12, q)
+        return en.sum(), w
+
+    e0, w0 = energy_and_grad(ls.crd)
+    h = 1e-5
+    for a in (0, 7, 200):
+        for d_ in range(3):
+            cp, cm = ls.crd.copy(), ls.crd.copy()
+            cp[a, d_] += h
+            cm[a, d_] -= h
+            fd = (energy_and_grad(cp)[0] - energy_and_grad(cm)[0]) / (2 * h)
+            assert fd == pytest.approx(w0[a, d_], rel=1e-5, abs=1e-6)
+
+
+def test_cell_list_covers_every_pair_once():
+    d = load_npz("minicell_cfg0.npz")
+    crd, box, rc = d["crd"], d["box"], 12.0
+    L = box[0]
+    o = oracle.Oracle(len(crd), box, 13.0)
+    c = o.build(crd)
+    cps, _ = o.enumerate(d["nb15off"], gap_bug=True)
+    seen = {}
+    ids = c.atomids
+    gx = c.crd.astype(np.float32)
+    for rec in cps:
+        c1, c2, img, m0, m1 = int(rec[0]), int(rec[1]), int(rec[2]), int(rec[3]), int(rec[4])
+        mask = (m0 & 0xffffffff) | ((m1 & 0xffffffff) << 32)
+        shift = np.array([(-L if img & 1 else L if img & 2 else 0.0), (-L if img & 4 else L if img & 8 else 0.0),
+                          (-L if img & 16 else L if img & 32 else 0.0)])
+        a1 = ids[c1 * 8:c1 * 8 + 8]
+        a2 = ids[c2 * 8:c2 * 8 + 8]
+        for j in range(8):
+            for i in range(8):
+                if (mask >> (j * 8 + i)) & 1:
+                    continue
+                r = np.linalg.norm(gx[c1 * 8 + i].astype(np.float64) - (gx[c2 * 8 + j] + shift))
+                if r >= rc:
+                    continue
+                key = (min(a1[i], a2[j]), max(a1[i], a2[j]))
+                seen[key] = seen.get(key, 0) + 1
+    assert max(seen.values()) == 1
+    # brute force over all pairs (float coordinates, minimum image)
+    x = gx[ids >= 0]
+    order = ids[ids >= 0]
+    pos = np.empty_like(x)
+    pos[order] = x
+    excl = set()
+    for a, row in enumerate(d["nb15off"]):
+        for b in row:
+            if b < 0:
+                break
+            excl.add((min(a, b), max(a, b)))
+    n_in = 0
+    for a in range(len(pos)):
+        dd = pos[a + 1:] - pos[a]
+        dd -= L * np.round(dd / L)
+        r = np.sqrt((dd.astype(np.float64) ** 2).sum(1))
+        for b in np.nonzero(r < rc)[0] + a + 1:
+            if (a, b) in excl:
+                continue
+            n_in += 1
+            assert (a, b) in seen
+    assert n_in > 0
+
+
+def test_settle_matches_converged_shake():
+    ls = gen.build_system(300, seed=9, L=21.0)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    rng = np.random.default_rng(0)
+    prev = ls.crd.copy()
+    cur = prev + rng.normal(scale=0.02, size=prev.shape)  # an unconstrained step
+    m32 = ls.mass.astype(np.float32)
+    minv = (1.0 / m32.astype(np.float64)).astype(np.float32)
+    d2 = ls.shake_trio_d ** 2
+    sh, nf = oracle.shake_trio(L, ls.shake_trio_ids, d2, cur, prev, minv, 1e-14, 1000)
+    assert nf == 0
+    st = oracle.settle(L, ls.shake_trio_ids, gen.D_OH, gen.D_HH, cur, prev, m32)
+    assert np.max(np.abs(st - sh)) < 1e-9
+    w = ls.shake_trio_ids
+    for a, b, dist in ((0, 1, gen.D_OH), (1, 2, gen.D_HH), (0, 2, gen.D_OH)):
+        r = np.linalg.norm(st[w[:, a]] - st[w[:, b]], axis=1)
+        assert np.max(np.abs(r - dist)) < 1e-12
+    # linear momentum of each water is untouched by the constraint forces
+    mom0 = (m32[w][..., None] * (cur[w] - prev[w])).sum(1)
+    mom1 = (m32[w][..., None] * (st[w] - prev[w])).sum(1)
+    assert np.max(np.abs(mom0 - mom1)) < 1e-12
+
+
+def test_langevin_zero_noise_zero_friction_is_verlet():
+    rng = np.random.default_rng(1)
+    n = 50
+    x0 = rng.uniform(0, 10, (n, 3))
+    x1 = x0 + rng.normal(scale=0.01, size=(n, 3))
+    work = rng.normal(size=(n, 3))
+    m = rng.uniform(1, 16, n).astype(np.float32)
+    minv = (1.0 / m.astype(np.float64)).astype(np.float32)
+    L = np.array([100.0, 100.0, 100.0], np.float32)
+    dt = 2.0
+    x2 = oracle.langevin_positions(dt / 2, 0.0, 300.0, L, x1, x0, work, m, minv, np.zeros((n, 3)))
+    verlet = 2 * x1 - x0 + dt * dt * minv[:, None].astype(np.float64) * (-4.184e-4 * work)
+    assert np.max(np.abs(x2 - verlet)) < 1e-12
