@@ -56,6 +56,7 @@ enum {
     OG_TERM_KINETIC,    /* kinetic energy                      (MmSystem::kinetic_e) */
     OG_TERM_VIRIAL,     /* sum over pairs of g*r^2 = r_ij . dE/dr_i (new; the reference has none) */
     OG_TERM_EXTEND,     /* VcMD bias energy                    (MmSystem::pote_extend) */
+    OG_TERM_NPAIRS,     /* in-cutoff unmasked atom pairs of the last evaluation (an exact integer) */
     OG_NTERMS
 };
 
@@ -125,6 +126,8 @@ void og_params_default(og_params *p);
  *      cuda_set_pbc :138-146; cuda_set_constant :193-203; cuda_zerodipole_constant :148-153;
  *      cuda_memcpy_htod_atom_info :115-119; cuda_free_atom_info :79-102) ---- */
 og_status og_set_device(int device_id);
+/* Wait for all work on the current device (cuda_thread_sync cuda_setup.cu:793-796). */
+og_status og_thread_sync(void);
 og_status og_system_create(const og_system_desc *desc, const og_params *params, og_system **out);
 og_status og_system_destroy(og_system *sys);
 

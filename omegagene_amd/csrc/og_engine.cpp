@@ -355,6 +355,8 @@ og_status check_flags(og_exec &x) {
         og_dev_counters c;
         OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
         s->host_step = c.step;
+        // the device flags are sticky: clear them once reported
+        if (c.flags & ~OG_F_OUTSIDE_BOX) OG_HIP(hipMemset(&s->d_cnt->flags, 0, sizeof(int)));
         if (c.flags & (OG_F_CP_OVERFLOW | OG_F_CELL_OVERFLOW | OG_F_COL_OVERFLOW)) {
             set_err("device list overflow (flags 0x%x): cells %d, cell pairs cap %d", c.flags, c.n_cells,
                     s->h.cap_cell_pairs);
@@ -526,7 +528,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
               A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) &&
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
-              A(&h.fx, 3 * (size_t)cap) && A(&h.e_cell, 3 * (size_t)h.cap_cells);
+              A(&h.fx, 3 * (size_t)cap) && A(&h.e_cell, 4 * (size_t)h.cap_cells);
     if (!ok) return OG_E_DEVICE;
     h.n_misc_blocks = cdiv(std::max(h.n_excess, std::max(h.n_bonds, h.n_pairs_c)), OG_THREADS);
     h.n_atom_blocks = cdiv(n, OG_THREADS);
@@ -655,6 +657,11 @@ og_status og_last_error(char *buf, int len) {
 
 og_status og_set_device(int device_id) {
     OG_HIP(hipSetDevice(device_id));
+    return OG_OK;
+}
+
+og_status og_thread_sync(void) {
+    OG_HIP(hipDeviceSynchronize());
     return OG_OK;
 }
 

@@ -24,7 +24,8 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_clear(const og_dev_sys *__r
         s.u2c_min[t] = OG_EMPTY_BIN;
         s.u2c_max[t] = -1;
     }
-    if (t == 0) s.cnt->flags &= ~(OG_F_CP_OVERFLOW | OG_F_COL_OVERFLOW | OG_F_CELL_OVERFLOW);
+    // overflow flags stay set until the host has read them (check_flags), so a
+    // failed rebuild inside a multi-step graph is never hidden by a later one
 }
 
 // ------------------------------------------------------------------ bin --

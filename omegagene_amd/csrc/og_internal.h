@@ -25,6 +25,7 @@ enum {
     OG_E_KIN,        // kinetic energy
     OG_E_VIRIAL,     // scalar pair virial  sum r_ij . f_ij
     OG_E_EXTEND,     // VcMD bias energy
+    OG_E_NPAIRS,     // in-cutoff unmasked atom pairs evaluated (exact count, stored as double)
     OG_E_NTERMS
 };
 
@@ -129,7 +130,7 @@ struct og_dev_sys {
     uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
     // ---- forces / energies ----
     long long *fx;          // [3*cap_atom_array] fixed-point gradient, SoA x|y|z
-    double    *e_cell;      // [cap_cells*3] per-cell1 vdw, ele, virial
+    double    *e_cell;      // [cap_cells*4] per-cell1 vdw, ele, virial, pair count
     double    *e_misc;      // [n_misc_blocks*4] excess, bond, ... partials
     double    *e_atom;      // [n_atom_blocks] kinetic partials
     double    *com_mom;     // [n_com_groups*3]

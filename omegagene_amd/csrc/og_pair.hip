@@ -71,6 +71,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
     const float  qi   = pi.w * OG_CHARGE_COEFF;
 
     double dfx = 0.0, dfy = 0.0, dfz = 0.0, ev = 0.0, ee = 0.0, vir = 0.0;
+    int    npair = 0;  // in-cutoff unmasked pairs of this cell1 (wave-uniform)
 
     for (int base = p0; base < p1; base += 64) {
         const int nb = min(64, p1 - base);
@@ -108,6 +109,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             const float    r2  = dx * dx + dy * dy + dz * dz;
             const bool     excl = __builtin_amdgcn_inverse_ballot_w64(mask);
             const bool     ok   = !excl && (r2 < rc2);
+            npair += __builtin_popcountll(__ballot(ok));
             const float    r2s  = ok ? r2 : 1.0f;
             const float    rinv  = __builtin_amdgcn_rsqf(r2s);
             const float    r2inv = rinv * rinv;
@@ -168,9 +170,10 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
         vir += __shfl_xor(vir, off, 64);
     }
     if (lane == 0) {
-        s.e_cell[3 * c1]     = ev;
-        s.e_cell[3 * c1 + 1] = ee;
-        s.e_cell[3 * c1 + 2] = vir;
+        s.e_cell[4 * c1]     = ev;
+        s.e_cell[4 * c1 + 1] = ee;
+        s.e_cell[4 * c1 + 2] = vir;
+        s.e_cell[4 * c1 + 3] = (double)npair;
     }
 }
 
@@ -298,26 +301,28 @@ __global__ void __launch_bounds__(OG_THREADS) k_energy_reduce(const og_dev_sys *
                                                               const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
     const int         n = s.cnt->n_cells;
-    double            a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = 0.0;
+    double            a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = 0.0, f = 0.0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        a += s.e_cell[3 * k];
-        b += s.e_cell[3 * k + 1];
-        c += s.e_cell[3 * k + 2];
+        a += s.e_cell[4 * k];
+        b += s.e_cell[4 * k + 1];
+        c += s.e_cell[4 * k + 2];
+        f += s.e_cell[4 * k + 3];
     }
     for (int k = threadIdx.x; k < s.n_misc_blocks; k += blockDim.x) {
         d += s.e_misc[2 * k];
         e += s.e_misc[2 * k + 1];
     }
-    __shared__ double sh[5 * OG_THREADS];
-    sh[5 * threadIdx.x]     = a;
-    sh[5 * threadIdx.x + 1] = b;
-    sh[5 * threadIdx.x + 2] = c;
-    sh[5 * threadIdx.x + 3] = d;
-    sh[5 * threadIdx.x + 4] = e;
+    __shared__ double sh[6 * OG_THREADS];
+    sh[6 * threadIdx.x]     = a;
+    sh[6 * threadIdx.x + 1] = b;
+    sh[6 * threadIdx.x + 2] = c;
+    sh[6 * threadIdx.x + 3] = d;
+    sh[6 * threadIdx.x + 4] = e;
+    sh[6 * threadIdx.x + 5] = f;
     __syncthreads();
     for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off)
-            for (int k = 0; k < 5; k++) sh[5 * threadIdx.x + k] += sh[5 * (threadIdx.x + off) + k];
+            for (int k = 0; k < 6; k++) sh[6 * threadIdx.x + k] += sh[6 * (threadIdx.x + off) + k];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -326,6 +331,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_energy_reduce(const og_dev_sys *
         s.energy[OG_E_VIRIAL] = sh[2];
         s.energy[OG_E_EXCESS] = sh[3];
         s.energy[OG_E_BOND]   = sh[4];
+        s.energy[OG_E_NPAIRS] = sh[5];
         s.energy[OG_E_SELF]   = s.e_self;
     }
 }

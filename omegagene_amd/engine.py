@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libog_engine.so")
 _LIB = None
 
 OG_OK, OG_E_ARG, OG_E_CAPACITY, OG_E_NAN, OG_E_NOCONV, OG_E_DEVICE, OG_E_STATE = range(7)
-TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend"]
+TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend", "npairs"]
 NTERMS = len(TERMS)
 INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
