@@ -28,6 +28,7 @@ SOURCES = [
     ("og_pair.hip", []),
     ("og_integrate.hip", ["-ffp-contract=off"]),
     ("og_engine.cpp", ["-x", "hip"]),
+    ("og_celeste.cpp", ["-x", "hip", "-ffp-contract=off"]),
 ]
 
 

@@ -157,6 +157,9 @@ int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
 }  // namespace
 
+// error text for the call-point layer (og_celeste.cpp), read by og_last_error
+void og_set_error(const char *msg) { g_last_error = msg; }
+
 // ----------------------------------------------------------------- types --
 enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_N };
 

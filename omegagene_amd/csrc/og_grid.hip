@@ -136,6 +136,8 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_scatter(const og_dev_sys *_
     s.tmp_ids[s.head_atom[col] + slot] = i;
 }
 
+__device__ void og_column_cells(const og_dev_sys &s, int col, int nz);
+
 // ----------------------------------------------------------------- sort --
 // One block per column.  Ascending float z with the atom id as tie-break
 // (the reference Hoare quicksort, MiniCell.cpp:636-691, orders distinct z
@@ -200,6 +202,15 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_sort(const og_dev_sys *__re
         (void)cap;
     }
     __syncthreads();
+    og_column_cells(s, col, nz);
+}
+
+// Cells of one grid column from grid-order coordinates: cell -> column and z
+// index, z extents from the first and last slot of each cell (dummies copy
+// the last real atom, so the last slot holds the top), and the uniform-z bins
+// the cell overlaps (MiniCell::set_uniform_grid, MiniCell.cpp:884-936).
+// Called by one block per column after the column's slots are written.
+__device__ void og_column_cells(const og_dev_sys &s, int col, int nz) {
     const int hc = s.head_cell[col];
     for (int c = threadIdx.x; c < nz; c += blockDim.x) {
         const int   cell = hc + c;
@@ -225,6 +236,18 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_sort(const og_dev_sys *__re
             atomicMax(&s.u2c_max[bin], c);
         }
     }
+}
+
+// Cell tables of a grid built on the host (the reference call points, where
+// MiniCell sorts the atoms and uploads atomids, column heads and float
+// coordinates): one block per column.
+__global__ void __launch_bounds__(OG_THREADS) k_cells_from_grid(const og_dev_sys *__restrict__ S,
+                                                                const int2 *__restrict__ map) {
+    const int2        bm  = map[blockIdx.x];
+    const og_dev_sys &s   = S[bm.x];
+    const int         col = bm.y;
+    if (s.head_cell[col] < 0) return;
+    og_column_cells(s, col, s.n_cells_z[col]);
 }
 
 // ---------------------------------------------------------- enumeration --
@@ -499,6 +522,9 @@ void og_launch_grid_scatter(const og_launch &a) {
 }
 void og_launch_grid_sort(const og_launch &a) {
     hipLaunchKernelGGL(k_grid_sort, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_cells_from_grid(const og_launch &a) {
+    hipLaunchKernelGGL(k_cells_from_grid, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_enum_count(const og_launch &a) {
     hipLaunchKernelGGL(k_enum_count, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);

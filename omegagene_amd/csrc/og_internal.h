@@ -218,6 +218,7 @@ void og_launch_grid_bin(const og_launch &atoms);
 void og_launch_grid_scan(const og_launch &sys);
 void og_launch_grid_scatter(const og_launch &atoms);
 void og_launch_grid_sort(const og_launch &cols);
+void og_launch_cells_from_grid(const og_launch &cols);
 void og_launch_enum_count(const og_launch &cellw);
 void og_launch_enum_scan(const og_launch &sys);
 void og_launch_enum_write(const og_launch &cellw);

@@ -297,8 +297,14 @@ def footprint(ls: LaunchSet, cfg: dict) -> int:
     return int(b.value)
 
 
-def exports():
-    """Names of the C-ABI symbols declared in include/og_engine.h."""
+def exports(header: str = None):
+    """Names of the C-ABI symbols declared in include/*.h (or one header)."""
+    import glob
     import re
-    hdr = open(os.path.join(os.path.dirname(_HERE), "include", "og_engine.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:og_status|void|const char \*)\s*(og_\w+)\s*\(", hdr, re.M)))
+    inc = os.path.join(os.path.dirname(_HERE), "include")
+    paths = [os.path.join(inc, header)] if header else sorted(glob.glob(os.path.join(inc, "*.h")))
+    names = set()
+    for p in paths:
+        hdr = open(p).read()
+        names |= set(re.findall(r"^\s*(?:og_status|void|const char \*)\s*(og_\w+)\s*\(", hdr, re.M))
+    return sorted(names)

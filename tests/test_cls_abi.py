@@ -79,3 +79,24 @@ def test_params_from_cfg_units():
     assert p.langevin_gamma == pytest.approx(1e-3)  # ps^-1 -> fs^-1 (Config.cpp:345-348)
     assert p.integrator == 1 and p.constraint == 2
     assert p.nsgrid_cutoff == 11.0                 # default cutoff + 1 when not given
+
+
+def test_call_point_header_mirrors_reference_abi():
+    """Every cuda_* call point of the reference's GPU path (declared at
+    SubBox.cpp:5-78 and MiniCell.cpp:7-35) has an og_ counterpart in
+    og_celeste.h that cites its definition, and the library exports it."""
+    hdr = open(os.path.join(os.path.dirname(engine.LIB_PATH), "..", "include", "og_celeste.h")).read()
+    points = ["hostalloc_atom_info", "hostfree_atom_info", "hostalloc_cell_info", "hostfree_cell_info",
+              "hostalloc_atom_type_charge", "hostfree_atom_type_charge", "alloc_atom_info", "free_atom_info",
+              "alloc_set_lj_params", "free_lj_params", "set_pbc", "set_constant", "zerodipole_constant",
+              "memcpy_htod_atom_info", "set_cell_constant", "memcpy_htod_atomids", "set_atominfo",
+              "memcpy_htod_crd", "set_crd", "enumerate_cell_pairs", "pairwise_ljzd", "pair_sync",
+              "memcpy_dtoh_work"]
+    declared = engine.exports("og_celeste.h")
+    for p in points:
+        assert "og_" + p in declared, p
+        assert re.search(r"cuda_%s cuda_setup\.cu:\d+" % p, hdr), p
+    # cuda_set_device / cuda_thread_sync map onto og_set_device / og_thread_sync (og_engine.h)
+    assert {"og_set_device", "og_thread_sync"} <= set(engine.exports("og_engine.h"))
+    lib = ctypes.CDLL(engine.LIB_PATH)
+    assert all(hasattr(lib, "og_" + p) for p in points)
