@@ -57,6 +57,8 @@ enum {
     OG_TERM_VIRIAL,     /* sum over pairs of g*r^2 = r_ij . dE/dr_i (new; the reference has none) */
     OG_TERM_EXTEND,     /* VcMD bias energy                    (MmSystem::pote_extend) */
     OG_TERM_NPAIRS,     /* in-cutoff unmasked atom pairs of the last evaluation (an exact integer) */
+    OG_TERM_LAMBDA,     /* VcMD reaction coordinate: distance between the two group mass centres (A) */
+    OG_TERM_VSTATE,     /* VcMD virtual state in force during the step (0-based) */
     OG_NTERMS
 };
 
@@ -95,6 +97,18 @@ typedef struct {
     const int    *com_group_off;    /* [n_com_groups+1] */
     const int    *com_group_atoms;
     int           d_free;           /* degrees of freedom for the temperature */
+    /* VcMD 1-D virtual-state bias on the distance between the mass centres of
+     * two atom groups (ExtendedVcMD, Extend.cpp:655-1102; input block
+     * Read.cpp:705-779).  vc_n_states = 0 disables it. */
+    int           vc_n_states;
+    const double *vc_min;           /* [n_states] lower edge of each state's lambda range */
+    const double *vc_max;           /* [n_states] upper edge */
+    const double *vc_q_cano;        /* [n_states] canonical weights; 0 = the default (Extend.cpp:548-566) */
+    int           vc_init_state;    /* 0-based */
+    int           vc_interval;      /* steps between transition trials */
+    int           vc_n_a, vc_n_b;   /* group sizes */
+    const int    *vc_atoms_a;       /* [vc_n_a] */
+    const int    *vc_atoms_b;       /* [vc_n_b] */
 } og_system_desc;
 
 /* Run parameters: the .cfg keys of the path (Config.cpp:37-367). */
@@ -117,6 +131,8 @@ typedef struct {
     int      log_stride;           /* energies kept every log_stride steps */
     int      log_cap;              /* entries kept in the device energy log */
     double   capacity_factor;      /* cell-pair list capacity multiplier (1.0 default) */
+    double   enhance_sigma;        /* --enhance-sigma (VcMD, A) */
+    double   enhance_recov_coef;   /* --enhance-recovery-coef (VcMD) */
 } og_params;
 
 void og_params_default(og_params *p);
@@ -156,6 +172,9 @@ og_status og_get_energies(og_system *sys, double *energies /* [OG_NTERMS] */);
 /* energy log of the last run: rows of OG_NTERMS doubles, one per log_stride steps */
 og_status og_get_energy_log(og_system *sys, double *out, int max_rows, int *n_rows);
 og_status og_get_step(og_system *sys, long long *step);
+/* VcMD state: current virtual state and the visit counts per state
+ * (q_raw, Extend.cpp:828-832); counts may be NULL */
+og_status og_get_vcmd(og_system *sys, int *vstate, double *counts, int n_states);
 
 /* Grid / cell-pair inspection in the reference layouts (parity tests):
  * atomids [cap >= n_atom_array] grid slot -> atom (-1 dummy);

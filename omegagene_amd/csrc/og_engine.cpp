@@ -191,6 +191,7 @@ struct og_exec {
     std::vector<hipEvent_t>  ev;          // pair-kernel timing events
     int                      alpha_mode = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
+    bool                     any_vcmd = false;
     int                      intvl = 1;
 };
 
@@ -232,7 +233,7 @@ og_status exec_prepare(og_exec &x) {
     std::vector<og_dev_sys> hs(ns);
     std::vector<int2>       maps[KIND_N];
     x.alpha_mode   = x.members[0]->alpha_mode;
-    x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = false;
+    x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = false;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
         og_system *s = x.members[si];
@@ -250,6 +251,7 @@ og_status exec_prepare(og_exec &x) {
         x.any_leapfrog |= d.integrator == 0;
         x.any_com |= d.com_cancel && d.n_com_groups > 0;
         x.any_constraint |= d.constraint != 0;
+        x.any_vcmd |= d.vc_n_states > 0;
         const int n_bins = (d.n_columns + 1) * d.n_uni_z;
         int       cnt[KIND_N];
         cnt[KIND_ATOM]  = cdiv(d.n_atoms, OG_THREADS);
@@ -307,6 +309,9 @@ void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1) {
 }
 
 void enqueue_integrate(og_exec &x) {
+    // the extended-ensemble bias acts on the gradient of the step
+    // (DynamicsMode.cpp:504-512, :824-834)
+    if (x.any_vcmd) og_launch_vcmd(L(x, KIND_SYS));
     if (x.any_leapfrog) {
         og_launch_leapfrog_vel(L(x, KIND_ATOM));
         if (x.any_com) {
@@ -502,7 +507,14 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.settle_dHH  = d->settle_dHH;
     h.seed_lo     = (uint32_t)p->random_seed;
     h.seed_hi     = (uint32_t)(p->random_seed >> 32) ^ 0x9E3779B9u;
-    h.vc_n_states = 0;
+    h.vc_n_states = d->vc_n_states;
+    h.vc_interval = std::max(1, d->vc_interval);
+    h.vc_na       = d->vc_n_states ? d->vc_n_a : 0;
+    h.vc_nb       = d->vc_n_states ? d->vc_n_b : 0;
+    h.vc_sigma    = p->enhance_sigma;
+    h.vc_coef     = p->enhance_recov_coef;
+    // ExtendedVcMD::set_temperature (Extend.cpp:573-576): k_B T in kcal/mol
+    h.vc_constk = (8.31451 / 4.184) * 1e-3 * p->temperature;
 
     std::vector<float> q(n), m(n), minv(n);
     for (int i = 0; i < n; i++) {
@@ -597,6 +609,37 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
             gminv[gi] = (float)(1.0 / (double)sum);
         }
         OG_HIP(hipMemcpy(h.com_minv, gminv.data(), sizeof(float) * ng, hipMemcpyHostToDevice));
+    }
+    if (h.vc_n_states) {
+        const int ns = h.vc_n_states, ng = h.vc_na + h.vc_nb;
+        ok = A(&h.vc_atoms, ng) && A(&h.vc_crdg, 3 * (size_t)ng) && A(&h.vc_lo, ns) && A(&h.vc_hi, ns) &&
+             A(&h.vc_qinv, ns) && A(&h.vc_hist, ns);
+        if (!ok) return OG_E_DEVICE;
+        std::vector<int>    ids(d->vc_atoms_a, d->vc_atoms_a + h.vc_na);
+        ids.insert(ids.end(), d->vc_atoms_b, d->vc_atoms_b + h.vc_nb);
+        std::vector<double> cg(3 * (size_t)ng);
+        float               ma = 0.0f, mb = 0.0f;  // float group masses (MmSystem.cpp:572-582)
+        for (int k = 0; k < ng; k++) {
+            for (int j = 0; j < 3; j++) cg[3 * k + j] = crd0[3 * (size_t)ids[k] + j];
+            (k < h.vc_na ? ma : mb) += m[ids[k]];
+        }
+        h.vc_minv_a = (double)(float)(1.0 / (double)ma);
+        h.vc_minv_b = (double)(float)(1.0 / (double)mb);
+        // ExtendedVcMD::set_default_q_cano (Extend.cpp:548-566): unset weights take the smallest positive one
+        double qmin = 1e10;
+        for (int k = 0; k < ns; k++)
+            if (d->vc_q_cano[k] > 0 && d->vc_q_cano[k] < qmin) qmin = d->vc_q_cano[k];
+        if (qmin == 1e10) qmin = 1.0;
+        std::vector<double> qinv(ns);
+        for (int k = 0; k < ns; k++) qinv[k] = 1.0 / (d->vc_q_cano[k] > 0 ? d->vc_q_cano[k] : qmin);
+        OG_HIP(hipMemcpy(h.vc_atoms, ids.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.vc_crdg, cg.data(), sizeof(double) * cg.size(), hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.vc_lo, d->vc_min, sizeof(double) * ns, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.vc_hi, d->vc_max, sizeof(double) * ns, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.vc_qinv, qinv.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+        og_dev_counters c0{};
+        c0.vs = d->vc_init_state;
+        OG_HIP(hipMemcpy(s->d_cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
     }
     OG_HIP(hipMemcpy(s->d_self, &h, sizeof h, hipMemcpyHostToDevice));
     return OG_OK;
@@ -740,6 +783,29 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
             seen[a] = 1;
         }
     }
+    if (d->vc_n_states) {
+        // two disjoint, non-empty groups; ranges and weights for every state
+        const int ns = d->vc_n_states;
+        if (ns < 0 || !d->vc_min || !d->vc_max || !d->vc_q_cano || d->vc_init_state < 0 || d->vc_init_state >= ns ||
+            d->vc_n_a <= 0 || d->vc_n_b <= 0 || !d->vc_atoms_a || !d->vc_atoms_b || d->vc_interval <= 0) {
+            set_err("VcMD: states, initial state, interval or groups missing or out of range");
+            return OG_E_ARG;
+        }
+        std::vector<char> seen(d->n_atoms, 0);
+        for (int k = 0; k < d->vc_n_a + d->vc_n_b; k++) {
+            const int a = k < d->vc_n_a ? d->vc_atoms_a[k] : d->vc_atoms_b[k - d->vc_n_a];
+            if (a < 0 || a >= d->n_atoms || seen[a]) {
+                set_err("VcMD groups must be disjoint sets of valid atoms");
+                return OG_E_ARG;
+            }
+            seen[a] = 1;
+        }
+        for (int k = 0; k < ns; k++)
+            if (!(d->vc_min[k] < d->vc_max[k])) {
+                set_err("VcMD state %d: empty lambda range", k);
+                return OG_E_ARG;
+            }
+    }
     s->d_free = d->d_free;
     og_status st = sys_alloc(s.get(), d, p);
     if (st != OG_OK) {
@@ -858,6 +924,22 @@ og_status og_get_step(og_system *s, long long *step) {
     og_dev_counters c;
     OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
     *step = c.step;
+    return OG_OK;
+}
+
+og_status og_get_vcmd(og_system *s, int *vstate, double *counts, int n_states) {
+    if (!s) return OG_E_ARG;
+    if (s->h.vc_n_states == 0) {
+        set_err("og_get_vcmd: the system has no VcMD block");
+        return OG_E_STATE;
+    }
+    og_dev_counters c;
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    if (vstate) *vstate = c.vs;
+    if (counts) {
+        if (n_states < s->h.vc_n_states) return OG_E_ARG;
+        OG_HIP(hipMemcpy(counts, s->h.vc_hist, sizeof(double) * s->h.vc_n_states, hipMemcpyDeviceToHost));
+    }
     return OG_OK;
 }
 

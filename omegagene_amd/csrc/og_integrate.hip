@@ -419,7 +419,114 @@ __global__ void __launch_bounds__(OG_THREADS) k_step_end(const og_dev_sys *__res
     }
 }
 
+// ----------------------------------------------------------------- VcMD --
+// One wave per system, after the forces and before the velocity update
+// (DynamicsModePresto/Langevin::calc_in_each_step, DynamicsMode.cpp:504-512,
+// :824-834):
+//  * ExtendedVcMD::set_crd_centers (Extend.cpp:753-797): group coordinates
+//    unwrapped against their previous values, mass centres;
+//  * set_struct_parameters_mass_center (:668-692): lambda = |minimum-image
+//    difference of the centres|, unit vector;
+//  * scale_force (:1035-1102): outside [min - sigma, max + sigma) of the
+//    current state the gradient of every group atom gets
+//    +-k_B T * coef * excess * u * m_i / M_group, and the reported energy is
+//    coef * excess^2 (the reference's energy/force pair, kept as is);
+//  * vs_step + trial_transition (:808-845, :907-1033): the state counts a
+//    visit while lambda is in its range, and every vc_interval steps moves
+//    to a neighbouring state whose range holds lambda with probability
+//    proportional to 1/q_cano (uniform from Philox keyed by seed and step).
+__global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (s.vc_n_states == 0) return;
+    const int lane = threadIdx.x;
+    const int n    = s.vc_na + s.vc_nb;
+    for (int k = lane; k < n; k += 64) {
+        const int a = s.vc_atoms[k];
+        for (int d = 0; d < 3; d++) {
+            const double Lh   = (double)(s.L[d] * 0.5f);
+            double       tmp  = s.crd[3 * a + d];
+            double       diff = tmp - s.vc_crdg[3 * k + d];
+            while (diff > Lh) {
+                diff -= (double)s.L[d];
+                tmp -= (double)s.L[d];
+            }
+            while (-diff > Lh) {
+                diff += (double)s.L[d];
+                tmp += (double)s.L[d];
+            }
+            s.vc_crdg[3 * k + d] = tmp;
+        }
+    }
+    __syncthreads();
+    __shared__ double bias[3];
+    __shared__ int    active;
+    if (lane == 0) {
+        // mass centres summed in group order, as the reference loop
+        double ca[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < n; k++) {
+            const double ms = (double)s.mass[s.vc_atoms[k]];
+            for (int d = 0; d < 3; d++) {
+                if (k < s.vc_na) ca[d] += s.vc_crdg[3 * k + d] * ms;
+                else cb[d] += s.vc_crdg[3 * k + d] * ms;
+            }
+        }
+        double diff[3];
+        for (int d = 0; d < 3; d++) diff[d] = og_mimg(ca[d] * s.vc_minv_a - cb[d] * s.vc_minv_b, s.L[d]);
+        const double lam = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+        const int    vs  = s.cnt->vs;
+        const double lo = s.vc_lo[vs], hi = s.vc_hi[vs];
+        double       ex = 0.0, rec = 0.0;
+        if (lam < lo - s.vc_sigma) {
+            ex  = lam - (lo - s.vc_sigma);
+            rec = s.vc_coef * ex;
+        } else if (lam >= hi + s.vc_sigma) {
+            ex  = lam - (hi + s.vc_sigma);
+            rec = s.vc_coef * ex;
+        }
+        const double dew = s.vc_constk * rec;
+        for (int d = 0; d < 3; d++) bias[d] = diff[d] / lam;  // unit vector, scaled below
+        for (int d = 0; d < 3; d++) bias[d] = dew * bias[d];
+        active                = rec != 0.0;
+        s.energy[OG_E_EXTEND] = rec * ex;
+        s.energy[OG_E_LAMBDA] = lam;
+        s.energy[OG_E_VSTATE] = (double)vs;
+        // visit count and transition trial
+        const long long step     = s.cnt->step;
+        const bool      in_range = lam >= lo && lam < hi;
+        if (in_range) s.vc_hist[vs] += 1.0;
+        if (step > 0 && step % s.vc_interval == 0 && in_range) {
+            int cand[2] = {vs, -1};
+            if (vs > 0) {
+                if (lam < s.vc_hi[vs - 1]) cand[1] = vs - 1;
+                else if (vs < s.vc_n_states - 1 && lam >= s.vc_lo[vs + 1]) cand[1] = vs + 1;
+            } else if (s.vc_n_states > 1 && lam >= s.vc_lo[1]) {
+                cand[1] = 1;
+            }
+            if (cand[1] >= 0) {
+                const uint4  r   = og_philox(make_uint4((uint32_t)step, (uint32_t)(step >> 32), 0u, 0x5643u),
+                                             make_uint2(s.seed_lo, s.seed_hi));
+                const double u   = ((double)r.x + 0.5) * 2.3283064365386963e-10;
+                const double w0  = s.vc_qinv[cand[0]], w1 = s.vc_qinv[cand[1]];
+                s.cnt->vs        = u * (w0 + w1) < w0 ? cand[0] : cand[1];
+            }
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    // gradient ("work") of the group atoms; the groups are disjoint
+    for (int k = lane; k < n; k += 64) {
+        const int    a    = s.vc_atoms[k];
+        const bool   ga   = k < s.vc_na;
+        const double minv = ga ? s.vc_minv_a : s.vc_minv_b;
+        const double dir  = ga ? 1.0 : -1.0;
+        for (int d = 0; d < 3; d++) s.work[3 * a + d] += dir * (bias[d] * minv) * (double)s.mass[a];
+    }
+}
+
 // ------------------------------------------------------------- launchers --
+void og_launch_vcmd(const og_launch &a) {
+    hipLaunchKernelGGL(k_vcmd, dim3(a.n_blocks), dim3(64), 0, a.stream, a.sys, a.map);
+}
 void og_launch_leapfrog_vel(const og_launch &a) {
     hipLaunchKernelGGL(k_leapfrog_vel, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }

@@ -26,6 +26,8 @@ enum {
     OG_E_VIRIAL,     // scalar pair virial  sum r_ij . f_ij
     OG_E_EXTEND,     // VcMD bias energy
     OG_E_NPAIRS,     // in-cutoff unmasked atom pairs evaluated (exact count, stored as double)
+    OG_E_LAMBDA,     // VcMD reaction coordinate
+    OG_E_VSTATE,     // VcMD virtual state during the step
     OG_E_NTERMS
 };
 
@@ -81,11 +83,13 @@ struct og_dev_sys {
     double e_self;
     uint32_t seed_lo, seed_hi;
     // ---- VcMD 1-D bias on the distance between two group mass centres ----
-    int    vc_n_states, vc_grp_a0, vc_grp_a1, vc_grp_b0, vc_grp_b1;   // index ranges into com_atoms
-    double vc_sigma, vc_coef, vc_kT;
-    double *vc_lo, *vc_hi;   // [n_states]
-    double *vc_hist;         // visit counts per state
-    double *vc_lambda;       // lambda per step (log)
+    int    vc_n_states, vc_interval, vc_na, vc_nb;
+    double vc_sigma, vc_coef, vc_constk;
+    double vc_minv_a, vc_minv_b;       // float 1/group mass, as MmSystem.cpp:572-582
+    int    *vc_atoms;                  // [vc_na + vc_nb], group a then group b
+    double *vc_crdg;                   // [3*(vc_na+vc_nb)] unwrapped group coordinates
+    double *vc_lo, *vc_hi, *vc_qinv;   // [n_states] ranges and 1/q_cano
+    double *vc_hist;                   // [n_states] visit counts (q_raw)
     // ---- atom-order arrays ----
     double *crd, *crd_prev, *crd_prev2, *vel, *vel_next, *work;
     float  *mass, *mass_inv, *charge;
@@ -237,5 +241,5 @@ void og_launch_langevin_positions(const og_launch &atoms);
 void og_launch_constraints(const og_launch &trio, const og_launch &misc);
 void og_launch_finish(const og_launch &atoms);
 void og_launch_step_end(const og_launch &sys);
-void og_launch_vcmd(const og_launch &sys, const og_launch &atoms);
+void og_launch_vcmd(const og_launch &sys);
 void og_launch_refresh_grid(const og_launch &slots);

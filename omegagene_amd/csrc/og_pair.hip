@@ -95,8 +95,9 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
                 pjn = X[a2n];
                 tjn = TY[a2n];
             }
-            const uint64_t mask =
-                ((uint64_t)__builtin_amdgcn_readlane(mhi, k) << 32) | (uint64_t)__builtin_amdgcn_readlane(mlo, k);
+            // readlane returns int: widen through uint32_t so the low word is not sign-extended
+            const uint64_t mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, k) << 32) |
+                                  (uint64_t)(uint32_t)__builtin_amdgcn_readlane(mlo, k);
             const unsigned img = c2i >> 24;
             const float    sx  = (img & 1u) ? -L0 : ((img & 2u) ? L0 : 0.0f);
             const float    sy  = (img & 4u) ? -L1 : ((img & 8u) ? L1 : 0.0f);

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libog_engine.so")
 _LIB = None
 
 OG_OK, OG_E_ARG, OG_E_CAPACITY, OG_E_NAN, OG_E_NOCONV, OG_E_DEVICE, OG_E_STATE = range(7)
-TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend", "npairs"]
+TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend", "npairs", "lambda", "vstate"]
 NTERMS = len(TERMS)
 INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
@@ -43,7 +43,11 @@ class Desc(C.Structure):
                 ("shake_trio_ids", C.POINTER(C.c_int)), ("shake_trio_d2", C.POINTER(C.c_double)),
                 ("n_settle", C.c_int), ("settle_ids", C.POINTER(C.c_int)), ("settle_dOH", C.c_double),
                 ("settle_dHH", C.c_double), ("n_com_groups", C.c_int), ("com_group_off", C.POINTER(C.c_int)),
-                ("com_group_atoms", C.POINTER(C.c_int)), ("d_free", C.c_int)]
+                ("com_group_atoms", C.POINTER(C.c_int)), ("d_free", C.c_int),
+                ("vc_n_states", C.c_int), ("vc_min", C.POINTER(C.c_double)), ("vc_max", C.POINTER(C.c_double)),
+                ("vc_q_cano", C.POINTER(C.c_double)), ("vc_init_state", C.c_int), ("vc_interval", C.c_int),
+                ("vc_n_a", C.c_int), ("vc_n_b", C.c_int), ("vc_atoms_a", C.POINTER(C.c_int)),
+                ("vc_atoms_b", C.POINTER(C.c_int))]
 
 
 class Params(C.Structure):
@@ -52,7 +56,8 @@ class Params(C.Structure):
                 ("constraint", C.c_int), ("const_tolerance", C.c_double), ("const_max_loops", C.c_int),
                 ("temperature", C.c_double), ("langevin_gamma", C.c_double), ("random_seed", C.c_uint64),
                 ("com_cancel", C.c_int), ("gap_bug", C.c_int), ("excess_bug", C.c_int), ("log_stride", C.c_int),
-                ("log_cap", C.c_int), ("capacity_factor", C.c_double)]
+                ("log_cap", C.c_int), ("capacity_factor", C.c_double), ("enhance_sigma", C.c_double),
+                ("enhance_recov_coef", C.c_double)]
 
 
 def lib():
@@ -97,6 +102,8 @@ def params_from_cfg(cfg: dict) -> Params:
             setattr(p, attr, int(cfg[k]))
     if "capacity-factor" in cfg:
         p.capacity_factor = float(cfg["capacity-factor"])
+    p.enhance_sigma = float(g("enhance-sigma", 0.0))                # Config.cpp --enhance-sigma
+    p.enhance_recov_coef = float(g("enhance-recovery-coef", 0.0))   # Config.cpp --enhance-recovery-coef
     return p
 
 
@@ -104,7 +111,7 @@ class _Keep:
     """Holds the numpy arrays a Desc points into."""
 
 
-def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None):
+def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcmd: bool = False):
     k = _Keep()
 
     def d(a, dt=np.float64):
@@ -152,6 +159,24 @@ def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None):
     x.com_group_atoms = d(np.concatenate([np.asarray(g_, np.int32) for g_ in groups]) if groups else
                           np.zeros(1, np.int32), np.int32)
     x.d_free = ls.d_free(constraint != 0)
+    vc = ls.vcmd if vcmd else None
+    if vc is not None:
+        # 1-D VcMD on the distance between the mass centres of the dimension's two groups (Read.cpp:705-779)
+        gids, _names, ranges = vc.dims[0]
+        ga = np.asarray(ls.groups[gids[0] - 1][1], np.int32)
+        gb = np.asarray(ls.groups[gids[1] - 1][1], np.int32)
+        q = np.ones(len(ranges))
+        for st, w in vc.q_cano:
+            q[st[0]] = w
+        x.vc_n_states = len(ranges)
+        x.vc_min = d(np.array([r[0] for r in ranges]))
+        x.vc_max = d(np.array([r[1] for r in ranges]))
+        x.vc_q_cano = d(q)
+        x.vc_init_state = int(vc.init_vs[0])
+        x.vc_interval = int(vc.interval)
+        x.vc_n_a, x.vc_n_b = len(ga), len(gb)
+        x.vc_atoms_a = d(ga, np.int32)
+        x.vc_atoms_b = d(gb, np.int32)
     return x, k
 
 
@@ -162,7 +187,8 @@ class System:
         L = lib()
         _check(L.og_set_device(C.c_int(device)))
         self.params = params_from_cfg(cfg)
-        self.desc, self._keep = desc_from_launchset(ls, self.params.constraint, com_groups)
+        vcmd = cfg.get("extended-ensemble", "none") == "vcmd" and ls.vcmd is not None
+        self.desc, self._keep = desc_from_launchset(ls, self.params.constraint, com_groups, vcmd=vcmd)
         self.n_atoms = ls.n_atoms
         self.d_free = self.desc.d_free
         self.h = C.c_void_p()
@@ -221,6 +247,13 @@ class System:
         s = C.c_longlong()
         _check(lib().og_get_step(self.h, C.byref(s)))
         return s.value
+
+    def vcmd(self, n_states: int):
+        """Current virtual state and visit counts per state."""
+        vs = C.c_int()
+        cnt = np.zeros(n_states)
+        _check(lib().og_get_vcmd(self.h, C.byref(vs), cnt.ctypes.data_as(C.c_void_p), C.c_int(n_states)))
+        return vs.value, cnt
 
     def grid_info(self):
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -355,3 +355,29 @@ def run_ref_minicell(n_atoms, box, nscut, crd, nb15off_table, workdir):
     cps = np.frombuffer(b, CELLPAIR_DTYPE, n_cp, o).copy()
     return dict(n_cells=int(n_cells), atomids=atomids, crd=gcrd, idx_xy_head_cell=head_xy,
                 idx_head_cell_pairs=head_cp, cell_pairs=cps)
+
+
+def vcmd_bias(L, atoms_a, atoms_b, crdg, crd, mass, n_states_lo, n_states_hi, vs, sigma, coef, T, work):
+    """orc_vcmd_bias: VcMD 1-D bias into work (in place) and crdg (in place).
+    Returns (energy, lambda, in_range, (cand0, cand1))."""
+    f = lib().orc_vcmd_bias
+    f.restype = C.c_double
+    m32 = np.ascontiguousarray(mass, np.float32)
+    a = np.ascontiguousarray(atoms_a, np.int32)
+    b = np.ascontiguousarray(atoms_b, np.int32)
+    ma = np.float32(m32[a].sum(dtype=np.float32))
+    mb = np.float32(m32[b].sum(dtype=np.float32))
+    minv_a = np.float32(1.0 / np.float64(ma))
+    minv_b = np.float32(1.0 / np.float64(mb))
+    lo = np.ascontiguousarray(n_states_lo, np.float64)
+    hi = np.ascontiguousarray(n_states_hi, np.float64)
+    lam = C.c_double()
+    inr = C.c_int()
+    cand = (C.c_int * 2)()
+    assert crdg.dtype == np.float64 and crdg.flags.c_contiguous and work.dtype == np.float64
+    e = f(_p(np.ascontiguousarray(L, np.float32), C.c_float), C.c_int(len(a)), _p(a, C.c_int), C.c_int(len(b)),
+          _p(b, C.c_int), _p(crdg, C.c_double), _p(np.ascontiguousarray(crd, np.float64), C.c_double),
+          _p(m32, C.c_float), C.c_float(minv_a), C.c_float(minv_b), C.c_int(len(lo)), _p(lo, C.c_double),
+          _p(hi, C.c_double), C.c_int(vs), C.c_double(sigma), C.c_double(coef), C.c_double(T), _p(work, C.c_double),
+          C.byref(lam), C.byref(inr), cand)
+    return e, lam.value, bool(inr.value), (cand[0], cand[1])

@@ -182,6 +182,14 @@ void orc_langevin_positions(int n, double dt_half, double gamma, double temperat
 void orc_langevin_velocity(int n, double dt, const float L[3], const double *crd, const double *crd_prev2,
                            double *vel);
 
+/* VcMD 1-D bias (Extend.cpp:668-692, :753-797, :1035-1102) and the
+ * transition candidates of vs_step/set_vs_next (:808-880).  Adds the bias to
+ * work[n*3]; returns the bias energy. */
+double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
+                     const double *crd, const float *mass, float minv_a, float minv_b, int n_states,
+                     const double *lo, const double *hi, int vs, double sigma, double coef, double temperature,
+                     double *work, double *lambda_out, int *in_range, int cand[2]);
+
 #ifdef __cplusplus
 }
 #endif

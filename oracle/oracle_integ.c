@@ -267,3 +267,89 @@ void orc_langevin_velocity(int n, double dt, const float L[3], const double *crd
             vel[i * 3 + d] = (p1 - p2) * dt2inv;
         }
 }
+
+/* ---- VcMD 1-D virtual-state bias ---------------------------------------
+ * ExtendedVcMD::set_crd_centers (Extend.cpp:753-797), set_struct_parameters_mass_center
+ * (:668-692), scale_force (:1035-1102) and the visit count / transition
+ * candidates of vs_step + set_vs_next (:808-832, :846-880), for one
+ * dimension with group a = enhance pair[0] and b = pair[1].
+ * crdg[(na+nb)*3] holds the unwrapped group coordinates between calls.
+ * minv_a/minv_b are the float group inverse masses (MmSystem.cpp:572-582).
+ * Returns the bias energy coef*excess^2 and writes lambda, whether lambda is
+ * inside the current state's range, and the transition candidates
+ * cand[0] = vs, cand[1] = neighbour or -1. */
+double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
+                     const double *crd, const float *mass, float minv_a, float minv_b, int n_states,
+                     const double *lo, const double *hi, int vs, double sigma, double coef, double temperature,
+                     double *work, double *lambda_out, int *in_range, int cand[2]) {
+    double c[2][3] = {{0, 0, 0}, {0, 0, 0}};
+    for (int g = 0; g < 2; g++) {
+        const int  n   = g == 0 ? na : nb;
+        const int *ids = g == 0 ? atoms_a : atoms_b;
+        double    *cg  = crdg + (g == 0 ? 0 : 3 * na);
+        for (int i = 0; i < n; i++) {
+            const int a = ids[i];
+            for (int d = 0; d < 3; d++) {
+                const float Lh   = L[d] * 0.5f;
+                double      tmp  = crd[3 * a + d];
+                double      diff = tmp - cg[3 * i + d];
+                while (diff > Lh) {
+                    diff -= L[d];
+                    tmp -= L[d];
+                }
+                while (-diff > Lh) {
+                    diff += L[d];
+                    tmp += L[d];
+                }
+                cg[3 * i + d] = tmp;
+                c[g][d] += tmp * mass[a];
+            }
+        }
+    }
+    double diff[3];
+    for (int d = 0; d < 3; d++) {
+        c[0][d] *= minv_a;
+        c[1][d] *= minv_b;
+        double      v   = c[0][d] - c[1][d];
+        const float Lhi = 1.0f / (L[d] * 0.5f);
+        /* PBC::diff_crd_minim_image, double overload (PBC.cpp:62-82) */
+        v       = v - L[d] * (float)(int)(v * Lhi);
+        v       = v - L[d] * (float)(int)(v * Lhi);
+        diff[d] = v;
+    }
+    const double lam = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+    double       u[3];
+    for (int d = 0; d < 3; d++) u[d] = diff[d] / lam;
+    double ex = 0.0, rec = 0.0;
+    if (lam < lo[vs] - sigma) {
+        ex  = lam - (lo[vs] - sigma);
+        rec = coef * ex;
+    } else if (lam >= hi[vs] + sigma) {
+        ex  = lam - (hi[vs] + sigma);
+        rec = coef * ex;
+    }
+    const double const_k = (8.31451 / 4.184) * 1e-3 * temperature; /* Extend.cpp:573-576 */
+    const double dew     = const_k * rec;
+    double       dir     = 1.0;
+    for (int g = 0; g < 2; g++) {
+        const int   n    = g == 0 ? na : nb;
+        const int  *ids  = g == 0 ? atoms_a : atoms_b;
+        const float minv = g == 0 ? minv_a : minv_b;
+        double      bias[3];
+        for (int d = 0; d < 3; d++) bias[d] = dew * u[d] * (double)minv;
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) work[3 * ids[i] + d] += dir * bias[d] * (double)mass[ids[i]];
+        dir *= -1.0;
+    }
+    *lambda_out = lam;
+    *in_range   = lam >= lo[vs] && lam < hi[vs];
+    cand[0]     = vs;
+    cand[1]     = -1;
+    if (vs > 0) {
+        if (lam < hi[vs - 1]) cand[1] = vs - 1;
+        else if (vs < n_states - 1 && lam >= lo[vs + 1]) cand[1] = vs + 1;
+    } else if (n_states > 1 && lam >= lo[1]) {
+        cand[1] = 1;
+    }
+    return rec * ex;
+}

@@ -262,3 +262,57 @@ def test_langevin_zero_noise_zero_friction_is_verlet():
     x2 = oracle.langevin_positions(dt / 2, 0.0, 300.0, L, x1, x0, work, m, minv, np.zeros((n, 3)))
     verlet = 2 * x1 - x0 + dt * dt * minv[:, None].astype(np.float64) * (-4.184e-4 * work)
     assert np.max(np.abs(x2 - verlet)) < 1e-12
+
+
+# ---------------------------------------------------------------- VcMD bias
+
+def test_vcmd_bias_closed_form_two_atoms():
+    """Two single-atom groups: lambda is their distance; outside
+    [min - sigma, max + sigma) the gradient of atom a is
+    +k_B T * coef * excess * u and of atom b its negative, and the energy is
+    coef * excess^2 (Extend.cpp:1035-1102; the energy/force pair of the
+    reference is kept as is)."""
+    L = np.array([40.0, 40.0, 40.0], np.float32)
+    crd = np.array([[1.0, 2.0, 3.0], [1.0 + 6.0, 2.0 + 8.0, 3.0]])  # distance 10
+    mass = np.array([12.0, 12.0])
+    lo, hi = np.array([2.0, 4.0]), np.array([5.0, 7.0])
+    T, sigma, coef = 300.0, 0.5, 2.0
+    kT = (8.31451 / 4.184) * 1e-3 * T
+    for vs, edge in ((0, 5.0 + 0.5), (1, 7.0 + 0.5)):
+        work = np.zeros((2, 3))
+        crdg = crd.copy()
+        e, lam, inr, cand = oracle.vcmd_bias(L, [0], [1], crdg, crd, mass, lo, hi, vs, sigma, coef, T, work)
+        # lambda carries the float group inverse mass (1/12 rounded to float)
+        assert lam == pytest.approx(10.0, rel=1e-7)
+        ex = lam - edge
+        u = (crd[0] - crd[1]) / 10.0
+        assert e == pytest.approx(coef * ex * ex, rel=1e-12)
+        assert np.allclose(work[0], kT * coef * ex * u, rtol=1e-6)  # float m/M_group
+        assert np.allclose(work[1], -work[0], rtol=1e-12)
+        assert not inr
+        # from state 0 the neighbour 1 is a candidate once lambda >= its min (4); state 1 has no upper neighbour
+        assert cand == ((0, 1) if vs == 0 else (1, -1))
+    # inside the widened range: no bias, and counted as a visit only inside [min, max)
+    work = np.zeros((2, 3))
+    crd2 = np.array([[0.0, 0.0, 0.0], [5.2, 0.0, 0.0]])
+    e, lam, inr, cand = oracle.vcmd_bias(L, [0], [1], crd2.copy(), crd2, mass, lo, hi, 0, sigma, coef, T, work)
+    assert e == 0.0 and not work.any() and not inr and cand == (0, 1)   # 5.2 >= min of state 1
+    e, lam, inr, cand = oracle.vcmd_bias(L, [0], [1], crd2.copy(), crd2, mass, lo, hi, 1, sigma, coef, T, work)
+    assert inr and cand == (1, -1)                                        # 5.2 >= max of state 0
+
+
+def test_vcmd_group_unwrap_and_mass_centre():
+    """A group straddling the periodic boundary: the mass centre follows the
+    unwrapped coordinates kept between calls, so lambda is continuous."""
+    Lf = 20.0
+    L = np.array([Lf, Lf, Lf], np.float32)
+    mass = np.array([1.0, 3.0, 2.0])
+    crd = np.array([[9.5, 0.0, 0.0], [-9.5, 0.0, 0.0], [0.0, 0.0, 0.0]])   # atoms 0,1 are 1 A apart across x
+    crdg = np.array([[9.5, 0.0, 0.0], [10.5, 0.0, 0.0], [0.0, 0.0, 0.0]])  # previous unwrapped positions
+    work = np.zeros((3, 3))
+    lo, hi = np.array([0.0]), np.array([100.0])
+    e, lam, inr, cand = oracle.vcmd_bias(L, [0, 1], [2], crdg, crd, mass, lo, hi, 0, 0.0, 1.0, 300.0, work)
+    com_a = (9.5 * 1.0 + 10.5 * 3.0) / 4.0           # 10.25 -> minimum image -9.75 from atom 2
+    assert crdg[1, 0] == 10.5
+    assert lam == pytest.approx(abs(com_a - Lf), rel=1e-6)
+    assert inr and e == 0.0
