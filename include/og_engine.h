@@ -196,6 +196,36 @@ og_status og_batch_remove(og_batch *b, og_system *sys);
 og_status og_batch_run(og_batch *b, int n_steps);
 og_status og_batch_destroy(og_batch *b);
 og_status og_batch_size(og_batch *b, int *n);
+/* steps per captured hipGraph of a batch (0 = automatic) */
+og_status og_batch_set_graph_steps(og_batch *b, int steps);
+/* End the system's current segment n_steps from now: past that step every
+ * kernel skips the system, so walkers with different segment lengths share
+ * one batch.  n_steps <= 0 removes the limit. */
+og_status og_set_segment(og_system *sys, long long n_steps);
+
+/* ---- serving: continuous batching of simulation segments (no reference
+ *      counterpart: the reference starts one process per walker and segment,
+ *      samples/mcmd_ala3/cal03_mcmd_inp/job_mpi.bash:16-32) ----
+ * Requests are admitted FIFO against an HBM budget; every og_server_pump
+ * advances all admitted segments by chunk_steps in one batch and retires
+ * the finished ones; walkers stay resident on the device between segments.
+ * Times are seconds on the server's clock: times[3] = {submit, admit, done}. */
+typedef struct og_server og_server;
+og_status og_server_create(uint64_t hbm_budget_bytes, int chunk_steps, og_server **out);
+og_status og_server_destroy(og_server *srv);
+/* a new walker (the description is copied) and its first segment */
+og_status og_server_submit_new(og_server *srv, const og_system_desc *desc, const og_params *params, int n_steps,
+                               long long *request_id, long long *walker_id);
+/* the next segment of a resident walker, continuing from its device state */
+og_status og_server_submit_next(og_server *srv, long long walker_id, int n_steps, long long *request_id);
+og_status og_server_pump(og_server *srv, int *n_active, int *n_queued, int *n_done);
+/* final coordinates and velocities [n*3] (atom order) and energy rows
+ * [n_steps][OG_NTERMS] of a finished request; any pointer may be NULL */
+og_status og_server_result(og_server *srv, long long request_id, double *crd, double *vel, double *energy_rows,
+                           int max_rows, int *n_rows, double *times);
+og_status og_server_forget(og_server *srv, long long request_id);
+og_status og_server_release_walker(og_server *srv, long long walker_id);
+og_status og_server_stats(og_server *srv, double *t_now, long long *steps_done, uint64_t *bytes_used, int *n_walkers);
 
 /* diagnostics */
 const char *og_status_string(og_status s);

@@ -29,6 +29,7 @@ SOURCES = [
     ("og_integrate.hip", ["-ffp-contract=off"]),
     ("og_engine.cpp", ["-x", "hip"]),
     ("og_celeste.cpp", ["-x", "hip", "-ffp-contract=off"]),
+    ("og_server.cpp", ["-x", "hip"]),
 ]
 
 

@@ -433,7 +433,8 @@ og_status og_set_cell_constant(int n_cells, int n_atoms_exbox, int n_atom_array,
         CS_TRY(cs_alloc(p, n_bins));
     }
     og_dev_counters c{};
-    c.n_cells = n_cells;
+    c.n_cells  = n_cells;
+    c.step_end = OG_NO_STEP_END;
     CS_HIP(hipMemcpy(g_cs.d_cnt, &c, sizeof c, hipMemcpyHostToDevice));
     CS_TRY(cs_maps());
     CS_TRY(cs_push_sys());
@@ -517,7 +518,8 @@ og_status og_enumerate_cell_pairs(int **h_atomids, int n_cells, int n_neighbor_c
     }
     CS_TRY(cs_push_sys());
     og_dev_counters c{};
-    c.n_cells = n_cells;
+    c.n_cells  = n_cells;
+    c.step_end = OG_NO_STEP_END;
     CS_HIP(hipMemcpy(g_cs.d_cnt, &c, sizeof c, hipMemcpyHostToDevice));
     og_launch_grid_clear(cs_launch(g_cs.map_slot, g_cs.nb_slot), cs_launch(g_cs.map_slot, g_cs.nb_slot));
     og_launch_cells_from_grid(cs_launch(g_cs.map_col, g_cs.nb_col));

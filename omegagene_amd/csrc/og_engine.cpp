@@ -177,6 +177,7 @@ struct og_system {
     double              pair_ms    = 0.0;
     int                 pair_launches = 0;
     bool                time_pair     = false;
+    long long           step_end      = OG_NO_STEP_END;  // host mirror of og_dev_counters::step_end
     struct og_exec     *ctx = nullptr;   // private batch-of-one
 };
 
@@ -193,6 +194,7 @@ struct og_exec {
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false;
     int                      intvl = 1;
+    int                      graph_steps = 0;     // steps per captured graph (0: automatic)
 };
 
 struct og_batch {
@@ -388,9 +390,17 @@ og_status exec_run(og_exec &x, int n_steps, bool timing) {
     og_status st = exec_prepare(x);
     if (st != OG_OK) return st;
     if (x.members.empty() || n_steps <= 0) return OG_OK;
-    long long step = x.members[0]->host_step;
+    // members whose segment is complete stand still; the others must share the rebuild phase
+    const og_system *lead = nullptr;
     for (og_system *s : x.members)
-        if (s->host_step % x.intvl != step % x.intvl) {
+        if (s->host_step < s->step_end) {
+            lead = s;
+            break;
+        }
+    if (!lead) return OG_OK;
+    long long step = lead->host_step;
+    for (og_system *s : x.members)
+        if (s->host_step < s->step_end && s->host_step % x.intvl != step % x.intvl) {
             set_err("batch members are at different rebuild phases");
             return OG_E_STATE;
         }
@@ -406,7 +416,8 @@ og_status exec_run(og_exec &x, int n_steps, bool timing) {
         done++;
         step++;
     }
-    const int K = std::max(x.intvl, std::min(100 / x.intvl, 10) * x.intvl);
+    const int K = x.graph_steps > 0 ? std::max(x.intvl, x.graph_steps / x.intvl * x.intvl)
+                                    : std::max(x.intvl, std::min(100 / x.intvl, 10) * x.intvl);
     while (n_steps - done >= K) {
         hipGraphExec_t ge;
         st = get_graph(x, K, timing, &ge);
@@ -552,6 +563,11 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
          A(&h.elog, (size_t)h.log_cap * OG_E_NTERMS) && A(&s->d_cnt, 1) && A(&s->d_self, 1);
     if (!ok) return OG_E_DEVICE;
     h.cnt = s->d_cnt;
+    {
+        og_dev_counters c0{};
+        c0.step_end = OG_NO_STEP_END;
+        OG_HIP(hipMemcpy(s->d_cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
+    }
 
     // uploads
     std::vector<float> lj6f((size_t)h.n_types * h.n_types), lj12f(lj6f.size());
@@ -638,7 +654,8 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         OG_HIP(hipMemcpy(h.vc_hi, d->vc_max, sizeof(double) * ns, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.vc_qinv, qinv.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
         og_dev_counters c0{};
-        c0.vs = d->vc_init_state;
+        c0.vs       = d->vc_init_state;
+        c0.step_end = OG_NO_STEP_END;
         OG_HIP(hipMemcpy(s->d_cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
     }
     OG_HIP(hipMemcpy(s->d_self, &h, sizeof h, hipMemcpyHostToDevice));
@@ -1046,6 +1063,26 @@ og_status og_batch_run(og_batch *b, int n_steps) {
     bool timing = false;
     for (og_system *s : b->ctx.members) timing |= s->time_pair;
     return exec_run(b->ctx, n_steps, timing);
+}
+og_status og_batch_set_graph_steps(og_batch *b, int steps) {
+    if (!b || steps < 0) return OG_E_ARG;
+    if (b->ctx.graph_steps != steps) {
+        if (b->ctx.stream) OG_HIP(hipStreamSynchronize(b->ctx.stream));
+        free_graphs(b->ctx);
+        b->ctx.graph_steps = steps;
+    }
+    return OG_OK;
+}
+og_status og_set_segment(og_system *s, long long n_steps) {
+    if (!s) return OG_E_ARG;
+    og_dev_counters c;
+    if (s->ctx && s->ctx->stream) OG_HIP(hipStreamSynchronize(s->ctx->stream));
+    OG_HIP(hipDeviceSynchronize());
+    OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
+    s->host_step = c.step;
+    s->step_end  = n_steps > 0 ? c.step + n_steps : OG_NO_STEP_END;
+    OG_HIP(hipMemcpy(&s->d_cnt->step_end, &s->step_end, sizeof(long long), hipMemcpyHostToDevice));
+    return OG_OK;
 }
 og_status og_batch_size(og_batch *b, int *n) {
     if (!b || !n) return OG_E_ARG;

@@ -14,6 +14,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_clear(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         t  = bm.y * OG_THREADS + threadIdx.x;
     if (t < s.n_columns) {
         s.n_atoms_col[t] = 0;
@@ -36,6 +37,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_bin(const og_dev_sys *__res
                                                          const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (i >= s.n_atoms) return;
     int  c[2];
@@ -75,6 +77,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_bin(const og_dev_sys *__res
 // exclusive scan to head cell/atom, -1 heads for empty columns.
 __global__ void __launch_bounds__(1024) k_grid_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s  = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
     const int         nc = s.n_columns;
     const int         T  = blockDim.x;
     const int         per = (nc + T - 1) / T;
@@ -129,6 +132,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_scatter(const og_dev_sys *_
                                                              const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (i >= s.n_atoms || s.cnt->n_cells == 0) return;
     const int col  = s.col_of_atom[i];
@@ -150,6 +154,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_sort(const og_dev_sys *__re
                                                           const int2 *__restrict__ map) {
     const int2        bm  = map[blockIdx.x];
     const og_dev_sys &s   = S[bm.x];
+    if (og_frozen(s)) return;
     const int         col = bm.y;
     __shared__ uint64_t keys[OG_SORT_CAP];
     if (s.cnt->n_cells == 0) return;
@@ -245,6 +250,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_cells_from_grid(const og_dev_sys
                                                                 const int2 *__restrict__ map) {
     const int2        bm  = map[blockIdx.x];
     const og_dev_sys &s   = S[bm.x];
+    if (og_frozen(s)) return;
     const int         col = bm.y;
     if (s.head_cell[col] < 0) return;
     og_column_cells(s, col, s.n_cells_z[col]);
@@ -421,6 +427,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_count(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells) return;
     og_enum_cell<false>(s, c1, &s.cp_count[c1]);
@@ -430,6 +437,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells) return;
     og_enum_cell<true>(s, c1, nullptr);
@@ -438,6 +446,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__r
 // Exclusive scan of per-cell pair counts -> cp_head (one 1024-thread block per system).
 __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s   = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
     const int         n   = s.cnt->n_cells;
     const int         T   = blockDim.x;
     const int         per = (n + T - 1) / T;
@@ -478,6 +487,7 @@ __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict
 __global__ void __launch_bounds__(OG_THREADS) k_masks(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const int2        bm   = map[blockIdx.x];
     const og_dev_sys &s    = S[bm.x];
+    if (og_frozen(s)) return;
     const int         lane = threadIdx.x & 63;
     const int         c1   = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells || s.cnt->n_cell_pairs == 0) return;

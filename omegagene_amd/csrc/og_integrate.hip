@@ -50,6 +50,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_leapfrog_vel(const og_dev_sys *_
                                                              const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (i >= s.n_atoms || s.integrator != 0) return;
     const double minv = (double)s.mass_inv[i];
@@ -65,6 +66,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_leapfrog_vel(const og_dev_sys *_
 // one block per system sums the momentum of each group in a fixed tree order.
 __global__ void __launch_bounds__(OG_THREADS) k_com_sum(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
     __shared__ double sh[3 * OG_THREADS];
     if (!s.com_cancel || s.integrator != 0) return;
     for (int g = 0; g < s.n_com_groups; g++) {
@@ -91,6 +93,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_com_apply(const og_dev_sys *__re
                                                           const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         t  = bm.y * OG_THREADS + threadIdx.x;
     if (!s.com_cancel || s.integrator != 0 || s.n_com_groups == 0 || t >= s.com_off[s.n_com_groups]) return;
     int g = 0;
@@ -105,6 +108,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_positions(const og_dev_sys *__re
                                                           const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (i >= s.n_atoms || s.integrator != 0) return;
     for (int d = 0; d < 3; d++) {
@@ -124,6 +128,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_langevin_positions(const og_dev_
                                                                    const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (i >= s.n_atoms || s.integrator != 1) return;
     const long long step = s.cnt->step;
@@ -167,6 +172,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_shake_pair(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         t  = bm.y * OG_THREADS + threadIdx.x;
     if (s.constraint == 0 || t >= s.n_pairs_c) return;
     const int a = s.pairc_ids[2 * t], b = s.pairc_ids[2 * t + 1];
@@ -314,6 +320,7 @@ __global__ void __launch_bounds__(OG_TRIO_THREADS) k_trio(const og_dev_sys *__re
                                                           const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         t  = bm.y * OG_TRIO_THREADS + threadIdx.x;
     if (s.constraint == 0) return;
     if (t < s.n_trios) og_shake_trio(s, s.trio_ids[3 * t], s.trio_ids[3 * t + 1], s.trio_ids[3 * t + 2],
@@ -333,6 +340,7 @@ __global__ void __launch_bounds__(OG_TRIO_THREADS) k_trio(const og_dev_sys *__re
 __global__ void __launch_bounds__(OG_THREADS) k_finish(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     double            ke = 0.0;
     if (i < s.n_atoms) {
@@ -399,6 +407,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_finish(const og_dev_sys *__restr
 // Kinetic energy, energy log entry, step counter (one thread block per system).
 __global__ void __launch_bounds__(OG_THREADS) k_step_end(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
     __shared__ double sh[OG_THREADS];
     double            t = 0.0;
     for (int k = threadIdx.x; k < s.n_atom_blocks; k += blockDim.x) t += s.e_atom[k];
@@ -437,6 +446,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_step_end(const og_dev_sys *__res
 //    proportional to 1/q_cano (uniform from Philox keyed by seed and step).
 __global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
     if (s.vc_n_states == 0) return;
     const int lane = threadIdx.x;
     const int n    = s.vc_na + s.vc_nb;

@@ -14,6 +14,7 @@
 #define OG_EMPTY_BIN 0x7fffffff  // uniform-z bin with no cell
 #define OG_FX_SCALE 4294967296.0 // fixed-point force scale, 2^32 units per kcal/mol/A
 #define OG_FX_INV (1.0 / 4294967296.0)
+#define OG_NO_STEP_END 0x7fffffffffffffffLL  // og_dev_counters::step_end of a system with no segment limit
 
 // energy terms kept per system and per step
 enum {
@@ -46,9 +47,10 @@ struct og_dev_counters {
     int       n_cell_pairs;
     int       flags;
     int       shake_fail;
-    long long step;   // steps completed since system creation
-    int       vs;     // current VcMD virtual state
+    long long step;      // steps completed since system creation
+    int       vs;        // current VcMD virtual state
     int       pad;
+    long long step_end;  // the system stands still once step reaches this (segment end)
 };
 
 struct og_dev_sys {
@@ -164,6 +166,11 @@ enum og_map_kind {
 #define OG_CP_BLOCKS_PER_SYS_MIN 64
 
 // ---- small device helpers ----
+// A system whose segment is complete skips every kernel of the step (each
+// kernel reads the counters before its first write; step only changes in
+// the last kernel of a step, so all kernels of one step agree).
+__device__ __forceinline__ bool og_frozen(const og_dev_sys &s) { return s.cnt->step >= s.cnt->step_end; }
+
 __device__ __forceinline__ int og_lane() { return __lane_id(); }
 
 __device__ __forceinline__ float og_dpp_sum8(float x) {

@@ -44,6 +44,7 @@ template <int ALPHA>
 __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
     const int lane = threadIdx.x & 63;
@@ -215,6 +216,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_misc(const og_dev_sys *__restric
 #pragma clang fp contract(off)
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         t  = bm.y * OG_THREADS + threadIdx.x;
     const int         cap = s.cap_atom_array;
     double            e_ex = 0.0, e_b = 0.0;
@@ -291,6 +293,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
                                                                const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (i >= s.n_atoms) return;
     const int g   = s.rev[i];
@@ -306,6 +309,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
 __global__ void __launch_bounds__(OG_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
                                                               const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
     const int         n = s.cnt->n_cells;
     double            a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = 0.0, f = 0.0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {

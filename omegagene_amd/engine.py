@@ -341,3 +341,73 @@ def exports(header: str = None):
         hdr = open(p).read()
         names |= set(re.findall(r"^\s*(?:og_status|void|const char \*)\s*(og_\w+)\s*\(", hdr, re.M))
     return sorted(names)
+
+
+class Server:
+    """Continuous-batching segment service (og_server_*): walkers stay on the
+    device; requests are admitted against an HBM budget and advanced
+    together, chunk_steps per pump."""
+
+    def __init__(self, hbm_budget_bytes: int, chunk_steps: int = 10, device: int = 0):
+        _check(lib().og_set_device(C.c_int(device)))
+        self.h = C.c_void_p()
+        _check(lib().og_server_create(C.c_uint64(int(hbm_budget_bytes)), C.c_int(chunk_steps), C.byref(self.h)))
+        self._n_atoms = {}   # walker -> atoms
+        self._req_walker = {}
+        self._req_steps = {}
+
+    def submit_new(self, ls: LaunchSet, cfg: dict, n_steps: int):
+        p = params_from_cfg(cfg)
+        vcmd = cfg.get("extended-ensemble", "none") == "vcmd" and ls.vcmd is not None
+        d, keep = desc_from_launchset(ls, p.constraint, vcmd=vcmd)
+        rid, wid = C.c_longlong(), C.c_longlong()
+        _check(lib().og_server_submit_new(self.h, C.byref(d), C.byref(p), C.c_int(n_steps), C.byref(rid),
+                                          C.byref(wid)))
+        self._n_atoms[wid.value] = ls.n_atoms
+        self._req_walker[rid.value] = wid.value
+        self._req_steps[rid.value] = n_steps
+        return rid.value, wid.value
+
+    def submit_next(self, walker: int, n_steps: int):
+        rid = C.c_longlong()
+        _check(lib().og_server_submit_next(self.h, C.c_longlong(walker), C.c_int(n_steps), C.byref(rid)))
+        self._req_walker[rid.value] = walker
+        self._req_steps[rid.value] = n_steps
+        return rid.value
+
+    def pump(self):
+        a, q, d = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().og_server_pump(self.h, C.byref(a), C.byref(q), C.byref(d)))
+        return a.value, q.value, d.value
+
+    def result(self, rid: int):
+        n = self._n_atoms[self._req_walker[rid]]
+        steps = self._req_steps[rid]
+        crd = np.zeros((n, 3))
+        vel = np.zeros((n, 3))
+        rows = np.zeros((steps, NTERMS))
+        nr = C.c_int()
+        t = np.zeros(3)
+        _check(lib().og_server_result(self.h, C.c_longlong(rid), crd.ctypes.data_as(C.c_void_p),
+                                      vel.ctypes.data_as(C.c_void_p), rows.ctypes.data_as(C.c_void_p),
+                                      C.c_int(steps), C.byref(nr), t.ctypes.data_as(C.c_void_p)))
+        return crd, vel, rows[:nr.value], t
+
+    def release_walker(self, walker: int):
+        _check(lib().og_server_release_walker(self.h, C.c_longlong(walker)))
+
+    def stats(self):
+        t, s, b, w = C.c_double(), C.c_longlong(), C.c_uint64(), C.c_int()
+        _check(lib().og_server_stats(self.h, C.byref(t), C.byref(s), C.byref(b), C.byref(w)))
+        return dict(t=t.value, steps_done=s.value, bytes_used=b.value, n_walkers=w.value)
+
+    def close(self):
+        if self.h:
+            lib().og_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -62,3 +62,19 @@ def oracle_forces(ls: LaunchSet, cutoff=12.0, alpha=0.0, nscut=13.0, excess_bug=
     eb = oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, crd, work) if len(ls.bond_ids) else 0.0
     eself, _ = oracle.self_energy(zd, q, ex)
     return work, dict(vdw=en[0], ele=en[1], excess=eex, bond=eb, self=eself, n_pairs=npair), (o, c, cps, head)
+
+
+def ele_scale(ls: LaunchSet, cutoff=12.0, alpha=0.0, nscut=13.0, crd=None):
+    """Sum over in-cutoff pairs of |q_i q_j| * 332.06378 * k(r), with
+    k(r) = erfc(alpha r)/r - zcore + bcoeff r^2 >= 0 on (0, rc]: the zero-dipole
+    pair energy with every charge replaced by its magnitude.  The net ele
+    energy is a near-cancelling sum of such terms, so its FP32 rounding error
+    scales with this sum, not with the net value."""
+    crd = ls.crd if crd is None else crd
+    zd = oracle.zd_setup(cutoff, alpha)
+    o = oracle.Oracle(ls.n_atoms, ls.box, nscut)
+    o.build(crd)
+    cps, _ = o.enumerate(ls.nb15off_table(), gap_bug=True)
+    lj6, lj12 = ls.lj_tables_f32()
+    _, en, _ = o.pairs_fp32(zd, cps, ls.atom_type, lj6, lj12, np.abs(ls.charge).astype(np.float32))
+    return float(en[1])

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import cls_from_fixture, load_npz, oracle_forces, rms_rel
+from helpers import cls_from_fixture, ele_scale, load_npz, oracle_forces, rms_rel
 from omegagene_amd import gen
 from omegagene_amd.cls import LaunchSet
 from omegagene_amd.engine import EngineError, System, Batch
@@ -71,13 +71,6 @@ def test_forces_energies_vs_oracle(case):
     wo, eo, (o, c, cps, head) = oracle_forces(ls, cutoff=cfg["cutoff"], alpha=cfg["ele-alpha"],
                                               nscut=cfg["nsgrid-cutoff"])
     err32 = rms_rel(w, wo)
-    say("  rms rel vs fp32 oracle %.3e" % err32)
-    assert err32 < 2e-6
-    for k in ("vdw", "ele"):
-        assert e[k] == pytest.approx(eo[k], rel=2e-6, abs=1e-4), k
-    assert e["excess"] == pytest.approx(eo["excess"], rel=1e-9, abs=1e-9)
-    assert e["bond"] == pytest.approx(eo["bond"], rel=1e-9, abs=1e-12)
-    assert e["self"] == eo["self"]
     # FP64 reference of the same physics
     zd = oracle.zd_setup(cfg["cutoff"], cfg["ele-alpha"])
     lj6, lj12 = ls.lj_tables_f32()
@@ -87,9 +80,23 @@ def test_forces_energies_vs_oracle(case):
     if len(ls.bond_ids):
         oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, ls.crd, w64)
     err64 = rms_rel(w, w64)
-    say("  rms rel vs fp64 oracle %.3e; vdw %.6e/%.6e ele %.6e/%.6e" % (err64, e["vdw"], e64[0], e["ele"], e64[1]))
+    S = ele_scale(ls, cfg["cutoff"], cfg["ele-alpha"], cfg["nsgrid-cutoff"])
+    say("  gradient rms rel: vs fp32 oracle %.3e, vs fp64 %.3e (fp32 oracle vs fp64 %.3e)" %
+        (err32, err64, rms_rel(wo, w64)))
+    say("  vdw  gpu %.6f  fp32 oracle %.6f  fp64 %.6f" % (e["vdw"], eo["vdw"], e64[0]))
+    say("  ele  gpu %.6f  fp32 oracle %.6f  fp64 %.6f  scale sum|q q k| %.4e" % (e["ele"], eo["ele"], e64[1], S))
+    # gradients: <= 2e-6 vs the FP32-faithful oracle, <= 1e-5 vs FP64
+    assert err32 < 2e-6
     assert err64 < 1e-5
-    assert e["vdw"] + e["ele"] == pytest.approx(e64[0] + e64[1], rel=1e-5)
+    # vdw is a sum of mostly same-sign terms: relative 2e-6
+    assert e["vdw"] == pytest.approx(eo["vdw"], rel=2e-6, abs=1e-4)
+    assert e["vdw"] == pytest.approx(e64[0], rel=2e-6, abs=1e-4)
+    # ele: within 1e-8 of the sum of term magnitudes, against both oracles
+    assert abs(e["ele"] - eo["ele"]) <= 1e-8 * S
+    assert abs(e["ele"] - e64[1]) <= 1e-8 * S
+    assert e["excess"] == pytest.approx(eo["excess"], rel=1e-9, abs=1e-9)
+    assert e["bond"] == pytest.approx(eo["bond"], rel=1e-9, abs=1e-12)
+    assert e["self"] == eo["self"]
     # Newton's third law on the fixed-point sums
     assert np.max(np.abs(w.sum(0))) < 1e-5 * np.abs(w).max()
     s.close()
@@ -127,8 +134,12 @@ def test_reference_trajectory_20_steps():
     ref = d["log"]
     assert len(log) == 20
     pot = log[:, 0] + log[:, 1] + log[:, 2] + log[:, 3] + log[:, 4]
+    # FP32 pair sums: vdw to relative 2e-6, ele to 1e-8 of the sum of its term magnitudes
+    S = ele_scale(ls)
     for k in range(20):
-        assert pot[k] == pytest.approx(ref[k, 1], rel=1e-6, abs=2e-3), k
+        tol = 2e-6 * abs(ref[k, 4]) + 1e-8 * S
+        say("  step %d pot %.6f ref %.6f (tol %.2e)" % (k, pot[k], ref[k, 1], tol)) if k in (0, 19) else None
+        assert abs(pot[k] - ref[k, 1]) <= tol, k
         assert log[k, 5] == pytest.approx(ref[k, 2], rel=1e-6), k
     crd, vn, _ = s.state()
     say("  max |dx| vs reference %.3e" % np.max(np.abs(crd - d["crd"])))
@@ -188,8 +199,8 @@ def test_nve_energy_conservation_and_settle():
     assert abs(drift) < 0.05
     crd, _, _ = s.state()
     w = ls.shake_trio_ids
-    r = np.linalg.norm(crd[w[:, 0]] - crd[w[:, 1]] - ls.box[0] * np.round((crd[w[:, 0]] - crd[w[:, 1]]) / ls.box[0]),
-                       axis=1)
+    Lf = float(np.float32(ls.box[0]))  # the engine's box edge is float (PBC.h:21-27); atoms wrap by it
+    r = np.linalg.norm(crd[w[:, 0]] - crd[w[:, 1]] - Lf * np.round((crd[w[:, 0]] - crd[w[:, 1]]) / Lf), axis=1)
     assert np.max(np.abs(r - gen.D_OH)) < 1e-6
     s.close()
 
