@@ -63,13 +63,144 @@ def cpu_baseline(ls, cfg, budget_s=12.0):
     return pairs / dt, n_eval, dt
 
 
+WALKER_CFG = {"cutoff": 12.0, "nsgrid-cutoff": 13.0, "nsgrid-update-intvl": 10, "time-step": 2.0,
+              "integrator": "langevin", "constraint": "shake-settle", "const-tolerance": 1e-8,
+              "langevin-gamma": 1.0, "temperature": 300.0, "ele-alpha": 0.0, "extended-ensemble": "vcmd",
+              "enhance-sigma": 0.5, "enhance-recovery-coef": 1.0}
+
+
+def _settle_waters(ls):
+    import numpy as np
+    ls.settle_ids, ls.settle_d = ls.shake_trio_ids, ls.shake_trio_d
+    ls.shake_trio_ids = np.zeros((0, 3), np.int32)
+    ls.shake_trio_d = np.zeros((0, 3))
+    return ls
+
+
+def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
+    """64 independent water + 200-bead-chain walkers (3,000-30,000 atoms,
+    log-uniform), Langevin 300 K, SETTLE, VcMD bias on the chain end-to-end
+    distance (8 windows).  Batched: one og_batch, one 500-step hipGraph per
+    launch.  Baseline: the same walkers one at a time through the
+    single-system path (og_md_run), seq_steps each."""
+    from omegagene_amd import gen
+    from omegagene_amd.engine import Batch, System, lib
+    sizes = gen.walker_sizes(n_walkers)
+    walkers = [_settle_waters(gen.walker_system(int(n), seed=5000 + i)) for i, n in enumerate(sizes)]
+    cfgs = [dict(WALKER_CFG, **{"random-seed": 100 + i, "log-cap": steps + warmup}) for i in range(n_walkers)]
+    n_atoms = sum(w.n_atoms for w in walkers)
+    log("walkers: %d systems, %d atoms in total (%d-%d)" % (n_walkers, n_atoms, min(sizes), max(sizes)))
+    # one at a time (the plain drop-in path)
+    seq_time, seq_steps_done = 0.0, 0
+    for w, c in zip(walkers, cfgs):
+        s = System(w, c)
+        s.run(10)
+        lib().og_thread_sync()
+        t0 = time.perf_counter()
+        s.run(seq_steps)
+        lib().og_thread_sync()
+        seq_time += time.perf_counter() - t0
+        seq_steps_done += seq_steps
+        s.close()
+    seq_rate = seq_steps_done / seq_time
+    log("walkers one at a time: %.1f walker-steps/s" % seq_rate)
+    # batched
+    systems = [System(w, c) for w, c in zip(walkers, cfgs)]
+    b = Batch()
+    for s in systems:
+        b.add(s)
+    lib().og_batch_set_graph_steps(b.h, steps)
+    b.run(warmup)
+    lib().og_thread_sync()
+    t0 = time.perf_counter()
+    b.run(steps)
+    lib().og_thread_sync()
+    wall = time.perf_counter() - t0
+    batch_rate = n_walkers * steps / wall
+    log("walkers batched: %.1f walker-steps/s (%.2f s for %d steps)" % (batch_rate, wall, steps))
+    b.close()
+    for s in systems:
+        s.close()
+    return {"workload": "64 walkers, water + 200-bead chain, 3,000-30,000 atoms log-uniform (%d atoms in total), "
+                        "Langevin 300 K, SETTLE, VcMD 8 windows, rc 12 A" % n_atoms,
+            "batched_walker_steps_per_s": batch_rate, "one_at_a_time_walker_steps_per_s": seq_rate,
+            "speedup": batch_rate / seq_rate, "atom_steps_per_s": n_atoms * steps / wall,
+            "graph_steps": steps, "timed_steps": steps}
+
+
+def bench_serving(n_requests=48, rate=200.0, budget_gb=200.0, chunk=20, seed=7):
+    """Poisson stream (mean 200 requests/s) of segments: each a new walker of
+    2,000-50,000 atoms (log-uniform, water + 200-bead chain), 200-2,000 steps,
+    280-360 K, admitted against a 200 GB HBM budget.  Latency = completion
+    time - arrival time."""
+    import numpy as np
+    from omegagene_amd import gen
+    from omegagene_amd.engine import Server
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    sizes = np.exp(rng.uniform(np.log(2000), np.log(50000), n_requests)).astype(int)
+    nsteps = rng.integers(200, 2001, n_requests)
+    temps = rng.uniform(280.0, 360.0, n_requests)
+    systems = [_settle_waters(gen.walker_system(int(n), seed=7000 + i, vcmd=False)) for i, n in enumerate(sizes)]
+    cfg = {k: v for k, v in WALKER_CFG.items() if k not in ("extended-ensemble", "enhance-sigma", "enhance-recovery-coef")}
+    srv = Server(int(budget_gb * (1 << 30)), chunk_steps=chunk)
+    t_start = srv.stats()["t"]
+    reqs = {}
+    nxt = 0
+    while True:
+        now = srv.stats()["t"] - t_start
+        while nxt < n_requests and arrivals[nxt] <= now:
+            c = dict(cfg, **{"temperature": float(temps[nxt]), "random-seed": 900 + nxt})
+            rid, _ = srv.submit_new(systems[nxt], c, int(nsteps[nxt]))
+            reqs[rid] = nxt
+            nxt += 1
+        active, queued, _ = srv.pump()
+        if nxt == n_requests and active == 0 and queued == 0:
+            break
+        if active == 0 and queued == 0:
+            time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)))
+    lat = []
+    for rid, k in reqs.items():
+        _, _, _, t = srv.result(rid)
+        lat.append(t[2] - t_start - arrivals[k])
+    st = srv.stats()
+    wall = st["t"] - t_start
+    srv.close()
+    lat = np.array(lat)
+    atom_steps = float(np.sum(sizes * nsteps))
+    return {"workload": "%d requests, Poisson %.0f req/s, 2,000-50,000 atoms, 200-2,000 steps, 280-360 K, "
+                        "200 GB budget, %d-step chunks" % (n_requests, rate, chunk),
+            "p50_latency_ms": float(np.percentile(lat, 50) * 1e3), "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+            "aggregate_steps_per_s": float(nsteps.sum() / wall), "atom_steps_per_s": atom_steps / wall,
+            "wall_s": wall}
+
+
+def copy_calibration(mib: int):
+    """A plain device copy of known size (mib MiB read + mib MiB written), run
+    under the same rocprofv3 counters as the engine to scale FETCH_SIZE /
+    WRITE_SIZE into bytes."""
+    import torch
+    x = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_mib": mib, "bytes_read_per_copy": mib << 20, "bytes_written_per_copy": mib << 20}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-walkers", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs each system (or walker batch) on a single "
               "MI355X; --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -120,6 +251,9 @@ def main():
                "sample": "%d single-point force evaluations (grid + cell-pair list + FP32 LJ/ZD pairs + "
                          "excluded pairs) of the same 99,200-atom system in %.1f s, one thread" % (n_eval, dt)}
 
+    walkers = None if args.no_walkers else bench_walkers()
+    serving = None if args.no_serving else bench_serving()
+
     out = {
         "metric": "pair interactions/s (99,200 atoms, 12 A, ZD alpha 0, Langevin, dt 2 fs)",
         "value": value,
@@ -143,6 +277,8 @@ def main():
                      "kernel": "k_pair<%d>" % alpha_mode, "kernel_ms": t_launch * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR[alpha_mode]},
         "cpu_baseline": cpu,
+        "walkers": walkers,
+        "serving": serving,
     }
     s.close()
     print(json.dumps(out))

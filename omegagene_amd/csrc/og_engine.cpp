@@ -330,25 +330,18 @@ void enqueue_integrate(og_exec &x) {
 
 // Capture K steps (rebuild at steps whose index within the chunk is a
 // multiple of intvl; chunks start aligned) into one graph.
-og_status get_graph(og_exec &x, int K, bool timing, hipGraphExec_t *out) {
-    const long long key = (long long)K * 2 + (timing ? 1 : 0);
+og_status get_graph(og_exec &x, int K, hipGraphExec_t *out) {
+    const long long key = K;
     auto            it  = x.graphs.find(key);
     if (it != x.graphs.end()) {
         *out = it->second;
         return OG_OK;
     }
-    if (timing && (int)x.ev.size() < 2 * K) {
-        for (int k = (int)x.ev.size(); k < 2 * K; k++) {
-            hipEvent_t e;
-            OG_HIP(hipEventCreate(&e));
-            x.ev.push_back(e);
-        }
-    }
     hipGraph_t g;
     OG_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < K; k++) {
         if (k % x.intvl == 0) enqueue_rebuild(x);
-        enqueue_forces(x, timing ? x.ev[2 * k] : nullptr, timing ? x.ev[2 * k + 1] : nullptr);
+        enqueue_forces(x, nullptr, nullptr);
         enqueue_integrate(x);
     }
     OG_HIP(hipStreamEndCapture(x.stream, &g));
@@ -416,26 +409,42 @@ og_status exec_run(og_exec &x, int n_steps, bool timing) {
         done++;
         step++;
     }
+    if (timing) {
+        // Pair-kernel timing: HIP events recorded on the run's stream around
+        // every pair launch.  Events recorded inside a captured graph cannot
+        // be timed, so a timed run launches its steps directly.
+        const int n = n_steps - done;
+        for (int k = (int)x.ev.size(); k < 2 * n; k++) {
+            hipEvent_t e;
+            OG_HIP(hipEventCreate(&e));
+            x.ev.push_back(e);
+        }
+        for (int k = 0; k < n; k++) {
+            if (step % x.intvl == 0) enqueue_rebuild(x);
+            enqueue_forces(x, x.ev[2 * k], x.ev[2 * k + 1]);
+            enqueue_integrate(x);
+            step++;
+        }
+        done = n_steps;
+        OG_HIP(hipStreamSynchronize(x.stream));
+        double ms = 0.0;
+        for (int k = 0; k < n; k++) {
+            float t = 0.0f;
+            OG_HIP(hipEventElapsedTime(&t, x.ev[2 * k], x.ev[2 * k + 1]));
+            ms += t;
+        }
+        for (og_system *s : x.members) {
+            s->pair_ms += ms;
+            s->pair_launches += n;
+        }
+    }
     const int K = x.graph_steps > 0 ? std::max(x.intvl, x.graph_steps / x.intvl * x.intvl)
                                     : std::max(x.intvl, std::min(100 / x.intvl, 10) * x.intvl);
     while (n_steps - done >= K) {
         hipGraphExec_t ge;
-        st = get_graph(x, K, timing, &ge);
+        st = get_graph(x, K, &ge);
         if (st != OG_OK) return st;
         OG_HIP(hipGraphLaunch(ge, x.stream));
-        if (timing) {
-            OG_HIP(hipStreamSynchronize(x.stream));
-            double ms = 0.0;
-            for (int k = 0; k < K; k++) {
-                float t = 0.0f;
-                OG_HIP(hipEventElapsedTime(&t, x.ev[2 * k], x.ev[2 * k + 1]));
-                ms += t;
-            }
-            for (og_system *s : x.members) {
-                s->pair_ms += ms;
-                s->pair_launches += K;
-            }
-        }
         done += K;
         step += K;
     }

@@ -230,9 +230,17 @@ int64_t orc_pairs_celllist_fp64(const orc_zd_t *zd, const orc_grid_t *g, const o
                 if (((uint32_t)cps[cp].pair_mask[bit / 32] >> (bit % 32)) & 1u) continue;
                 int    id1 = c->atomids[c1 * NAC + i1];
                 double d[3];
+                /* the image is the cell pair's (a small box lists one cell pair
+                 * under several images): take the grid-coordinate separation
+                 * with that image and carry the FP64 atom separation to the
+                 * same periodic copy */
+                const int img = cps[cp].image;
                 for (int k = 0; k < 3; k++) {
-                    d[k] = crd_atom[id1 * 3 + k] - crd_atom[id2 * 3 + k];
-                    d[k] -= (double)g->L[k] * nearbyint(d[k] / (double)g->L[k]);
+                    const int    lo = 1 << (2 * k), hi = 2 << (2 * k);
+                    const double sh = (img & lo) ? -(double)g->L[k] : (img & hi) ? (double)g->L[k] : 0.0;
+                    const double dg = c->crd[(c1 * NAC + i1) * 3 + k] - (c->crd[(c2 * NAC + i2) * 3 + k] + sh);
+                    d[k]            = crd_atom[id1 * 3 + k] - crd_atom[id2 * 3 + k];
+                    d[k] -= (double)g->L[k] * nearbyint((d[k] - dg) / (double)g->L[k]);
                 }
                 double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 double r  = sqrt(r2);

@@ -316,3 +316,44 @@ def test_vcmd_group_unwrap_and_mass_centre():
     assert crdg[1, 0] == 10.5
     assert lam == pytest.approx(abs(com_a - Lf), rel=1e-6)
     assert inr and e == 0.0
+
+
+def test_fp64_cell_list_equals_brute_force():
+    """The FP64 error reference (orc_pairs_celllist_fp64) against a numpy
+    all-pairs minimum-image sum on a small box where one cell pair is listed
+    under several z images (so an image taken from the coordinates instead
+    of the cell pair would double-count)."""
+    ls = gen.build_system(400, seed=12, L=24.5)
+    zd = oracle.zd_setup(12.0, 0.0)
+    o = oracle.Oracle(ls.n_atoms, ls.box, 13.0)
+    c = o.build(ls.crd)
+    cps, _ = o.enumerate(ls.nb15off_table(), gap_bug=False)
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    w64, e64, n64 = o.pairs_fp64(zd, cps, ls.crd, ls.atom_type, lj6, lj12, q)
+    # brute force, FP64, every unexcluded pair within the cutoff once
+    L = float(np.float32(ls.box[0]))
+    x = ls.crd
+    n = ls.n_atoms
+    tab = ls.nb15off_table()
+    excl = np.zeros((n, n), bool)
+    for i in range(n):
+        excl[i, tab[i][tab[i] >= 0]] = True
+    d = x[:, None, :] - x[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d * d).sum(-1)
+    iu = np.triu_indices(n, 1)
+    ok = (r2[iu] < 144.0) & ~excl[iu]
+    i, j = iu[0][ok], iu[1][ok]
+    r = np.sqrt(r2[i, j])
+    t = ls.atom_type[i] * ls.n_lj_types + ls.atom_type[j]
+    a6, a12 = lj6.reshape(-1)[t].astype(np.float64), lj12.reshape(-1)[t].astype(np.float64)
+    cc = q[i].astype(np.float64) * q[j].astype(np.float64) * 332.06378
+    ri = 1.0 / r
+    g = ri ** 2 * (-12 * a12 * ri ** 12 + 6 * a6 * ri ** 6) - cc * (ri ** 3 - 1.0 / 12.0 ** 3)
+    wb = np.zeros((n, 3))
+    np.add.at(wb, i, g[:, None] * d[i, j])
+    np.add.at(wb, j, -g[:, None] * d[i, j])
+    assert n64 == len(r)
+    assert rms_rel(w64, wb) < 1e-12
+    assert e64[0] == pytest.approx(np.sum(a12 * ri ** 12 - a6 * ri ** 6), rel=1e-12)
