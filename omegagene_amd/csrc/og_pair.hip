@@ -26,18 +26,25 @@ __device__ __forceinline__ float og_erfc_as(float x, float expmx2) {
     return p * t * expmx2;
 }
 
-// Adds one tile's j-gradient (already summed over the 8 i-lanes) for the
-// 8 lanes with a1 == 0.  The fixed-point conversion is branch free; the
-// atomics are skipped when the whole column of the tile contributed nothing.
-__device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, int a2, float rx, float ry, float rz,
-                                           int *flag) {
-    const long long vx = og_fx_fast(-(double)rx), vy = og_fx_fast(-(double)ry), vz = og_fx_fast(-(double)rz);
-    if ((vx | vy | vz) != 0) {
-        og_atomic_add_fx_g(FX + a2, vx);
-        og_atomic_add_fx_g(FX + cap + a2, vy);
-        og_atomic_add_fx_g(FX + 2 * cap + a2, vz);
-    }
-    if (fabsf(rx) + fabsf(ry) + fabsf(rz) > (float)OG_FX_LIMIT) atomicOr(flag, OG_F_NAN);
+// Adds one tile's j-gradient.  After the 8-lane butterfly every lane of a
+// j row holds the row's (x, y, z) sums bit for bit, so lane k < 3 of the row
+// converts and adds component k: one conversion and one atomic instruction
+// serve all three components of the 8 j atoms (24 active lanes).
+__device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, int a2, int i, float rx, float ry,
+                                           float rz, int *flag) {
+    const float     v  = i == 0 ? rx : (i == 1 ? ry : rz);
+    const long long fv = og_fx_fast(-(double)v);
+    if (i < 3 && fv != 0) og_atomic_add_fx_g(FX + i * cap + a2, fv);
+    if (i < 3 && !(fabsf(v) <= (float)OG_FX_LIMIT)) atomicOr(flag, OG_F_NAN);
+}
+
+// Periodic shift of a tile's j cell (image bits of MiniCell::add_cell_pair,
+// MiniCell.cpp:1174-1187: neg bit -> -L, pos bit -> +L) as integer selects on
+// wave-uniform values, so it stays on the scalar unit.
+__device__ __forceinline__ float og_img_shift(unsigned img, unsigned neg, unsigned pos, float L) {
+    const int Lb = __builtin_amdgcn_readfirstlane(__float_as_int(L));
+    const int b  = (img & neg) ? (Lb ^ (int)0x80000000) : ((img & pos) ? Lb : 0);
+    return __int_as_float(__builtin_amdgcn_readfirstlane(b));
 }
 
 template <int ALPHA>
@@ -99,10 +106,11 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             // readlane returns int: widen through uint32_t so the low word is not sign-extended
             const uint64_t mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, k) << 32) |
                                   (uint64_t)(uint32_t)__builtin_amdgcn_readlane(mlo, k);
+            // image shift of the tile: wave-uniform, selected on the scalar unit
             const unsigned img = c2i >> 24;
-            const float    sx  = (img & 1u) ? -L0 : ((img & 2u) ? L0 : 0.0f);
-            const float    sy  = (img & 4u) ? -L1 : ((img & 8u) ? L1 : 0.0f);
-            const float    sz  = (img & 16u) ? -L2 : ((img & 32u) ? L2 : 0.0f);
+            const float    sx  = og_img_shift(img, 1u, 2u, L0);
+            const float    sy  = og_img_shift(img, 4u, 8u, L1);
+            const float    sz  = og_img_shift(img, 16u, 32u, L2);
             const float    dx  = pi.x - (pj.x + sx);
             const float    dy  = pi.y - (pj.y + sy);
             const float    dz  = pi.z - (pj.z + sz);
@@ -147,7 +155,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             const float rjx = og_dpp_sum8(wx);
             const float rjy = og_dpp_sum8(wy);
             const float rjz = og_dpp_sum8(wz);
-            if (i == 0) og_flush_j(FX, cap, a2, rjx, rjy, rjz, &s.cnt->flags);
+            og_flush_j(FX, cap, a2, i, rjx, rjy, rjz, &s.cnt->flags);
         }
         dfx += (double)fx;
         dfy += (double)fy;

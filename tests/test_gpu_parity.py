@@ -94,7 +94,9 @@ def test_forces_energies_vs_oracle(case):
     # ele: within 1e-8 of the sum of term magnitudes, against both oracles
     assert abs(e["ele"] - eo["ele"]) <= 1e-8 * S
     assert abs(e["ele"] - e64[1]) <= 1e-8 * S
-    assert e["excess"] == pytest.approx(eo["excess"], rel=1e-9, abs=1e-9)
+    # excluded pairs: FP32 arithmetic as the reference; at alpha > 0 device erfcf and
+    # host erfc (float) differ by an ulp per pair
+    assert e["excess"] == pytest.approx(eo["excess"], rel=1e-9 if cfg["ele-alpha"] == 0 else 2e-6, abs=1e-9)
     assert e["bond"] == pytest.approx(eo["bond"], rel=1e-9, abs=1e-12)
     assert e["self"] == eo["self"]
     # Newton's third law on the fixed-point sums
