@@ -77,6 +77,32 @@ def _settle_waters(ls):
     return ls
 
 
+def equilibrate(systems_cfgs, steps=300, gamma=20.0):
+    """Untimed relaxation of freshly generated systems: the seeded lattice
+    waters start far from equilibrium and would heat to ~1,200 K.  Runs
+    position Langevin with a strong friction (gamma ps^-1) on all systems
+    in one batch and returns launch sets carrying the relaxed coordinates
+    and velocities."""
+    import copy
+    from omegagene_amd.engine import Batch, System
+    systems = [System(ls, dict(c, **{"langevin-gamma": gamma, "extended-ensemble": "none"}))
+               for ls, c in systems_cfgs]
+    b = Batch()
+    for x in systems:
+        b.add(x)
+    b.run(steps)
+    out = []
+    for (ls, _), x in zip(systems_cfgs, systems):
+        crd, _, vj = x.state()
+        ls2 = copy.copy(ls)
+        ls2.crd, ls2.vel = crd, vj
+        out.append(ls2)
+    b.close()
+    for x in systems:
+        x.close()
+    return out
+
+
 def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     """64 independent water + 200-bead-chain walkers (3,000-30,000 atoms,
     log-uniform), Langevin 300 K, SETTLE, VcMD bias on the chain end-to-end
@@ -88,6 +114,7 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     sizes = gen.walker_sizes(n_walkers)
     walkers = [_settle_waters(gen.walker_system(int(n), seed=5000 + i)) for i, n in enumerate(sizes)]
     cfgs = [dict(WALKER_CFG, **{"random-seed": 100 + i, "log-cap": steps + warmup}) for i in range(n_walkers)]
+    walkers = equilibrate(list(zip(walkers, cfgs)))
     n_atoms = sum(w.n_atoms for w in walkers)
     log("walkers: %d systems, %d atoms in total (%d-%d)" % (n_walkers, n_atoms, min(sizes), max(sizes)))
     # one at a time (the plain drop-in path)
@@ -219,6 +246,8 @@ def main():
     cfg["log-cap"] = max(args.steps, 1)
     log("system: %d atoms, box %.3f A, cutoff %.1f / list %.1f, rebuild every %d steps" %
         (ls.n_atoms, ls.box[0], cfg["cutoff"], cfg["nsgrid-cutoff"], cfg["nsgrid-update-intvl"]))
+    ls = equilibrate([(ls, cfg)])[0]
+    log("equilibrated 300 steps at gamma 20/ps (untimed)")
     s = System(ls, cfg)
     alpha_mode = 1 if cfg["ele-alpha"] > 0 else 0
     s.run(args.warmup)

@@ -289,10 +289,15 @@ __device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double 
     const double xb1 = OG_P(X, b1), yb1 = OG_P(Y, b1), zb1 = OG_P(Z, b1);
     const double xc1 = OG_P(X, c1), yc1 = OG_P(Y, c1), zc1 = OG_P(Z, c1);
 #undef OG_P
+    // A displacement too large for a rigid solution makes one of these square
+    // roots negative: the water is then left unconstrained and counted as a
+    // constraint failure (OG_E_NOCONV) instead of turning into NaN.
     const double sinphi = za1 / ra;
-    const double cosphi = sqrt(1.0 - sinphi * sinphi);
+    const double q1     = 1.0 - sinphi * sinphi;
+    const double cosphi = sqrt(q1);
     const double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi);
-    const double cospsi = sqrt(1.0 - sinpsi * sinpsi);
+    const double q2     = 1.0 - sinpsi * sinpsi;
+    const double cospsi = sqrt(q2);
     const double ya2 = ra * cosphi;
     const double xb2 = -rc * cospsi;
     const double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
@@ -301,8 +306,14 @@ __device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double 
     const double beta   = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
     const double gama   = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
     const double al2be2 = alpa * alpa + beta * beta;
-    const double sinthe = (alpa * gama - beta * sqrt(al2be2 - gama * gama)) / al2be2;
-    const double costhe = sqrt(1.0 - sinthe * sinthe);
+    const double q3     = al2be2 - gama * gama;
+    const double sinthe = (alpa * gama - beta * sqrt(q3)) / al2be2;
+    const double q4     = 1.0 - sinthe * sinthe;
+    const double costhe = sqrt(q4);
+    if (!(q1 > 0.0 && q2 >= 0.0 && q3 >= 0.0 && q4 >= 0.0)) {
+        atomicAdd(&s.cnt->shake_fail, 1);
+        return;
+    }
     const double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
     const double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
     const double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};

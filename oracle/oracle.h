@@ -162,8 +162,8 @@ int orc_shake_trio(const float L[3], int n, const int *ids, const double *dist2 
 /* Analytic SETTLE (Miyamoto & Kollman 1992) for rigid 3-site waters; new code
  * with no reference counterpart (the reference SETTLE is a stub,
  * ConstraintShake.cpp:284-286).  ids = (O,H1,H2); dOH, dHH distances. */
-void orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH, double *crd,
-                const double *crd_prev, const float *mass);
+int orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH, double *crd,
+               const double *crd_prev, const float *mass); /* returns waters with no rigid solution (left unconstrained) */
 /* v' = (x' - x)/dt (SubBox.cpp:1462-1481). */
 void orc_velocity_from_crd(int n, double dt_inv, const double *crd, const double *crd_prev, double *vel_next);
 /* Wrap once into [lb, lb+L) (SubBox.cpp:1482-1502). */

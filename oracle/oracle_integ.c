@@ -122,8 +122,9 @@ int orc_shake_trio(const float L[3], int n, const int *ids, const double *dist2,
 
 /* Analytic SETTLE (S. Miyamoto and P. A. Kollman, J. Comput. Chem. 13, 952 (1992)).
  * Works on minimum-image unwrapped copies of each water, relative to its oxygen. */
-void orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH, double *crd,
-                const double *crd_prev, const float *mass) {
+int orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH, double *crd,
+               const double *crd_prev, const float *mass) {
+    int n_fail = 0;
     for (int k = 0; k < n; k++) {
         int    iO = ids[3 * k], iH1 = ids[3 * k + 1], iH2 = ids[3 * k + 2];
         double mO = mass[iO], mH = mass[iH1];
@@ -170,9 +171,11 @@ void orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH,
 #undef PY
 #undef PZ
         double sinphi = za1 / ra;
-        double cosphi = sqrt(1.0 - sinphi * sinphi);
+        double q1     = 1.0 - sinphi * sinphi;
+        double cosphi = sqrt(q1);
         double sinpsi = (zb1 - zc1) / (2.0 * rc * cosphi);
-        double cospsi = sqrt(1.0 - sinpsi * sinpsi);
+        double q2     = 1.0 - sinpsi * sinpsi;
+        double cospsi = sqrt(q2);
         double ya2 = ra * cosphi;
         double xb2 = -rc * cospsi;
         double t1 = -rb * cosphi, t2 = rc * sinpsi * sinphi;
@@ -181,8 +184,15 @@ void orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH,
         double beta   = xb2 * (yc0 - yb0) + xb0 * yb2 + xc0 * yc2;
         double gama   = xb0 * yb1 - xb1 * yb0 + xc0 * yc1 - xc1 * yc0;
         double al2be2 = alpa * alpa + beta * beta;
-        double sinthe = (alpa * gama - beta * sqrt(al2be2 - gama * gama)) / al2be2;
-        double costhe = sqrt(1.0 - sinthe * sinthe);
+        double q3     = al2be2 - gama * gama;
+        double sinthe = (alpa * gama - beta * sqrt(q3)) / al2be2;
+        double q4     = 1.0 - sinthe * sinthe;
+        double costhe = sqrt(q4);
+        /* no rigid solution: leave the water unconstrained and count it */
+        if (!(q1 > 0.0 && q2 >= 0.0 && q3 >= 0.0 && q4 >= 0.0)) {
+            n_fail++;
+            continue;
+        }
         double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
         double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
         double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};
@@ -195,6 +205,7 @@ void orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH,
             crd[iH2 * 3 + d] = base + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
         }
     }
+    return n_fail;
 }
 
 /* SubBox::set_velocity_from_crd (SubBox.cpp:1462-1481) */

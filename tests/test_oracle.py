@@ -2,6 +2,8 @@
 this path: the ala3 sample log (samples/ala3/log03_md.txt:99-135), MiniCell
 grids and cell-pair lists from the reference sources, reference all-pairs
 runs (one-step gradients, a 20-step SHAKE trajectory), and closed forms."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -357,3 +359,28 @@ def test_fp64_cell_list_equals_brute_force():
     assert n64 == len(r)
     assert rms_rel(w64, wb) < 1e-12
     assert e64[0] == pytest.approx(np.sum(a12 * ri ** 12 - a6 * ri ** 6), rel=1e-12)
+
+
+def test_settle_reports_waters_without_rigid_solution():
+    """A water pushed so far that no rotation of the rigid model reaches its
+    displaced positions is left unconstrained and counted, never NaN."""
+    L = np.array([30.0, 30.0, 30.0], np.float32)
+    dOH, dHH = gen.D_OH, gen.D_HH
+    prev = np.array([[0.0, 0.0, 0.0], [dOH, 0.0, 0.0], [dOH * np.cos(np.radians(104.52)), dOH * np.sin(np.radians(104.52)), 0.0]])
+    cur = prev.copy()
+    cur[1, 2] += 3.0    # H1 thrown 3 A out of the plane
+    ids = np.array([[0, 1, 2]], np.int32)
+    mass = np.array([15.9994, 1.008, 1.008], np.float32)
+    c = np.ascontiguousarray(cur.copy())
+    f = oracle.lib().orc_settle
+    f.restype = ctypes.c_int
+    n_fail = f(oracle._p(L, ctypes.c_float), ctypes.c_int(1), oracle._p(ids, ctypes.c_int), ctypes.c_double(dOH),
+               ctypes.c_double(dHH), oracle._p(c, ctypes.c_double), oracle._p(np.ascontiguousarray(prev), ctypes.c_double),
+               oracle._p(mass, ctypes.c_float))
+    assert n_fail == 1
+    assert np.array_equal(c, cur)
+    # a small displacement is solved exactly
+    cur2 = prev + 0.01
+    cur2[2, 0] += 0.02
+    c2 = oracle.settle(L, ids, dOH, dHH, cur2, prev, mass)
+    assert abs(np.linalg.norm(c2[1] - c2[0]) - dOH) < 1e-12 and abs(np.linalg.norm(c2[2] - c2[1]) - dHH) < 1e-12
