@@ -303,6 +303,13 @@ og_status og_alloc_set_lj_params(const float *h_lj_6term, const float *h_lj_12te
     CS_TRY(cs_alloc(&h.lj12, (size_t)n_lj_types * n_lj_types));
     CS_HIP(hipMemcpy(h.lj6, h_lj_6term, sizeof(float) * n_lj_types * n_lj_types, hipMemcpyHostToDevice));
     CS_HIP(hipMemcpy(h.lj12, h_lj_12term, sizeof(float) * n_lj_types * n_lj_types, hipMemcpyHostToDevice));
+    {
+        const size_t       nt2 = (size_t)n_lj_types * n_lj_types;
+        std::vector<float2> ljp(nt2);
+        for (size_t k = 0; k < nt2; k++) ljp[k] = make_float2(h_lj_6term[k], h_lj_12term[k]);
+        CS_TRY(cs_alloc(&h.lj_pair, nt2));
+        CS_HIP(hipMemcpy(h.lj_pair, ljp.data(), sizeof(float2) * nt2, hipMemcpyHostToDevice));
+    }
     // the reference stride is max_n_nb15off (MmSystem.cpp:478-525); the kernels read 32
     std::vector<int> tab((size_t)n * OG_MAX_NB15, -1);
     for (int a = 0; a < n; a++)
@@ -324,7 +331,7 @@ og_status og_alloc_set_lj_params(const float *h_lj_6term, const float *h_lj_12te
 
 og_status og_free_lj_params(void) {
     og_dev_sys &h = g_cs.h;
-    for (void **p : {(void **)&h.lj6, (void **)&h.lj12, (void **)&h.nb15off}) {
+    for (void **p : {(void **)&h.lj6, (void **)&h.lj12, (void **)&h.lj_pair, (void **)&h.nb15off}) {
         if (*p) {
             auto it = std::find(g_cs.allocs.begin(), g_cs.allocs.end(), *p);
             if (it != g_cs.allocs.end()) g_cs.allocs.erase(it);

@@ -549,6 +549,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.vel_next, 3 * n) && A(&h.work, 3 * n) && A(&h.mass, n) && A(&h.mass_inv, n) && A(&h.charge, n) &&
               A(&h.atom_type, n) && A(&h.nb15off, (size_t)n * OG_MAX_NB15) && A(&h.comp, n) &&
               A(&h.lj6, (size_t)h.n_types * h.n_types) && A(&h.lj12, (size_t)h.n_types * h.n_types) &&
+              A(&h.lj_pair, (size_t)h.n_types * h.n_types) &&
               A(&h.excess, 2 * (size_t)h.n_excess) && A(&h.bond_ids, 2 * (size_t)h.n_bonds) &&
               A(&h.bond_k, h.n_bonds) && A(&h.bond_r0, h.n_bonds) && A(&h.trio_ids, 3 * (size_t)h.n_trios) &&
               A(&h.trio_d2, 3 * (size_t)h.n_trios) && A(&h.pairc_ids, 2 * (size_t)h.n_pairs_c) &&
@@ -608,6 +609,11 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     OG_HIP(hipMemcpy(h.comp, comp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.lj6, lj6f.data(), sizeof(float) * lj6f.size(), hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.lj12, lj12f.data(), sizeof(float) * lj12f.size(), hipMemcpyHostToDevice));
+    {
+        std::vector<float2> ljp(lj6f.size());
+        for (size_t k = 0; k < ljp.size(); k++) ljp[k] = make_float2(lj6f[k], lj12f[k]);
+        OG_HIP(hipMemcpy(h.lj_pair, ljp.data(), sizeof(float2) * ljp.size(), hipMemcpyHostToDevice));
+    }
     if (h.n_excess) OG_HIP(hipMemcpy(h.excess, d->excess, sizeof(int) * 2 * h.n_excess, hipMemcpyHostToDevice));
     if (h.n_bonds) {
         OG_HIP(hipMemcpy(h.bond_ids, d->bond_ids, sizeof(int) * 2 * h.n_bonds, hipMemcpyHostToDevice));

@@ -317,13 +317,16 @@ __device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double 
     const double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
     const double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
     const double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};
-    double Onew[3];
-    for (int d = 0; d < 3; d++) Onew[d] = s.crd[3 * iO + d];
+    // Each atom moves by its correction (constrained minus unconstrained
+    // position, both relative to the oxygen), so a hydrogen wrapped to the
+    // other side of the box stays in its own periodic frame.
     for (int d = 0; d < 3; d++) {
-        const double base   = Onew[d] + com[d];
-        s.crd[3 * iO + d]  = base + X[d] * a3[0] + Y[d] * a3[1] + Z[d] * a3[2];
-        s.crd[3 * iH1 + d] = base + X[d] * b3[0] + Y[d] * b3[1] + Z[d] * b3[2];
-        s.crd[3 * iH2 + d] = base + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
+        const double ca = com[d] + X[d] * a3[0] + Y[d] * a3[1] + Z[d] * a3[2];
+        const double cb = com[d] + X[d] * b3[0] + Y[d] * b3[1] + Z[d] * b3[2];
+        const double cc = com[d] + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
+        s.crd[3 * iO + d] += ca;
+        s.crd[3 * iH1 + d] += cb - xb[d];
+        s.crd[3 * iH2 + d] += cc - xc[d];
     }
 }
 

@@ -99,6 +99,7 @@ struct og_dev_sys {
     int    *nb15off;        // [n_atoms*32], -1 terminated
     int    *comp;           // exclusion-connectivity component of each atom
     float  *lj6, *lj12;     // [n_types*n_types]
+    float2 *lj_pair;        // [n_types*n_types] {A6, A12} interleaved for the pair kernel
     int    *excess;         // [n_excess*2]
     int    *bond_ids;       // [n_bonds*2]
     double *bond_k, *bond_r0;

@@ -14,6 +14,7 @@
 #define OG_CHARGE_COEFF 332.06378f   // CelesteObject.cpp:36
 
 typedef float og_f4 __attribute__((ext_vector_type(4)));
+typedef float og_f2 __attribute__((ext_vector_type(2)));
 
 // erfc(x) for x >= 0 from Abramowitz & Stegun 7.1.26 (|abs error| <= 1.5e-7),
 // sharing exp(-x^2) with the force term.
@@ -30,12 +31,16 @@ __device__ __forceinline__ float og_erfc_as(float x, float expmx2) {
 // j row holds the row's (x, y, z) sums bit for bit, so lane k < 3 of the row
 // converts and adds component k: one conversion and one atomic instruction
 // serve all three components of the 8 j atoms (24 active lanes).
+// The largest converted magnitude is kept in vmax and checked once per cell
+// against the conversion range; fmaxf drops a NaN operand, so a NaN is
+// caught instead by the running sum vsum.
 __device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, int a2, int i, float rx, float ry,
-                                           float rz, int *flag) {
+                                           float rz, float &vmax, float &vsum) {
     const float     v  = i == 0 ? rx : (i == 1 ? ry : rz);
     const long long fv = og_fx_fast(-(double)v);
     if (i < 3 && fv != 0) og_atomic_add_fx_g(FX + i * cap + a2, fv);
-    if (i < 3 && !(fabsf(v) <= (float)OG_FX_LIMIT)) atomicOr(flag, OG_F_NAN);
+    vmax = fmaxf(vmax, fabsf(v));
+    vsum += v;
 }
 
 // Periodic shift of a tile's j cell (image bits of MiniCell::add_cell_pair,
@@ -60,8 +65,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
 
     const OG_GLOBAL(og_f4) *__restrict__ X      = (const OG_GLOBAL(og_f4) *)s.gxyzq;
     const OG_GLOBAL(int) *__restrict__ TY       = (const OG_GLOBAL(int) *)s.gtype;
-    const OG_GLOBAL(float) *__restrict__ LJ6    = (const OG_GLOBAL(float) *)s.lj6;
-    const OG_GLOBAL(float) *__restrict__ LJ12   = (const OG_GLOBAL(float) *)s.lj12;
+    const OG_GLOBAL(og_f2) *__restrict__ LJP    = (const OG_GLOBAL(og_f2) *)s.lj_pair;
     const OG_GLOBAL(unsigned) *__restrict__ CP  = (const OG_GLOBAL(unsigned) *)s.cp_c2;
     const OG_GLOBAL(uint64_t) *__restrict__ CM  = (const OG_GLOBAL(uint64_t) *)s.cp_mask;
     OG_GLOBAL(long long) *__restrict__ FX       = (OG_GLOBAL(long long) *)s.fx;
@@ -75,11 +79,12 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
     const int   p1 = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
 
     const og_f4  pi   = X[c1 * OG_NAC + i];
-    const int    trow = TY[c1 * OG_NAC + i] * nt;
+    const unsigned trow = (unsigned)(TY[c1 * OG_NAC + i] * nt);
     const float  qi   = pi.w * OG_CHARGE_COEFF;
 
     double dfx = 0.0, dfy = 0.0, dfz = 0.0, ev = 0.0, ee = 0.0, vir = 0.0;
     int    npair = 0;  // in-cutoff unmasked pairs of this cell1 (wave-uniform)
+    float  vmax = 0.0f, vsum = 0.0f;  // j-gradient range check (see og_flush_j)
 
     for (int base = p0; base < p1; base += 64) {
         const int nb = min(64, p1 - base);
@@ -123,8 +128,9 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             // 12 A cutoff: skip them.  Their contributions are exact zeros, so
             // the result is bit-identical to evaluating them.
             if (okb == 0) continue;
-            const float a6  = LJ6[trow + tj];
-            const float a12 = LJ12[trow + tj];
+            const og_f2  lj  = LJP[trow + (unsigned)tj];  // {A6, A12}, one 8-byte load
+            const float  a6  = lj.x;
+            const float  a12 = lj.y;
             const float    r2s  = ok ? r2 : 1.0f;
             const float    rinv  = __builtin_amdgcn_rsqf(r2s);
             const float    r2inv = rinv * rinv;
@@ -155,7 +161,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             const float rjx = og_dpp_sum8(wx);
             const float rjy = og_dpp_sum8(wy);
             const float rjz = og_dpp_sum8(wz);
-            og_flush_j(FX, cap, a2, i, rjx, rjy, rjz, &s.cnt->flags);
+            og_flush_j(FX, cap, a2, i, rjx, rjy, rjz, vmax, vsum);
         }
         dfx += (double)fx;
         dfy += (double)fy;
@@ -169,6 +175,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
         dfy += __shfl_xor(dfy, off, 64);
         dfz += __shfl_xor(dfz, off, 64);
     }
+    if (!(vmax <= (float)OG_FX_LIMIT) || !(fabsf(vsum) < INFINITY)) atomicOr(&s.cnt->flags, OG_F_NAN);
     if (jl == 0) {
         const int a1 = c1 * OG_NAC + i;
         if (!(fabs(dfx) + fabs(dfy) + fabs(dfz) <= OG_FX_LIMIT)) atomicOr(&s.cnt->flags, OG_F_NAN);

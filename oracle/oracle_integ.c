@@ -196,13 +196,14 @@ int orc_settle(const float L[3], int n, const int *ids, double dOH, double dHH, 
         double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
         double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
         double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};
-        double Onew[3];
-        for (int d = 0; d < 3; d++) Onew[d] = crd[iO * 3 + d];
+        /* each atom moves by its correction, so it stays in its own periodic frame */
         for (int d = 0; d < 3; d++) {
-            double base = Onew[d] + com[d];
-            crd[iO * 3 + d]  = base + X[d] * a3[0] + Y[d] * a3[1] + Z[d] * a3[2];
-            crd[iH1 * 3 + d] = base + X[d] * b3[0] + Y[d] * b3[1] + Z[d] * b3[2];
-            crd[iH2 * 3 + d] = base + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
+            double ca = com[d] + X[d] * a3[0] + Y[d] * a3[1] + Z[d] * a3[2];
+            double cb = com[d] + X[d] * b3[0] + Y[d] * b3[1] + Z[d] * b3[2];
+            double cc = com[d] + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
+            crd[iO * 3 + d] += ca;
+            crd[iH1 * 3 + d] += cb - xb[d];
+            crd[iH2 * 3 + d] += cc - xc[d];
         }
     }
     return n_fail;

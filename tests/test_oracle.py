@@ -384,3 +384,23 @@ def test_settle_reports_waters_without_rigid_solution():
     cur2[2, 0] += 0.02
     c2 = oracle.settle(L, ids, dOH, dHH, cur2, prev, mass)
     assert abs(np.linalg.norm(c2[1] - c2[0]) - dOH) < 1e-12 and abs(np.linalg.norm(c2[2] - c2[1]) - dHH) < 1e-12
+
+
+def test_settle_keeps_each_atom_in_its_periodic_frame():
+    """A water whose hydrogen is wrapped to the other side of the box: SETTLE
+    moves every atom by a small correction in its own frame (no jump of one
+    box length), and the constrained geometry holds under the minimum image."""
+    Lf = 20.0
+    L = np.array([Lf, Lf, Lf], np.float32)
+    dOH, dHH = gen.D_OH, gen.D_HH
+    th = np.radians(104.52)
+    O = np.array([9.7, 1.0, 1.0])
+    prev = np.array([O, O + [dOH, 0, 0], O + [dOH * np.cos(th), dOH * np.sin(th), 0]])
+    prev[1, 0] -= Lf                       # H1 stored wrapped: x = -9.54 instead of 10.46
+    rng = np.random.default_rng(3)
+    cur = prev + rng.normal(scale=0.02, size=(3, 3))
+    mass = np.array([15.9994, 1.008, 1.008], np.float32)
+    new = oracle.settle(L, np.array([[0, 1, 2]], np.int32), dOH, dHH, cur, prev, mass)
+    assert np.max(np.abs(new - cur)) < 0.1
+    d = lambda a, b: np.linalg.norm((new[a] - new[b]) - Lf * np.round((new[a] - new[b]) / Lf))
+    assert abs(d(0, 1) - dOH) < 1e-12 and abs(d(0, 2) - dOH) < 1e-12 and abs(d(1, 2) - dHH) < 1e-12
