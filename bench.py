@@ -280,6 +280,12 @@ def main():
                "sample": "%d single-point force evaluations (grid + cell-pair list + FP32 LJ/ZD pairs + "
                          "excluded pairs) of the same 99,200-atom system in %.1f s, one thread" % (n_eval, dt)}
 
+    # HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+    # (a counter run cannot be nested in this process)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_pair_water_chain_99k_traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
     walkers = None if args.no_walkers else bench_walkers()
     serving = None if args.no_serving else bench_serving()
 
@@ -302,7 +308,7 @@ def main():
         "ns_per_day": ns_day,
         "temperature_K": T,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_pair<%d>" % alpha_mode, "kernel_ms": t_launch * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR[alpha_mode]},
         "cpu_baseline": cpu,
