@@ -261,7 +261,7 @@ og_status exec_prepare(og_exec &x) {
         cnt[KIND_CELLW] = cdiv(d.cap_cells, OG_THREADS / 64);
         cnt[KIND_COL]   = d.n_columns;
         cnt[KIND_SYS]   = 1;
-        cnt[KIND_MISC]  = d.n_misc_blocks;
+        cnt[KIND_MISC]  = cdiv(d.n_pairs_c, OG_THREADS);  // SHAKE pair constraints
         cnt[KIND_TRIO]  = cdiv(std::max(d.n_trios, d.n_settle), OG_TRIO_THREADS);
         cnt[KIND_COMA] = 0;  // filled below
         for (int k = 0; k < KIND_N; k++)
@@ -305,7 +305,6 @@ void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1) {
     if (e0) hipEventRecord(e0, x.stream);
     og_launch_pair(L(x, KIND_CELLW), x.alpha_mode);
     if (e1) hipEventRecord(e1, x.stream);
-    og_launch_misc_forces(L(x, KIND_MISC));
     og_launch_forces_to_work(L(x, KIND_ATOM));
     og_launch_energy_reduce(L(x, KIND_SYS));
 }
@@ -566,7 +565,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
               A(&h.fx, 3 * (size_t)cap) && A(&h.e_cell, 4 * (size_t)h.cap_cells);
     if (!ok) return OG_E_DEVICE;
-    h.n_misc_blocks = cdiv(std::max(h.n_excess, std::max(h.n_bonds, h.n_pairs_c)), OG_THREADS);
+    h.n_misc_blocks = cdiv(n, OG_THREADS);  // excess/bond energy partials, one per atom block
     h.n_atom_blocks = cdiv(n, OG_THREADS);
     ok = A(&h.e_misc, 2 * (size_t)std::max(h.n_misc_blocks, 1)) && A(&h.e_atom, std::max(h.n_atom_blocks, 1)) &&
          A(&h.com_mom, 3 * (size_t)(h.n_com_groups + 1)) && A(&h.energy, OG_E_NTERMS) &&
@@ -619,6 +618,25 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         OG_HIP(hipMemcpy(h.bond_ids, d->bond_ids, sizeof(int) * 2 * h.n_bonds, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.bond_k, d->bond_k, sizeof(double) * h.n_bonds, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.bond_r0, d->bond_r0, sizeof(double) * h.n_bonds, hipMemcpyHostToDevice));
+    }
+    {
+        // per-atom lists of the excess pairs and bonds each atom takes part in,
+        // in ascending list order (the per-atom gradient sums follow it)
+        auto csr = [&](const int *pairs, int np, int **off_d, int **at_d) -> og_status {
+            std::vector<int> off(n + 1, 0), at(2 * (size_t)np);
+            for (int t = 0; t < np; t++) off[pairs[2 * t] + 1]++, off[pairs[2 * t + 1] + 1]++;
+            for (int i = 0; i < n; i++) off[i + 1] += off[i];
+            std::vector<int> fill(off.begin(), off.end() - 1);
+            for (int t = 0; t < np; t++) at[fill[pairs[2 * t]]++] = t, at[fill[pairs[2 * t + 1]]++] = t;
+            if (!A(off_d, n + 1) || !A(at_d, std::max<size_t>(at.size(), 1))) return OG_E_DEVICE;
+            OG_HIP(hipMemcpy(*off_d, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+            if (np) OG_HIP(hipMemcpy(*at_d, at.data(), sizeof(int) * at.size(), hipMemcpyHostToDevice));
+            return OG_OK;
+        };
+        og_status st = csr(d->excess, h.n_excess, &h.exc_off, &h.exc_at);
+        if (st != OG_OK) return st;
+        st = csr(d->bond_ids, h.n_bonds, &h.bond_off, &h.bond_at);
+        if (st != OG_OK) return st;
     }
     if (h.n_trios) {
         OG_HIP(hipMemcpy(h.trio_ids, d->shake_trio_ids, sizeof(int) * 3 * h.n_trios, hipMemcpyHostToDevice));

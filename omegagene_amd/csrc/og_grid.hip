@@ -480,10 +480,11 @@ __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict
 }
 
 // ---------------------------------------------------------------- masks --
-// MiniCell::set_cell_pair_bitmask (MiniCell.cpp:1198-1237): one wave per cell
-// pair, lane = bit a2*8+a1; bit set (= skip) for a dummy atom, the lower
-// triangle of a self pair, or an nb15off partner.  nb15off is scanned only
-// when both atoms lie in the same exclusion component.
+// MiniCell::set_cell_pair_bitmask (MiniCell.cpp:1198-1237): bit a2*8+a1 set
+// (= skip) for a dummy atom, the lower triangle of a self pair, or an nb15off
+// partner.  One wave per cell1; each lane builds the whole 64-bit mask of one
+// of the cell's pairs, so 64 pairs are done at once.  nb15off is scanned only
+// when both atoms lie in the same exclusion component (same molecule).
 __global__ void __launch_bounds__(OG_THREADS) k_masks(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const int2        bm   = map[blockIdx.x];
     const og_dev_sys &s    = S[bm.x];
@@ -491,29 +492,38 @@ __global__ void __launch_bounds__(OG_THREADS) k_masks(const og_dev_sys *__restri
     const int         lane = threadIdx.x & 63;
     const int         c1   = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells || s.cnt->n_cell_pairs == 0) return;
-    const int i1 = lane & 7, i2 = lane >> 3;
     const int p0 = s.cp_head[c1], p1 = s.cp_head[c1 + 1];
-    const int a1   = c1 * OG_NAC + i1;
-    const int id1  = s.atomids[a1];
-    const int cmp1 = s.gcomp[a1];
-    for (int p = p0; p < p1; p++) {
-        const int c2  = s.cp_c2[p] & 0xffffff;
-        const int a2  = c2 * OG_NAC + i2;
-        const int id2 = s.atomids[a2];
-        bool skip     = id1 < 0 || id2 < 0 || (c1 == c2 && a1 >= a2);
-        if (!skip && cmp1 == s.gcomp[a2]) {
-            const int *l = s.nb15off + (size_t)id1 * OG_MAX_NB15;
-            for (int k = 0; k < OG_MAX_NB15; k++) {
-                const int v = l[k];
-                if (v == -1) break;
-                if (v == id2) {
-                    skip = true;
-                    break;
+    // the 8 atoms of cell1, wave-uniform
+    const int idl = s.atomids[c1 * OG_NAC + (lane & 7)];
+    const int cml = s.gcomp[c1 * OG_NAC + (lane & 7)];
+    int       id1[OG_NAC], cm1[OG_NAC];
+    for (int a = 0; a < OG_NAC; a++) {
+        id1[a] = __builtin_amdgcn_readlane(idl, a);
+        cm1[a] = __builtin_amdgcn_readlane(cml, a);
+    }
+    for (int p = p0 + lane; p < p1; p += 64) {
+        const int c2 = s.cp_c2[p] & 0xffffff;
+        uint64_t  m  = 0;
+        for (int a2 = 0; a2 < OG_NAC; a2++) {
+            const int id2 = s.atomids[c2 * OG_NAC + a2];
+            const int cm2 = s.gcomp[c2 * OG_NAC + a2];
+            for (int a1 = 0; a1 < OG_NAC; a1++) {
+                bool skip = id1[a1] < 0 || id2 < 0 || (c1 == c2 && a1 >= a2);
+                if (!skip && cm1[a1] == cm2) {
+                    const int *l = s.nb15off + (size_t)id1[a1] * OG_MAX_NB15;
+                    for (int k = 0; k < OG_MAX_NB15; k++) {
+                        const int v = l[k];
+                        if (v == -1) break;
+                        if (v == id2) {
+                            skip = true;
+                            break;
+                        }
+                    }
                 }
+                m |= (uint64_t)skip << (a2 * OG_NAC + a1);
             }
         }
-        const uint64_t m = __ballot(skip);
-        if (lane == 0) s.cp_mask[p] = m;
+        s.cp_mask[p] = m;
     }
 }
 

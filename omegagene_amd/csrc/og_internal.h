@@ -103,6 +103,8 @@ struct og_dev_sys {
     int    *excess;         // [n_excess*2]
     int    *bond_ids;       // [n_bonds*2]
     double *bond_k, *bond_r0;
+    int    *exc_off, *exc_at;    // per-atom CSR of excess-pair indices: [n_atoms+1], [2*n_excess]
+    int    *bond_off, *bond_at;  // per-atom CSR of bond indices: [n_atoms+1], [2*n_bonds]
     int    *trio_ids;       // [n_trios*3]
     double *trio_d2;        // [n_trios*3]
     int    *pairc_ids;      // [n_pairs_c*2]
@@ -237,7 +239,6 @@ void og_launch_enum_write(const og_launch &cellw);
 void og_launch_masks(const og_launch &cp);
 // forces
 void og_launch_pair(const og_launch &cellw, int alpha_mode);
-void og_launch_misc_forces(const og_launch &misc);
 void og_launch_forces_to_work(const og_launch &atoms);
 void og_launch_energy_reduce(const og_launch &sys);
 // integration

@@ -227,72 +227,110 @@ __device__ __forceinline__ void og_block_sum2(double &a, double &b, double *sh) 
     }
 }
 
-__global__ void __launch_bounds__(OG_THREADS) k_misc(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+// Excluded pair (a, b), a < b in the excess list: gradient w added to a and
+// subtracted from b, energy returned (reference float arithmetic).
+__device__ __forceinline__ float og_excess_pair(const og_dev_sys &s, int a, int b, float *w) {
+    float d[3];
+    for (int k = 0; k < 3; k++) {
+        const float Lh  = s.L[k] * 0.5f;
+        const float Lhi = 1.0f / Lh;
+        float       v   = (float)s.crd[3 * a + k] - (float)s.crd[3 * b + k];
+        v               = v - s.L[k] * (float)(int)(v * Lhi);
+        v               = v - s.L[k] * (float)(int)(v * Lhi);
+        d[k]            = v;
+    }
+    const float r2    = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const float r     = sqrtf(r2);
+    const float rinv  = 1.0f / r;
+    const float r2inv = rinv * rinv;
+    const float r3inv = r2inv * rinv;
+    const float cc    = (float)((double)(s.charge[a] * s.charge[b]) * 332.06378);
+    float       ene, gc;
+    if (s.alpha == 0.0f) {
+        ene = cc * s.bcoeff * r2;
+        gc  = cc * s.fcoeff;
+    } else {
+        const float x     = s.alpha * r;
+        const float erfcv = erfcf(x);
+        const float erfv  = (float)(1.0 - (double)erfcv);
+        ene = s.excess_bug ? cc * (-erfv * r + s.bcoeff * r2) : cc * (-erfv * rinv + s.bcoeff * r2);
+        gc  = -cc * (erfcv * r3inv + s.piewald * expf(-x * x) * r2inv - s.fcoeff - r3inv);
+    }
+    for (int k = 0; k < 3; k++) w[k] = gc * d[k];
+    return ene;
+}
+
+// Harmonic bond (a, b): gradient w added to a and subtracted from b, energy returned (FP64).
+__device__ __forceinline__ double og_bond_pair(const og_dev_sys &s, int t, int a, int b, double *w) {
+    double d[3];
+    for (int k = 0; k < 3; k++) {
+        const float Lh  = s.L[k] * 0.5f;
+        const float Lhi = 1.0f / Lh;
+        double      v   = s.crd[3 * a + k] - s.crd[3 * b + k];
+        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
+        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
+        d[k]            = v;
+    }
+    const double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const double dif  = r - s.bond_r0[t];
+    const double coef = s.bond_k[t] * dif;
+    const double gcf  = 2.0 * coef / r;
+    for (int k = 0; k < 3; k++) w[k] = d[k] * gcf;
+    return coef * dif;
+}
+
+// Per atom: fixed-point pair gradient (grid order) -> FP64 "work" in atom order
+// (SubBox::add_work_from_minicell, SubBox.cpp:1398-1407), plus the atom's
+// excluded-pair ZD terms (SubBox::calc_energy_ele_excess SubBox.cpp:1347-1367,
+// ForceField::calc_zms_excess ForceField.cpp:420-437,
+// calc_zero02pole_excess(_alpha0) ZeroMultipoleSum.cpp:191-217) and bonds
+// E = k (r - r0)^2 (ForceField::calc_bond ForceField.cpp:108-136) through
+// per-atom lists: every pair term is evaluated for both of its atoms in the
+// orientation of the reference list, so no atomics are needed and the sums
+// are deterministic.  Energies are counted on the pair's first atom.
+// Clears the fixed-point accumulators for the next evaluation.
+__global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys *__restrict__ S,
+                                                               const int2 *__restrict__ map) {
 #pragma clang fp contract(off)
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
     if (og_frozen(s)) return;
-    const int         t  = bm.y * OG_THREADS + threadIdx.x;
-    const int         cap = s.cap_atom_array;
-    double            e_ex = 0.0, e_b = 0.0;
-    if (t < s.n_excess) {
-        const int a = s.excess[2 * t], b = s.excess[2 * t + 1];
-        float     d[3];
+    const int i   = bm.y * OG_THREADS + threadIdx.x;
+    double    e_ex = 0.0, e_b = 0.0;
+    if (i < s.n_atoms) {
+        const int g   = s.rev[i];
+        const int cap = s.cap_atom_array;
+        double    w[3];
         for (int k = 0; k < 3; k++) {
-            const float Lh  = s.L[k] * 0.5f;
-            const float Lhi = 1.0f / Lh;
-            float       v   = (float)s.crd[3 * a + k] - (float)s.crd[3 * b + k];
-            v               = v - s.L[k] * (float)(int)(v * Lhi);
-            v               = v - s.L[k] * (float)(int)(v * Lhi);
-            d[k]            = v;
+            const long long v = s.fx[k * cap + g];
+            s.fx[k * cap + g] = 0;
+            w[k]              = (double)v * OG_FX_INV;
         }
-        const float r2     = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-        const float r      = sqrtf(r2);
-        const float rinv   = 1.0f / r;
-        const float r2inv  = rinv * rinv;
-        const float r3inv  = r2inv * rinv;
-        const float cc     = (float)((double)(s.charge[a] * s.charge[b]) * 332.06378);
-        float       ene, gc;
-        if (s.alpha == 0.0f) {
-            ene = cc * s.bcoeff * r2;
-            gc  = cc * s.fcoeff;
-        } else {
-            const float x      = s.alpha * r;
-            const float erfcv  = erfcf(x);
-            const float erfv   = (float)(1.0 - (double)erfcv);
-            ene = s.excess_bug ? cc * (-erfv * r + s.bcoeff * r2) : cc * (-erfv * rinv + s.bcoeff * r2);
-            gc  = -cc * (erfcv * r3inv + s.piewald * expf(-x * x) * r2inv - s.fcoeff - r3inv);
+        for (int e = s.exc_off[i]; e < s.exc_off[i + 1]; e++) {
+            const int t = s.exc_at[e];
+            const int a = s.excess[2 * t], b = s.excess[2 * t + 1];
+            float     we[3];
+            const float ene = og_excess_pair(s, a, b, we);
+            if (i == a) {
+                for (int k = 0; k < 3; k++) w[k] += (double)we[k];
+                e_ex += (double)ene;
+            } else {
+                for (int k = 0; k < 3; k++) w[k] -= (double)we[k];
+            }
         }
-        e_ex       = (double)ene;
-        const int ga = s.rev[a], gb = s.rev[b];
-        for (int k = 0; k < 3; k++) {
-            const double w = (double)(gc * d[k]);
-            og_atomic_add_fx(&s.fx[k * cap + ga], og_to_fixed(w));
-            og_atomic_add_fx(&s.fx[k * cap + gb], og_to_fixed(-w));
+        for (int e = s.bond_off[i]; e < s.bond_off[i + 1]; e++) {
+            const int t = s.bond_at[e];
+            const int a = s.bond_ids[2 * t], b = s.bond_ids[2 * t + 1];
+            double    wb[3];
+            const double ene = og_bond_pair(s, t, a, b, wb);
+            if (i == a) {
+                for (int k = 0; k < 3; k++) w[k] += wb[k];
+                e_b += ene;
+            } else {
+                for (int k = 0; k < 3; k++) w[k] -= wb[k];
+            }
         }
-    }
-    if (t < s.n_bonds) {
-        const int a = s.bond_ids[2 * t], b = s.bond_ids[2 * t + 1];
-        double    d[3];
-        for (int k = 0; k < 3; k++) {
-            const float Lh  = s.L[k] * 0.5f;
-            const float Lhi = 1.0f / Lh;
-            double      v   = s.crd[3 * a + k] - s.crd[3 * b + k];
-            v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-            v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-            d[k]            = v;
-        }
-        const double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        const double dif  = r - s.bond_r0[t];
-        const double coef = s.bond_k[t] * dif;
-        e_b               = coef * dif;
-        const double gcf  = 2.0 * coef / r;
-        const int    ga = s.rev[a], gb = s.rev[b];
-        for (int k = 0; k < 3; k++) {
-            const double w = d[k] * gcf;
-            og_atomic_add_fx(&s.fx[k * cap + ga], og_to_fixed(w));
-            og_atomic_add_fx(&s.fx[k * cap + gb], og_to_fixed(-w));
-        }
+        for (int k = 0; k < 3; k++) s.work[3 * i + k] = w[k];
     }
     __shared__ double sh[2 * (OG_THREADS / 64)];
     og_block_sum2(e_ex, e_b, sh);
@@ -302,61 +340,48 @@ __global__ void __launch_bounds__(OG_THREADS) k_misc(const og_dev_sys *__restric
     }
 }
 
-// fixed point (grid order) -> FP64 gradient "work" in atom order (SubBox::add_work_from_minicell,
-// SubBox.cpp:1398-1407); clears the accumulators for the next evaluation.
-__global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys *__restrict__ S,
-                                                               const int2 *__restrict__ map) {
-    const int2        bm = map[blockIdx.x];
-    const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
-    const int         i  = bm.y * OG_THREADS + threadIdx.x;
-    if (i >= s.n_atoms) return;
-    const int g   = s.rev[i];
-    const int cap = s.cap_atom_array;
-    for (int k = 0; k < 3; k++) {
-        const long long v   = s.fx[k * cap + g];
-        s.fx[k * cap + g]   = 0;
-        s.work[3 * i + k]   = (double)v * OG_FX_INV;
-    }
-}
-
-// Deterministic per-system energy reduction (fixed thread/tree order).
-__global__ void __launch_bounds__(OG_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
-                                                              const int2 *__restrict__ map) {
+// Deterministic per-system energy reduction: 1024 threads, each with a fixed
+// strided share of the per-cell and per-block partials (16-byte loads, four in
+// flight), then a fixed wave and block tree.
+#define OG_RED_THREADS 1024
+__global__ void __launch_bounds__(OG_RED_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
+                                                                  const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
     if (og_frozen(s)) return;
-    const int         n = s.cnt->n_cells;
-    double            a = 0.0, b = 0.0, c = 0.0, d = 0.0, e = 0.0, f = 0.0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        a += s.e_cell[4 * k];
-        b += s.e_cell[4 * k + 1];
-        c += s.e_cell[4 * k + 2];
-        f += s.e_cell[4 * k + 3];
+    const int n = s.cnt->n_cells;
+    double    v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // vdw, ele, virial, pairs, excess, bond
+    const double2 *E = (const double2 *)s.e_cell;
+#pragma unroll 4
+    for (int k = threadIdx.x; k < n; k += OG_RED_THREADS) {
+        const double2 u = E[2 * k], w = E[2 * k + 1];
+        v[0] += u.x;
+        v[1] += u.y;
+        v[2] += w.x;
+        v[3] += w.y;
     }
-    for (int k = threadIdx.x; k < s.n_misc_blocks; k += blockDim.x) {
-        d += s.e_misc[2 * k];
-        e += s.e_misc[2 * k + 1];
+    const double2 *M = (const double2 *)s.e_misc;
+    for (int k = threadIdx.x; k < s.n_misc_blocks; k += OG_RED_THREADS) {
+        const double2 u = M[k];
+        v[4] += u.x;
+        v[5] += u.y;
     }
-    __shared__ double sh[6 * OG_THREADS];
-    sh[6 * threadIdx.x]     = a;
-    sh[6 * threadIdx.x + 1] = b;
-    sh[6 * threadIdx.x + 2] = c;
-    sh[6 * threadIdx.x + 3] = d;
-    sh[6 * threadIdx.x + 4] = e;
-    sh[6 * threadIdx.x + 5] = f;
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 6; q++) v[q] += __shfl_down(v[q], off, 64);
+    __shared__ double sh[6 * (OG_RED_THREADS / 64)];
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 6; q++) sh[6 * wv + q] = v[q];
     __syncthreads();
-    for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off)
-            for (int k = 0; k < 6; k++) sh[6 * threadIdx.x + k] += sh[6 * (threadIdx.x + off) + k];
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
-        s.energy[OG_E_VDW]    = sh[0];
-        s.energy[OG_E_ELE]    = sh[1];
-        s.energy[OG_E_VIRIAL] = sh[2];
-        s.energy[OG_E_EXCESS] = sh[3];
-        s.energy[OG_E_BOND]   = sh[4];
-        s.energy[OG_E_NPAIRS] = sh[5];
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < OG_RED_THREADS / 64; w++)
+            for (int q = 0; q < 6; q++) t[q] += sh[6 * w + q];
+        s.energy[OG_E_VDW]    = t[0];
+        s.energy[OG_E_ELE]    = t[1];
+        s.energy[OG_E_VIRIAL] = t[2];
+        s.energy[OG_E_NPAIRS] = t[3];
+        s.energy[OG_E_EXCESS] = t[4];
+        s.energy[OG_E_BOND]   = t[5];
         s.energy[OG_E_SELF]   = s.e_self;
     }
 }
@@ -368,13 +393,9 @@ void og_launch_pair(const og_launch &a, int alpha_mode) {
     else
         hipLaunchKernelGGL(k_pair<0>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
-void og_launch_misc_forces(const og_launch &a) {
-    if (a.n_blocks == 0) return;
-    hipLaunchKernelGGL(k_misc, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
-}
 void og_launch_forces_to_work(const og_launch &a) {
     hipLaunchKernelGGL(k_forces_to_work, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_energy_reduce(const og_launch &a) {
-    hipLaunchKernelGGL(k_energy_reduce, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+    hipLaunchKernelGGL(k_energy_reduce, dim3(a.n_blocks), dim3(OG_RED_THREADS), 0, a.stream, a.sys, a.map);
 }
