@@ -234,6 +234,7 @@ og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_c
     h.cap_cell_pairs       = max_n_cell_pairs;
     h.n_atoms              = max_n_atoms_exbox;
     h.gap_bug              = 1;
+    h.log_stride           = 1;  // energies on every evaluation, as cuda_pairwise_ljzd
     CS_TRY(cs_alloc(&g_cs.d_sys, 1));
     CS_TRY(cs_alloc(&g_cs.d_cnt, 1));
     h.cnt = g_cs.d_cnt;

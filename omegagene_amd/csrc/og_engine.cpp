@@ -893,9 +893,13 @@ og_status og_calc_energy_force(og_system *s, int rebuild, double *work_atom, dou
     og_status st = ensure_ctx(s);
     if (st != OG_OK) return st;
     og_exec &x = *s->ctx;
+    // a single-point evaluation always carries its energies
+    const int one = 1, zero = 0;
+    OG_HIP(hipMemcpy(&s->d_cnt->force_energy, &one, sizeof(int), hipMemcpyHostToDevice));
     if (rebuild || !s->grid_valid) enqueue_rebuild(x);
     enqueue_forces(x, nullptr, nullptr);
     OG_HIP(hipStreamSynchronize(x.stream));
+    OG_HIP(hipMemcpy(&s->d_cnt->force_energy, &zero, sizeof(int), hipMemcpyHostToDevice));
     s->grid_valid = true;
     st            = check_flags(x);
     if (st != OG_OK) return st;

@@ -49,7 +49,7 @@ struct og_dev_counters {
     int       shake_fail;
     long long step;      // steps completed since system creation
     int       vs;        // current VcMD virtual state
-    int       pad;
+    int       force_energy;  // 1: evaluate energies on the next force call whatever the step
     long long step_end;  // the system stands still once step reaches this (segment end)
 };
 
@@ -173,6 +173,11 @@ enum og_map_kind {
 // kernel reads the counters before its first write; step only changes in
 // the last kernel of a step, so all kernels of one step agree).
 __device__ __forceinline__ bool og_frozen(const og_dev_sys &s) { return s.cnt->step >= s.cnt->step_end; }
+// Energies (pair, virial) are evaluated on logged steps and on request; the
+// forces and the pair count on every step.
+__device__ __forceinline__ bool og_energy_step(const og_dev_sys &s) {
+    return s.cnt->force_energy || (s.cnt->step % s.log_stride) == 0;
+}
 
 __device__ __forceinline__ int og_lane() { return __lane_id(); }
 

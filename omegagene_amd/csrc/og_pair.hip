@@ -52,13 +52,9 @@ __device__ __forceinline__ float og_img_shift(unsigned img, unsigned neg, unsign
     return __int_as_float(__builtin_amdgcn_readfirstlane(b));
 }
 
-template <int ALPHA>
-__global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
-    const int2        bm = map[blockIdx.x];
-    const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
-    const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
-    if (c1 >= s.cnt->n_cells || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
+// One cell1 (one wave).  ENERGY = false on steps whose energies are not
+// logged: forces and the pair count only.
+template <int ALPHA, bool ENERGY> __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1) {
     const int lane = threadIdx.x & 63;
     const int i    = lane & 7;
     const int jl   = lane >> 3;
@@ -155,9 +151,11 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             fx += wx;
             fy += wy;
             fz += wz;
-            fe += ok ? (t12 - t6) : 0.0f;
-            fq += ok ? eel : 0.0f;
-            fv += g * r2s;
+            if (ENERGY) {
+                fe += ok ? (t12 - t6) : 0.0f;
+                fq += ok ? eel : 0.0f;
+                fv += g * r2s;
+            }
             const float rjx = og_dpp_sum8(wx);
             const float rjy = og_dpp_sum8(wy);
             const float rjz = og_dpp_sum8(wz);
@@ -166,9 +164,11 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
         dfx += (double)fx;
         dfy += (double)fy;
         dfz += (double)fz;
-        ev += (double)fe;
-        ee += (double)fq;
-        vir += (double)fv;
+        if (ENERGY) {
+            ev += (double)fe;
+            ee += (double)fq;
+            vir += (double)fv;
+        }
     }
     for (int off = 8; off < 64; off <<= 1) {
         dfx += __shfl_xor(dfx, off, 64);
@@ -186,6 +186,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
             og_atomic_add_fx_g(FX + 2 * cap + a1, vz);
         }
     }
+    if (!ENERGY) return;
     for (int off = 1; off < 64; off <<= 1) {
         ev += __shfl_xor(ev, off, 64);
         ee += __shfl_xor(ee, off, 64);
@@ -197,6 +198,17 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
         s.e_cell[4 * c1 + 2] = vir;
         s.e_cell[4 * c1 + 3] = (double)npair;
     }
+}
+
+template <int ALPHA>
+__global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
+    const int c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
+    if (c1 >= s.cnt->n_cells || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
+    if (og_energy_step(s)) og_pair_cell<ALPHA, true>(s, c1);
+    else og_pair_cell<ALPHA, false>(s, c1);
 }
 
 // ------------------------------------------------------------ misc terms --
@@ -230,6 +242,7 @@ __device__ __forceinline__ void og_block_sum2(double &a, double &b, double *sh) 
 // Excluded pair (a, b), a < b in the excess list: gradient w added to a and
 // subtracted from b, energy returned (reference float arithmetic).
 __device__ __forceinline__ float og_excess_pair(const og_dev_sys &s, int a, int b, float *w) {
+#pragma clang fp contract(off)
     float d[3];
     for (int k = 0; k < 3; k++) {
         const float Lh  = s.L[k] * 0.5f;
@@ -262,6 +275,7 @@ __device__ __forceinline__ float og_excess_pair(const og_dev_sys &s, int a, int 
 
 // Harmonic bond (a, b): gradient w added to a and subtracted from b, energy returned (FP64).
 __device__ __forceinline__ double og_bond_pair(const og_dev_sys &s, int t, int a, int b, double *w) {
+#pragma clang fp contract(off)
     double d[3];
     for (int k = 0; k < 3; k++) {
         const float Lh  = s.L[k] * 0.5f;
@@ -347,7 +361,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
 __global__ void __launch_bounds__(OG_RED_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
                                                                   const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
-    if (og_frozen(s)) return;
+    if (og_frozen(s) || !og_energy_step(s)) return;
     const int n = s.cnt->n_cells;
     double    v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // vdw, ele, virial, pairs, excess, bond
     const double2 *E = (const double2 *)s.e_cell;
