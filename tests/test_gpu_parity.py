@@ -279,3 +279,21 @@ def test_vcmd_bias_matches_oracle():
     say("  after 301 steps: state %d, visits %s" % (vs, counts))
     assert 0 <= vs < 3 and counts.sum() <= 301
     s.close()
+
+
+def test_energy_log_stride_keeps_trajectory_and_logged_energies():
+    """Pair energies are evaluated only on logged steps (forces every step):
+    the trajectory and the energies of the logged steps are bitwise those of
+    a run that logs every step."""
+    ls = gen.build_system(1200, seed=44)
+    cfg = dict(BASE, **{"nsgrid-update-intvl": 5, "log-cap": 64})
+    say("log stride", ls.n_atoms)
+    runs = []
+    for stride in (1, 4):
+        s = System(ls, dict(cfg, **{"log-stride": stride}))
+        s.run(24)
+        runs.append((s.state()[0], s.energy_log()))
+        s.close()
+    (c1, l1), (c4, l4) = runs
+    assert np.array_equal(c1, c4)
+    assert len(l4) == 6 and np.array_equal(l4, l1[::4])
