@@ -2,16 +2,28 @@
 // and bond terms, fixed-point gradient reduction and energy reductions.
 //
 // Pair kernel (reference: kernel_pairwise_ljzd + cal_pair, cuda_setup.cu:487-675;
-// CPU: SubBox::calc_energy_pairwise, SubBox.cpp:956-1125): one wave64 per
-// cell1; every cell pair is one 8x8 tile with lane = a1 + 8*a2, so one pass
-// over 64 lanes covers the tile (the reference needs two half passes of a
-// warp32).  FP32 pair math, j-gradients reduced over the 8 i-lanes with DPP
-// and added as 64-bit fixed point, i-gradients kept in registers across the
-// cell1's pairs, energies accumulated per lane in FP64 and reduced per cell
-// in a fixed order (deterministic, no floating-point atomics).
+// CPU: SubBox::calc_energy_pairwise, SubBox.cpp:956-1125).  It evaluates the
+// same pairs as the reference -- every unmasked atom pair of every listed
+// cell pair closer than the cutoff -- but schedules them for a wave64:
+//
+// * one wave per cell1; its 8 atoms ("i") sit in scalar registers;
+// * filter: the wave reads its cell pairs 8 at a time (lane = 8*pair + j) and
+//   keeps the cell2 atoms ("rows") within the cutoff of the cell1 bounding
+//   box, appending them (shifted to the pair's periodic image) to a per-wave
+//   LDS queue.  About half of the rows of a 13 A list survive at 12 A;
+// * rows: every 64 queued rows are one pass with lane = row, looping over
+//   the 8 i atoms.  The row's j-gradient stays in its lane (no cross-lane
+//   reduction) and is added once as 64-bit fixed point; each lane keeps 24
+//   partial i-gradients, reduced over the wave once per cell1.
+//
+// FP32 pair math as the reference, fixed-point atomics (order-independent,
+// so every run gives the same bits), energies summed per lane in FP64 and
+// reduced per cell in a fixed order.
 #include "og_internal.h"
 
 #define OG_CHARGE_COEFF 332.06378f   // CelesteObject.cpp:36
+#define OG_QCAP 128                  // per-wave row queue (two passes of 64)
+#define OG_LJ_LDS_MAX 4096           // n_types^2 at most this: LJ table in LDS
 
 typedef float og_f4 __attribute__((ext_vector_type(4)));
 typedef float og_f2 __attribute__((ext_vector_type(2)));
@@ -27,37 +39,135 @@ __device__ __forceinline__ float og_erfc_as(float x, float expmx2) {
     return p * t * expmx2;
 }
 
-// Adds one tile's j-gradient.  After the 8-lane butterfly every lane of a
-// j row holds the row's (x, y, z) sums bit for bit, so lane k < 3 of the row
-// converts and adds component k: one conversion and one atomic instruction
-// serve all three components of the 8 j atoms (24 active lanes).
-// The largest converted magnitude is kept in vmax and checked once per cell
-// against the conversion range; fmaxf drops a NaN operand, so a NaN is
-// caught instead by the running sum vsum.
-__device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, int a2, int i, float rx, float ry,
-                                           float rz, float &vmax, float &vsum) {
-    const float     v  = i == 0 ? rx : (i == 1 ? ry : rz);
-    const long long fv = og_fx_fast(-(double)v);
-    if (i < 3 && fv != 0) og_atomic_add_fx_g(FX + i * cap + a2, fv);
-    vmax = fmaxf(vmax, fabsf(v));
-    vsum += v;
+__device__ __forceinline__ float og_rlane(float v, int k) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// Periodic shift of a tile's j cell (image bits of MiniCell::add_cell_pair,
-// MiniCell.cpp:1174-1187: neg bit -> -L, pos bit -> +L) as integer selects on
-// wave-uniform values, so it stays on the scalar unit.
-__device__ __forceinline__ float og_img_shift(unsigned img, unsigned neg, unsigned pos, float L) {
-    const int Lb = __builtin_amdgcn_readfirstlane(__float_as_int(L));
-    const int b  = (img & neg) ? (Lb ^ (int)0x80000000) : ((img & pos) ? Lb : 0);
-    return __int_as_float(__builtin_amdgcn_readfirstlane(b));
+// Cell1 data held by the wave: atom coordinates, charge * 332.06378, LJ row
+// (byte offset of the type's row in the LDS table, or its index in the
+// global table) and the bounding box.
+struct og_cell1 {
+    float    x[OG_NAC], y[OG_NAC], z[OG_NAC], q[OG_NAC];
+    unsigned row[OG_NAC];
+    float    lo[3], hi[3];
+};
+
+// Fixed-point add of a lane's gradient (x, y, z) at grid slot a (the
+// j-gradient enters with a minus sign).  Zero contributions are skipped.
+__device__ __forceinline__ void og_add_fx3(OG_GLOBAL(long long) *FX, int cap, int a, double sx, double sy,
+                                           double sz) {
+    const long long vx = og_fx_fast(sx), vy = og_fx_fast(sy), vz = og_fx_fast(sz);
+    if (vx != 0) og_atomic_add_fx_g(FX + a, vx);
+    if (vy != 0) og_atomic_add_fx_g(FX + cap + a, vy);
+    if (vz != 0) og_atomic_add_fx_g(FX + 2 * cap + a, vz);
+}
+
+// One pass over up to 64 queued rows (lane = row; lanes >= cnt idle).
+// m8: the row's excluded pairs (bit i set = skip atom i).
+template <int ALPHA, bool ENERGY, bool LJ_LDS>
+__device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, const og_f4 pj, const unsigned tj,
+                                        const unsigned m8, const int a2, const bool live,
+                                        const OG_GLOBAL(og_f2) *__restrict__ LJP, const og_f2 *__restrict__ ljs,
+                                        float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
+                                        double &ee, double &vir) {
+    const float rc2 = s.cutoff2;
+    const float zcore = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
+    const float alpha = s.alpha, piewald = s.piewald;
+    float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, fe = 0.0f, fq = 0.0f, fv = 0.0f;
+    float px = pj.x, py = pj.y, pz = pj.z, pw = pj.w;
+#pragma unroll
+    for (int i = 0; i < OG_NAC; i++) {
+        const float dx = c.x[i] - px;
+        const float dy = c.y[i] - py;
+        const float dz = c.z[i] - pz;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
+        npair += __builtin_popcountll(__ballot(ok));
+        const og_f2 lj = LJ_LDS ? *(const og_f2 *)((const char *)ljs + c.row[i] + tj) : LJP[c.row[i] + tj];
+        const float rinv  = __builtin_amdgcn_rsqf(r2);
+        const float r2inv = rinv * rinv;
+        const float r6inv = r2inv * r2inv * r2inv;
+        const float t6    = lj.x * r6inv;
+        const float t12   = lj.y * r6inv * r6inv;
+        const float cc    = c.q[i] * pw;
+        float       g, eel;
+        if (ALPHA == 0) {
+            eel = cc * (rinv - zcore + bcoeff * r2);
+            g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv - fcoeff);
+        } else {
+            const float r   = r2 * rinv;
+            const float ar  = alpha * r;
+            const float ex  = __expf(-ar * ar);
+            const float erc = og_erfc_as(ar, ex);
+            eel = cc * (rinv * erc - zcore + bcoeff * r2);
+            g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (erc * rinv * r2inv + piewald * ex * r2inv - fcoeff);
+        }
+        // lanes without a pair (masked, out of range, idle) contribute exact zeros
+        g                 = ok ? g : 0.0f;
+        fjx               = fmaf(g, dx, fjx);
+        fjy               = fmaf(g, dy, fjy);
+        fjz               = fmaf(g, dz, fjz);
+        fi[3 * i]         = fmaf(g, dx, fi[3 * i]);
+        fi[3 * i + 1]     = fmaf(g, dy, fi[3 * i + 1]);
+        fi[3 * i + 2]     = fmaf(g, dz, fi[3 * i + 2]);
+        if (ENERGY) {
+            fe += ok ? (t12 - t6) : 0.0f;
+            fq += ok ? eel : 0.0f;
+            fv = fmaf(g, r2, fv);
+        }
+        // finish atom i here: without this the compiler defers the
+        // accumulations and keeps all 8 atoms' distances live
+        asm volatile("" : "+v"(fjx), "+v"(fjy), "+v"(fjz), "+v"(fi[3 * i]), "+v"(fi[3 * i + 1]), "+v"(fi[3 * i + 2]),
+                     "+v"(px), "+v"(py), "+v"(pz), "+v"(pw), "+s"(npair));
+        if constexpr (ENERGY) asm volatile("" : "+v"(fe), "+v"(fq), "+v"(fv));
+    }
+    if (live) {
+        OG_GLOBAL(long long) *FX = (OG_GLOBAL(long long) *)s.fx;
+        og_add_fx3(FX, s.cap_atom_array, a2, -(double)fjx, -(double)fjy, -(double)fjz);
+    }
+    // range check of the converted values; fmaxf drops NaN, the sum keeps it
+    vmax = fmaxf(vmax, fmaxf(fabsf(fjx), fmaxf(fabsf(fjy), fabsf(fjz))));
+    vsum += fjx + fjy + fjz;
+    if (ENERGY) {
+        ev += (double)fe;
+        ee += (double)fq;
+        vir += (double)fv;
+    }
+}
+
+// Pass over queue entries [q0, q0 + cnt).
+template <int ALPHA, bool ENERGY, bool LJ_LDS>
+__device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, const og_f4 *qp, const int2 *qa,
+                                        const int q0, const int cnt, const OG_GLOBAL(og_f2) *__restrict__ LJP,
+                                        const og_f2 *__restrict__ ljs, float (&fi)[3 * OG_NAC], int &npair,
+                                        float &vmax, float &vsum, double &ev, double &ee, double &vir) {
+    const int  lane = threadIdx.x & 63;
+    const int  slot = (q0 + lane) & (OG_QCAP - 1);
+    const bool live = lane < cnt;
+    og_f4      pj   = qp[slot];
+    const int2 ax   = qa[slot];
+    // idle lanes (stale queue slots): finite coordinates, LJ column 0, every pair masked
+    if (!live) pj = og_f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const unsigned m8 = live ? ((unsigned)ax.y >> 24) : 0xffu;
+    const unsigned tj = live ? ((unsigned)ax.y & 0xffffffu) : 0u;
+    og_rows<ALPHA, ENERGY, LJ_LDS>(s, c, pj, tj, m8, ax.x, live, LJP, ljs, fi, npair, vmax, vsum, ev, ee, vir);
+}
+
+// a <- the kept half of (a, b) summed with the partner lane's other half
+// (partner = lane ^ off, off = 32 or 16); "upper" lanes keep b.
+__device__ __forceinline__ float og_keep_add(float a, float b, bool upper, int off) {
+    const float keep = upper ? b : a;
+    const float send = upper ? a : b;
+    return keep + __shfl_xor(send, off, 64);
 }
 
 // One cell1 (one wave).  ENERGY = false on steps whose energies are not
 // logged: forces and the pair count only.
-template <int ALPHA, bool ENERGY> __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1) {
+template <int ALPHA, bool ENERGY, bool LJ_LDS>
+__device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, const og_f4 *__restrict__ shift,
+                                             og_f4 *qp, int2 *qa, const og_f2 *__restrict__ ljs) {
     const int lane = threadIdx.x & 63;
-    const int i    = lane & 7;
-    const int jl   = lane >> 3;
+    const int jl   = lane & 7;
 
     const OG_GLOBAL(og_f4) *__restrict__ X      = (const OG_GLOBAL(og_f4) *)s.gxyzq;
     const OG_GLOBAL(int) *__restrict__ TY       = (const OG_GLOBAL(int) *)s.gtype;
@@ -65,126 +175,101 @@ template <int ALPHA, bool ENERGY> __device__ __forceinline__ void og_pair_cell(c
     const OG_GLOBAL(unsigned) *__restrict__ CP  = (const OG_GLOBAL(unsigned) *)s.cp_c2;
     const OG_GLOBAL(uint64_t) *__restrict__ CM  = (const OG_GLOBAL(uint64_t) *)s.cp_mask;
     OG_GLOBAL(long long) *__restrict__ FX       = (OG_GLOBAL(long long) *)s.fx;
-    const int   cap    = s.cap_atom_array;
-    const int   nt     = s.n_types;
-    const float rc2    = s.cutoff2;
-    const float zcore  = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
-    const float alpha  = s.alpha, piewald = s.piewald;
-    const float L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
-    const int   p0 = __builtin_amdgcn_readfirstlane(s.cp_head[c1]);
-    const int   p1 = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
+    const int   cap = s.cap_atom_array;
+    const int   nt  = s.n_types;
+    const int   p0  = __builtin_amdgcn_readfirstlane(s.cp_head[c1]);
+    const int   p1  = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
+    // keep-test radius: the cutoff plus 10 ppm, so float rounding of the box
+    // distance can never drop a row that holds an in-range pair
+    const float rcb2 = s.cutoff2 * 1.00001f;
 
-    const og_f4  pi   = X[c1 * OG_NAC + i];
-    const unsigned trow = (unsigned)(TY[c1 * OG_NAC + i] * nt);
-    const float  qi   = pi.w * OG_CHARGE_COEFF;
-
-    double dfx = 0.0, dfy = 0.0, dfz = 0.0, ev = 0.0, ee = 0.0, vir = 0.0;
-    int    npair = 0;  // in-cutoff unmasked pairs of this cell1 (wave-uniform)
-    float  vmax = 0.0f, vsum = 0.0f;  // j-gradient range check (see og_flush_j)
-
-    for (int base = p0; base < p1; base += 64) {
-        const int nb = min(64, p1 - base);
-        // 64 cell-pair records per vector load; broadcast with readlane
-        const unsigned cpv = lane < nb ? CP[base + lane] : 0u;
-        const uint64_t mkv = lane < nb ? CM[base + lane] : ~0ull;
-        const unsigned mlo = (unsigned)mkv, mhi = (unsigned)(mkv >> 32);
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f, fe = 0.0f, fq = 0.0f, fv = 0.0f;
-        unsigned c2n = __builtin_amdgcn_readlane(cpv, 0);
-        int      a2n = (int)(c2n & 0xffffffu) * OG_NAC + jl;
-        og_f4    pjn = X[a2n];
-        int      tjn = TY[a2n];
-        for (int k = 0; k < nb; k++) {
-            const unsigned c2i  = c2n;
-            const int      a2   = a2n;
-            const og_f4    pj   = pjn;
-            const int      tj   = tjn;
-            if (k + 1 < nb) {  // prefetch the next tile's j atoms
-                c2n = __builtin_amdgcn_readlane(cpv, k + 1);
-                a2n = (int)(c2n & 0xffffffu) * OG_NAC + jl;
-                pjn = X[a2n];
-                tjn = TY[a2n];
-            }
-            // readlane returns int: widen through uint32_t so the low word is not sign-extended
-            const uint64_t mask = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, k) << 32) |
-                                  (uint64_t)(uint32_t)__builtin_amdgcn_readlane(mlo, k);
-            // image shift of the tile: wave-uniform, selected on the scalar unit
-            const unsigned img = c2i >> 24;
-            const float    sx  = og_img_shift(img, 1u, 2u, L0);
-            const float    sy  = og_img_shift(img, 4u, 8u, L1);
-            const float    sz  = og_img_shift(img, 16u, 32u, L2);
-            const float    dx  = pi.x - (pj.x + sx);
-            const float    dy  = pi.y - (pj.y + sy);
-            const float    dz  = pi.z - (pj.z + sz);
-            const float    r2  = dx * dx + dy * dy + dz * dz;
-            const bool     excl = __builtin_amdgcn_inverse_ballot_w64(mask);
-            const bool     ok   = !excl && (r2 < rc2);
-            const uint64_t okb  = __ballot(ok);
-            npair += __builtin_popcountll(okb);
-            // About 40% of the tiles of a 13 A list hold no pair inside the
-            // 12 A cutoff: skip them.  Their contributions are exact zeros, so
-            // the result is bit-identical to evaluating them.
-            if (okb == 0) continue;
-            const og_f2  lj  = LJP[trow + (unsigned)tj];  // {A6, A12}, one 8-byte load
-            const float  a6  = lj.x;
-            const float  a12 = lj.y;
-            const float    r2s  = ok ? r2 : 1.0f;
-            const float    rinv  = __builtin_amdgcn_rsqf(r2s);
-            const float    r2inv = rinv * rinv;
-            const float    r6inv = r2inv * r2inv * r2inv;
-            const float    t6    = a6 * r6inv;
-            const float    t12   = a12 * r6inv * r6inv;
-            const float    cc    = qi * pj.w;
-            float          g, eel;
-            if (ALPHA == 0) {
-                eel = cc * (rinv - zcore + bcoeff * r2s);
-                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv - fcoeff);
-            } else {
-                const float r   = r2s * rinv;
-                const float ar  = alpha * r;
-                const float ex  = __expf(-ar * ar);
-                const float erc = og_erfc_as(ar, ex);
-                eel = cc * (rinv * erc - zcore + bcoeff * r2s);
-                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (erc * rinv * r2inv + piewald * ex * r2inv - fcoeff);
-            }
-            g              = ok ? g : 0.0f;
-            const float wx = g * dx, wy = g * dy, wz = g * dz;
-            fx += wx;
-            fy += wy;
-            fz += wz;
-            if (ENERGY) {
-                fe += ok ? (t12 - t6) : 0.0f;
-                fq += ok ? eel : 0.0f;
-                fv += g * r2s;
-            }
-            const float rjx = og_dpp_sum8(wx);
-            const float rjy = og_dpp_sum8(wy);
-            const float rjz = og_dpp_sum8(wz);
-            og_flush_j(FX, cap, a2, i, rjx, rjy, rjz, vmax, vsum);
+    og_cell1 c;
+    {
+        const og_f4 p  = X[c1 * OG_NAC + jl];
+        const int   ty = TY[c1 * OG_NAC + jl];
+        const float qc = p.w * OG_CHARGE_COEFF;
+#pragma unroll
+        for (int i = 0; i < OG_NAC; i++) {
+            c.x[i]   = og_rlane(p.x, i);
+            c.y[i]   = og_rlane(p.y, i);
+            c.z[i]   = og_rlane(p.z, i);
+            c.q[i]   = og_rlane(qc, i);
+            const unsigned t = (unsigned)__builtin_amdgcn_readlane(ty, i) * (unsigned)nt;
+            c.row[i] = LJ_LDS ? t * (unsigned)sizeof(og_f2) : t;
         }
-        dfx += (double)fx;
-        dfy += (double)fy;
-        dfz += (double)fz;
-        if (ENERGY) {
-            ev += (double)fe;
-            ee += (double)fq;
-            vir += (double)fv;
+        c.lo[0] = c.hi[0] = c.x[0];
+        c.lo[1] = c.hi[1] = c.y[0];
+        c.lo[2] = c.hi[2] = c.z[0];
+#pragma unroll
+        for (int i = 1; i < OG_NAC; i++) {
+            c.lo[0] = fminf(c.lo[0], c.x[i]), c.hi[0] = fmaxf(c.hi[0], c.x[i]);
+            c.lo[1] = fminf(c.lo[1], c.y[i]), c.hi[1] = fmaxf(c.hi[1], c.y[i]);
+            c.lo[2] = fminf(c.lo[2], c.z[i]), c.hi[2] = fmaxf(c.hi[2], c.z[i]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {  // wave-uniform: keep in scalar registers
+            c.lo[k] = og_rlane(c.lo[k], 0);
+            c.hi[k] = og_rlane(c.hi[k], 0);
         }
     }
-    for (int off = 8; off < 64; off <<= 1) {
-        dfx += __shfl_xor(dfx, off, 64);
-        dfy += __shfl_xor(dfy, off, 64);
-        dfz += __shfl_xor(dfz, off, 64);
+
+    float  fi[3 * OG_NAC];
+#pragma unroll
+    for (int k = 0; k < 3 * OG_NAC; k++) fi[k] = 0.0f;
+    double ev = 0.0, ee = 0.0, vir = 0.0;
+    int    npair = 0;                    // in-cutoff unmasked pairs of this cell1 (wave-uniform)
+    float  vmax = 0.0f, vsum = 0.0f;     // j-gradient range check
+    int    qn = 0, qd = 0;               // rows queued / rows evaluated (wave-uniform)
+
+    for (int base = p0; base < p1; base += OG_NAC) {
+        // lane = 8 * (cell pair in this group) + j atom
+        const int      t     = base + (lane >> 3);
+        const bool     valid = t < p1;
+        const int      tc    = valid ? t : p1 - 1;
+        const unsigned cp    = CP[tc];
+        const uint64_t mk    = CM[tc];
+        const int      a2    = (int)(cp & 0xffffffu) * OG_NAC + jl;
+        const og_f4    pj    = X[a2];
+        const int      ty    = TY[a2];
+        const og_f4    sh    = shift[(cp >> 24) & 63u];  // periodic image of the cell pair
+        const float    px = pj.x + sh.x, py = pj.y + sh.y, pz = pj.z + sh.z;
+        const float    gx = fmaxf(fmaxf(c.lo[0] - px, px - c.hi[0]), 0.0f);
+        const float    gy = fmaxf(fmaxf(c.lo[1] - py, py - c.hi[1]), 0.0f);
+        const float    gz = fmaxf(fmaxf(c.lo[2] - pz, pz - c.hi[2]), 0.0f);
+        const unsigned m8 = valid ? (unsigned)(mk >> (8 * jl)) & 0xffu : 0xffu;
+        const bool     use = (gx * gx + gy * gy + gz * gz < rcb2) && m8 != 0xffu;
+        const uint64_t ub  = __ballot(use);
+        if (use) {
+            const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(ub >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ub, 0u));
+            const int slot = (qn + rank) & (OG_QCAP - 1);
+            const unsigned tj = LJ_LDS ? (unsigned)ty * (unsigned)sizeof(og_f2) : (unsigned)ty;
+            qp[slot] = og_f4{px, py, pz, pj.w};
+            qa[slot] = make_int2(a2, (int)(tj | (m8 << 24)));
+        }
+        qn += __builtin_popcountll(ub);
+        if (qn - qd >= 64) {
+            og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, 64, LJP, ljs, fi, npair, vmax, vsum, ev, ee, vir);
+            qd += 64;
+        }
     }
+    if (qn > qd) og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, qn - qd, LJP, ljs, fi, npair, vmax, vsum, ev, ee, vir);
+
+    // i-gradients: reduce-scatter the 24 partial sums over the wave.  After
+    // the xor-32/16/8 steps lane L holds atom L>>3's (x, y, z) summed over 8
+    // lanes; an 8-lane butterfly completes the sum.
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    float      h12[12], h6[6], h3[3];
+#pragma unroll
+    for (int k = 0; k < 12; k++) h12[k] = og_keep_add(fi[k], fi[12 + k], b5, 32);
+#pragma unroll
+    for (int k = 0; k < 6; k++) h6[k] = og_keep_add(h12[k], h12[6 + k], b4, 16);
+#pragma unroll
+    for (int k = 0; k < 3; k++) h3[k] = og_dpp_sum8(og_keep_add(h6[k], h6[3 + k], b3, 8));
     if (!(vmax <= (float)OG_FX_LIMIT) || !(fabsf(vsum) < INFINITY)) atomicOr(&s.cnt->flags, OG_F_NAN);
     if (jl == 0) {
-        const int a1 = c1 * OG_NAC + i;
-        if (!(fabs(dfx) + fabs(dfy) + fabs(dfz) <= OG_FX_LIMIT)) atomicOr(&s.cnt->flags, OG_F_NAN);
-        const long long vx = og_fx_fast(dfx), vy = og_fx_fast(dfy), vz = og_fx_fast(dfz);
-        if ((vx | vy | vz) != 0) {
-            og_atomic_add_fx_g(FX + a1, vx);
-            og_atomic_add_fx_g(FX + cap + a1, vy);
-            og_atomic_add_fx_g(FX + 2 * cap + a1, vz);
-        }
+        const int a1 = c1 * OG_NAC + (lane >> 3);
+        if (!(fabsf(h3[0]) + fabsf(h3[1]) + fabsf(h3[2]) <= (float)OG_FX_LIMIT)) atomicOr(&s.cnt->flags, OG_F_NAN);
+        og_add_fx3(FX, cap, a1, (double)h3[0], (double)h3[1], (double)h3[2]);
     }
     if (!ENERGY) return;
     for (int off = 1; off < 64; off <<= 1) {
@@ -200,15 +285,33 @@ template <int ALPHA, bool ENERGY> __device__ __forceinline__ void og_pair_cell(c
     }
 }
 
-template <int ALPHA>
+template <int ALPHA, bool LJ_LDS>
 __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    extern __shared__ og_f2 ljs[];                         // [n_types^2] when LJ_LDS
+    __shared__ og_f4        shift[64];                     // image code -> shift of cell2
+    __shared__ og_f4        qpos[OG_THREADS / 64][OG_QCAP];  // row queue: shifted x, y, z, charge
+    __shared__ int2         qaux[OG_THREADS / 64][OG_QCAP];  // grid slot, LJ column | mask byte << 24
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
-    const int c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
-    if (c1 >= s.cnt->n_cells || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
-    if (og_energy_step(s)) og_pair_cell<ALPHA, true>(s, c1);
-    else og_pair_cell<ALPHA, false>(s, c1);
+    // block-uniform exits (one system per block)
+    if (og_frozen(s) || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
+    if (LJ_LDS) {
+        const OG_GLOBAL(og_f2) *LJP = (const OG_GLOBAL(og_f2) *)s.lj_pair;
+        for (int t = threadIdx.x; t < s.n_types * s.n_types; t += OG_THREADS) ljs[t] = LJP[t];
+    }
+    if (threadIdx.x < 64) {
+        // image bits of MiniCell::add_cell_pair (MiniCell.cpp:1174-1187): neg bit -> -L, pos bit -> +L
+        const unsigned img = threadIdx.x;
+        const float    L0 = s.L[0], L1 = s.L[1], L2 = s.L[2];
+        shift[img] = og_f4{(img & 1u) ? -L0 : ((img & 2u) ? L0 : 0.0f), (img & 4u) ? -L1 : ((img & 8u) ? L1 : 0.0f),
+                           (img & 16u) ? -L2 : ((img & 32u) ? L2 : 0.0f), 0.0f};
+    }
+    __syncthreads();
+    const int w  = threadIdx.x >> 6;
+    const int c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + w);
+    if (c1 >= s.cnt->n_cells) return;
+    if (og_energy_step(s)) og_pair_cell<ALPHA, true, LJ_LDS>(s, c1, shift, qpos[w], qaux[w], ljs);
+    else og_pair_cell<ALPHA, false, LJ_LDS>(s, c1, shift, qpos[w], qaux[w], ljs);
 }
 
 // ------------------------------------------------------------ misc terms --
@@ -401,11 +504,16 @@ __global__ void __launch_bounds__(OG_RED_THREADS) k_energy_reduce(const og_dev_s
 }
 
 // ------------------------------------------------------------- launchers --
-void og_launch_pair(const og_launch &a, int alpha_mode) {
-    if (alpha_mode)
-        hipLaunchKernelGGL(k_pair<1>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
-    else
-        hipLaunchKernelGGL(k_pair<0>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+void og_launch_pair(const og_launch &a, int alpha_mode, int max_types) {
+    const dim3 g(a.n_blocks), b(OG_THREADS);
+    if (max_types * max_types <= OG_LJ_LDS_MAX) {
+        const size_t lds = sizeof(og_f2) * (size_t)max_types * (size_t)max_types;
+        if (alpha_mode) hipLaunchKernelGGL((k_pair<1, true>), g, b, lds, a.stream, a.sys, a.map);
+        else hipLaunchKernelGGL((k_pair<0, true>), g, b, lds, a.stream, a.sys, a.map);
+    } else {
+        if (alpha_mode) hipLaunchKernelGGL((k_pair<1, false>), g, b, 0, a.stream, a.sys, a.map);
+        else hipLaunchKernelGGL((k_pair<0, false>), g, b, 0, a.stream, a.sys, a.map);
+    }
 }
 void og_launch_forces_to_work(const og_launch &a) {
     hipLaunchKernelGGL(k_forces_to_work, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
