@@ -191,7 +191,6 @@ struct og_exec {
     std::map<long long, hipGraphExec_t> graphs;
     std::vector<hipEvent_t>  ev;          // pair-kernel timing events
     int                      alpha_mode = 0;
-    int                      max_types  = 0;     // largest LJ type count of the members
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false;
     int                      intvl = 1;
@@ -236,7 +235,6 @@ og_status exec_prepare(og_exec &x) {
     std::vector<og_dev_sys> hs(ns);
     std::vector<int2>       maps[KIND_N];
     x.alpha_mode   = x.members[0]->alpha_mode;
-    x.max_types    = 0;
     x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = false;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
@@ -251,7 +249,6 @@ og_status exec_prepare(og_exec &x) {
         }
         hs[si] = s->h;
         const og_dev_sys &d = s->h;
-        x.max_types = std::max(x.max_types, d.n_types);
         x.any_langevin |= d.integrator == 1;
         x.any_leapfrog |= d.integrator == 0;
         x.any_com |= d.com_cancel && d.n_com_groups > 0;
@@ -306,7 +303,7 @@ void enqueue_rebuild(og_exec &x) {
 
 void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1) {
     if (e0) hipEventRecord(e0, x.stream);
-    og_launch_pair(L(x, KIND_CELLW), x.alpha_mode, x.max_types);
+    og_launch_pair(L(x, KIND_CELLW), x.alpha_mode);
     if (e1) hipEventRecord(e1, x.stream);
     og_launch_forces_to_work(L(x, KIND_ATOM));
     og_launch_energy_reduce(L(x, KIND_SYS));

@@ -482,48 +482,71 @@ __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict
 // ---------------------------------------------------------------- masks --
 // MiniCell::set_cell_pair_bitmask (MiniCell.cpp:1198-1237): bit a2*8+a1 set
 // (= skip) for a dummy atom, the lower triangle of a self pair, or an nb15off
-// partner.  One wave per cell1; each lane builds the whole 64-bit mask of one
-// of the cell's pairs, so 64 pairs are done at once.  nb15off is scanned only
-// when both atoms lie in the same exclusion component (same molecule).
+// partner (the partner list read up to its first -1).  One wave per cell1.
+// The exclusion bits are found from the atoms' side: the cell's partner
+// lists are read once (lane = 8 * atom + partner) and each partner's grid
+// slot becomes a (cell2, bit) entry in LDS; each lane then builds the masks
+// of 64 of the cell's pairs from the dummy flags, the self-pair triangle and
+// the entries whose cell2 matches.  Same bits as testing every atom pair.
 __global__ void __launch_bounds__(OG_THREADS) k_masks(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
-    const int2        bm   = map[blockIdx.x];
-    const og_dev_sys &s    = S[bm.x];
+    __shared__ int exq[OG_THREADS / 64][OG_NAC * OG_MAX_NB15];  // (cell2 << 6) | bit, per wave
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
     if (og_frozen(s)) return;
-    const int         lane = threadIdx.x & 63;
-    const int         c1   = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int c1   = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + w);
     if (c1 >= s.cnt->n_cells || s.cnt->n_cell_pairs == 0) return;
-    const int p0 = s.cp_head[c1], p1 = s.cp_head[c1 + 1];
-    // the 8 atoms of cell1, wave-uniform
-    const int idl = s.atomids[c1 * OG_NAC + (lane & 7)];
-    const int cml = s.gcomp[c1 * OG_NAC + (lane & 7)];
-    int       id1[OG_NAC], cm1[OG_NAC];
-    for (int a = 0; a < OG_NAC; a++) {
-        id1[a] = __builtin_amdgcn_readlane(idl, a);
-        cm1[a] = __builtin_amdgcn_readlane(cml, a);
-    }
-    for (int p = p0 + lane; p < p1; p += 64) {
-        const int c2 = s.cp_c2[p] & 0xffffff;
-        uint64_t  m  = 0;
-        for (int a2 = 0; a2 < OG_NAC; a2++) {
-            const int id2 = s.atomids[c2 * OG_NAC + a2];
-            const int cm2 = s.gcomp[c2 * OG_NAC + a2];
-            for (int a1 = 0; a1 < OG_NAC; a1++) {
-                bool skip = id1[a1] < 0 || id2 < 0 || (c1 == c2 && a1 >= a2);
-                if (!skip && cm1[a1] == cm2) {
-                    const int *l = s.nb15off + (size_t)id1[a1] * OG_MAX_NB15;
-                    for (int k = 0; k < OG_MAX_NB15; k++) {
-                        const int v = l[k];
-                        if (v == -1) break;
-                        if (v == id2) {
-                            skip = true;
-                            break;
-                        }
-                    }
-                }
-                m |= (uint64_t)skip << (a2 * OG_NAC + a1);
-            }
+    int      *ex = exq[w];
+    const int p0 = __builtin_amdgcn_readfirstlane(s.cp_head[c1]);
+    const int p1 = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
+
+    // exclusion entries: lane = 8 * a1 + (partner index within this round of 8)
+    const int a1  = lane >> 3;
+    const int id1 = s.atomids[c1 * OG_NAC + a1];
+    bool      open = id1 >= 0;  // a1's list has not ended yet
+    int       nex  = 0;
+    for (int kb = 0; kb < OG_MAX_NB15; kb += 8) {
+        const int      k    = kb + (lane & 7);
+        const int      t    = open ? s.nb15off[(size_t)id1 * OG_MAX_NB15 + k] : -1;
+        const uint64_t neg  = __ballot(open && t < 0);
+        const unsigned term = (unsigned)(neg >> (8 * a1)) & 0xffu;  // -1 entries of a1 in this round
+        const bool     keep = open && (lane & 7) < (term ? __builtin_ctz(term) : 8);
+        open                = open && term == 0;
+        const uint64_t kbal = __ballot(keep);
+        if (keep) {
+            const int g2   = s.rev[t];
+            const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(kbal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)kbal, 0u));
+            ex[nex + rank] = ((g2 >> 3) << 6) | ((g2 & 7) << 3) | a1;
         }
-        s.cp_mask[p] = m;
+        nex += __builtin_popcountll(kbal);
+        if (__ballot(open) == 0) break;
+    }
+
+    // dummy atoms of cell1: bit a1 of every row byte
+    const int      idl  = s.atomids[c1 * OG_NAC + (lane & 7)];
+    const unsigned dmy1 = (unsigned)__ballot(idl < 0) & 0xffu;
+    const uint64_t cols = (uint64_t)dmy1 * 0x0101010101010101ull;
+    // self pair: a1 >= a2, i.e. bits a2..7 of row byte a2
+    uint64_t self = 0;
+    for (int a2 = 0; a2 < OG_NAC; a2++) self |= (uint64_t)((0xffu << a2) & 0xffu) << (8 * a2);
+    const int4 *A4 = (const int4 *)s.atomids;
+    for (int base = p0; base < p1; base += 64) {
+        const int p = base + lane;
+        if (p < p1) {
+            const int  c2 = s.cp_c2[p] & 0xffffff;
+            const int4 u = A4[2 * c2], v = A4[2 * c2 + 1];
+            uint64_t   m = cols | (c2 == c1 ? self : 0ull);
+            // dummy atoms of cell2: the whole row byte
+            m |= (u.x < 0 ? 0xffull : 0ull) | (u.y < 0 ? 0xffull << 8 : 0ull) | (u.z < 0 ? 0xffull << 16 : 0ull) |
+                 (u.w < 0 ? 0xffull << 24 : 0ull) | (v.x < 0 ? 0xffull << 32 : 0ull) |
+                 (v.y < 0 ? 0xffull << 40 : 0ull) | (v.z < 0 ? 0xffull << 48 : 0ull) |
+                 (v.w < 0 ? 0xffull << 56 : 0ull);
+            for (int e = 0; e < nex; e++) {
+                const int q = ex[e];
+                if ((q >> 6) == c2) m |= 1ull << (q & 63);
+            }
+            s.cp_mask[p] = m;
+        }
     }
 }
 

@@ -23,7 +23,6 @@
 
 #define OG_CHARGE_COEFF 332.06378f   // CelesteObject.cpp:36
 #define OG_QCAP 128                  // per-wave row queue (two passes of 64)
-#define OG_LJ_LDS_MAX 4096           // n_types^2 at most this: LJ table in LDS
 
 typedef float og_f4 __attribute__((ext_vector_type(4)));
 typedef float og_f2 __attribute__((ext_vector_type(2)));
@@ -64,10 +63,10 @@ __device__ __forceinline__ void og_add_fx3(OG_GLOBAL(long long) *FX, int cap, in
 
 // One pass over up to 64 queued rows (lane = row; lanes >= cnt idle).
 // m8: the row's excluded pairs (bit i set = skip atom i).
-template <int ALPHA, bool ENERGY, bool LJ_LDS>
+template <int ALPHA, bool ENERGY>
 __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, const og_f4 pj, const unsigned tj,
                                         const unsigned m8, const int a2, const bool live,
-                                        const OG_GLOBAL(og_f2) *__restrict__ LJP, const og_f2 *__restrict__ ljs,
+                                        const OG_GLOBAL(og_f2) *__restrict__ LJP,
                                         float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
                                         double &ee, double &vir) {
     const float rc2 = s.cutoff2;
@@ -83,7 +82,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
         npair += __builtin_popcountll(__ballot(ok));
-        const og_f2 lj = LJ_LDS ? *(const og_f2 *)((const char *)ljs + c.row[i] + tj) : LJP[c.row[i] + tj];
+        const og_f2 lj = LJP[c.row[i] + tj];  // {A6, A12}, one 8-byte load
         const float rinv  = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
@@ -136,10 +135,10 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
 }
 
 // Pass over queue entries [q0, q0 + cnt).
-template <int ALPHA, bool ENERGY, bool LJ_LDS>
+template <int ALPHA, bool ENERGY>
 __device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, const og_f4 *qp, const int2 *qa,
                                         const int q0, const int cnt, const OG_GLOBAL(og_f2) *__restrict__ LJP,
-                                        const og_f2 *__restrict__ ljs, float (&fi)[3 * OG_NAC], int &npair,
+                                        float (&fi)[3 * OG_NAC], int &npair,
                                         float &vmax, float &vsum, double &ev, double &ee, double &vir) {
     const int  lane = threadIdx.x & 63;
     const int  slot = (q0 + lane) & (OG_QCAP - 1);
@@ -150,7 +149,7 @@ __device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, 
     if (!live) pj = og_f4{0.0f, 0.0f, 0.0f, 0.0f};
     const unsigned m8 = live ? ((unsigned)ax.y >> 24) : 0xffu;
     const unsigned tj = live ? ((unsigned)ax.y & 0xffffffu) : 0u;
-    og_rows<ALPHA, ENERGY, LJ_LDS>(s, c, pj, tj, m8, ax.x, live, LJP, ljs, fi, npair, vmax, vsum, ev, ee, vir);
+    og_rows<ALPHA, ENERGY>(s, c, pj, tj, m8, ax.x, live, LJP, fi, npair, vmax, vsum, ev, ee, vir);
 }
 
 // a <- the kept half of (a, b) summed with the partner lane's other half
@@ -163,9 +162,9 @@ __device__ __forceinline__ float og_keep_add(float a, float b, bool upper, int o
 
 // One cell1 (one wave).  ENERGY = false on steps whose energies are not
 // logged: forces and the pair count only.
-template <int ALPHA, bool ENERGY, bool LJ_LDS>
+template <int ALPHA, bool ENERGY>
 __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, const og_f4 *__restrict__ shift,
-                                             og_f4 *qp, int2 *qa, const og_f2 *__restrict__ ljs) {
+                                             og_f4 *qp, int2 *qa) {
     const int lane = threadIdx.x & 63;
     const int jl   = lane & 7;
 
@@ -195,7 +194,7 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
             c.z[i]   = og_rlane(p.z, i);
             c.q[i]   = og_rlane(qc, i);
             const unsigned t = (unsigned)__builtin_amdgcn_readlane(ty, i) * (unsigned)nt;
-            c.row[i] = LJ_LDS ? t * (unsigned)sizeof(og_f2) : t;
+            c.row[i] = t;
         }
         c.lo[0] = c.hi[0] = c.x[0];
         c.lo[1] = c.hi[1] = c.y[0];
@@ -242,17 +241,17 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         if (use) {
             const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(ub >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ub, 0u));
             const int slot = (qn + rank) & (OG_QCAP - 1);
-            const unsigned tj = LJ_LDS ? (unsigned)ty * (unsigned)sizeof(og_f2) : (unsigned)ty;
+            const unsigned tj = (unsigned)ty;
             qp[slot] = og_f4{px, py, pz, pj.w};
             qa[slot] = make_int2(a2, (int)(tj | (m8 << 24)));
         }
         qn += __builtin_popcountll(ub);
         if (qn - qd >= 64) {
-            og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, 64, LJP, ljs, fi, npair, vmax, vsum, ev, ee, vir);
+            og_pass<ALPHA, ENERGY>(s, c, qp, qa, qd, 64, LJP, fi, npair, vmax, vsum, ev, ee, vir);
             qd += 64;
         }
     }
-    if (qn > qd) og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, qn - qd, LJP, ljs, fi, npair, vmax, vsum, ev, ee, vir);
+    if (qn > qd) og_pass<ALPHA, ENERGY>(s, c, qp, qa, qd, qn - qd, LJP, fi, npair, vmax, vsum, ev, ee, vir);
 
     // i-gradients: reduce-scatter the 24 partial sums over the wave.  After
     // the xor-32/16/8 steps lane L holds atom L>>3's (x, y, z) summed over 8
@@ -285,20 +284,16 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     }
 }
 
-template <int ALPHA, bool LJ_LDS>
+// Four waves (four cells) per block sharing the image table.
+template <int ALPHA>
 __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
-    extern __shared__ og_f2 ljs[];                         // [n_types^2] when LJ_LDS
-    __shared__ og_f4        shift[64];                     // image code -> shift of cell2
-    __shared__ og_f4        qpos[OG_THREADS / 64][OG_QCAP];  // row queue: shifted x, y, z, charge
-    __shared__ int2         qaux[OG_THREADS / 64][OG_QCAP];  // grid slot, LJ column | mask byte << 24
+    __shared__ og_f4 shift[64];                        // image code -> shift of cell2
+    __shared__ og_f4 qpos[OG_THREADS / 64][OG_QCAP];  // row queue: shifted x, y, z, charge
+    __shared__ int2  qaux[OG_THREADS / 64][OG_QCAP];  // grid slot, LJ column | mask byte << 24
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    // block-uniform exits (one system per block)
+    // block-uniform exits (one system per block; this kernel never sets the overflow flag)
     if (og_frozen(s) || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
-    if (LJ_LDS) {
-        const OG_GLOBAL(og_f2) *LJP = (const OG_GLOBAL(og_f2) *)s.lj_pair;
-        for (int t = threadIdx.x; t < s.n_types * s.n_types; t += OG_THREADS) ljs[t] = LJP[t];
-    }
     if (threadIdx.x < 64) {
         // image bits of MiniCell::add_cell_pair (MiniCell.cpp:1174-1187): neg bit -> -L, pos bit -> +L
         const unsigned img = threadIdx.x;
@@ -310,8 +305,8 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
     const int w  = threadIdx.x >> 6;
     const int c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + w);
     if (c1 >= s.cnt->n_cells) return;
-    if (og_energy_step(s)) og_pair_cell<ALPHA, true, LJ_LDS>(s, c1, shift, qpos[w], qaux[w], ljs);
-    else og_pair_cell<ALPHA, false, LJ_LDS>(s, c1, shift, qpos[w], qaux[w], ljs);
+    if (og_energy_step(s)) og_pair_cell<ALPHA, true>(s, c1, shift, qpos[w], qaux[w]);
+    else og_pair_cell<ALPHA, false>(s, c1, shift, qpos[w], qaux[w]);
 }
 
 // ------------------------------------------------------------ misc terms --
@@ -504,16 +499,11 @@ __global__ void __launch_bounds__(OG_RED_THREADS) k_energy_reduce(const og_dev_s
 }
 
 // ------------------------------------------------------------- launchers --
-void og_launch_pair(const og_launch &a, int alpha_mode, int max_types) {
-    const dim3 g(a.n_blocks), b(OG_THREADS);
-    if (max_types * max_types <= OG_LJ_LDS_MAX) {
-        const size_t lds = sizeof(og_f2) * (size_t)max_types * (size_t)max_types;
-        if (alpha_mode) hipLaunchKernelGGL((k_pair<1, true>), g, b, lds, a.stream, a.sys, a.map);
-        else hipLaunchKernelGGL((k_pair<0, true>), g, b, lds, a.stream, a.sys, a.map);
-    } else {
-        if (alpha_mode) hipLaunchKernelGGL((k_pair<1, false>), g, b, 0, a.stream, a.sys, a.map);
-        else hipLaunchKernelGGL((k_pair<0, false>), g, b, 0, a.stream, a.sys, a.map);
-    }
+void og_launch_pair(const og_launch &a, int alpha_mode) {
+    if (alpha_mode)
+        hipLaunchKernelGGL(k_pair<1>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+    else
+        hipLaunchKernelGGL(k_pair<0>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_forces_to_work(const og_launch &a) {
     hipLaunchKernelGGL(k_forces_to_work, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
