@@ -33,43 +33,67 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_clear(const og_dev_sys *__r
 // MiniCell::set_atoms_into_grid_xy binning (MiniCell.cpp:316-359):
 //   float x = crd - lb (double subtraction rounded to float), periodic wrap
 //   in float, column = floor(x / L_cell) with IEEE float division.
+// Wave-aggregated counter add: lanes with the same key share one atomic
+// (atoms in input order are mostly spatial neighbours, so a wave touches a
+// few columns).  Returns the old counter value plus the lane's rank among
+// the lanes of its key, i.e. what a per-lane atomicAdd(ctr + key, 1) could
+// have returned.
+__device__ __forceinline__ int og_wave_count_add(int *ctr, int key, bool active) {
+    const int lane = threadIdx.x & 63;
+    uint64_t  todo = __ballot(active);
+    int       res  = 0;
+    while (todo) {
+        const int      leader = __builtin_ctzll(todo);
+        const int      lkey   = __builtin_amdgcn_readlane(key, leader);
+        const uint64_t m      = __ballot(active && key == lkey) & todo;
+        int            base   = 0;
+        if (lane == leader) base = atomicAdd(ctr + lkey, __builtin_popcountll(m));
+        base = __builtin_amdgcn_readlane(base, leader);
+        if ((m >> lane) & 1ull) res = base + __builtin_popcountll(m & ((1ull << lane) - 1ull));
+        todo &= ~m;
+    }
+    return res;
+}
+
 __global__ void __launch_bounds__(OG_THREADS) k_grid_bin(const og_dev_sys *__restrict__ S,
                                                          const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
     if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
-    if (i >= s.n_atoms) return;
-    int  c[2];
+    const bool        in = i < s.n_atoms;
+    int  c[2] = {0, 0};
     bool bad = false;
-    for (int d = 0; d < 2; d++) {
-        const double cd = s.crd[3 * i + d];
-        if (!(cd == cd) || cd > 1e30 || cd < -1e30) {
-            bad = true;
-            c[d] = 0;
-            continue;
+    if (in) {
+        for (int d = 0; d < 2; d++) {
+            const double cd = s.crd[3 * i + d];
+            if (!(cd == cd) || cd > 1e30 || cd < -1e30) {
+                bad = true;
+                c[d] = 0;
+                continue;
+            }
+            if (cd > (double)s.ub[d] || cd < (double)s.lb[d]) atomicOr(&s.cnt->flags, OG_F_OUTSIDE_BOX);
+            float x = (float)(cd - (double)s.lb[d]);
+            if (x < -4.0f * s.L[d] || x >= 5.0f * s.L[d]) {  // far outside: flag instead of looping
+                atomicOr(&s.cnt->flags, OG_F_OUTSIDE_BOX);
+                x = 0.0f;
+            }
+            while (x < 0.0f) x += s.L[d];
+            while (x >= s.L[d]) x -= s.L[d];
+            int k = (int)floorf(x / s.L_cell[d]);
+            const int n = d == 0 ? s.nx : s.ny;
+            while (k < 0) k += n;
+            while (k >= n) k -= n;
+            c[d] = k;
         }
-        if (cd > (double)s.ub[d] || cd < (double)s.lb[d]) atomicOr(&s.cnt->flags, OG_F_OUTSIDE_BOX);
-        float x = (float)(cd - (double)s.lb[d]);
-        if (x < -4.0f * s.L[d] || x >= 5.0f * s.L[d]) {  // far outside: flag instead of looping
-            atomicOr(&s.cnt->flags, OG_F_OUTSIDE_BOX);
-            x = 0.0f;
-        }
-        while (x < 0.0f) x += s.L[d];
-        while (x >= s.L[d]) x -= s.L[d];
-        int k = (int)floorf(x / s.L_cell[d]);
-        const int n = d == 0 ? s.nx : s.ny;
-        while (k < 0) k += n;
-        while (k >= n) k -= n;
-        c[d] = k;
+        const double z = s.crd[3 * i + 2];
+        if (!(z == z) || z > 1e30 || z < -1e30) bad = true;
+        if (bad) atomicOr(&s.cnt->flags, OG_F_NAN);
+        const int col    = c[0] + c[1] * s.nx;
+        s.col_of_atom[i] = col;
+        s.zkey[i]        = og_float_key(bad ? 0.0f : (float)z);
     }
-    const double z = s.crd[3 * i + 2];
-    if (!(z == z) || z > 1e30 || z < -1e30) bad = true;
-    if (bad) atomicOr(&s.cnt->flags, OG_F_NAN);
-    const int col    = c[0] + c[1] * s.nx;
-    s.col_of_atom[i] = col;
-    s.zkey[i]        = og_float_key(bad ? 0.0f : (float)z);
-    atomicAdd(&s.n_atoms_col[col], 1);
+    og_wave_count_add(s.n_atoms_col, c[0] + c[1] * s.nx, in);
 }
 
 // ----------------------------------------------------------------- scan --
@@ -134,10 +158,11 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_scatter(const og_dev_sys *_
     const og_dev_sys &s  = S[bm.x];
     if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
-    if (i >= s.n_atoms || s.cnt->n_cells == 0) return;
-    const int col  = s.col_of_atom[i];
-    const int slot = atomicAdd(&s.col_fill[col], 1);
-    s.tmp_ids[s.head_atom[col] + slot] = i;
+    if (s.cnt->n_cells == 0) return;
+    const bool in   = i < s.n_atoms;
+    const int  col  = in ? s.col_of_atom[i] : 0;
+    const int  slot = og_wave_count_add(s.col_fill, col, in);
+    if (in) s.tmp_ids[s.head_atom[col] + slot] = i;
 }
 
 __device__ void og_column_cells(const og_dev_sys &s, int col, int nz);
