@@ -63,7 +63,12 @@ def cpu_baseline(ls, cfg, budget_s=12.0):
     return pairs / dt, n_eval, dt
 
 
-WALKER_CFG = {"cutoff": 12.0, "nsgrid-cutoff": 13.0, "nsgrid-update-intvl": 10, "time-step": 2.0,
+# The walker and serving workloads build their cell-pair lists with the z-bin
+# scan over every bin ("gap-bug": 0).  The reference's scan stops at the first
+# uniform-z bin that no cell overlaps (MiniCell.cpp:1063-1077) and so drops
+# interacting cell pairs wherever such a gap opens; over hundreds of steps
+# atoms of a dropped pair drift into each other and the run ends in a NaN.
+WALKER_CFG = {"cutoff": 12.0, "nsgrid-cutoff": 13.0, "nsgrid-update-intvl": 10, "time-step": 2.0, "gap-bug": 0,
               "integrator": "langevin", "constraint": "shake-settle", "const-tolerance": 1e-8,
               "langevin-gamma": 1.0, "temperature": 300.0, "ele-alpha": 0.0, "extended-ensemble": "vcmd",
               "enhance-sigma": 0.5, "enhance-recovery-coef": 1.0}

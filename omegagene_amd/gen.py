@@ -30,6 +30,10 @@ D_OH = 0.9572
 THETA_HOH = 104.52
 D_HH = 2.0 * D_OH * math.sin(math.radians(THETA_HOH) / 2.0)
 SIG_O, EPS_O = 3.1507, 0.1521
+# water hydrogen LJ of the CHARMM TIP3P variant (Rmin/2 0.2245 A): without it a
+# charged solute site can collapse onto a hydrogen (no repulsion against the
+# Coulomb attraction)
+SIG_H, EPS_H = 0.4, 0.046
 Q_O, Q_H = -0.834, 0.417
 M_O, M_H = 15.9994, 1.008
 M_BEAD = 14.027
@@ -142,6 +146,7 @@ def build_system(n_waters: int, seed: int, L: Optional[float] = None, n_beads: i
     sig = np.zeros(n_types)
     eps = np.zeros(n_types)
     sig[0], eps[0] = SIG_O, EPS_O
+    sig[1], eps[1] = SIG_H, EPS_H
     if n_beads:
         sig[2:] = rng.uniform(3.0, 4.0, n_beads)
         eps[2:] = rng.uniform(0.05, 0.2, n_beads)

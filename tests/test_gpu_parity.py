@@ -54,6 +54,24 @@ def test_grid_and_cell_pairs_bitexact_vs_reference_minicell(name):
     s.close()
 
 
+@pytest.mark.parametrize("gap", [0, 1])
+def test_cell_pairs_both_bin_scans_match_oracle(gap):
+    """Generated water + chain system: the list and masks equal the oracle's
+    enumeration with the reference's gap rule (the scan of uniform-z bins stops
+    at the first empty bin) and without it (every bin of the window)."""
+    ls = gen.build_system(2500, seed=17, n_beads=40)
+    say("cell pairs, gap rule", gap, ls.n_atoms)
+    s = System(ls, dict(BASE, **{"gap-bug": gap, "constraint": "none"}))
+    s.nsgrid_update()
+    cp, cph = s.cell_pairs()
+    o = oracle.Oracle(ls.n_atoms, ls.box, BASE["nsgrid-cutoff"])
+    o.build(ls.crd)
+    cps, head = o.enumerate(ls.nb15off_table(), gap_bug=bool(gap))
+    assert np.array_equal(cph, head)
+    assert np.array_equal(cp, cps.view(np.int32).reshape(-1, 5))
+    s.close()
+
+
 def _systems():
     ls0, _ = gen.config0()
     yield "cfg0", ls0, dict(BASE)
