@@ -42,8 +42,9 @@ __device__ __forceinline__ float og_rlane(float v, int k) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), k));
 }
 
-// Cell1 data held by the wave: atom coordinates, charge * 332.06378, byte
-// offset of the atom type's row in the LJ table, and the bounding box.
+// Cell1 data held by the wave: atom coordinates, charge * 332.06378, LJ row
+// (byte offset of the type's row in the LDS table, or its index in the
+// global table) and the bounding box.
 struct og_cell1 {
     float    x[OG_NAC], y[OG_NAC], z[OG_NAC], q[OG_NAC];
     unsigned row[OG_NAC];
@@ -68,9 +69,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
                                         const OG_GLOBAL(og_f2) *__restrict__ LJP,
                                         float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
                                         double &ee, double &vir) {
-    const float    rc2  = s.cutoff2;
-    const unsigned tj8  = tj * (unsigned)sizeof(og_f2);
-    const bool     excl = __builtin_amdgcn_ballot_w64(m8 != 0u) != 0;  // wave-uniform
+    const float rc2 = s.cutoff2;
     const float zcore = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
     const float alpha = s.alpha, piewald = s.piewald;
     float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, fe = 0.0f, fq = 0.0f, fv = 0.0f;
@@ -81,14 +80,9 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         const float dy = c.y[i] - py;
         const float dz = c.z[i] - pz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        // in-cutoff lanes, minus the rows that exclude atom i (tested only
-        // in passes that carry an exclusion)
-        uint64_t okb = __builtin_amdgcn_ballot_w64(r2 < rc2);
-        if (excl) okb &= ~__builtin_amdgcn_ballot_w64(((m8 >> i) & 1u) != 0u);
-        const bool ok = __builtin_amdgcn_inverse_ballot_w64(okb);
-        npair += __builtin_popcountll(okb);
-        // {A6, A12}: SGPR base + 32-bit byte offset
-        const og_f2 lj = *(const OG_GLOBAL(og_f2) *)((const OG_GLOBAL(char) *)LJP + (c.row[i] + tj8));
+        const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
+        npair += __builtin_popcountll(__ballot(ok));
+        const og_f2 lj = LJP[c.row[i] + tj];  // {A6, A12}, one 8-byte load
         const float rinv  = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
@@ -123,7 +117,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         // finish atom i here: without this the compiler defers the
         // accumulations and keeps all 8 atoms' distances live
         asm volatile("" : "+v"(fjx), "+v"(fjy), "+v"(fjz), "+v"(fi[3 * i]), "+v"(fi[3 * i + 1]), "+v"(fi[3 * i + 2]),
-                     "+v"(px), "+v"(py), "+v"(pz), "+v"(pw));
+                     "+v"(px), "+v"(py), "+v"(pz), "+v"(pw), "+s"(npair));
         if constexpr (ENERGY) asm volatile("" : "+v"(fe), "+v"(fq), "+v"(fv));
     }
     if (live) {
@@ -199,7 +193,8 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
             c.y[i]   = og_rlane(p.y, i);
             c.z[i]   = og_rlane(p.z, i);
             c.q[i]   = og_rlane(qc, i);
-            c.row[i] = (unsigned)__builtin_amdgcn_readlane(ty, i) * (unsigned)nt * (unsigned)sizeof(og_f2);
+            const unsigned t = (unsigned)__builtin_amdgcn_readlane(ty, i) * (unsigned)nt;
+            c.row[i] = t;
         }
         c.lo[0] = c.hi[0] = c.x[0];
         c.lo[1] = c.hi[1] = c.y[0];
@@ -230,23 +225,19 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         const int      t     = base + (lane >> 3);
         const bool     valid = t < p1;
         const int      tc    = valid ? t : p1 - 1;
-        // 32-bit byte offsets from the scalar base pointers
-        const unsigned cp = *(const OG_GLOBAL(unsigned) *)((const OG_GLOBAL(char) *)CP + 4u * (unsigned)tc);
-        const uint64_t mk = *(const OG_GLOBAL(uint64_t) *)((const OG_GLOBAL(char) *)CM + 8u * (unsigned)tc);
-        const int      a2 = (int)(cp & 0xffffffu) * OG_NAC + jl;
-        const og_f4    pj = *(const OG_GLOBAL(og_f4) *)((const OG_GLOBAL(char) *)X + 16u * (unsigned)a2);
-        int            ty = *(const OG_GLOBAL(int) *)((const OG_GLOBAL(char) *)TY + 4u * (unsigned)a2);
-        const og_f4    sh = shift[(cp >> 24) & 63u];  // periodic image of the cell pair
+        const unsigned cp    = CP[tc];
+        const uint64_t mk    = CM[tc];
+        const int      a2    = (int)(cp & 0xffffffu) * OG_NAC + jl;
+        const og_f4    pj    = X[a2];
+        const int      ty    = TY[a2];
+        const og_f4    sh    = shift[(cp >> 24) & 63u];  // periodic image of the cell pair
         const float    px = pj.x + sh.x, py = pj.y + sh.y, pz = pj.z + sh.z;
-        asm volatile("" : "+v"(ty));  // the type load completes with the coordinates, not inside the branch
         const float    gx = fmaxf(fmaxf(c.lo[0] - px, px - c.hi[0]), 0.0f);
         const float    gy = fmaxf(fmaxf(c.lo[1] - py, py - c.hi[1]), 0.0f);
         const float    gz = fmaxf(fmaxf(c.lo[2] - pz, pz - c.hi[2]), 0.0f);
         const unsigned m8 = valid ? (unsigned)(mk >> (8 * jl)) & 0xffu : 0xffu;
-        // ballots of the two comparisons (kept in scalar registers)
-        const uint64_t ub  = __builtin_amdgcn_ballot_w64(gx * gx + gy * gy + gz * gz < rcb2) &
-                            __builtin_amdgcn_ballot_w64(m8 != 0xffu);
-        const bool     use = __builtin_amdgcn_inverse_ballot_w64(ub);
+        const bool     use = (gx * gx + gy * gy + gz * gz < rcb2) && m8 != 0xffu;
+        const uint64_t ub  = __ballot(use);
         if (use) {
             const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(ub >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ub, 0u));
             const int slot = (qn + rank) & (OG_QCAP - 1);
