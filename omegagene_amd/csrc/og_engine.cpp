@@ -301,12 +301,14 @@ void enqueue_rebuild(og_exec &x) {
     og_launch_masks(L(x, KIND_CELLW));
 }
 
-void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1) {
+// reduce: reduce the energies now (a single-point call); an MD step reduces
+// them in its closing kernel instead
+void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1, bool reduce = false) {
     if (e0) hipEventRecord(e0, x.stream);
     og_launch_pair(L(x, KIND_CELLW), x.alpha_mode);
     if (e1) hipEventRecord(e1, x.stream);
     og_launch_forces_to_work(L(x, KIND_ATOM));
-    og_launch_energy_reduce(L(x, KIND_SYS));
+    if (reduce) og_launch_energy_reduce(L(x, KIND_SYS));
 }
 
 void enqueue_integrate(og_exec &x) {
@@ -324,7 +326,7 @@ void enqueue_integrate(og_exec &x) {
     if (x.any_langevin) og_launch_langevin_positions(L(x, KIND_ATOM));
     if (x.any_constraint) og_launch_constraints(L(x, KIND_TRIO), L(x, KIND_MISC));
     og_launch_finish(L(x, KIND_ATOM));
-    og_launch_step_end(L(x, KIND_SYS));
+    og_launch_step_close(L(x, KIND_SYS));
 }
 
 // Capture K steps (rebuild at steps whose index within the chunk is a
@@ -897,7 +899,7 @@ og_status og_calc_energy_force(og_system *s, int rebuild, double *work_atom, dou
     const int one = 1, zero = 0;
     OG_HIP(hipMemcpy(&s->d_cnt->force_energy, &one, sizeof(int), hipMemcpyHostToDevice));
     if (rebuild || !s->grid_valid) enqueue_rebuild(x);
-    enqueue_forces(x, nullptr, nullptr);
+    enqueue_forces(x, nullptr, nullptr, true);
     OG_HIP(hipStreamSynchronize(x.stream));
     OG_HIP(hipMemcpy(&s->d_cnt->force_energy, &zero, sizeof(int), hipMemcpyHostToDevice));
     s->grid_valid = true;

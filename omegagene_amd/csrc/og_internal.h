@@ -204,6 +204,53 @@ __device__ __forceinline__ long long og_fx_fast(double v) {
 
 #define OG_GLOBAL(T) __attribute__((address_space(1))) T
 
+#define OG_RED_THREADS 1024  // per-system reduction block
+
+// Pair, excluded-pair and bond energies of one system into s.energy: each of
+// the OG_RED_THREADS threads sums a fixed strided share of the per-cell and
+// per-block partials, then a fixed wave and block tree (deterministic).
+// sh: 6 * OG_RED_THREADS / 64 doubles of shared memory.
+__device__ __forceinline__ void og_reduce_pair_energies(const og_dev_sys &s, double *sh) {
+    const int n = s.cnt->n_cells;
+    double    v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // vdw, ele, virial, pairs, excess, bond
+    const double2 *E = (const double2 *)s.e_cell;
+#pragma unroll 4
+    for (int k = threadIdx.x; k < n; k += OG_RED_THREADS) {
+        const double2 u = E[2 * k], w = E[2 * k + 1];
+        v[0] += u.x;
+        v[1] += u.y;
+        v[2] += w.x;
+        v[3] += w.y;
+    }
+    const double2 *M = (const double2 *)s.e_misc;
+    for (int k = threadIdx.x; k < s.n_misc_blocks; k += OG_RED_THREADS) {
+        const double2 u = M[k];
+        v[4] += u.x;
+        v[5] += u.y;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 6; q++) v[q] += __shfl_down(v[q], off, 64);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 6; q++) sh[6 * wv + q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int w = 0; w < OG_RED_THREADS / 64; w++)
+            for (int q = 0; q < 6; q++) t[q] += sh[6 * w + q];
+        s.energy[OG_E_VDW]    = t[0];
+        s.energy[OG_E_ELE]    = t[1];
+        s.energy[OG_E_VIRIAL] = t[2];
+        s.energy[OG_E_NPAIRS] = t[3];
+        s.energy[OG_E_EXCESS] = t[4];
+        s.energy[OG_E_BOND]   = t[5];
+        s.energy[OG_E_SELF]   = s.e_self;
+    }
+    __syncthreads();
+}
+
+
+
 __device__ __forceinline__ void og_atomic_add_fx(long long *p, long long v) {
     if (v != 0) atomicAdd((unsigned long long *)p, (unsigned long long)v);
 }
@@ -254,6 +301,6 @@ void og_launch_positions(const og_launch &atoms);
 void og_launch_langevin_positions(const og_launch &atoms);
 void og_launch_constraints(const og_launch &trio, const og_launch &misc);
 void og_launch_finish(const og_launch &atoms);
-void og_launch_step_end(const og_launch &sys);
+void og_launch_step_close(const og_launch &sys);  // energies of the step, log entry, step counter
 void og_launch_vcmd(const og_launch &sys);
 void og_launch_refresh_grid(const og_launch &slots);

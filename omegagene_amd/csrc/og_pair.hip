@@ -454,48 +454,14 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
 
 // Deterministic per-system energy reduction: 1024 threads, each with a fixed
 // strided share of the per-cell and per-block partials (16-byte loads, four in
-// flight), then a fixed wave and block tree.
-#define OG_RED_THREADS 1024
+// flight), then a fixed wave and block tree.  Used by single-point force
+// calls; an MD step reduces its energies in k_step_close.
 __global__ void __launch_bounds__(OG_RED_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
                                                                   const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
     if (og_frozen(s) || !og_energy_step(s)) return;
-    const int n = s.cnt->n_cells;
-    double    v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // vdw, ele, virial, pairs, excess, bond
-    const double2 *E = (const double2 *)s.e_cell;
-#pragma unroll 4
-    for (int k = threadIdx.x; k < n; k += OG_RED_THREADS) {
-        const double2 u = E[2 * k], w = E[2 * k + 1];
-        v[0] += u.x;
-        v[1] += u.y;
-        v[2] += w.x;
-        v[3] += w.y;
-    }
-    const double2 *M = (const double2 *)s.e_misc;
-    for (int k = threadIdx.x; k < s.n_misc_blocks; k += OG_RED_THREADS) {
-        const double2 u = M[k];
-        v[4] += u.x;
-        v[5] += u.y;
-    }
-    for (int off = 32; off > 0; off >>= 1)
-        for (int q = 0; q < 6; q++) v[q] += __shfl_down(v[q], off, 64);
     __shared__ double sh[6 * (OG_RED_THREADS / 64)];
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 6; q++) sh[6 * wv + q] = v[q];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int w = 0; w < OG_RED_THREADS / 64; w++)
-            for (int q = 0; q < 6; q++) t[q] += sh[6 * w + q];
-        s.energy[OG_E_VDW]    = t[0];
-        s.energy[OG_E_ELE]    = t[1];
-        s.energy[OG_E_VIRIAL] = t[2];
-        s.energy[OG_E_NPAIRS] = t[3];
-        s.energy[OG_E_EXCESS] = t[4];
-        s.energy[OG_E_BOND]   = t[5];
-        s.energy[OG_E_SELF]   = s.e_self;
-    }
+    og_reduce_pair_energies(s, sh);
 }
 
 // ------------------------------------------------------------- launchers --
