@@ -624,15 +624,21 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     {
         // per-atom lists of the excess pairs and bonds each atom takes part in,
         // in ascending list order (the per-atom gradient sums follow it)
-        auto csr = [&](const int *pairs, int np, int **off_d, int **at_d) -> og_status {
-            std::vector<int> off(n + 1, 0), at(2 * (size_t)np);
+        // (entries carry both atoms and the pair index, one 16-byte load each)
+        auto csr = [&](const int *pairs, int np, int **off_d, int4 **at_d) -> og_status {
+            std::vector<int>  off(n + 1, 0);
+            std::vector<int4> at(2 * (size_t)np);
             for (int t = 0; t < np; t++) off[pairs[2 * t] + 1]++, off[pairs[2 * t + 1] + 1]++;
             for (int i = 0; i < n; i++) off[i + 1] += off[i];
             std::vector<int> fill(off.begin(), off.end() - 1);
-            for (int t = 0; t < np; t++) at[fill[pairs[2 * t]]++] = t, at[fill[pairs[2 * t + 1]]++] = t;
+            for (int t = 0; t < np; t++) {
+                const int4 e            = make_int4(pairs[2 * t], pairs[2 * t + 1], t, 0);
+                at[fill[pairs[2 * t]]++] = e;
+                at[fill[pairs[2 * t + 1]]++] = e;
+            }
             if (!A(off_d, n + 1) || !A(at_d, std::max<size_t>(at.size(), 1))) return OG_E_DEVICE;
             OG_HIP(hipMemcpy(*off_d, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-            if (np) OG_HIP(hipMemcpy(*at_d, at.data(), sizeof(int) * at.size(), hipMemcpyHostToDevice));
+            if (np) OG_HIP(hipMemcpy(*at_d, at.data(), sizeof(int4) * at.size(), hipMemcpyHostToDevice));
             return OG_OK;
         };
         og_status st = csr(d->excess, h.n_excess, &h.exc_off, &h.exc_at);

@@ -103,8 +103,8 @@ struct og_dev_sys {
     int    *excess;         // [n_excess*2]
     int    *bond_ids;       // [n_bonds*2]
     double *bond_k, *bond_r0;
-    int    *exc_off, *exc_at;    // per-atom CSR of excess-pair indices: [n_atoms+1], [2*n_excess]
-    int    *bond_off, *bond_at;  // per-atom CSR of bond indices: [n_atoms+1], [2*n_bonds]
+    int    *exc_off, *bond_off;  // per-atom CSR offsets of the excess pairs / bonds: [n_atoms+1]
+    int4   *exc_at, *bond_at;    // entries {a, b, pair index, 0}: [2*n_excess], [2*n_bonds]
     int    *trio_ids;       // [n_trios*3]
     double *trio_d2;        // [n_trios*3]
     int    *pairc_ids;      // [n_pairs_c*2]

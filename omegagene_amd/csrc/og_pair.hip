@@ -419,8 +419,8 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
             w[k]              = (double)v * OG_FX_INV;
         }
         for (int e = s.exc_off[i]; e < s.exc_off[i + 1]; e++) {
-            const int t = s.exc_at[e];
-            const int a = s.excess[2 * t], b = s.excess[2 * t + 1];
+            const int4 ent = s.exc_at[e];
+            const int  a = ent.x, b = ent.y;
             float     we[3];
             const float ene = og_excess_pair(s, a, b, we);
             if (i == a) {
@@ -431,8 +431,8 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
             }
         }
         for (int e = s.bond_off[i]; e < s.bond_off[i + 1]; e++) {
-            const int t = s.bond_at[e];
-            const int a = s.bond_ids[2 * t], b = s.bond_ids[2 * t + 1];
+            const int4 ent = s.bond_at[e];
+            const int  a = ent.x, b = ent.y, t = ent.z;
             double    wb[3];
             const double ene = og_bond_pair(s, t, a, b, wb);
             if (i == a) {

@@ -122,6 +122,27 @@ def test_forces_energies_vs_oracle(case):
     s.close()
 
 
+def test_forces_at_full_bench_size_vs_oracle():
+    """The 99,200-atom water + chain system of the headline benchmark (one
+    force evaluation): exact in-cutoff pair count, gradient RMS relative error
+    <= 2e-6 against the FP32-faithful oracle, energies as in the small cases,
+    and Newton's third law on the fixed-point sums."""
+    ls, cfg = gen.config1()
+    say("forces, full size", ls.n_atoms)
+    s = System(ls, dict(BASE, **{"cutoff": cfg["cutoff"], "nsgrid-cutoff": cfg["nsgrid-cutoff"]}))
+    w, e = s.energy_force(rebuild=True)
+    wo, eo, _ = oracle_forces(ls, cutoff=cfg["cutoff"], nscut=cfg["nsgrid-cutoff"])
+    err = rms_rel(w, wo)
+    say("  pairs gpu %d oracle %d, gradient rms rel %.3e" % (e["npairs"], eo["n_pairs"], err))
+    assert int(e["npairs"]) == int(eo["n_pairs"])
+    assert err < 2e-6
+    assert e["vdw"] == pytest.approx(eo["vdw"], rel=2e-6, abs=1e-4)
+    S = ele_scale(ls, cfg["cutoff"], 0.0, cfg["nsgrid-cutoff"])
+    assert abs(e["ele"] - eo["ele"]) <= 1e-8 * S
+    assert np.max(np.abs(w.sum(0))) < 1e-5 * np.abs(w).max()
+    s.close()
+
+
 def test_integration_step_bitexact_vs_oracle():
     ls = gen.build_system(1000, seed=88)
     say("integrator", ls.n_atoms)
