@@ -264,10 +264,8 @@ og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_c
     CS_TRY(cs_alloc(&h.cp_c2, max_n_cell_pairs));
     CS_TRY(cs_alloc(&h.cp_mask, max_n_cell_pairs));
     CS_TRY(cs_alloc(&h.fx, 3 * (size_t)cap));
-    CS_TRY(cs_alloc(&h.e_cell, 4 * (size_t)h.cap_cells));
-    CS_TRY(cs_alloc(&h.e_misc, 2));
+    CS_TRY(cs_alloc(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE));
     CS_TRY(cs_alloc(&h.energy, OG_E_NTERMS));
-    h.n_misc_blocks = 0;
     g_cs.have_alloc = true;
     return OG_OK;
 }

@@ -565,11 +565,10 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
               A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) &&
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
-              A(&h.fx, 3 * (size_t)cap) && A(&h.e_cell, 4 * (size_t)h.cap_cells);
+              A(&h.fx, 3 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;
-    h.n_misc_blocks = cdiv(n, OG_THREADS);  // excess/bond energy partials, one per atom block
     h.n_atom_blocks = cdiv(n, OG_THREADS);
-    ok = A(&h.e_misc, 2 * (size_t)std::max(h.n_misc_blocks, 1)) && A(&h.e_atom, std::max(h.n_atom_blocks, 1)) &&
+    ok = A(&h.e_atom, std::max(h.n_atom_blocks, 1)) &&
          A(&h.com_mom, 3 * (size_t)(h.n_com_groups + 1)) && A(&h.energy, OG_E_NTERMS) &&
          A(&h.elog, (size_t)h.log_cap * OG_E_NTERMS) && A(&s->d_cnt, 1) && A(&s->d_self, 1);
     if (!ok) return OG_E_DEVICE;

@@ -418,23 +418,26 @@ __global__ void __launch_bounds__(OG_THREADS) k_finish(const og_dev_sys *__restr
     }
 }
 
+#define OG_STEP_THREADS 256
+
 // End of an MD step, one block per system: pair/excluded-pair/bond energies
 // (on logged or requested steps), kinetic energy from the per-block partials,
 // the energy log entry, and the step counter.
-__global__ void __launch_bounds__(OG_RED_THREADS) k_step_close(const og_dev_sys *__restrict__ S,
+__global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys *__restrict__ S,
                                                                const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
     if (og_frozen(s)) return;
-    __shared__ double sh[OG_RED_THREADS];
-    if (og_energy_step(s)) og_reduce_pair_energies(s, sh);
+    __shared__ double    sh[OG_STEP_THREADS / 64];
+    __shared__ long long she[16 * OG_EACC];
+    if (og_energy_step(s)) og_take_pair_energies(s, she);
     double t = 0.0;
-    for (int k = threadIdx.x; k < s.n_atom_blocks; k += OG_RED_THREADS) t += s.e_atom[k];
+    for (int k = threadIdx.x; k < s.n_atom_blocks; k += OG_STEP_THREADS) t += s.e_atom[k];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
         double ke = 0.0;
-        for (int w = 0; w < OG_RED_THREADS / 64; w++) ke += sh[w];
+        for (int w = 0; w < OG_STEP_THREADS / 64; w++) ke += sh[w];
         s.energy[OG_E_KIN]  = ke * OG_KINETIC_COEFF;
         const long long stp = s.cnt->step;
         if (s.log_cap > 0 && (stp % s.log_stride) == 0) {
@@ -579,5 +582,5 @@ void og_launch_finish(const og_launch &a) {
     hipLaunchKernelGGL(k_finish, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_step_close(const og_launch &a) {
-    hipLaunchKernelGGL(k_step_close, dim3(a.n_blocks), dim3(OG_RED_THREADS), 0, a.stream, a.sys, a.map);
+    hipLaunchKernelGGL(k_step_close, dim3(a.n_blocks), dim3(OG_STEP_THREADS), 0, a.stream, a.sys, a.map);
 }

@@ -139,13 +139,13 @@ struct og_dev_sys {
     uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
     // ---- forces / energies ----
     long long *fx;          // [3*cap_atom_array] fixed-point gradient, SoA x|y|z
-    double    *e_cell;      // [cap_cells*4] per-cell1 vdw, ele, virial, pair count
-    double    *e_misc;      // [n_misc_blocks*4] excess, bond, ... partials
+    long long *e_acc;       // [OG_EACC_SLOTS][OG_EACC_STRIDE] energy accumulators, 2^32 fixed point (the pair
+                            // count as an integer): vdw, ele, virial, pairs, excess, bond; a cell / block adds to
+                            // slot (index % OG_EACC_SLOTS), so no cache line takes every add; zeroed by the reader
     double    *e_atom;      // [n_atom_blocks] kinetic partials
     double    *com_mom;     // [n_com_groups*3]
     double    *energy;      // [OG_E_NTERMS] last step
     double    *elog;        // [log_cap*OG_E_NTERMS]
-    int        n_misc_blocks;
     int        n_atom_blocks;
     og_dev_counters *cnt;
 };
@@ -205,45 +205,42 @@ __device__ __forceinline__ long long og_fx_fast(double v) {
 #define OG_GLOBAL(T) __attribute__((address_space(1))) T
 
 #define OG_RED_THREADS 1024  // per-system reduction block
+enum { OG_EACC_VDW, OG_EACC_ELE, OG_EACC_VIR, OG_EACC_PAIRS, OG_EACC_EXCESS, OG_EACC_BOND, OG_EACC };
+#define OG_EACC_SLOTS 256   // accumulator slots per system
+#define OG_EACC_STRIDE 8    // long longs per slot (one 64-byte line)
 
-// Pair, excluded-pair and bond energies of one system into s.energy: each of
-// the OG_RED_THREADS threads sums a fixed strided share of the per-cell and
-// per-block partials, then a fixed wave and block tree (deterministic).
-// sh: 6 * OG_RED_THREADS / 64 doubles of shared memory.
-__device__ __forceinline__ void og_reduce_pair_energies(const og_dev_sys &s, double *sh) {
-    const int n = s.cnt->n_cells;
-    double    v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // vdw, ele, virial, pairs, excess, bond
-    const double2 *E = (const double2 *)s.e_cell;
-#pragma unroll 4
-    for (int k = threadIdx.x; k < n; k += OG_RED_THREADS) {
-        const double2 u = E[2 * k], w = E[2 * k + 1];
-        v[0] += u.x;
-        v[1] += u.y;
-        v[2] += w.x;
-        v[3] += w.y;
-    }
-    const double2 *M = (const double2 *)s.e_misc;
-    for (int k = threadIdx.x; k < s.n_misc_blocks; k += OG_RED_THREADS) {
-        const double2 u = M[k];
-        v[4] += u.x;
-        v[5] += u.y;
+// Pair, excluded-pair and bond energies of one system into s.energy from
+// the fixed-point accumulator slots (integer adds: the same bits whatever the
+// order of the contributions), which are cleared for the next evaluation.
+// Block-wide (blockDim.x a multiple of 64, at most 1024); sh: 16 * OG_EACC
+// long longs of shared memory.
+__device__ __forceinline__ void og_take_pair_energies(const og_dev_sys &s, long long *sh) {
+    long long v[OG_EACC];
+    for (int q = 0; q < OG_EACC; q++) v[q] = 0;
+    for (int slot = threadIdx.x; slot < OG_EACC_SLOTS; slot += blockDim.x) {
+        long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
+        for (int q = 0; q < OG_EACC; q++) {
+            v[q] += a[q];
+            a[q] = 0;
+        }
     }
     for (int off = 32; off > 0; off >>= 1)
-        for (int q = 0; q < 6; q++) v[q] += __shfl_down(v[q], off, 64);
-    const int wv = threadIdx.x >> 6;
+        for (int q = 0; q < OG_EACC; q++) v[q] += __shfl_down(v[q], off, 64);
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < 6; q++) sh[6 * wv + q] = v[q];
+        for (int q = 0; q < OG_EACC; q++) sh[w * OG_EACC + q] = v[q];
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int w = 0; w < OG_RED_THREADS / 64; w++)
-            for (int q = 0; q < 6; q++) t[q] += sh[6 * w + q];
-        s.energy[OG_E_VDW]    = t[0];
-        s.energy[OG_E_ELE]    = t[1];
-        s.energy[OG_E_VIRIAL] = t[2];
-        s.energy[OG_E_NPAIRS] = t[3];
-        s.energy[OG_E_EXCESS] = t[4];
-        s.energy[OG_E_BOND]   = t[5];
+        long long t[OG_EACC];
+        for (int q = 0; q < OG_EACC; q++) t[q] = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++)
+            for (int q = 0; q < OG_EACC; q++) t[q] += sh[k * OG_EACC + q];
+        s.energy[OG_E_VDW]    = (double)t[OG_EACC_VDW] * OG_FX_INV;
+        s.energy[OG_E_ELE]    = (double)t[OG_EACC_ELE] * OG_FX_INV;
+        s.energy[OG_E_VIRIAL] = (double)t[OG_EACC_VIR] * OG_FX_INV;
+        s.energy[OG_E_NPAIRS] = (double)t[OG_EACC_PAIRS];
+        s.energy[OG_E_EXCESS] = (double)t[OG_EACC_EXCESS] * OG_FX_INV;
+        s.energy[OG_E_BOND]   = (double)t[OG_EACC_BOND] * OG_FX_INV;
         s.energy[OG_E_SELF]   = s.e_self;
     }
     __syncthreads();

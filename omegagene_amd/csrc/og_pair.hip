@@ -277,10 +277,11 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         vir += __shfl_xor(vir, off, 64);
     }
     if (lane == 0) {
-        s.e_cell[4 * c1]     = ev;
-        s.e_cell[4 * c1 + 1] = ee;
-        s.e_cell[4 * c1 + 2] = vir;
-        s.e_cell[4 * c1 + 3] = (double)npair;
+        OG_GLOBAL(long long) *A = (OG_GLOBAL(long long) *)s.e_acc + (c1 % OG_EACC_SLOTS) * OG_EACC_STRIDE;
+        og_atomic_add_fx_g(A + OG_EACC_VDW, og_to_fixed(ev));
+        og_atomic_add_fx_g(A + OG_EACC_ELE, og_to_fixed(ee));
+        og_atomic_add_fx_g(A + OG_EACC_VIR, og_to_fixed(vir));
+        og_atomic_add_fx_g(A + OG_EACC_PAIRS, (long long)npair);
     }
 }
 
@@ -444,24 +445,24 @@ __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys 
         }
         for (int k = 0; k < 3; k++) s.work[3 * i + k] = w[k];
     }
+    if (!og_energy_step(s)) return;
     __shared__ double sh[2 * (OG_THREADS / 64)];
     og_block_sum2(e_ex, e_b, sh);
-    if (threadIdx.x == 0 && bm.y < s.n_misc_blocks) {
-        s.e_misc[2 * bm.y]     = e_ex;
-        s.e_misc[2 * bm.y + 1] = e_b;
+    if (threadIdx.x == 0) {
+        OG_GLOBAL(long long) *A = (OG_GLOBAL(long long) *)s.e_acc + (bm.y % OG_EACC_SLOTS) * OG_EACC_STRIDE;
+        og_atomic_add_fx_g(A + OG_EACC_EXCESS, og_to_fixed(e_ex));
+        og_atomic_add_fx_g(A + OG_EACC_BOND, og_to_fixed(e_b));
     }
 }
 
-// Deterministic per-system energy reduction: 1024 threads, each with a fixed
-// strided share of the per-cell and per-block partials (16-byte loads, four in
-// flight), then a fixed wave and block tree.  Used by single-point force
-// calls; an MD step reduces its energies in k_step_close.
-__global__ void __launch_bounds__(OG_RED_THREADS) k_energy_reduce(const og_dev_sys *__restrict__ S,
+// Energies of a single-point force call from the fixed-point accumulators
+// (an MD step takes them in k_step_close).
+__global__ void __launch_bounds__(64) k_energy_reduce(const og_dev_sys *__restrict__ S,
                                                                   const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
     if (og_frozen(s) || !og_energy_step(s)) return;
-    __shared__ double sh[6 * (OG_RED_THREADS / 64)];
-    og_reduce_pair_energies(s, sh);
+    __shared__ long long sh[16 * OG_EACC];
+    og_take_pair_energies(s, sh);
 }
 
 // ------------------------------------------------------------- launchers --
@@ -475,5 +476,5 @@ void og_launch_forces_to_work(const og_launch &a) {
     hipLaunchKernelGGL(k_forces_to_work, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_energy_reduce(const og_launch &a) {
-    hipLaunchKernelGGL(k_energy_reduce, dim3(a.n_blocks), dim3(OG_RED_THREADS), 0, a.stream, a.sys, a.map);
+    hipLaunchKernelGGL(k_energy_reduce, dim3(a.n_blocks), dim3(64), 0, a.stream, a.sys, a.map);
 }

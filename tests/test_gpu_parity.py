@@ -124,9 +124,13 @@ def test_forces_energies_vs_oracle(case):
 
 def test_forces_at_full_bench_size_vs_oracle():
     """The 99,200-atom water + chain system of the headline benchmark (one
-    force evaluation): exact in-cutoff pair count, gradient RMS relative error
-    <= 2e-6 against the FP32-faithful oracle, energies as in the small cases,
-    and Newton's third law on the fixed-point sums."""
+    force evaluation): gradient RMS relative error <= 2e-6 against the
+    FP32-faithful oracle, energies as in the small cases, and Newton's third
+    law on the fixed-point sums.  The in-cutoff pair count agrees to 1e-7: the
+    kernel forms r^2 with fused multiply-adds and the oracle (like the
+    reference's CPU build) without, so a pair whose r^2 is within an ulp of
+    rc^2 can fall on either side (2 of 35.5 M pairs here; such a pair adds
+    ~1e-4 kcal/mol at 12 A)."""
     ls, cfg = gen.config1()
     say("forces, full size", ls.n_atoms)
     s = System(ls, dict(BASE, **{"cutoff": cfg["cutoff"], "nsgrid-cutoff": cfg["nsgrid-cutoff"]}))
@@ -134,7 +138,7 @@ def test_forces_at_full_bench_size_vs_oracle():
     wo, eo, _ = oracle_forces(ls, cutoff=cfg["cutoff"], nscut=cfg["nsgrid-cutoff"])
     err = rms_rel(w, wo)
     say("  pairs gpu %d oracle %d, gradient rms rel %.3e" % (e["npairs"], eo["n_pairs"], err))
-    assert int(e["npairs"]) == int(eo["n_pairs"])
+    assert abs(int(e["npairs"]) - int(eo["n_pairs"])) <= 1e-7 * eo["n_pairs"]
     assert err < 2e-6
     assert e["vdw"] == pytest.approx(eo["vdw"], rel=2e-6, abs=1e-4)
     S = ele_scale(ls, cfg["cutoff"], 0.0, cfg["nsgrid-cutoff"])
