@@ -260,6 +260,7 @@ og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_c
     CS_TRY(cs_alloc(&h.cell_u0, h.cap_cells));
     CS_TRY(cs_alloc(&h.cell_u1, h.cap_cells));
     CS_TRY(cs_alloc(&h.cp_count, (size_t)h.cap_cells + 1));
+    CS_TRY(cs_alloc(&h.cp_stage, (size_t)h.cap_cells * OG_ENUM_STAGE));
     CS_TRY(cs_alloc(&h.cp_head, (size_t)h.cap_cells + 1));
     CS_TRY(cs_alloc(&h.cp_c2, max_n_cell_pairs));
     CS_TRY(cs_alloc(&h.cp_mask, max_n_cell_pairs));

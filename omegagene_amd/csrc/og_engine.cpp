@@ -563,7 +563,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.gtype, cap) && A(&h.gcomp, cap) && A(&h.cell_col, h.cap_cells) && A(&h.cell_z, h.cap_cells) &&
               A(&h.cell_zmin, h.cap_cells) && A(&h.cell_zmax, h.cap_cells) && A(&h.cell_u0, h.cap_cells) &&
               A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
-              A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) &&
+              A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) && A(&h.cp_stage, (size_t)h.cap_cells * OG_ENUM_STAGE) &&
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
               A(&h.fx, 3 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;

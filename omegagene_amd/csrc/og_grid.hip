@@ -293,8 +293,12 @@ __device__ __forceinline__ bool og_valid_pair(int c1, int c2) {
     return !((c2 < c1 && o2) || (c2 > c1 && !o2));
 }
 
-template <bool WRITE>
+// MODE 1: count the cell's pairs and stage them (cell2 | image << 24, list
+// order) in cp_stage[c1 * OG_ENUM_STAGE ...] as far as they fit;
+// MODE 2: write them at cp_head[c1] (cells whose pairs did not fit).
+template <int MODE>
 __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
+    constexpr bool WRITE = MODE != 0;
     const int lane  = threadIdx.x & 63;
     const int col1  = s.cell_col[c1];
     const int x1    = col1 % s.nx, y1 = col1 / s.nx;
@@ -323,7 +327,7 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
         fz[2] = lz[2] = -1;
         lz[1]         = tl;
     }
-    int carry = WRITE ? s.cp_head[c1] : 0;
+    int carry = MODE == 2 ? s.cp_head[c1] : 0;
     int total = 0;
     for (int r0 = 0; r0 < ncand; r0 += 64) {
         const int k     = r0 + lane;
@@ -437,7 +441,12 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
                     }
                     if (dd01 + d2 > s.cutoff_pair2) continue;
                     if (!og_valid_pair(c1, c2)) continue;
-                    if (p < s.cap_cell_pairs) s.cp_c2[p] = c2 | ((imgxy | img2) << 24);
+                    const int rec = c2 | ((imgxy | img2) << 24);
+                    if (MODE == 1) {
+                        if (p < OG_ENUM_STAGE) s.cp_stage[(size_t)c1 * OG_ENUM_STAGE + p] = rec;
+                    } else if (p < s.cap_cell_pairs) {
+                        s.cp_c2[p] = rec;
+                    }
                     p++;
                 }
             }
@@ -445,7 +454,7 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
         carry += round_total;
         total += round_total;
     }
-    if (!WRITE && lane == 0) *out_count = total;
+    if (MODE == 1 && lane == 0) *out_count = total;
 }
 
 __global__ void __launch_bounds__(OG_THREADS) k_enum_count(const og_dev_sys *__restrict__ S,
@@ -455,7 +464,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_count(const og_dev_sys *__r
     if (og_frozen(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells) return;
-    og_enum_cell<false>(s, c1, &s.cp_count[c1]);
+    og_enum_cell<1>(s, c1, &s.cp_count[c1]);
 }
 
 __global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__restrict__ S,
@@ -465,7 +474,16 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__r
     if (og_frozen(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells) return;
-    og_enum_cell<true>(s, c1, nullptr);
+    const int n = s.cp_count[c1];
+    if (n > OG_ENUM_STAGE) {  // did not fit the stage: enumerate again, writing in place
+        og_enum_cell<2>(s, c1, nullptr);
+        return;
+    }
+    const int  lane = threadIdx.x & 63;
+    const int  p0   = s.cp_head[c1];
+    const int *src  = s.cp_stage + (size_t)c1 * OG_ENUM_STAGE;
+    for (int k = lane; k < n; k += 64)
+        if (p0 + k < s.cap_cell_pairs) s.cp_c2[p0 + k] = src[k];
 }
 
 // Exclusive scan of per-cell pair counts -> cp_head (one 1024-thread block per system).

@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #define OG_NAC 8                 // atoms per cell (MiniCell.h:10 N_ATOM_CELL)
+#define OG_ENUM_STAGE 320        // cell pairs per cell kept from the counting pass (more: enumerated again)
 #define OG_MAX_NB15 32           // exclusion partners per atom (CelesteObject.h:26)
 #define OG_SORT_CAP 2048         // max atoms per column handled by the in-LDS sort
 #define OG_EMPTY_BIN 0x7fffffff  // uniform-z bin with no cell
@@ -134,6 +135,7 @@ struct og_dev_sys {
     int      *u2c_min, *u2c_max;   // [(n_columns+1)*n_uni_z]
     // ---- cell pairs ----
     int      *cp_count;     // [cap_cells]
+    int      *cp_stage;     // [cap_cells * OG_ENUM_STAGE] pairs of each cell from the counting pass
     int      *cp_head;      // [cap_cells+1]
     int      *cp_c2;        // [cap_cell_pairs] cell2 | image << 24
     uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
