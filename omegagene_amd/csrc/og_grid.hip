@@ -494,24 +494,27 @@ __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict
     const int         T   = blockDim.x;
     const int         per = (n + T - 1) / T;
     const int         k0  = threadIdx.x * per, k1 = min(k0 + per, n);
-    __shared__ long long sh[1024];
+    const int         lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ long long wsum[16];
     long long local = 0;
     for (int k = k0; k < k1; k++) local += s.cp_count[k];
-    sh[threadIdx.x] = local;
-    __syncthreads();
-    for (int off = 1; off < T; off <<= 1) {
-        const long long v = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += v;
-        __syncthreads();
+    // inclusive scan within the wave (shuffles), then over the 16 wave totals
+    long long incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const long long v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
     }
-    long long p = sh[threadIdx.x] - local;
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    long long before = 0;
+    for (int q = 0; q < w; q++) before += wsum[q];
+    long long p = before + incl - local;
     for (int k = k0; k < k1; k++) {
         s.cp_head[k] = (int)min(p, (long long)s.cap_cell_pairs);
         p += s.cp_count[k];
     }
     if (threadIdx.x == T - 1) {
-        const long long total = sh[T - 1];
+        const long long total = before + incl;
         s.cp_head[n]          = (int)min(total, (long long)s.cap_cell_pairs);
         if (total > s.cap_cell_pairs) {
             atomicOr(&s.cnt->flags, OG_F_CP_OVERFLOW);
