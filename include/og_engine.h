@@ -133,6 +133,8 @@ typedef struct {
     double   capacity_factor;      /* cell-pair list capacity multiplier (1.0 default) */
     double   enhance_sigma;        /* --enhance-sigma (VcMD, A) */
     double   enhance_recov_coef;   /* --enhance-recovery-coef (VcMD) */
+    int      split_step;           /* 1: run a Langevin step as separate kernels even where the fused
+                                      one-kernel step applies (same trajectory; for comparison) */
 } og_params;
 
 void og_params_default(og_params *p);
@@ -234,6 +236,9 @@ og_status   og_last_error(char *buf, int len);
  * recorded with HIP events around each pair launch on the run's stream when timing is on */
 og_status   og_set_pair_timing(og_system *sys, int on);
 og_status   og_pair_kernel_time(og_system *sys, double *ms, int *n_launches);
+/* 1 when an MD step of this system runs as the fused one-kernel Langevin step
+ * (Langevin, SETTLE waters bound to nothing outside the water, at most 4096 other atoms) */
+og_status   og_step_fused(og_system *sys, int *fused);
 /* bytes of device memory a system with this description would hold */
 og_status   og_system_footprint(const og_system_desc *desc, const og_params *params, uint64_t *bytes);
 

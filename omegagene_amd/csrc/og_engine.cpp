@@ -161,7 +161,7 @@ int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 void og_set_error(const char *msg) { g_last_error = msg; }
 
 // ----------------------------------------------------------------- types --
-enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_N };
+enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_LSTEP, KIND_N };
 
 struct og_system {
     og_dev_sys          h{};           // host copy of the descriptor (device pointers)
@@ -193,6 +193,7 @@ struct og_exec {
     int                      alpha_mode = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false;
+    bool                     fused = false;       // every member takes the fused Langevin step
     int                      intvl = 1;
     int                      graph_steps = 0;     // steps per captured graph (0: automatic)
 };
@@ -236,6 +237,7 @@ og_status exec_prepare(og_exec &x) {
     std::vector<int2>       maps[KIND_N];
     x.alpha_mode   = x.members[0]->alpha_mode;
     x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = false;
+    x.fused        = true;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
         og_system *s = x.members[si];
@@ -254,6 +256,7 @@ og_status exec_prepare(og_exec &x) {
         x.any_com |= d.com_cancel && d.n_com_groups > 0;
         x.any_constraint |= d.constraint != 0;
         x.any_vcmd |= d.vc_n_states > 0;
+        x.fused &= d.integrator == 1 && d.n_water_blocks > 0;
         const int n_bins = (d.n_columns + 1) * d.n_uni_z;
         int       cnt[KIND_N];
         cnt[KIND_ATOM]  = cdiv(d.n_atoms, OG_THREADS);
@@ -264,6 +267,7 @@ og_status exec_prepare(og_exec &x) {
         cnt[KIND_MISC]  = cdiv(d.n_pairs_c, OG_THREADS);  // SHAKE pair constraints
         cnt[KIND_TRIO]  = cdiv(std::max(d.n_trios, d.n_settle), OG_TRIO_THREADS);
         cnt[KIND_COMA] = 0;  // filled below
+        cnt[KIND_LSTEP] = d.n_water_blocks ? d.n_water_blocks + 1 : 0;  // free-atom block + water blocks
         for (int k = 0; k < KIND_N; k++)
             for (int b = 0; b < cnt[k]; b++) maps[k].push_back(make_int2(si, b));
     }
@@ -307,11 +311,17 @@ void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1, bool reduce = fals
     if (e0) hipEventRecord(e0, x.stream);
     og_launch_pair(L(x, KIND_CELLW), x.alpha_mode);
     if (e1) hipEventRecord(e1, x.stream);
-    og_launch_forces_to_work(L(x, KIND_ATOM));
+    // an MD step of a fused context converts the gradient inside its step kernel
+    if (reduce || !x.fused) og_launch_forces_to_work(L(x, KIND_ATOM));
     if (reduce) og_launch_energy_reduce(L(x, KIND_SYS));
 }
 
 void enqueue_integrate(og_exec &x) {
+    if (x.fused) {
+        og_launch_langevin_step(L(x, KIND_LSTEP));
+        og_launch_step_close(L(x, KIND_SYS));
+        return;
+    }
     // the extended-ensemble bias acts on the gradient of the step
     // (DynamicsMode.cpp:504-512, :824-834)
     if (x.any_vcmd) og_launch_vcmd(L(x, KIND_SYS));
@@ -568,7 +578,8 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.fx, 3 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;
     h.n_atom_blocks = cdiv(n, OG_THREADS);
-    ok = A(&h.e_atom, std::max(h.n_atom_blocks, 1)) &&
+    h.n_ke_parts    = std::max(h.n_atom_blocks, cdiv(h.n_settle, OG_WATERS_PER_BLOCK) + 1);
+    ok = A(&h.e_atom, std::max(h.n_ke_parts, 1)) &&
          A(&h.com_mom, 3 * (size_t)(h.n_com_groups + 1)) && A(&h.energy, OG_E_NTERMS) &&
          A(&h.elog, (size_t)h.log_cap * OG_E_NTERMS) && A(&s->d_cnt, 1) && A(&s->d_self, 1);
     if (!ok) return OG_E_DEVICE;
@@ -654,6 +665,42 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         OG_HIP(hipMemcpy(h.pairc_d2, d->shake_pair_d2, sizeof(double) * h.n_pairs_c, hipMemcpyHostToDevice));
     }
     if (h.n_settle) OG_HIP(hipMemcpy(h.settle_ids, d->settle_ids, sizeof(int) * 3 * h.n_settle, hipMemcpyHostToDevice));
+    // The fused Langevin step (one kernel per MD step after the pair kernel)
+    // takes SETTLE waters one per thread, so each water must interact with no
+    // other atom through exclusions and carry no bond, SHAKE or VcMD group;
+    // the other atoms (at most OG_FREE_MAX) go through one block.
+    h.n_free         = 0;
+    h.n_water_blocks = 0;
+    if (h.integrator == 1 && h.n_settle > 0 && !p->split_step) {
+        std::vector<int> water(n, -1);
+        bool             fuse = true;
+        for (int t = 0; t < h.n_settle && fuse; t++)
+            for (int k = 0; k < 3 && fuse; k++) {
+                int &w = water[d->settle_ids[3 * t + k]];
+                fuse   = w < 0;
+                w      = t;
+            }
+        auto outside = [&](const int *ids, long long cnt) {
+            for (long long k = 0; k < cnt; k++)
+                if (water[ids[k]] >= 0) return false;
+            return true;
+        };
+        for (int t = 0; t < h.n_excess && fuse; t++) fuse = water[d->excess[2 * t]] == water[d->excess[2 * t + 1]];
+        fuse = fuse && outside(d->bond_ids, 2LL * h.n_bonds) && outside(d->shake_pair_ids, 2LL * h.n_pairs_c) &&
+               outside(d->shake_trio_ids, 3LL * h.n_trios) && outside(d->vc_atoms_a, h.vc_na) &&
+               outside(d->vc_atoms_b, h.vc_nb);
+        std::vector<int> free_ids;
+        for (int i = 0; i < n && fuse; i++)
+            if (water[i] < 0) free_ids.push_back(i);
+        const int nwb = cdiv(h.n_settle, OG_WATERS_PER_BLOCK);
+        if (fuse && (int)free_ids.size() <= OG_FREE_MAX) {
+            if (!A(&h.free_ids, std::max<size_t>(free_ids.size(), 1))) return OG_E_DEVICE;
+            if (!free_ids.empty())
+                OG_HIP(hipMemcpy(h.free_ids, free_ids.data(), sizeof(int) * free_ids.size(), hipMemcpyHostToDevice));
+            h.n_free         = (int)free_ids.size();
+            h.n_water_blocks = nwb;
+        }
+    }
     if (h.n_com_groups) {
         const int ng = h.n_com_groups;
         OG_HIP(hipMemcpy(h.com_off, d->com_group_off, sizeof(int) * (ng + 1), hipMemcpyHostToDevice));
@@ -1070,6 +1117,12 @@ og_status og_pair_kernel_time(og_system *s, double *ms, int *n) {
     if (!s) return OG_E_ARG;
     if (ms) *ms = s->pair_ms;
     if (n) *n = s->pair_launches;
+    return OG_OK;
+}
+
+og_status og_step_fused(og_system *s, int *fused) {
+    if (!s || !fused) return OG_E_ARG;
+    *fused = s->h.integrator == 1 && s->h.n_water_blocks > 0;
     return OG_OK;
 }
 

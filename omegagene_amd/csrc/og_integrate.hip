@@ -5,7 +5,7 @@
 // Compiled with -ffp-contract=off: each expression rounds like the reference
 // host code it follows, so these kernels match the CPU oracle bit for bit on
 // identical inputs (reductions aside, which use a fixed tree order).
-#include "og_internal.h"
+#include "og_terms.h"
 
 #define OG_FORCE_VEL 4.184e-4                          // CelesteObject.cpp:37
 #define OG_GAS_CONST 8.31451                           // CelesteObject.cpp:38
@@ -124,26 +124,13 @@ __global__ void __launch_bounds__(OG_THREADS) k_positions(const og_dev_sys *__re
 // crd_prev2 <- crd_prev <- crd; first step x1 = x0 + dt v0; later steps
 // SubBox::update_coordinates_langevin (SubBox.cpp:2077-2114) with N(0,1)
 // noise from Philox keyed by (seed, step, atom).
-__global__ void __launch_bounds__(OG_THREADS) k_langevin_positions(const og_dev_sys *__restrict__ S,
-                                                                   const int2 *__restrict__ map) {
-    const int2        bm = map[blockIdx.x];
-    const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
-    const int         i  = bm.y * OG_THREADS + threadIdx.x;
-    if (i >= s.n_atoms || s.integrator != 1) return;
-    const long long step = s.cnt->step;
-    double          x[3];
-    for (int d = 0; d < 3; d++) {
-        x[d]                   = s.crd[3 * i + d];
-        s.crd_prev2[3 * i + d] = s.crd_prev[3 * i + d];
-        s.crd_prev[3 * i + d]  = x[d];
-    }
+// Core on values: x = current position, xp = previous position, vn = initial
+// velocity (first step only), wk = gradient; the new position goes to xn.
+__device__ __forceinline__ void og_langevin_core(const og_dev_sys &s, const int i, const long long step,
+                                                 const double *x, const double *xp, const double *vn,
+                                                 const double *wk, double mass, double minv, double *xn) {
     if (step == 0) {
-        for (int d = 0; d < 3; d++) {
-            const double v     = s.vel_next[3 * i + d];
-            s.vel[3 * i + d]   = v;
-            s.crd[3 * i + d]   = x[d] + v * s.dt;
-        }
+        for (int d = 0; d < 3; d++) xn[d] = x[d] + vn[d] * s.dt;
         return;
     }
     const uint4   r  = og_philox(make_uint4((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)i, 0x4c414e47u),
@@ -153,28 +140,48 @@ __global__ void __launch_bounds__(OG_THREADS) k_langevin_positions(const og_dev_
     const double  zeta[3] = {z0.x, z0.y, z1.x};
     const double  dth     = s.dt * 0.5;
     const double  gm      = s.gamma;
-    const double  mass    = (double)s.mass[i];
-    const double  minv    = (double)s.mass_inv[i];
     const double  sig     = sqrt(OG_GAS_CONST * s.temperature * gm * mass / dth * 1e-7);
     for (int d = 0; d < 3; d++) {
-        const double det_f = -OG_FORCE_VEL * s.work[3 * i + d];
+        const double det_f = -OG_FORCE_VEL * wk[d];
         const double stc_f = zeta[d] * sig;
         const double p1    = x[d];
-        double       p2    = s.crd_prev2[3 * i + d];
+        double       p2    = xp[d];
         p2 += nearbyint((p1 - p2) / (double)s.L[d]) * (double)s.L[d];
-        s.crd[3 * i + d] = 1.0 / (1.0 + gm * dth) * (2 * p1 - p2 + gm * dth * p2 + 4 * dth * dth * minv * (det_f + stc_f));
+        xn[d] = 1.0 / (1.0 + gm * dth) * (2 * p1 - p2 + gm * dth * p2 + 4 * dth * dth * minv * (det_f + stc_f));
     }
+}
+
+__device__ __forceinline__ void og_langevin_atom(const og_dev_sys &s, const int i, const long long step,
+                                                 const double *wk) {
+    double x[3], xp[3], vn[3], xn[3];
+    for (int d = 0; d < 3; d++) {
+        x[d]  = s.crd[3 * i + d];
+        xp[d] = s.crd_prev[3 * i + d];
+        vn[d] = step == 0 ? s.vel_next[3 * i + d] : 0.0;
+    }
+    og_langevin_core(s, i, step, x, xp, vn, wk, (double)s.mass[i], (double)s.mass_inv[i], xn);
+    for (int d = 0; d < 3; d++) {
+        s.crd_prev2[3 * i + d] = xp[d];
+        s.crd_prev[3 * i + d]  = x[d];
+        if (step == 0) s.vel[3 * i + d] = vn[d];
+        s.crd[3 * i + d] = xn[d];
+    }
+}
+
+__global__ void __launch_bounds__(OG_THREADS) k_langevin_positions(const og_dev_sys *__restrict__ S,
+                                                                   const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
+    const int         i  = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms || s.integrator != 1) return;
+    const double wk[3] = {s.work[3 * i], s.work[3 * i + 1], s.work[3 * i + 2]};
+    og_langevin_atom(s, i, s.cnt->step, wk);
 }
 
 // --------------------------------------------------------- constraints --
 // ConstraintShake::shake_pair (ConstraintShake.cpp:18-49), one thread per constraint.
-__global__ void __launch_bounds__(OG_THREADS) k_shake_pair(const og_dev_sys *__restrict__ S,
-                                                           const int2 *__restrict__ map) {
-    const int2        bm = map[blockIdx.x];
-    const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
-    const int         t  = bm.y * OG_THREADS + threadIdx.x;
-    if (s.constraint == 0 || t >= s.n_pairs_c) return;
+__device__ __forceinline__ void og_shake_pair(const og_dev_sys &s, const int t) {
     const int a = s.pairc_ids[2 * t], b = s.pairc_ids[2 * t + 1];
     double    d[3], dp[3];
     for (int k = 0; k < 3; k++) {
@@ -194,8 +201,19 @@ __global__ void __launch_bounds__(OG_THREADS) k_shake_pair(const og_dev_sys *__r
     }
 }
 
+__global__ void __launch_bounds__(OG_THREADS) k_shake_pair(const og_dev_sys *__restrict__ S,
+                                                           const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
+    const int         t  = bm.y * OG_THREADS + threadIdx.x;
+    if (s.constraint == 0 || t >= s.n_pairs_c) return;
+    og_shake_pair(s, t);
+}
+
 // ConstraintShake::shake_trio (ConstraintShake.cpp:51-149), one thread per trio.
-__device__ void og_shake_trio(const og_dev_sys &s, int a1, int a2, int a3, const double *D) {
+// inlined: as a called function it takes 248 VGPRs, which would set every caller's count
+__device__ __forceinline__ void og_shake_trio(const og_dev_sys &s, int a1, int a2, int a3, const double *D) {
     double d12[3], d23[3], d31[3], p12[3], p23[3], p31[3];
     double g[6] = {0, 0, 0, 0, 0, 0};
     for (int k = 0; k < 3; k++) {
@@ -252,8 +270,12 @@ __device__ void og_shake_trio(const og_dev_sys &s, int a1, int a2, int a3, const
 
 // Analytic SETTLE (S. Miyamoto, P. A. Kollman, J. Comput. Chem. 13 (1992) 952-962)
 // for a rigid 3-site water (O, H1, H2); same arithmetic as the oracle's orc_settle.
-__device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double dOH, double dHH) {
-    const double mO = s.mass[iO], mH = s.mass[iH1];
+// Core on values: xO/xH1/xH2 are the unconstrained positions (moved to the
+// constrained ones), pO/pH1/pH2 the previous positions.  Returns false, with
+// the positions untouched, when there is no rigid solution.
+__device__ __forceinline__ bool og_settle_core(const og_dev_sys &s, double mO, double mH, double dOH, double dHH,
+                                               double *xO, double *xH1, double *xH2, const double *pO,
+                                               const double *pH1, const double *pH2) {
     const double wt = mO + 2.0 * mH;
     const double rc = 0.5 * dHH;
     const double h  = sqrt(dOH * dOH - rc * rc);
@@ -261,10 +283,10 @@ __device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double 
     const double rb = h - ra;
     double b0[3], c0[3], xb[3], xc[3], com[3], a1[3], b1[3], c1[3];
     for (int d = 0; d < 3; d++) {
-        b0[d] = og_mimg(s.crd_prev[3 * iH1 + d] - s.crd_prev[3 * iO + d], s.L[d]);
-        c0[d] = og_mimg(s.crd_prev[3 * iH2 + d] - s.crd_prev[3 * iO + d], s.L[d]);
-        xb[d] = og_mimg(s.crd[3 * iH1 + d] - s.crd[3 * iO + d], s.L[d]);
-        xc[d] = og_mimg(s.crd[3 * iH2 + d] - s.crd[3 * iO + d], s.L[d]);
+        b0[d] = og_mimg(pH1[d] - pO[d], s.L[d]);
+        c0[d] = og_mimg(pH2[d] - pO[d], s.L[d]);
+        xb[d] = og_mimg(xH1[d] - xO[d], s.L[d]);
+        xc[d] = og_mimg(xH2[d] - xO[d], s.L[d]);
     }
     for (int d = 0; d < 3; d++) {
         com[d] = (mH * xb[d] + mH * xc[d]) / wt;
@@ -310,10 +332,7 @@ __device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double 
     const double sinthe = (alpa * gama - beta * sqrt(q3)) / al2be2;
     const double q4     = 1.0 - sinthe * sinthe;
     const double costhe = sqrt(q4);
-    if (!(q1 > 0.0 && q2 >= 0.0 && q3 >= 0.0 && q4 >= 0.0)) {
-        atomicAdd(&s.cnt->shake_fail, 1);
-        return;
-    }
+    if (!(q1 > 0.0 && q2 >= 0.0 && q3 >= 0.0 && q4 >= 0.0)) return false;
     const double a3[3] = {-ya2 * sinthe, ya2 * costhe, za1};
     const double b3[3] = {xb2 * costhe - yb2 * sinthe, xb2 * sinthe + yb2 * costhe, zb1};
     const double c3[3] = {-xb2 * costhe - yc2 * sinthe, -xb2 * sinthe + yc2 * costhe, zc1};
@@ -324,10 +343,28 @@ __device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double 
         const double ca = com[d] + X[d] * a3[0] + Y[d] * a3[1] + Z[d] * a3[2];
         const double cb = com[d] + X[d] * b3[0] + Y[d] * b3[1] + Z[d] * b3[2];
         const double cc = com[d] + X[d] * c3[0] + Y[d] * c3[1] + Z[d] * c3[2];
-        s.crd[3 * iO + d] += ca;
-        s.crd[3 * iH1 + d] += cb - xb[d];
-        s.crd[3 * iH2 + d] += cc - xc[d];
+        xO[d] += ca;
+        xH1[d] += cb - xb[d];
+        xH2[d] += cc - xc[d];
     }
+    return true;
+}
+
+
+__device__ void og_settle(const og_dev_sys &s, int iO, int iH1, int iH2, double dOH, double dHH) {
+    double    x[3][3], p[3][3];
+    const int ia[3] = {iO, iH1, iH2};
+    for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++) {
+            x[a][d] = s.crd[3 * ia[a] + d];
+            p[a][d] = s.crd_prev[3 * ia[a] + d];
+        }
+    if (!og_settle_core(s, s.mass[iO], s.mass[iH1], dOH, dHH, x[0], x[1], x[2], p[0], p[1], p[2])) {
+        atomicAdd(&s.cnt->shake_fail, 1);
+        return;
+    }
+    for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++) s.crd[3 * ia[a] + d] = x[a][d];
 }
 
 __global__ void __launch_bounds__(OG_TRIO_THREADS) k_trio(const og_dev_sys *__restrict__ S,
@@ -351,70 +388,250 @@ __global__ void __launch_bounds__(OG_TRIO_THREADS) k_trio(const og_dev_sys *__re
 // MiniCell.cpp:1312-1325) and republished as float; PBC wrap
 // (SubBox::revise_coordinates_pbc, SubBox.cpp:1482-1502); kinetic-energy
 // partials (DynamicsMode::cal_kinetic_energy, DynamicsMode.cpp:358-372).
+// Kinetic-energy partial of one block (fixed order) into its e_atom slot.
+__device__ __forceinline__ void og_ke_partial(const og_dev_sys &s, const int slot, double ke, double *sh) {
+    for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ke;
+    __syncthreads();
+    if (threadIdx.x == 0 && slot < s.n_ke_parts) {
+        double t = 0.0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += sh[k];
+        s.e_atom[slot] = t;
+    }
+}
+
+// Langevin velocity of one atom by the central difference over two steps
+// (first step: the initial velocity vn); returns sum_d v_d^2.
+__device__ __forceinline__ double og_langevin_vel(const og_dev_sys &s, const long long step, const double *x,
+                                                  const double *xp2, const double *vn, double *v) {
+    double ka = 0.0;
+    if (step > 0) {
+        const double dt2inv = 1.0 / (2 * s.dt);
+        for (int d = 0; d < 3; d++) {
+            double p2 = xp2[d];
+            p2 += nearbyint((x[d] - p2) / (double)s.L[d]) * (double)s.L[d];
+            v[d] = (x[d] - p2) * dt2inv;
+            ka += v[d] * v[d];
+        }
+    } else {
+        for (int d = 0; d < 3; d++) ka += vn[d] * vn[d];
+    }
+    return ka;
+}
+
+// Grid coordinates gc advanced by the step's displacement x - xp (float copy
+// in f) and the position wrapped once into the box (xw).
+__device__ __forceinline__ void og_advance(const og_dev_sys &s, const double *x, const double *xp, double *gc,
+                                           float *f, double *xw) {
+    for (int d = 0; d < 3; d++) {
+        const double diff = x[d] - xp[d];
+        gc[d]             = gc[d] + diff;
+        f[d]              = (float)gc[d];
+        xw[d]             = x[d];
+        if (xw[d] >= (double)s.ub[d]) xw[d] -= (double)s.L[d];
+        else if (xw[d] < (double)s.lb[d]) xw[d] += (double)s.L[d];
+    }
+}
+
+__device__ __forceinline__ double og_finish_atom(const og_dev_sys &s, const int i, const long long step) {
+    double ke;
+    const int       g    = s.rev[i];
+    double          x[3], xp[3];
+    for (int d = 0; d < 3; d++) {
+        x[d]  = s.crd[3 * i + d];
+        xp[d] = s.crd_prev[3 * i + d];
+    }
+    if (s.integrator == 0) {
+        if (s.constraint) {
+            const double dt_inv = 1.0 / s.dt;
+            for (int d = 0; d < 3; d++) s.vel_next[3 * i + d] = (x[d] - xp[d]) * dt_inv;
+        }
+        double ka = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double vj = (s.vel[3 * i + d] + s.vel_next[3 * i + d]) * 0.5;
+            ka += vj * vj;
+        }
+        ke = ka * (double)s.mass[i];
+    } else {
+        double xp2[3], vn[3], v[3];
+        for (int d = 0; d < 3; d++) {
+            xp2[d] = s.crd_prev2[3 * i + d];
+            vn[d]  = s.vel_next[3 * i + d];
+        }
+        const double ka = og_langevin_vel(s, step, x, xp2, vn, v);
+        if (step > 0)
+            for (int d = 0; d < 3; d++) s.vel[3 * i + d] = v[d];
+        ke = ka * (double)s.mass[i];
+    }
+    double gc[3], xw[3];
+    float  f[3];
+    for (int d = 0; d < 3; d++) gc[d] = s.gcrd[3 * g + d];
+    og_advance(s, x, xp, gc, f, xw);
+    for (int d = 0; d < 3; d++) {
+        s.gcrd[3 * g + d] = gc[d];
+        s.crd[3 * i + d]  = xw[d];
+    }
+    const float4 old = s.gxyzq[g];
+    s.gxyzq[g]       = make_float4(f[0], f[1], f[2], old.w);
+    return ke;
+}
+
 __global__ void __launch_bounds__(OG_THREADS) k_finish(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
     if (og_frozen(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     double            ke = 0.0;
-    if (i < s.n_atoms) {
-        const long long step = s.cnt->step;
-        const int       g    = s.rev[i];
-        double          x[3], xp[3];
-        for (int d = 0; d < 3; d++) {
-            x[d]  = s.crd[3 * i + d];
-            xp[d] = s.crd_prev[3 * i + d];
-        }
-        if (s.integrator == 0) {
-            if (s.constraint) {
-                const double dt_inv = 1.0 / s.dt;
-                for (int d = 0; d < 3; d++) s.vel_next[3 * i + d] = (x[d] - xp[d]) * dt_inv;
-            }
-            double ka = 0.0;
-            for (int d = 0; d < 3; d++) {
-                const double vj = (s.vel[3 * i + d] + s.vel_next[3 * i + d]) * 0.5;
-                ka += vj * vj;
-            }
-            ke = ka * (double)s.mass[i];
-        } else {
-            double ka = 0.0;
-            if (step > 0) {
-                const double dt2inv = 1.0 / (2 * s.dt);
-                for (int d = 0; d < 3; d++) {
-                    double p2 = s.crd_prev2[3 * i + d];
-                    p2 += nearbyint((x[d] - p2) / (double)s.L[d]) * (double)s.L[d];
-                    const double v   = (x[d] - p2) * dt2inv;
-                    s.vel[3 * i + d] = v;
-                    ka += v * v;
-                }
-            } else {
-                for (int d = 0; d < 3; d++) ka += s.vel_next[3 * i + d] * s.vel_next[3 * i + d];
-            }
-            ke = ka * (double)s.mass[i];
-        }
-        float f[3];
-        for (int d = 0; d < 3; d++) {
-            const double diff = x[d] - xp[d];
-            const double gx   = s.gcrd[3 * g + d] + diff;
-            s.gcrd[3 * g + d] = gx;
-            f[d]              = (float)gx;
-            double xw         = x[d];
-            if (xw >= (double)s.ub[d]) xw -= (double)s.L[d];
-            else if (xw < (double)s.lb[d]) xw += (double)s.L[d];
-            s.crd[3 * i + d] = xw;
-        }
-        const float4 old = s.gxyzq[g];
-        s.gxyzq[g]       = make_float4(f[0], f[1], f[2], old.w);
-    }
-    // block partial of the kinetic sum, fixed order
+    if (i < s.n_atoms) ke = og_finish_atom(s, i, s.cnt->step);
+    if (bm.y == 0)  // slots beyond this kernel's blocks (used by the fused step) count zero
+        for (int k = s.n_atom_blocks + threadIdx.x; k < s.n_ke_parts; k += OG_THREADS) s.e_atom[k] = 0.0;
     __shared__ double sh[OG_THREADS / 64];
-    for (int off = 32; off > 0; off >>= 1) ke += __shfl_down(ke, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = ke;
-    __syncthreads();
-    if (threadIdx.x == 0 && bm.y < s.n_atom_blocks) {
-        double t = 0.0;
-        for (int k = 0; k < OG_THREADS / 64; k++) t += sh[k];
-        s.e_atom[bm.y] = t;
+    og_ke_partial(s, bm.y, ke, sh);
+}
+
+// ------------------------------------------------- fused Langevin step --
+// One launch for the whole integration step of a Langevin system whose
+// SETTLE waters interact with nothing but their own atoms through exclusions
+// and carry no bonds, VcMD groups or SHAKE groups:
+//  * block 0 of a system takes every other ("free") atom through the same
+//    sequence as the separate kernels -- gradient conversion, VcMD bias,
+//    Langevin positions, SHAKE pairs then trios, finish -- with a block
+//    barrier between the stages;
+//  * blocks 1.. take the waters, three lanes per water (og_water_lanes):
+//    gradient conversion, Langevin positions, SETTLE and finish.
+// Each atom goes through the same device functions, in the same order, as in
+// the separate kernels, so the trajectory is bitwise the same; only the
+// FP64 block partials of the kinetic, excluded-pair and bond energies are
+// grouped differently (block 0, then the water blocks; the remaining e_atom
+// slots are zeroed).
+// SETTLE waters of the fused Langevin step, three lanes per water (lane
+// 3k + a holds atom a of the wave's water k; lane 63 is an idle copy): each
+// lane loads its atom, takes the other two atoms' positions by shuffles for
+// the excluded pairs and for SETTLE (solved identically in all three lanes),
+// runs the value cores shared with the separate kernels and stores its atom
+// once.  e_ex and ke collect the lane's contributions.
+__device__ __forceinline__ void og_water_lanes(const og_dev_sys &s, const int blk, const long long step,
+                                               double &e_ex, double &ke) {
+    const int  lane = threadIdx.x & 63;
+    const int  grp  = min(lane / 3, OG_WATERS_PER_WAVE - 1);
+    const int  ac   = min(lane - 3 * grp, 2);
+    const int  base = 3 * grp;
+    const int  tr   = blk * OG_WATERS_PER_BLOCK + (threadIdx.x >> 6) * OG_WATERS_PER_WAVE + grp;
+    const bool live = lane < 3 * OG_WATERS_PER_WAVE && tr < s.n_settle;
+    const int  t    = min(tr, s.n_settle - 1);
+    const int  cap  = s.cap_atom_array;
+    int        ia[3];
+    for (int k = 0; k < 3; k++) ia[k] = s.settle_ids[3 * t + k];
+    const int   i  = ia[ac];
+    const int   g  = s.rev[i];
+    const int   e0 = s.exc_off[i], e1 = s.exc_off[i + 1];
+    const float q  = s.charge[i];
+    const float m  = s.mass[i];
+    const float mi = s.mass_inv[i];
+    double      x[3], xp[3], vn[3], w[3], gc[3];
+    for (int d = 0; d < 3; d++) {
+        x[d]  = s.crd[3 * i + d];
+        xp[d] = s.crd_prev[3 * i + d];
+        vn[d] = step == 0 ? s.vel_next[3 * i + d] : 0.0;
+        w[d]  = (double)s.fx[d * cap + g] * OG_FX_INV;
+        gc[d] = s.gcrd[3 * g + d];
+    }
+    const float gw = s.gxyzq[g].w;
+    int4        ent[2];
+    for (int j = 0; j < 2; j++)
+        if (e0 + j < e1) ent[j] = s.exc_at[e0 + j];
+    double X[3][3];
+    float  Q[3];
+    for (int k = 0; k < 3; k++) {
+        for (int d = 0; d < 3; d++) X[k][d] = __shfl(x[d], base + k, 64);
+        Q[k] = __shfl(q, base + k, 64);
+    }
+    // excluded pairs of the atom, in its list order (og_atom_work_calc)
+    auto excl = [&](const int4 en) {
+        const int   la = en.x == ia[0] ? 0 : (en.x == ia[1] ? 1 : 2);
+        const int   lb = en.y == ia[0] ? 0 : (en.y == ia[1] ? 1 : 2);
+        float       we[3];
+        const float ene = og_excess_pair(s, X[la], X[lb], Q[la], Q[lb], we);
+        if (la == ac) {
+            for (int d = 0; d < 3; d++) w[d] += (double)we[d];
+            if (live) e_ex += (double)ene;
+        } else {
+            for (int d = 0; d < 3; d++) w[d] -= (double)we[d];
+        }
+    };
+    for (int j = 0; j < 2; j++)
+        if (e0 + j < e1) excl(ent[j]);
+    for (int e = e0 + 2; e < e1; e++) excl(s.exc_at[e]);
+    double xn[3];
+    og_langevin_core(s, i, step, x, xp, vn, w, (double)m, (double)mi, xn);
+    double XN[3][3];
+    for (int k = 0; k < 3; k++)
+        for (int d = 0; d < 3; d++) XN[k][d] = __shfl(xn[d], base + k, 64);
+    const double mO = __shfl((double)m, base, 64), mH = __shfl((double)m, base + 1, 64);
+    if (!og_settle_core(s, mO, mH, s.settle_dOH, s.settle_dHH, XN[0], XN[1], XN[2], X[0], X[1], X[2]) && live &&
+        ac == 0)
+        atomicAdd(&s.cnt->shake_fail, 1);
+    double v[3], xw[3];
+    float  f[3];
+    const double ka = og_langevin_vel(s, step, XN[ac], xp, vn, v);
+    og_advance(s, XN[ac], x, gc, f, xw);
+    if (!live) return;
+    ke += ka * (double)m;
+    for (int d = 0; d < 3; d++) {
+        s.fx[d * cap + g]      = 0;
+        s.work[3 * i + d]      = w[d];
+        s.crd_prev2[3 * i + d] = xp[d];
+        s.crd_prev[3 * i + d]  = x[d];
+        s.crd[3 * i + d]       = xw[d];
+        s.vel[3 * i + d]       = step == 0 ? vn[d] : v[d];
+        s.gcrd[3 * g + d]      = gc[d];
+    }
+    s.gxyzq[g] = make_float4(f[0], f[1], f[2], gw);
+}
+
+__device__ void og_vcmd_block(const og_dev_sys &s, double *bias, int &active);  // VcMD section below
+
+__global__ void __launch_bounds__(OG_WATER_THREADS) k_langevin_step(const og_dev_sys *__restrict__ S,
+                                                                    const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
+    const long long step = s.cnt->step;
+    const int       tid  = threadIdx.x;
+    double          e_ex = 0.0, e_b = 0.0, ke = 0.0;
+    if (bm.y == 0) {
+        const int nf = s.n_free;
+        for (int k = tid; k < nf; k += OG_WATER_THREADS) og_atom_work(s, s.free_ids[k], e_ex, e_b);
+        __syncthreads();
+        if (s.vc_n_states) {
+            __shared__ double bias[3];
+            __shared__ int    active;
+            og_vcmd_block(s, bias, active);
+            __syncthreads();
+        }
+        for (int k = tid; k < nf; k += OG_WATER_THREADS) {
+            const int    i     = s.free_ids[k];
+            const double wk[3] = {s.work[3 * i], s.work[3 * i + 1], s.work[3 * i + 2]};
+            og_langevin_atom(s, i, step, wk);
+        }
+        __syncthreads();
+        if (s.constraint) {
+            for (int t = tid; t < s.n_pairs_c; t += OG_WATER_THREADS) og_shake_pair(s, t);
+            __syncthreads();
+            for (int t = tid; t < s.n_trios; t += OG_WATER_THREADS)
+                og_shake_trio(s, s.trio_ids[3 * t], s.trio_ids[3 * t + 1], s.trio_ids[3 * t + 2], s.trio_d2 + 3 * t);
+            __syncthreads();
+        }
+        for (int k = tid; k < nf; k += OG_WATER_THREADS) ke += og_finish_atom(s, s.free_ids[k], step);
+        for (int k = s.n_water_blocks + 1 + tid; k < s.n_ke_parts; k += OG_WATER_THREADS) s.e_atom[k] = 0.0;
+    } else {
+        og_water_lanes(s, bm.y - 1, step, e_ex, ke);
+    }
+    __shared__ double sh[2 * (OG_WATER_THREADS / 64)];
+    og_ke_partial(s, bm.y, ke, sh);
+    if (og_energy_step(s)) {
+        __syncthreads();  // sh is reused
+        og_add_misc_energies(s, bm.y, e_ex, e_b, sh);
     }
 }
 
@@ -431,7 +648,7 @@ __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys
     __shared__ long long she[16 * OG_EACC];
     if (og_energy_step(s)) og_take_pair_energies(s, she);
     double t = 0.0;
-    for (int k = threadIdx.x; k < s.n_atom_blocks; k += OG_STEP_THREADS) t += s.e_atom[k];
+    for (int k = threadIdx.x; k < s.n_ke_parts; k += OG_STEP_THREADS) t += s.e_atom[k];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
     __syncthreads();
@@ -464,13 +681,11 @@ __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys
 //    visit while lambda is in its range, and every vc_interval steps moves
 //    to a neighbouring state whose range holds lambda with probability
 //    proportional to 1/q_cano (uniform from Philox keyed by seed and step).
-__global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
-    const og_dev_sys &s = S[map[blockIdx.x].x];
-    if (og_frozen(s)) return;
-    if (s.vc_n_states == 0) return;
-    const int lane = threadIdx.x;
+// Called by every thread of a block (any size); bias and active live in LDS.
+__device__ void og_vcmd_block(const og_dev_sys &s, double *bias, int &active) {
+    const int lane = threadIdx.x, nt = blockDim.x;
     const int n    = s.vc_na + s.vc_nb;
-    for (int k = lane; k < n; k += 64) {
+    for (int k = lane; k < n; k += nt) {
         const int a = s.vc_atoms[k];
         for (int d = 0; d < 3; d++) {
             const double Lh   = (double)(s.L[d] * 0.5f);
@@ -488,8 +703,6 @@ __global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, c
         }
     }
     __syncthreads();
-    __shared__ double bias[3];
-    __shared__ int    active;
     if (lane == 0) {
         // mass centres summed in group order, as the reference loop
         double ca[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
@@ -544,13 +757,22 @@ __global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, c
     __syncthreads();
     if (!active) return;
     // gradient ("work") of the group atoms; the groups are disjoint
-    for (int k = lane; k < n; k += 64) {
+    for (int k = lane; k < n; k += nt) {
         const int    a    = s.vc_atoms[k];
         const bool   ga   = k < s.vc_na;
         const double minv = ga ? s.vc_minv_a : s.vc_minv_b;
         const double dir  = ga ? 1.0 : -1.0;
         for (int d = 0; d < 3; d++) s.work[3 * a + d] += dir * (bias[d] * minv) * (double)s.mass[a];
     }
+}
+
+__global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (og_frozen(s)) return;
+    if (s.vc_n_states == 0) return;
+    __shared__ double bias[3];
+    __shared__ int    active;
+    og_vcmd_block(s, bias, active);
 }
 
 // ------------------------------------------------------------- launchers --
@@ -577,6 +799,9 @@ void og_launch_constraints(const og_launch &trio, const og_launch &misc) {
         hipLaunchKernelGGL(k_shake_pair, dim3(misc.n_blocks), dim3(OG_THREADS), 0, misc.stream, misc.sys, misc.map);
     if (trio.n_blocks)
         hipLaunchKernelGGL(k_trio, dim3(trio.n_blocks), dim3(OG_TRIO_THREADS), 0, trio.stream, trio.sys, trio.map);
+}
+void og_launch_langevin_step(const og_launch &a) {
+    hipLaunchKernelGGL(k_langevin_step, dim3(a.n_blocks), dim3(OG_WATER_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_finish(const og_launch &a) {
     hipLaunchKernelGGL(k_finish, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);

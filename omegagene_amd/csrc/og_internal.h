@@ -61,6 +61,8 @@ struct og_dev_sys {
     int cap_atom_array, cap_cells, cap_cell_pairs;
     int n_types;
     int n_excess, n_bonds, n_trios, n_pairs_c, n_settle;
+    int n_free;            // atoms outside SETTLE waters (fused Langevin step)
+    int n_water_blocks;    // fused Langevin step: blocks of OG_WATERS_PER_BLOCK waters; 0 = not fusable
     int n_com_groups;
     int integrator;        // 0 leapfrog-presto, 1 langevin
     int constraint;        // 0 none, 1 shake, 2 shake+settle
@@ -111,6 +113,7 @@ struct og_dev_sys {
     int    *pairc_ids;      // [n_pairs_c*2]
     double *pairc_d2;
     int    *settle_ids;     // [n_settle*3]
+    int    *free_ids;       // [n_free] atoms outside SETTLE waters
     int    *com_off, *com_atoms;   // COM-cancel groups
     float  *com_minv;
     // ---- grid ----
@@ -144,11 +147,12 @@ struct og_dev_sys {
     long long *e_acc;       // [OG_EACC_SLOTS][OG_EACC_STRIDE] energy accumulators, 2^32 fixed point (the pair
                             // count as an integer): vdw, ele, virial, pairs, excess, bond; a cell / block adds to
                             // slot (index % OG_EACC_SLOTS), so no cache line takes every add; zeroed by the reader
-    double    *e_atom;      // [n_atom_blocks] kinetic partials
+    double    *e_atom;      // [n_ke_parts] kinetic partials (one per block of the finishing kernel)
     double    *com_mom;     // [n_com_groups*3]
     double    *energy;      // [OG_E_NTERMS] last step
     double    *elog;        // [log_cap*OG_E_NTERMS]
     int        n_atom_blocks;
+    int        n_ke_parts;   // max(n_atom_blocks, water blocks + 1 of the fused step)
     og_dev_counters *cnt;
 };
 
@@ -168,6 +172,10 @@ enum og_map_kind {
 
 #define OG_THREADS 256
 #define OG_TRIO_THREADS 128
+#define OG_WATER_THREADS 256  // fused Langevin step: threads per block
+#define OG_WATERS_PER_WAVE 21  // fused Langevin step: three lanes per water
+#define OG_WATERS_PER_BLOCK (OG_WATERS_PER_WAVE * (OG_WATER_THREADS / 64))
+#define OG_FREE_MAX 4096      // fused Langevin step: most atoms outside SETTLE waters (one block takes them)
 #define OG_CP_BLOCKS_PER_SYS_MIN 64
 
 // ---- small device helpers ----
@@ -299,6 +307,7 @@ void og_launch_com_apply(const og_launch &atoms);
 void og_launch_positions(const og_launch &atoms);
 void og_launch_langevin_positions(const og_launch &atoms);
 void og_launch_constraints(const og_launch &trio, const og_launch &misc);
+void og_launch_langevin_step(const og_launch &a);
 void og_launch_finish(const og_launch &atoms);
 void og_launch_step_close(const og_launch &sys);  // energies of the step, log entry, step counter
 void og_launch_vcmd(const og_launch &sys);

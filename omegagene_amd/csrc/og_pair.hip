@@ -19,7 +19,7 @@
 // FP32 pair math as the reference, fixed-point atomics (order-independent,
 // so every run gives the same bits), energies summed per lane in FP64 and
 // reduced per cell in a fixed order.
-#include "og_internal.h"
+#include "og_terms.h"
 
 #define OG_CHARGE_COEFF 332.06378f   // CelesteObject.cpp:36
 #define OG_QCAP 128                  // per-wave row queue (two passes of 64)
@@ -310,149 +310,19 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
     else og_pair_cell<ALPHA, false>(s, c1, shift, qpos[w], qaux[w]);
 }
 
-// ------------------------------------------------------------ misc terms --
-// Excluded-pair ZD terms (SubBox::calc_energy_ele_excess SubBox.cpp:1347-1367,
-// ForceField::calc_zms_excess ForceField.cpp:420-437,
-// calc_zero02pole_excess(_alpha0) ZeroMultipoleSum.cpp:191-217) in the
-// reference float arithmetic, and harmonic bonds E = k (r - r0)^2 in FP64
-// (ForceField::calc_bond ForceField.cpp:108-136).  Both add into the
-// fixed-point gradient at the atoms' grid slots.
-__device__ __forceinline__ void og_block_sum2(double &a, double &b, double *sh) {
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_down(a, off, 64);
-        b += __shfl_down(b, off, 64);
-    }
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) {
-        sh[2 * w]     = a;
-        sh[2 * w + 1] = b;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        a = 0.0;
-        b = 0.0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
-            a += sh[2 * k];
-            b += sh[2 * k + 1];
-        }
-    }
-}
-
-// Excluded pair (a, b), a < b in the excess list: gradient w added to a and
-// subtracted from b, energy returned (reference float arithmetic).
-__device__ __forceinline__ float og_excess_pair(const og_dev_sys &s, int a, int b, float *w) {
-#pragma clang fp contract(off)
-    float d[3];
-    for (int k = 0; k < 3; k++) {
-        const float Lh  = s.L[k] * 0.5f;
-        const float Lhi = 1.0f / Lh;
-        float       v   = (float)s.crd[3 * a + k] - (float)s.crd[3 * b + k];
-        v               = v - s.L[k] * (float)(int)(v * Lhi);
-        v               = v - s.L[k] * (float)(int)(v * Lhi);
-        d[k]            = v;
-    }
-    const float r2    = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-    const float r     = sqrtf(r2);
-    const float rinv  = 1.0f / r;
-    const float r2inv = rinv * rinv;
-    const float r3inv = r2inv * rinv;
-    const float cc    = (float)((double)(s.charge[a] * s.charge[b]) * 332.06378);
-    float       ene, gc;
-    if (s.alpha == 0.0f) {
-        ene = cc * s.bcoeff * r2;
-        gc  = cc * s.fcoeff;
-    } else {
-        const float x     = s.alpha * r;
-        const float erfcv = erfcf(x);
-        const float erfv  = (float)(1.0 - (double)erfcv);
-        ene = s.excess_bug ? cc * (-erfv * r + s.bcoeff * r2) : cc * (-erfv * rinv + s.bcoeff * r2);
-        gc  = -cc * (erfcv * r3inv + s.piewald * expf(-x * x) * r2inv - s.fcoeff - r3inv);
-    }
-    for (int k = 0; k < 3; k++) w[k] = gc * d[k];
-    return ene;
-}
-
-// Harmonic bond (a, b): gradient w added to a and subtracted from b, energy returned (FP64).
-__device__ __forceinline__ double og_bond_pair(const og_dev_sys &s, int t, int a, int b, double *w) {
-#pragma clang fp contract(off)
-    double d[3];
-    for (int k = 0; k < 3; k++) {
-        const float Lh  = s.L[k] * 0.5f;
-        const float Lhi = 1.0f / Lh;
-        double      v   = s.crd[3 * a + k] - s.crd[3 * b + k];
-        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-        d[k]            = v;
-    }
-    const double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const double dif  = r - s.bond_r0[t];
-    const double coef = s.bond_k[t] * dif;
-    const double gcf  = 2.0 * coef / r;
-    for (int k = 0; k < 3; k++) w[k] = d[k] * gcf;
-    return coef * dif;
-}
-
-// Per atom: fixed-point pair gradient (grid order) -> FP64 "work" in atom order
-// (SubBox::add_work_from_minicell, SubBox.cpp:1398-1407), plus the atom's
-// excluded-pair ZD terms (SubBox::calc_energy_ele_excess SubBox.cpp:1347-1367,
-// ForceField::calc_zms_excess ForceField.cpp:420-437,
-// calc_zero02pole_excess(_alpha0) ZeroMultipoleSum.cpp:191-217) and bonds
-// E = k (r - r0)^2 (ForceField::calc_bond ForceField.cpp:108-136) through
-// per-atom lists: every pair term is evaluated for both of its atoms in the
-// orientation of the reference list, so no atomics are needed and the sums
-// are deterministic.  Energies are counted on the pair's first atom.
-// Clears the fixed-point accumulators for the next evaluation.
+// Per atom: fixed-point pair gradient -> FP64 "work" with the atom's
+// excluded-pair and bond terms (og_atom_work).
 __global__ void __launch_bounds__(OG_THREADS) k_forces_to_work(const og_dev_sys *__restrict__ S,
                                                                const int2 *__restrict__ map) {
-#pragma clang fp contract(off)
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
     if (og_frozen(s)) return;
-    const int i   = bm.y * OG_THREADS + threadIdx.x;
+    const int i    = bm.y * OG_THREADS + threadIdx.x;
     double    e_ex = 0.0, e_b = 0.0;
-    if (i < s.n_atoms) {
-        const int g   = s.rev[i];
-        const int cap = s.cap_atom_array;
-        double    w[3];
-        for (int k = 0; k < 3; k++) {
-            const long long v = s.fx[k * cap + g];
-            s.fx[k * cap + g] = 0;
-            w[k]              = (double)v * OG_FX_INV;
-        }
-        for (int e = s.exc_off[i]; e < s.exc_off[i + 1]; e++) {
-            const int4 ent = s.exc_at[e];
-            const int  a = ent.x, b = ent.y;
-            float     we[3];
-            const float ene = og_excess_pair(s, a, b, we);
-            if (i == a) {
-                for (int k = 0; k < 3; k++) w[k] += (double)we[k];
-                e_ex += (double)ene;
-            } else {
-                for (int k = 0; k < 3; k++) w[k] -= (double)we[k];
-            }
-        }
-        for (int e = s.bond_off[i]; e < s.bond_off[i + 1]; e++) {
-            const int4 ent = s.bond_at[e];
-            const int  a = ent.x, b = ent.y, t = ent.z;
-            double    wb[3];
-            const double ene = og_bond_pair(s, t, a, b, wb);
-            if (i == a) {
-                for (int k = 0; k < 3; k++) w[k] += wb[k];
-                e_b += ene;
-            } else {
-                for (int k = 0; k < 3; k++) w[k] -= wb[k];
-            }
-        }
-        for (int k = 0; k < 3; k++) s.work[3 * i + k] = w[k];
-    }
+    if (i < s.n_atoms) og_atom_work(s, i, e_ex, e_b);
     if (!og_energy_step(s)) return;
     __shared__ double sh[2 * (OG_THREADS / 64)];
-    og_block_sum2(e_ex, e_b, sh);
-    if (threadIdx.x == 0) {
-        OG_GLOBAL(long long) *A = (OG_GLOBAL(long long) *)s.e_acc + (bm.y % OG_EACC_SLOTS) * OG_EACC_STRIDE;
-        og_atomic_add_fx_g(A + OG_EACC_EXCESS, og_to_fixed(e_ex));
-        og_atomic_add_fx_g(A + OG_EACC_BOND, og_to_fixed(e_b));
-    }
+    og_add_misc_energies(s, bm.y, e_ex, e_b, sh);
 }
 
 // Energies of a single-point force call from the fixed-point accumulators

@@ -57,7 +57,7 @@ class Params(C.Structure):
                 ("temperature", C.c_double), ("langevin_gamma", C.c_double), ("random_seed", C.c_uint64),
                 ("com_cancel", C.c_int), ("gap_bug", C.c_int), ("excess_bug", C.c_int), ("log_stride", C.c_int),
                 ("log_cap", C.c_int), ("capacity_factor", C.c_double), ("enhance_sigma", C.c_double),
-                ("enhance_recov_coef", C.c_double)]
+                ("enhance_recov_coef", C.c_double), ("split_step", C.c_int)]
 
 
 def lib():
@@ -97,7 +97,7 @@ def params_from_cfg(cfg: dict) -> Params:
     p.random_seed = int(g("random-seed", 0)) & 0xFFFFFFFFFFFFFFFF
     p.com_cancel = 1 if g("com-motion", "none") == "cancel" else 0
     for k, attr in (("gap-bug", "gap_bug"), ("excess-bug", "excess_bug"), ("log-stride", "log_stride"),
-                    ("log-cap", "log_cap")):
+                    ("log-cap", "log_cap"), ("split-step", "split_step")):
         if k in cfg:
             setattr(p, attr, int(cfg[k]))
     if "capacity-factor" in cfg:
@@ -284,6 +284,12 @@ class System:
 
     def set_pair_timing(self, on=True):
         _check(lib().og_set_pair_timing(self.h, C.c_int(int(on))))
+
+    def step_fused(self):
+        """True when an MD step runs as the fused one-kernel Langevin step."""
+        f = C.c_int()
+        _check(lib().og_step_fused(self.h, C.byref(f)))
+        return bool(f.value)
 
     def pair_time(self):
         ms, n = C.c_double(), C.c_int()

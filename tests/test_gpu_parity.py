@@ -276,6 +276,61 @@ def test_langevin_reaches_target_temperature():
     s.close()
 
 
+@pytest.mark.parametrize("vcmd", [False, True])
+def test_fused_langevin_step_equals_split_kernels(vcmd):
+    """The one-kernel Langevin step (SETTLE waters one per thread; the chain,
+    its bonds and the VcMD groups in one block) gives bitwise the trajectory,
+    potential terms and VcMD walk of the separate kernels, alone and in a
+    batch.  The excluded-pair, bond and kinetic energies are block sums
+    grouped by water instead of by atom index (FP64, then 2^-32 fixed point
+    for the first two), so they agree to relative 1e-13 or 64 fixed-point
+    quanta."""
+    lss = [gen.walker_system(n, seed=61 + k, vcmd=vcmd) for k, n in enumerate((4000, 6500))]
+    for ls in lss:
+        ls.settle_ids, ls.settle_d = ls.shake_trio_ids, ls.shake_trio_d
+        ls.shake_trio_ids = np.zeros((0, 3), np.int32)
+        ls.shake_trio_d = np.zeros((0, 3))
+    cfg = dict(BASE, **{"integrator": "langevin", "constraint": "shake-settle", "langevin-gamma": 20.0,
+                        "temperature": 300.0, "nsgrid-update-intvl": 5, "random-seed": 3, "log-cap": 64,
+                        "gap-bug": 0})
+    if vcmd:
+        cfg.update({"extended-ensemble": "vcmd", "enhance-sigma": 0.5, "enhance-recovery-coef": 1.0})
+    say("fused step", [ls.n_atoms for ls in lss], "vcmd" if vcmd else "")
+    runs = {}
+    for split in (1, 0):
+        out = []
+        for ls in lss:
+            s = System(ls, dict(cfg, **{"split-step": split}))
+            assert s.step_fused() == (split == 0)
+            s.run(40)
+            out.append((s.state(), s.energy_log(), s.vcmd(len(ls.vcmd.dims[0][2])) if vcmd else None))
+            s.close()
+        runs[split] = out
+    b = Batch()
+    ss = [System(ls, cfg) for ls in lss]
+    for s in ss:
+        b.add(s)
+    b.run(40)
+    batch = [(s.state(), s.energy_log(), s.vcmd(len(ls.vcmd.dims[0][2])) if vcmd else None)
+             for s, ls in zip(ss, lss)]
+    b.close()
+    for s in ss:
+        s.close()
+    grouped = [2, 4, 5]  # excess, bond, kinetic
+    for (st1, log1, vc1), (st0, log0, vc0), (stb, logb, vcb) in zip(runs[1], runs[0], batch):
+        for a, c, d in zip(st1, st0, stb):
+            assert np.array_equal(a, c) and np.array_equal(c, d)
+        other = [k for k in range(log1.shape[1]) if k not in grouped]
+        assert np.array_equal(log1[:, other], log0[:, other])
+        assert np.array_equal(log0, logb)
+        for k in grouped:
+            rel = np.max(np.abs(log1[:, k] - log0[:, k]) / np.maximum(np.abs(log1[:, k]), 1e-300))
+            say("  40 steps: positions and velocities identical; energy term %d rel diff %.2e" % (k, rel))
+            assert rel <= 1e-13 or np.max(np.abs(log1[:, k] - log0[:, k])) <= 64 * 2.0 ** -32
+        if vcmd:
+            assert vc1[0] == vc0[0] == vcb[0] and np.array_equal(vc1[1], vc0[1]) and np.array_equal(vc0[1], vcb[1])
+
+
 def test_box_too_small_is_an_error():
     ls = gen.build_system(60, seed=1, L=12.0)
     with pytest.raises(EngineError):
