@@ -13,7 +13,8 @@ per step, counted by the pair kernel, times steps/s), ns/day beside it, the
 pair-kernel roofline (algorithmic 41 FLOP per pair at alpha 0 over the
 pair-kernel time measured with HIP events on the engine's stream) and the
 CPU baseline (the oracle's single-thread cell-list force evaluation of the
-same system).
+same system).  Beside it ("alpha_large", skipped with --no-large): the pair
+kernel at ZD alpha 0.1 on 400,000 waters (1.2 M atoms, rc 10 A, list 11 A).
 """
 import argparse
 import json
@@ -106,6 +107,39 @@ def equilibrate(systems_cfgs, steps=300, gamma=20.0):
     for x in systems:
         x.close()
     return out
+
+
+def bench_large(steps=30, warmup=10):
+    """400,000 waters (1,200,000 atoms, 228.7 A box), rc 10 A, list 11 A, ZD
+    alpha 0.1: the pair kernel at alpha > 0 timed with HIP events over MD
+    steps (Langevin 300 K at gamma 20/ps straight from the seeded lattice,
+    SETTLE, rebuild every 10 steps)."""
+    from omegagene_amd import gen
+    from omegagene_amd.engine import System, lib
+    ls, cfg = gen.config2()
+    ls = _settle_waters(ls)
+    cfg = dict(cfg, **{"integrator": "langevin", "langevin-gamma": 20.0, "log-cap": steps})
+    s = System(ls, cfg)
+    s.run(warmup)
+    s.set_pair_timing(True)
+    lib().og_thread_sync()
+    t0 = time.perf_counter()
+    s.run(steps)
+    lib().og_thread_sync()
+    wall = time.perf_counter() - t0
+    pair_ms, n_launch = s.pair_time()
+    elog = s.energy_log(max_rows=steps)
+    s.close()
+    pairs = float(elog[:, 8].mean())
+    t_launch = pair_ms / max(n_launch, 1) * 1e-3
+    tflops = FLOP_PER_PAIR[1] * pairs / t_launch / 1e12
+    log("1.2 M atoms, alpha 0.1: %d steps in %.3f s, pair kernel %.3f ms" % (steps, wall, t_launch * 1e3))
+    return {"workload": "400,000 waters, 1,200,000 atoms, 228.7 A box, rc 10 A, list 11 A, ZD alpha 0.1, "
+                        "Langevin 300 K (gamma 20/ps from the lattice), SETTLE, rebuild every 10 steps",
+            "pairs_per_step": pairs, "ms_per_step": wall / steps * 1e3,
+            "pair_interactions_per_s": pairs * steps / wall, "pair_kernel_ms": t_launch * 1e3,
+            "pair_kernel_tflops": tflops, "flop_per_pair": FLOP_PER_PAIR[1],
+            "frac_of_fp32_peak": tflops / PEAK_FP32_TFLOPS, "timed_steps": steps}
 
 
 def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
@@ -228,6 +262,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-walkers", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -293,6 +328,7 @@ def main():
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
     walkers = None if args.no_walkers else bench_walkers()
     serving = None if args.no_serving else bench_serving()
+    large = None if args.no_large else bench_large()
 
     out = {
         "metric": "pair interactions/s (99,200 atoms, 12 A, ZD alpha 0, Langevin, dt 2 fs)",
@@ -319,6 +355,7 @@ def main():
         "cpu_baseline": cpu,
         "walkers": walkers,
         "serving": serving,
+        "alpha_large": large,
     }
     s.close()
     print(json.dumps(out))
