@@ -109,7 +109,7 @@ def equilibrate(systems_cfgs, steps=300, gamma=20.0):
     return out
 
 
-def bench_large(steps=30, warmup=10):
+def bench_large(steps=30, warmup=10, split=False):
     """400,000 waters (1,200,000 atoms, 228.7 A box), rc 10 A, list 11 A, ZD
     alpha 0.1: the pair kernel at alpha > 0 timed with HIP events over MD
     steps (Langevin 300 K at gamma 20/ps straight from the seeded lattice,
@@ -118,7 +118,8 @@ def bench_large(steps=30, warmup=10):
     from omegagene_amd.engine import System, lib
     ls, cfg = gen.config2()
     ls = _settle_waters(ls)
-    cfg = dict(cfg, **{"integrator": "langevin", "langevin-gamma": 20.0, "log-cap": steps})
+    cfg = dict(cfg, **{"integrator": "langevin", "langevin-gamma": 20.0, "log-cap": steps,
+                       "split-step": int(split)})
     s = System(ls, cfg)
     s.run(warmup)
     s.set_pair_timing(True)
@@ -263,10 +264,18 @@ def main():
     ap.add_argument("--no-walkers", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--large-only", action="store_true",
+                    help="only the 1.2 M-atom alpha 0.1 measurement (its own JSON line; for profiling)")
+    ap.add_argument("--split-step", action="store_true",
+                    help="run the integration step as separate kernels (for profiling them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.large_only:
+        print(json.dumps({"alpha_large": bench_large(steps=args.steps, warmup=args.warmup,
+                                                     split=args.split_step)}))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs each system (or walker batch) on a single "
@@ -284,6 +293,7 @@ def main():
     ls.shake_trio_ids = np.zeros((0, 3), np.int32)
     ls.shake_trio_d = np.zeros((0, 3))
     cfg["log-cap"] = max(args.steps, 1)
+    cfg["split-step"] = int(args.split_step)
     log("system: %d atoms, box %.3f A, cutoff %.1f / list %.1f, rebuild every %d steps" %
         (ls.n_atoms, ls.box[0], cfg["cutoff"], cfg["nsgrid-cutoff"], cfg["nsgrid-update-intvl"]))
     ls = equilibrate([(ls, cfg)])[0]

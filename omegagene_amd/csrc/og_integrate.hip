@@ -648,7 +648,15 @@ __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys
     __shared__ long long she[16 * OG_EACC];
     if (og_energy_step(s)) og_take_pair_energies(s, she);
     double t = 0.0;
-    for (int k = threadIdx.x; k < s.n_ke_parts; k += OG_STEP_THREADS) t += s.e_atom[k];
+    // loads issued four strides at a time, added in stride order
+    for (int k = threadIdx.x; k < s.n_ke_parts; k += 4 * OG_STEP_THREADS) {
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = k + j * OG_STEP_THREADS < s.n_ke_parts ? s.e_atom[k + j * OG_STEP_THREADS] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (k + j * OG_STEP_THREADS < s.n_ke_parts) t += v[j];
+    }
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
     __syncthreads();
