@@ -143,6 +143,56 @@ def bench_large(steps=30, warmup=10, split=False):
             "frac_of_fp32_peak": tflops / PEAK_FP32_TFLOPS, "timed_steps": steps}
 
 
+def bench_nve(steps=1000, relax=300, gap_bug=1):
+    """NVE energy drift per configuration: each system is relaxed (position
+    Langevin, gamma 20/ps, SETTLE or SHAKE as configured) and then run
+    `steps` leapfrog steps at dt 2 fs with energies logged every step.  Drift
+    is the slope of a least-squares line through the total energy
+    (potential + kinetic), in kcal/mol per ns and per atom per ns; the
+    fluctuation is the RMS residual about that line."""
+    import numpy as np
+    from omegagene_amd import gen
+    from omegagene_amd.engine import EngineError, System
+    out = []
+    cases = [("6,000 atoms (2,000 waters), rc 12 A, alpha 0, SHAKE", gen.config0(), False),
+             ("99,200 atoms (33,000 waters + 200-bead chain), rc 12 A, alpha 0, SETTLE", gen.config1(), True),
+             ("1,200,000 atoms (400,000 waters), rc 10 A, alpha 0.1, SETTLE", gen.config2(), True)]
+    for name, (ls, cfg), settle in cases:
+        if settle:
+            ls = _settle_waters(ls)
+        cfg = dict(cfg, **{"constraint": "shake-settle" if settle else "shake", "integrator": "langevin",
+                           "temperature": 300.0, "gap-bug": gap_bug})
+        rule = "reference z-bin scan" if gap_bug else "full z-bin scan"
+        try:
+            ls = equilibrate([(ls, cfg)], steps=relax)[0]
+            nve = dict(cfg, **{"integrator": "leapfrog-presto", "log-cap": steps})
+            s = System(ls, nve)
+            try:
+                s.run(steps)
+                el = s.energy_log(max_rows=steps)
+            finally:
+                s.close()
+        except EngineError as e:
+            out.append({"workload": name, "n_atoms": int(ls.n_atoms), "list_rule": rule, "error": str(e)})
+            log("nve %s (%s): %s" % (name, rule, e))
+            continue
+        tot = el[:, 0] + el[:, 1] + el[:, 2] + el[:, 3] + el[:, 4] + el[:, 5] + el[:, 7]
+        t_ns = np.arange(len(tot)) * cfg["time-step"] * 1e-6
+        slope, icpt = np.polyfit(t_ns, tot, 1)
+        resid = tot - (slope * t_ns + icpt)
+        out.append({"workload": name, "n_atoms": int(ls.n_atoms), "steps": steps, "dt_fs": cfg["time-step"],
+                    "list_rule": rule,
+                    "drift_kcal_per_mol_per_ns": float(slope),
+                    "drift_kcal_per_mol_per_ns_per_atom": float(slope / ls.n_atoms),
+                    "fluctuation_kcal_per_mol": float(resid.std()), "mean_kinetic_kcal_per_mol": float(el[:, 5].mean()),
+                    "total_start": float(tot[0]), "total_end": float(tot[-1])})
+        r = out[-1]
+        log("nve %s (%s): drift %.3f kcal/mol/ns (%.2e per atom), fluctuation %.3f" %
+            (name, r["list_rule"], r["drift_kcal_per_mol_per_ns"], r["drift_kcal_per_mol_per_ns_per_atom"],
+             r["fluctuation_kcal_per_mol"]))
+    return out
+
+
 def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     """64 independent water + 200-bead-chain walkers (3,000-30,000 atoms,
     log-uniform), Langevin 300 K, SETTLE, VcMD bias on the chain end-to-end
@@ -266,12 +316,17 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 1.2 M-atom alpha 0.1 measurement (its own JSON line; for profiling)")
+    ap.add_argument("--nve-drift", action="store_true",
+                    help="only the NVE energy drift of the three single-system configurations (own JSON line)")
     ap.add_argument("--split-step", action="store_true",
                     help="run the integration step as separate kernels (for profiling them)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.nve_drift:
+        print(json.dumps({"nve_drift": bench_nve(gap_bug=0) + bench_nve(gap_bug=1)}))
         return
     if args.large_only:
         print(json.dumps({"alpha_large": bench_large(steps=args.steps, warmup=args.warmup,

@@ -140,7 +140,7 @@ class Oracle:
     def pairs_fp64(self, zd, cps, crd, atom_type, lj6, lj12, charge):
         n_types = lj6.shape[0]
         work = np.zeros((self.n_atoms, 3))
-        en = np.zeros(2)
+        en = np.zeros(3)  # vdw, ele, virial
         cps = np.ascontiguousarray(cps)
         f = lib().orc_pairs_celllist_fp64
         f.restype = C.c_int64

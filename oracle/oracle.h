@@ -129,11 +129,13 @@ int64_t orc_pairs_allpairs_fp32(const orc_zd_t *zd, const float L[3], int n_atom
 
 /* FP64 reference of the same physics (not a restatement of reference
  * rounding): double coordinates, exact minimum image, erfc in double.
- * Used to state force RMS relative error.  Uses the cell list for O(N). */
+ * Used to state force RMS relative error.  Uses the cell list for O(N).
+ * energy: vdw, ele, virial (sum over pairs of g r^2 = r_ij . dE/dr_i; the
+ * reference has no virial). */
 int64_t orc_pairs_celllist_fp64(const orc_zd_t *zd, const orc_grid_t *g, const orc_cells_t *c,
                                 const orc_cellpair_t *cps, int n_cp, const double *crd_atom,
                                 const int *atom_type, int n_types, const float *lj6, const float *lj12,
-                                const float *charge, double *work_atom, double energy[2]);
+                                const float *charge, double *work_atom, double energy[3]);
 
 /* ZD excess over excluded pairs (SubBox.cpp:1347-1367, ForceField.cpp:420-437,
  * ZeroMultipoleSum.cpp:191-217).  bug_compat != 0 keeps the alpha>0 energy

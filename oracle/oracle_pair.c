@@ -201,13 +201,13 @@ int64_t orc_pairs_allpairs_fp32(const orc_zd_t *zd, const float L[3], int n, con
 int64_t orc_pairs_celllist_fp64(const orc_zd_t *zd, const orc_grid_t *g, const orc_cells_t *c,
                                 const orc_cellpair_t *cps, int n_cp, const double *crd_atom,
                                 const int *atom_type, int n_types, const float *lj6, const float *lj12,
-                                const float *charge, double *work, double energy[2]) {
+                                const float *charge, double *work, double energy[3]) {
     int64_t n_in = 0;
     int     n_atoms = 0;
     for (int i = 0; i < c->n_atom_array; i++)
         if (c->atomids[i] + 1 > n_atoms) n_atoms = c->atomids[i] + 1;
     memset(work, 0, sizeof(double) * (size_t)n_atoms * 3);
-    energy[0] = energy[1] = 0.0;
+    energy[0] = energy[1] = energy[2] = 0.0;
     const double rc = zd->cutoff, alpha = zd->alpha;
     /* constants in double from the same closed forms */
     double fcoeff, bcoeff, zcore, piewald = 0.0;
@@ -261,6 +261,7 @@ int64_t orc_pairs_celllist_fp64(const orc_zd_t *zd, const orc_grid_t *g, const o
                 }
                 energy[0] += ev;
                 energy[1] += ee;
+                energy[2] += gcoef * r2; /* virial: r_ij . dE/dr_i */
                 for (int k = 0; k < 3; k++) {
                     work[id1 * 3 + k] += gcoef * d[k];
                     work[id2 * 3 + k] -= gcoef * d[k];

@@ -117,6 +117,9 @@ def test_forces_energies_vs_oracle(case):
     assert e["excess"] == pytest.approx(eo["excess"], rel=1e-9 if cfg["ele-alpha"] == 0 else 2e-6, abs=1e-9)
     assert e["bond"] == pytest.approx(eo["bond"], rel=1e-9, abs=1e-12)
     assert e["self"] == eo["self"]
+    # virial (sum over pairs of g r^2; new, the reference has none) against the FP64 oracle
+    say("  virial gpu %.6f  fp64 %.6f" % (e["virial"], e64[2]))
+    assert e["virial"] == pytest.approx(e64[2], rel=2e-6)
     # Newton's third law on the fixed-point sums
     assert np.max(np.abs(w.sum(0))) < 1e-5 * np.abs(w).max()
     s.close()

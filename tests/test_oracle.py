@@ -164,7 +164,7 @@ def test_fp64_pair_gradient_finite_difference():
         o.build(crd)
         cps, _ = o.enumerate(ls.nb15off_table(), gap_bug=False)
         w, en, _ = o.pairs_fp64(zd, cps, crd, ls.atom_type, lj6, lj12, q)
-        return en.sum(), w
+        return en[0] + en[1], w
 
     e0, w0 = energy_and_grad(ls.crd)
     h = 1e-5
@@ -359,6 +359,8 @@ def test_fp64_cell_list_equals_brute_force():
     assert n64 == len(r)
     assert rms_rel(w64, wb) < 1e-12
     assert e64[0] == pytest.approx(np.sum(a12 * ri ** 12 - a6 * ri ** 6), rel=1e-12)
+    # virial: sum over pairs of g r^2 (closed form per pair, brute force)
+    assert e64[2] == pytest.approx(np.sum(g * r * r), rel=1e-12)
 
 
 def test_settle_reports_waters_without_rigid_solution():
