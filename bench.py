@@ -228,6 +228,9 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
         b.add(s)
     lib().og_batch_set_graph_steps(b.h, steps)
     b.run(warmup)
+    tune = b.autotune(5)
+    log("walkers: autotuned pair kernel %d waves per block, round robin %s" % (tune["waves_per_block"],
+                                                                              tune["round_robin"]))
     lib().og_thread_sync()
     t0 = time.perf_counter()
     b.run(steps)
@@ -242,7 +245,7 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
                         "Langevin 300 K, SETTLE, VcMD 8 windows, rc 12 A" % n_atoms,
             "batched_walker_steps_per_s": batch_rate, "one_at_a_time_walker_steps_per_s": seq_rate,
             "speedup": batch_rate / seq_rate, "atom_steps_per_s": n_atoms * steps / wall,
-            "graph_steps": steps, "timed_steps": steps}
+            "graph_steps": steps, "timed_steps": steps, "pair_shape": tune}
 
 
 def bench_serving(n_requests=48, rate=200.0, budget_gb=200.0, chunk=20, seed=7):
@@ -357,6 +360,8 @@ def main():
     alpha_mode = 1 if cfg["ele-alpha"] > 0 else 0
     s.run(args.warmup)
     log("warmup %d steps done" % args.warmup)
+    tune = s.autotune(5)
+    log("autotuned pair kernel: %d waves per block (%.3f ms)" % (tune["waves_per_block"], tune["pair_ms"]))
 
     s.set_pair_timing(True)
     lib().og_thread_sync()
@@ -415,7 +420,7 @@ def main():
         "temperature_K": T,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                     "kernel": "k_pair<%d>" % alpha_mode, "kernel_ms": t_launch * 1e3,
+                     "kernel": "k_pair<%d, %d>" % (alpha_mode, tune["waves_per_block"]), "kernel_ms": t_launch * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR[alpha_mode]},
         "cpu_baseline": cpu,
         "walkers": walkers,

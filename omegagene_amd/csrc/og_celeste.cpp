@@ -551,7 +551,7 @@ og_status og_pairwise_ljzd(int flg_mod_15mask) {
     og_dev_sys &h = g_cs.h;
     CS_TRY(cs_push_sys());
     CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 3 * (size_t)h.cap_atom_array, g_cs.stream));
-    og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), 0);
+    og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), 0, 4);
     og_launch_energy_reduce(cs_launch(g_cs.map_sys, 1));
     CS_HIP(hipGetLastError());
     return OG_OK;

@@ -161,7 +161,10 @@ int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 void og_set_error(const char *msg) { g_last_error = msg; }
 
 // ----------------------------------------------------------------- types --
-enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_LSTEP, KIND_N };
+enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_LSTEP,
+       KIND_PAIR,  // 8 pair-kernel maps: index log2(waves per block) + 4 * (blocks round-robin across systems)
+       KIND_N = KIND_PAIR + 8 };
+constexpr int OG_PAIR_DEFAULT = 2;  // 4 waves per block, blocks in system order
 
 struct og_system {
     og_dev_sys          h{};           // host copy of the descriptor (device pointers)
@@ -194,6 +197,7 @@ struct og_exec {
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false;
     bool                     fused = false;       // every member takes the fused Langevin step
+    int                      pair_shape = OG_PAIR_DEFAULT;  // pair-kernel launch shape (KIND_PAIR + pair_shape)
     int                      intvl = 1;
     int                      graph_steps = 0;     // steps per captured graph (0: automatic)
 };
@@ -268,8 +272,20 @@ og_status exec_prepare(og_exec &x) {
         cnt[KIND_TRIO]  = cdiv(std::max(d.n_trios, d.n_settle), OG_TRIO_THREADS);
         cnt[KIND_COMA] = 0;  // filled below
         cnt[KIND_LSTEP] = d.n_water_blocks ? d.n_water_blocks + 1 : 0;  // free-atom block + water blocks
-        for (int k = 0; k < KIND_N; k++)
+        for (int k = 0; k < KIND_PAIR; k++)
             for (int b = 0; b < cnt[k]; b++) maps[k].push_back(make_int2(si, b));
+        for (int lw = 0; lw < 4; lw++) {  // pair kernel, blocks in system order
+            const int nb = cdiv(d.cap_cells, 1 << lw);
+            for (int b = 0; b < nb; b++) maps[KIND_PAIR + lw].push_back(make_int2(si, b));
+        }
+    }
+    // pair kernel, blocks round-robin across the systems of a batch
+    for (int lw = 0; lw < 4; lw++) {
+        int most = 0;
+        for (int si = 0; si < ns; si++) most = std::max(most, cdiv(x.members[si]->h.cap_cells, 1 << lw));
+        for (int b = 0; b < most; b++)
+            for (int si = 0; si < ns; si++)
+                if (b < cdiv(x.members[si]->h.cap_cells, 1 << lw)) maps[KIND_PAIR + 4 + lw].push_back(make_int2(si, b));
     }
     // COM-apply map from per-system totals (kept on the system)
     maps[KIND_COMA].clear();
@@ -309,7 +325,7 @@ void enqueue_rebuild(og_exec &x) {
 // them in its closing kernel instead
 void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1, bool reduce = false) {
     if (e0) hipEventRecord(e0, x.stream);
-    og_launch_pair(L(x, KIND_CELLW), x.alpha_mode);
+    og_launch_pair(L(x, KIND_PAIR + x.pair_shape), x.alpha_mode, 1 << (x.pair_shape & 3));
     if (e1) hipEventRecord(e1, x.stream);
     // an MD step of a fused context converts the gradient inside its step kernel
     if (reduce || !x.fused) og_launch_forces_to_work(L(x, KIND_ATOM));
@@ -749,6 +765,73 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     return OG_OK;
 }
 
+// Autotuner: times the pair kernel of an execution context over its launch
+// shapes (1, 2, 4 or 8 waves per block; for a batch also blocks in system
+// order or round-robin across systems), `trials` launches each after one
+// warm-up, and keeps the fastest.  The forces are fixed-point integer sums,
+// so every shape gives the same bits and the choice never changes a result.
+// The trial launches add into the fixed-point gradient and energy slots,
+// which are zero between steps; they are cleared again afterwards.
+og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) {
+    og_status st = exec_prepare(x);
+    if (st != OG_OK) return st;
+    if (x.members.empty()) return OG_E_STATE;
+    trials = std::max(trials, 1);
+    bool need = false;
+    for (og_system *s : x.members) need |= !s->grid_valid;
+    if (need) enqueue_rebuild(x);  // a fresh system rebuilds on its first step anyway
+    hipEvent_t e0, e1;
+    OG_HIP(hipEventCreate(&e0));
+    OG_HIP(hipEventCreate(&e1));
+    double    best     = 1e300;
+    int       best_shp = x.pair_shape;
+    const int n_shapes = x.members.size() > 1 ? 8 : 4;
+    for (int shp = 0; shp < n_shapes; shp++) {
+        const og_launch a = L(x, KIND_PAIR + shp);
+        const int       w = 1 << (shp & 3);
+        og_launch_pair(a, x.alpha_mode, w);
+        OG_HIP(hipEventRecord(e0, x.stream));
+        for (int t = 0; t < trials; t++) og_launch_pair(a, x.alpha_mode, w);
+        OG_HIP(hipEventRecord(e1, x.stream));
+        OG_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        OG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / trials < best) best = ms / trials, best_shp = shp;
+    }
+    for (og_system *s : x.members) {
+        OG_HIP(hipMemsetAsync(s->h.fx, 0, sizeof(long long) * 3 * (size_t)s->h.cap_atom_array, x.stream));
+        OG_HIP(hipMemsetAsync(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE, x.stream));
+    }
+    OG_HIP(hipStreamSynchronize(x.stream));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (og_system *s : x.members) s->grid_valid = true;
+    if (best_shp != x.pair_shape) {
+        free_graphs(x);
+        x.pair_shape = best_shp;
+    }
+    if (shape_out) *shape_out = best_shp;
+    if (ms_out) *ms_out = best;
+    return check_flags(x);
+}
+
+og_status exec_set_pair_shape(og_exec &x, int waves, int round_robin) {
+    int lw = -1;
+    for (int k = 0; k < 4; k++)
+        if (waves == 1 << k) lw = k;
+    if (lw < 0) {
+        set_err("pair kernel waves per block must be 1, 2, 4 or 8");
+        return OG_E_ARG;
+    }
+    const int shp = lw + (round_robin ? 4 : 0);
+    if (shp != x.pair_shape) {
+        if (x.stream) OG_HIP(hipStreamSynchronize(x.stream));
+        free_graphs(x);
+        x.pair_shape = shp;
+    }
+    return OG_OK;
+}
+
 og_status ensure_ctx(og_system *s) {
     if (!s->ctx) {
         s->ctx = new og_exec();
@@ -1160,6 +1243,32 @@ og_status og_batch_run(og_batch *b, int n_steps) {
     bool timing = false;
     for (og_system *s : b->ctx.members) timing |= s->time_pair;
     return exec_run(b->ctx, n_steps, timing);
+}
+og_status og_autotune(og_system *s, int trials, int *waves, double *pair_ms) {
+    if (!s) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    if (st != OG_OK) return st;
+    int shp = OG_PAIR_DEFAULT;
+    st      = exec_autotune(*s->ctx, trials, &shp, pair_ms);
+    if (waves) *waves = 1 << (shp & 3);
+    return st;
+}
+og_status og_set_pair_shape(og_system *s, int waves) {
+    if (!s) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    return st != OG_OK ? st : exec_set_pair_shape(*s->ctx, waves, 0);
+}
+og_status og_batch_autotune(og_batch *b, int trials, int *waves, int *round_robin, double *pair_ms) {
+    if (!b) return OG_E_ARG;
+    int       shp = OG_PAIR_DEFAULT;
+    og_status st  = exec_autotune(b->ctx, trials, &shp, pair_ms);
+    if (waves) *waves = 1 << (shp & 3);
+    if (round_robin) *round_robin = shp >> 2;
+    return st;
+}
+og_status og_batch_set_pair_shape(og_batch *b, int waves, int round_robin) {
+    if (!b) return OG_E_ARG;
+    return exec_set_pair_shape(b->ctx, waves, round_robin);
 }
 og_status og_batch_set_graph_steps(og_batch *b, int steps) {
     if (!b || steps < 0) return OG_E_ARG;

@@ -297,7 +297,7 @@ void og_launch_enum_scan(const og_launch &sys);
 void og_launch_enum_write(const og_launch &cellw);
 void og_launch_masks(const og_launch &cp);
 // forces
-void og_launch_pair(const og_launch &cellw, int alpha_mode);
+void og_launch_pair(const og_launch &cells, int alpha_mode, int waves);  // waves per block: 1, 2, 4 or 8
 void og_launch_forces_to_work(const og_launch &atoms);
 void og_launch_energy_reduce(const og_launch &sys);
 // integration

@@ -285,12 +285,13 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     }
 }
 
-// Four waves (four cells) per block sharing the image table.
-template <int ALPHA>
-__global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
-    __shared__ og_f4 shift[64];                        // image code -> shift of cell2
-    __shared__ og_f4 qpos[OG_THREADS / 64][OG_QCAP];  // row queue: shifted x, y, z, charge
-    __shared__ int2  qaux[OG_THREADS / 64][OG_QCAP];  // grid slot, LJ column | mask byte << 24
+// WAVES waves (one cell each) per block sharing the image table; the block
+// size is a launch shape the autotuner chooses (results do not depend on it).
+template <int ALPHA, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    __shared__ og_f4 shift[64];              // image code -> shift of cell2
+    __shared__ og_f4 qpos[WAVES][OG_QCAP];  // row queue: shifted x, y, z, charge
+    __shared__ int2  qaux[WAVES][OG_QCAP];  // grid slot, LJ column | mask byte << 24
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
     // block-uniform exits (one system per block; this kernel never sets the overflow flag)
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_pair(const og_dev_sys *__restric
     }
     __syncthreads();
     const int w  = threadIdx.x >> 6;
-    const int c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + w);
+    const int c1 = __builtin_amdgcn_readfirstlane(bm.y * WAVES + w);
     if (c1 >= s.cnt->n_cells) return;
     if (og_energy_step(s)) og_pair_cell<ALPHA, true>(s, c1, shift, qpos[w], qaux[w]);
     else og_pair_cell<ALPHA, false>(s, c1, shift, qpos[w], qaux[w]);
@@ -336,11 +337,21 @@ __global__ void __launch_bounds__(64) k_energy_reduce(const og_dev_sys *__restri
 }
 
 // ------------------------------------------------------------- launchers --
-void og_launch_pair(const og_launch &a, int alpha_mode) {
+template <int WAVES>
+static void og_launch_pair_w(const og_launch &a, int alpha_mode) {
     if (alpha_mode)
-        hipLaunchKernelGGL(k_pair<1>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+        hipLaunchKernelGGL((k_pair<1, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
     else
-        hipLaunchKernelGGL(k_pair<0>, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+        hipLaunchKernelGGL((k_pair<0, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
+}
+
+void og_launch_pair(const og_launch &a, int alpha_mode, int waves) {
+    switch (waves) {
+    case 1: og_launch_pair_w<1>(a, alpha_mode); break;
+    case 2: og_launch_pair_w<2>(a, alpha_mode); break;
+    case 8: og_launch_pair_w<8>(a, alpha_mode); break;
+    default: og_launch_pair_w<4>(a, alpha_mode); break;
+    }
 }
 void og_launch_forces_to_work(const og_launch &a) {
     hipLaunchKernelGGL(k_forces_to_work, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);

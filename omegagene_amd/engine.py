@@ -285,6 +285,16 @@ class System:
     def set_pair_timing(self, on=True):
         _check(lib().og_set_pair_timing(self.h, C.c_int(int(on))))
 
+    def autotune(self, trials=5):
+        """Time the pair kernel over its launch shapes and keep the fastest
+        (results do not depend on it); returns the chosen shape and time."""
+        w, ms = C.c_int(), C.c_double()
+        _check(lib().og_autotune(self.h, C.c_int(trials), C.byref(w), C.byref(ms)))
+        return {"waves_per_block": w.value, "pair_ms": ms.value}
+
+    def set_pair_shape(self, waves):
+        _check(lib().og_set_pair_shape(self.h, C.c_int(waves)))
+
     def step_fused(self):
         """True when an MD step runs as the fused one-kernel Langevin step."""
         f = C.c_int()
@@ -312,6 +322,16 @@ class Batch:
     def remove(self, s: System):
         _check(lib().og_batch_remove(self.h, s.h))
         self.members.remove(s)
+
+    def autotune(self, trials=5):
+        """Time the pair kernel over its launch shapes (waves per block, block
+        order across systems) and keep the fastest; results do not depend on it."""
+        w, rr, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().og_batch_autotune(self.h, C.c_int(trials), C.byref(w), C.byref(rr), C.byref(ms)))
+        return {"waves_per_block": w.value, "round_robin": bool(rr.value), "pair_ms": ms.value}
+
+    def set_pair_shape(self, waves, round_robin=False):
+        _check(lib().og_batch_set_pair_shape(self.h, C.c_int(waves), C.c_int(int(round_robin))))
 
     def run(self, n_steps: int):
         _check(lib().og_batch_run(self.h, C.c_int(n_steps)))

@@ -382,6 +382,59 @@ def test_fused_langevin_step_equals_split_kernels(vcmd):
             assert vc1[0] == vc0[0] == vcb[0] and np.array_equal(vc1[1], vc0[1]) and np.array_equal(vc0[1], vcb[1])
 
 
+def test_pair_launch_shapes_and_autotune_keep_results():
+    """The pair kernel's launch shapes (1, 2, 4 or 8 waves per block; blocks in
+    system order or round-robin across a batch) give bitwise the same forces
+    and energies, and autotuning in the middle of a run, alone or in a batch,
+    leaves the trajectory bitwise unchanged."""
+    ls = gen.build_system(1500, seed=8, n_beads=200)
+    cfg = dict(BASE, **{"nsgrid-update-intvl": 5})
+    say("pair launch shapes", ls.n_atoms)
+    ref = None
+    for w in (1, 2, 4, 8):
+        s = System(ls, cfg)
+        s.set_pair_shape(w)
+        wk, e = s.energy_force(rebuild=True)
+        s.close()
+        if ref is None:
+            ref = (wk, e)
+        assert np.array_equal(wk, ref[0]) and all(e[k] == ref[1][k] for k in e), w
+    finals = []
+    for tune in (False, True):
+        s = System(ls, cfg)
+        s.run(12)
+        if tune:
+            t = s.autotune(3)
+            say("  autotune alone: %d waves per block, %.3f ms" % (t["waves_per_block"], t["pair_ms"]))
+            assert t["waves_per_block"] in (1, 2, 4, 8)
+        s.run(13)
+        finals.append(s.state()[0])
+        s.close()
+    assert np.array_equal(finals[0], finals[1])
+    lss = [gen.walker_system(n, seed=70 + k, vcmd=False) for k, n in enumerate((3000, 5000))]
+    outs = []
+    for shape in ((4, False), (2, True), (8, True), "tune"):
+        ss = [System(x, cfg) for x in lss]
+        b = Batch()
+        for x in ss:
+            b.add(x)
+        if shape == "tune":
+            b.run(4)
+            t = b.autotune(3)
+            say("  autotune batch: %d waves per block, round robin %s" % (t["waves_per_block"], t["round_robin"]))
+            b.run(11)
+        else:
+            b.set_pair_shape(*shape)
+            b.run(15)
+        outs.append([x.state()[0] for x in ss])
+        b.close()
+        for x in ss:
+            x.close()
+    for o in outs[1:]:
+        for a, c in zip(outs[0], o):
+            assert np.array_equal(a, c)
+
+
 def test_box_too_small_is_an_error():
     ls = gen.build_system(60, seed=1, L=12.0)
     with pytest.raises(EngineError):
