@@ -246,7 +246,9 @@ og_status   og_batch_set_pair_shape(og_batch *b, int waves, int round_robin);
 /* 1 when an MD step of this system runs as the fused one-kernel Langevin step
  * (Langevin, SETTLE waters bound to nothing outside the water, at most 4096 other atoms) */
 og_status   og_step_fused(og_system *sys, int *fused);
-/* bytes of device memory a system with this description would hold */
+/* device memory a created system holds (bytes) */
+og_status   og_system_device_bytes(og_system *sys, uint64_t *bytes);
+/* bytes of device memory a system with this description would hold (an upper estimate, used for admission) */
 og_status   og_system_footprint(const og_system_desc *desc, const og_params *params, uint64_t *bytes);
 
 #ifdef __cplusplus

@@ -162,7 +162,8 @@ void og_set_error(const char *msg) { g_last_error = msg; }
 
 // ----------------------------------------------------------------- types --
 enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_LSTEP,
-       KIND_PAIR,  // 8 pair-kernel maps: index log2(waves per block) + 4 * (blocks round-robin across systems)
+       KIND_SCANB,  // blocks of 1024 cells over cap_cells + 1 (multi-block offset scan)
+       KIND_PAIR,   // 8 pair-kernel maps: index log2(waves per block) + 4 * (blocks round-robin across systems)
        KIND_N = KIND_PAIR + 8 };
 constexpr int OG_PAIR_DEFAULT = 2;  // 4 waves per block, blocks in system order
 
@@ -182,6 +183,7 @@ struct og_system {
     bool                time_pair     = false;
     long long           step_end      = OG_NO_STEP_END;  // host mirror of og_dev_counters::step_end
     struct og_exec     *ctx = nullptr;   // private batch-of-one
+    uint64_t            dev_bytes = 0;   // device memory allocated for the system
 };
 
 struct og_exec {
@@ -272,6 +274,7 @@ og_status exec_prepare(og_exec &x) {
         cnt[KIND_TRIO]  = cdiv(std::max(d.n_trios, d.n_settle), OG_TRIO_THREADS);
         cnt[KIND_COMA] = 0;  // filled below
         cnt[KIND_LSTEP] = d.n_water_blocks ? d.n_water_blocks + 1 : 0;  // free-atom block + water blocks
+        cnt[KIND_SCANB] = d.cap_cells / 1024 + 1;
         for (int k = 0; k < KIND_PAIR; k++)
             for (int b = 0; b < cnt[k]; b++) maps[k].push_back(make_int2(si, b));
         for (int lw = 0; lw < 4; lw++) {  // pair kernel, blocks in system order
@@ -316,7 +319,7 @@ void enqueue_rebuild(og_exec &x) {
     og_launch_grid_scatter(L(x, KIND_ATOM));
     og_launch_grid_sort(L(x, KIND_COL));
     og_launch_enum_count(L(x, KIND_CELLW));
-    og_launch_enum_scan(L(x, KIND_SYS));
+    og_launch_enum_scan_blocks(L(x, KIND_SCANB));
     og_launch_enum_write(L(x, KIND_CELLW));
     og_launch_masks(L(x, KIND_CELLW));
 }
@@ -488,6 +491,26 @@ og_status exec_run(og_exec &x, int n_steps, bool timing) {
 }
 
 // ------------------------------------------------------------ allocation --
+// Cell-pair list capacity from the candidate geometry: cells per column
+// height, candidate cells per cell1 within the list radius, half of them
+// owned, 25% headroom.
+int pair_capacity(const GridGeom &g, int n, int cap_cells, double capacity_factor) {
+    const double per_col = (double)n / g.n_columns;
+    const double hcell   = (double)g.L[2] * OG_NAC / std::max(per_col, 1.0);
+    double       cand    = 0.0;
+    for (int dx = -g.nnx; dx <= g.nnx; dx++)
+        for (int dy = -g.nny; dy <= g.nny; dy++) {
+            const double ex = std::max(0, std::abs(dx) - 1) * (double)g.L_cell[0];
+            const double ey = std::max(0, std::abs(dy) - 1) * (double)g.L_cell[1];
+            const double r2 = (double)g.cutoff_pair2 - ex * ex - ey * ey;
+            if (r2 < 0) continue;
+            cand += (2.0 * std::sqrt(r2) + 2.0 * hcell) / hcell + 1.0;
+        }
+    const double est = 0.5 * cand * (double)cap_cells;
+    const double fac = capacity_factor > 0 ? capacity_factor : 1.0;
+    return (int)std::min(2.0e9, est * 1.25 * fac + 4096.0);
+}
+
 og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     og_dev_sys &h = s->h;
     auto        A = [&](auto **ptr, size_t n) -> bool {
@@ -497,6 +520,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
             return false;
         }
         s->allocs.push_back((void *)*ptr);
+        s->dev_bytes += n * sizeof(**ptr);
         return true;
     };
     const int n = d->n_atoms;
@@ -506,21 +530,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.nx = g.nx, h.ny = g.ny, h.nnx = g.nnx, h.nny = g.nny, h.n_uni_z = g.n_uni_z;
     h.cap_atom_array = n + g.n_columns * OG_NAC;
     h.cap_cells      = h.cap_atom_array / OG_NAC;
-    // cell-pair capacity from the candidate geometry: cells per column-height h
-    const double per_col  = (double)n / g.n_columns;
-    const double hcell    = (double)g.L[2] * OG_NAC / std::max(per_col, 1.0);
-    double       cand     = 0.0;
-    for (int dx = -g.nnx; dx <= g.nnx; dx++)
-        for (int dy = -g.nny; dy <= g.nny; dy++) {
-            const double ex = std::max(0, std::abs(dx) - 1) * (double)g.L_cell[0];
-            const double ey = std::max(0, std::abs(dy) - 1) * (double)g.L_cell[1];
-            const double r2 = (double)g.cutoff_pair2 - ex * ex - ey * ey;
-            if (r2 < 0) continue;
-            cand += (2.0 * std::sqrt(r2) + 2.0 * hcell) / hcell + 1.0;
-        }
-    const double est   = 0.5 * cand * (double)h.cap_cells;
-    const double fac   = p->capacity_factor > 0 ? p->capacity_factor : 1.0;
-    h.cap_cell_pairs   = (int)std::min(2.0e9, est * 1.25 * fac + 4096.0);
+    h.cap_cell_pairs   = pair_capacity(g, n, h.cap_cells, p->capacity_factor);
     h.n_types          = d->n_lj_types;
     h.n_excess         = d->n_excess;
     h.n_bonds          = d->n_bonds;
@@ -590,7 +600,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cell_zmin, h.cap_cells) && A(&h.cell_zmax, h.cap_cells) && A(&h.cell_u0, h.cap_cells) &&
               A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
               A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) && A(&h.cp_stage, (size_t)h.cap_cells * OG_ENUM_STAGE) &&
-              A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
+              A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_bsum, h.cap_cells / 1024 + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
               A(&h.fx, 3 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;
     h.n_atom_blocks = cdiv(n, OG_THREADS);
@@ -902,11 +912,32 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     if (!d || !p || !bytes) return OG_E_ARG;
     GridGeom g;
     if (grid_geom(g, d->n_atoms, d->box, p->nsgrid_cutoff) != 0) return OG_E_ARG;
-    const double n = d->n_atoms, cap = n + g.n_columns * OG_NAC, cells = cap / OG_NAC;
-    const double cand  = (2 * g.nnx + 1.0) * (2 * g.nny + 1.0) * (2.0 * g.cutoff_pair / (g.L[2] * OG_NAC * g.n_columns / n) + 3.0);
-    const double pairs = 0.5 * cand * cells * 1.25 * (p->capacity_factor > 0 ? p->capacity_factor : 1.0);
-    *bytes = (uint64_t)(n * (6 * 24 + 3 * 4 + 4 * 4 + 128) + cap * (4 * 3 + 24 + 16 + 8 + 24) + cells * 48 + pairs * 12 +
-                        (double)d->n_lj_types * d->n_lj_types * 8 + 4096);
+    // the allocations of sys_alloc, term by term (the fused step's free-atom list counted at its bound)
+    const double n = d->n_atoms, nt = (double)d->n_lj_types * d->n_lj_types;
+    const double cols = g.n_columns, cap = n + cols * OG_NAC, cells = cap / OG_NAC;
+    const double pairs = pair_capacity(g, d->n_atoms, (int)cells, p->capacity_factor);
+    const double ex = d->n_excess, nb = d->n_bonds;
+    const double trios = p->constraint ? d->n_shake_trios : 0, pc = p->constraint ? d->n_shake_pairs : 0;
+    const double settle = p->constraint == OG_CONSTRAINT_SHAKE_SETTLE ? d->n_settle : 0;
+    const double ng = p->com_cancel ? d->n_com_groups : 0;
+    const double ng_atoms = ng > 0 ? d->com_group_off[d->n_com_groups] : 1;
+    const double ke_parts = std::max(std::ceil(n / OG_THREADS), std::floor(settle / OG_WATERS_PER_BLOCK) + 2);
+    const double vc = d->vc_n_states ? d->vc_n_a + d->vc_n_b : 0, vs = d->vc_n_states;
+    const double log_rows = std::max(1, p->log_cap);
+    double b = 0.0;
+    b += n * (6 * 24 + 3 * 4 + 4 + 4.0 * OG_MAX_NB15 + 4 + 4 + 4 + 4);   // state, masses, types, table, comp, col, zkey, rev
+    b += nt * (4 + 4 + 8);                                               // lj6, lj12, lj_pair
+    b += ex * 8 + nb * (8 + 16) + trios * (12 + 24) + pc * (8 + 8) + settle * 12;
+    b += 2 * (n + 1) * 4 + std::max(2 * ex, 1.0) * 16 + std::max(2 * nb, 1.0) * 16;      // per-atom pair lists
+    b += (ng + 1) * 4 + ng_atoms * 4 + (ng + 1) * 4 + 3 * (ng + 1) * 8;
+    b += cols * (4 * 6 + 8) + 2 * (cols + 2) * g.n_uni_z * 4;
+    b += cap * (4 + 4 + 24 + 16 + 4 + 4 + 24);                            // tmp_ids, atomids, gcrd, gxyzq, gtype, gcomp, fx
+    b += cells * (6 * 4 + 4.0 * OG_ENUM_STAGE + 8) + (cells / 1024 + 1) * 8;
+    b += pairs * 12;
+    b += 8.0 * OG_EACC_SLOTS * OG_EACC_STRIDE + ke_parts * 8 + OG_E_NTERMS * 8 + log_rows * OG_E_NTERMS * 8;
+    b += std::min(n, (double)OG_FREE_MAX) * 4 + vc * 28 + vs * 32;
+    b += sizeof(og_dev_counters) + sizeof(og_dev_sys);
+    *bytes = (uint64_t)(b * 1.02) + 65536;                                // allocation granularity
     return OG_OK;
 }
 
@@ -1200,6 +1231,12 @@ og_status og_pair_kernel_time(og_system *s, double *ms, int *n) {
     if (!s) return OG_E_ARG;
     if (ms) *ms = s->pair_ms;
     if (n) *n = s->pair_launches;
+    return OG_OK;
+}
+
+og_status og_system_device_bytes(og_system *s, uint64_t *bytes) {
+    if (!s || !bytes) return OG_E_ARG;
+    *bytes = s->dev_bytes;
     return OG_OK;
 }
 

@@ -487,6 +487,64 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__r
 }
 
 // Exclusive scan of per-cell pair counts -> cp_head (one 1024-thread block per system).
+// The same scan over many blocks of 1024 cells (a large system has 150 k
+// cells): k_enum_scan_part writes each cell's exclusive prefix within its
+// block and the block total; k_enum_scan_add adds the sum of the earlier
+// blocks' totals, clamps to the list capacity and records the total.
+__global__ void __launch_bounds__(1024) k_enum_scan_part(const og_dev_sys *__restrict__ S,
+                                                         const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
+    const int n = s.cnt->n_cells;
+    if (bm.y * 1024 > n) return;
+    const int k    = bm.y * 1024 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ int wsum[16];
+    const int      v    = k < n ? s.cp_count[k] : 0;
+    int            incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int q = 0; q < w; q++) before += wsum[q];
+    if (k <= n) s.cp_head[k] = before + incl - v;
+    if (threadIdx.x == 1023) s.cp_bsum[bm.y] = before + incl;
+}
+
+__global__ void __launch_bounds__(1024) k_enum_scan_add(const og_dev_sys *__restrict__ S,
+                                                        const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s)) return;
+    const int n = s.cnt->n_cells;
+    if (bm.y * 1024 > n) return;
+    const int k    = bm.y * 1024 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ long long wsum[16];
+    long long o = 0;
+    for (int q = threadIdx.x; q < bm.y; q += 1024) o += s.cp_bsum[q];
+    for (int off = 32; off > 0; off >>= 1) o += __shfl_xor(o, off, 64);
+    if (lane == 0) wsum[w] = o;
+    __syncthreads();
+    long long offset = 0;
+    for (int q = 0; q < 16; q++) offset += wsum[q];
+    if (k > n) return;
+    const long long p = offset + s.cp_head[k];
+    s.cp_head[k]      = (int)min(p, (long long)s.cap_cell_pairs);
+    if (k == n) {
+        if (p > s.cap_cell_pairs) {
+            atomicOr(&s.cnt->flags, OG_F_CP_OVERFLOW);
+            s.cnt->n_cell_pairs = 0;
+        } else {
+            s.cnt->n_cell_pairs = (int)p;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s   = S[map[blockIdx.x].x];
     if (og_frozen(s)) return;
@@ -617,6 +675,10 @@ void og_launch_cells_from_grid(const og_launch &a) {
 }
 void og_launch_enum_count(const og_launch &a) {
     hipLaunchKernelGGL(k_enum_count, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_enum_scan_blocks(const og_launch &a) {
+    hipLaunchKernelGGL(k_enum_scan_part, dim3(a.n_blocks), dim3(1024), 0, a.stream, a.sys, a.map);
+    hipLaunchKernelGGL(k_enum_scan_add, dim3(a.n_blocks), dim3(1024), 0, a.stream, a.sys, a.map);
 }
 void og_launch_enum_scan(const og_launch &a) {
     hipLaunchKernelGGL(k_enum_scan, dim3(a.n_blocks), dim3(1024), 0, a.stream, a.sys, a.map);

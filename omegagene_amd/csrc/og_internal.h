@@ -140,6 +140,7 @@ struct og_dev_sys {
     int      *cp_count;     // [cap_cells]
     int      *cp_stage;     // [cap_cells * OG_ENUM_STAGE] pairs of each cell from the counting pass
     int      *cp_head;      // [cap_cells+1]
+    long long *cp_bsum;     // [cap_cells / 1024 + 1] block totals of the multi-block offset scan
     int      *cp_c2;        // [cap_cell_pairs] cell2 | image << 24
     uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
     // ---- forces / energies ----
@@ -294,6 +295,7 @@ void og_launch_grid_sort(const og_launch &cols);
 void og_launch_cells_from_grid(const og_launch &cols);
 void og_launch_enum_count(const og_launch &cellw);
 void og_launch_enum_scan(const og_launch &sys);
+void og_launch_enum_scan_blocks(const og_launch &blocks);  // blocks of 1024 cells (cap_cells + 1)
 void og_launch_enum_write(const og_launch &cellw);
 void og_launch_masks(const og_launch &cp);
 // forces

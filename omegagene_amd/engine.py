@@ -295,6 +295,11 @@ class System:
     def set_pair_shape(self, waves):
         _check(lib().og_set_pair_shape(self.h, C.c_int(waves)))
 
+    def device_bytes(self):
+        b = C.c_uint64()
+        _check(lib().og_system_device_bytes(self.h, C.byref(b)))
+        return b.value
+
     def step_fused(self):
         """True when an MD step runs as the fused one-kernel Langevin step."""
         f = C.c_int()

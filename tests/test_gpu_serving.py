@@ -70,3 +70,20 @@ def test_segment_same_alone_in_mix_and_single_path():
     assert np.array_equal(crd_a, ref55)
     assert np.array_equal(crd_a2, ref75)
     assert st["steps_done"] == 55 + 40 + 70 + 20
+
+
+def test_admission_footprint_bounds_the_allocation():
+    """The footprint the server admits against (og_system_footprint, from the
+    description alone) is an upper estimate of the device memory a created
+    system holds, within a factor of two, from walker sizes to 1.2 M atoms."""
+    from omegagene_amd.engine import footprint
+    cases = [(_walker(3000, 11), CFG), (_walker(30000, 12), CFG), (_walker(99200, 13), CFG)]
+    ls2, cfg2 = gen.config2()
+    cases.append((ls2, dict(CFG, **{"cutoff": 10.0, "nsgrid-cutoff": 11.0, "ele-alpha": 0.1})))
+    for ls, cfg in cases:
+        est = footprint(ls, cfg)
+        s = System(ls, cfg)
+        used = s.device_bytes()
+        s.close()
+        say("footprint %d atoms: estimate %.1f MB, allocated %.1f MB" % (ls.n_atoms, est / 1e6, used / 1e6))
+        assert used <= est <= 2 * used
