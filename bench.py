@@ -14,7 +14,9 @@ pair-kernel roofline (algorithmic 41 FLOP per pair at alpha 0 over the
 pair-kernel time measured with HIP events on the engine's stream) and the
 CPU baseline (the oracle's single-thread cell-list force evaluation of the
 same system).  Beside it ("alpha_large", skipped with --no-large): the pair
-kernel at ZD alpha 0.1 on 400,000 waters (1.2 M atoms, rc 10 A, list 11 A).
+kernel at ZD alpha 0.1 on 400,000 waters (1.2 M atoms, rc 10 A, list 11 A);
+and ("energies_at_print_interval", skipped with --no-sparse) the same step
+with energies only every print-interval-log steps of the .cfg (1,000).
 """
 import argparse
 import json
@@ -317,6 +319,8 @@ def main():
     ap.add_argument("--no-walkers", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true",
+                    help="skip the run with energies only at the .cfg print interval")
     ap.add_argument("--large-only", action="store_true",
                     help="only the 1.2 M-atom alpha 0.1 measurement (its own JSON line; for profiling)")
     ap.add_argument("--nve-drift", action="store_true",
@@ -383,6 +387,30 @@ def main():
     achieved = flops / t_launch / 1e12 if t_launch > 0 else 0.0
     T = elog[-1, 5] * 2 * 4184.0 / (8.31451 * s.d_free)
 
+    # the same workload with energies only at the .cfg's print interval (the trajectory is the same; the pair
+    # kernel then skips the energy terms on the other steps): a second system continued from the final state
+    sparse = None
+    if not args.no_sparse:
+        crd, _, vj = s.state()
+        import copy
+        ls2 = copy.copy(ls)
+        ls2.crd, ls2.vel = crd, vj
+        stride = int(cfg.get("print-interval-log", 1000))
+        s2 = System(ls2, dict(cfg, **{"log-stride": stride, "log-cap": max(args.steps // stride, 1)}))
+        s2.run(args.warmup)
+        s2.set_pair_shape(tune["waves_per_block"])
+        lib().og_thread_sync()
+        t1 = time.perf_counter()
+        s2.run(args.steps)
+        lib().og_thread_sync()
+        wall2 = time.perf_counter() - t1
+        s2.close()
+        sparse = {"energies_every_steps": stride, "ms_per_step": wall2 / args.steps * 1e3,
+                  "pair_interactions_per_s": pairs_per_step * args.steps / wall2,
+                  "ns_per_day": args.steps / wall2 * cfg["time-step"] * 86400.0 / 1e6,
+                  "note": "pairs per step from the energies-every-step run (same trajectory rule)"}
+        log("energies every %d steps: %.4f ms/step" % (stride, sparse["ms_per_step"]))
+
     cpu = None
     if not args.no_cpu_baseline:
         v, n_eval, dt = cpu_baseline(ls, cfg)
@@ -426,6 +454,7 @@ def main():
         "walkers": walkers,
         "serving": serving,
         "alpha_large": large,
+        "energies_at_print_interval": sparse,
     }
     s.close()
     print(json.dumps(out))
