@@ -643,12 +643,9 @@ __global__ void __launch_bounds__(OG_WATER_THREADS) k_langevin_step(const og_dev
 __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys *__restrict__ S,
                                                                const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
-    if (og_frozen(s)) return;
-    __shared__ double    sh[OG_STEP_THREADS / 64];
-    __shared__ long long she[16 * OG_EACC];
-    if (og_energy_step(s)) og_take_pair_energies(s, she);
+    // the kinetic partials are read first (loads issued four strides at a
+    // time, added in stride order) so they overlap the counter reads
     double t = 0.0;
-    // loads issued four strides at a time, added in stride order
     for (int k = threadIdx.x; k < s.n_ke_parts; k += 4 * OG_STEP_THREADS) {
         double v[4];
 #pragma unroll
@@ -657,6 +654,10 @@ __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys
         for (int j = 0; j < 4; j++)
             if (k + j * OG_STEP_THREADS < s.n_ke_parts) t += v[j];
     }
+    if (og_frozen(s)) return;
+    __shared__ double    sh[OG_STEP_THREADS / 64];
+    __shared__ long long she[16 * OG_EACC];
+    if (og_energy_step(s)) og_take_pair_energies(s, she);
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
     __syncthreads();
