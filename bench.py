@@ -424,6 +424,11 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "k_pair_water_chain_99k_traffic.json")
     if os.path.exists(tpath):
         traffic = json.load(open(tpath))["traffic_bytes_per_launch"]
+    # FP32 work the pair kernel issues per launch, from the committed SQ_INSTS_VALU_*_F32 counter pass
+    issued = None
+    vpath = os.path.join(ROOT, "profiles", "k_pair_water_chain_99k_valu.json")
+    if os.path.exists(vpath) and t_launch > 0:
+        issued = json.load(open(vpath))["issued_fp32_flop_per_launch"] / t_launch / 1e12
     walkers = None if args.no_walkers else bench_walkers()
     serving = None if args.no_serving else bench_serving()
     large = None if args.no_large else bench_large()
@@ -449,7 +454,8 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_pair<%d, %d>" % (alpha_mode, tune["waves_per_block"]), "kernel_ms": t_launch * 1e3,
-                     "flop_per_pair": FLOP_PER_PAIR[alpha_mode]},
+                     "flop_per_pair": FLOP_PER_PAIR[alpha_mode],
+                     "issued_tflops_counters": issued},
         "cpu_baseline": cpu,
         "walkers": walkers,
         "serving": serving,
