@@ -327,26 +327,50 @@ def test_langevin_reaches_target_temperature():
     s.close()
 
 
-@pytest.mark.parametrize("vcmd", [False, True])
-def test_fused_langevin_step_equals_split_kernels(vcmd):
-    """The one-kernel Langevin step (SETTLE waters one per thread; the chain,
-    its bonds and the VcMD groups in one block) gives bitwise the trajectory,
-    potential terms and VcMD walk of the separate kernels, alone and in a
-    batch.  The excluded-pair, bond and kinetic energies are block sums
+def _chain_constraints(ls):
+    """SHAKE pairs on every tenth bond of the first half of the chain and
+    SHAKE trios (b, b+1, b+2) every tenth bead of the second half, at the
+    current geometry (disjoint groups)."""
+    water = np.zeros(ls.n_atoms, bool)
+    water[np.asarray(ls.settle_ids).reshape(-1)] = True
+    beads = np.flatnonzero(~water)
+    pairs = np.array([[beads[b], beads[b + 1]] for b in range(0, 100, 10)], np.int32)
+    trios = np.array([[beads[b], beads[b + 1], beads[b + 2]] for b in range(100, 198, 10)], np.int32)
+    x, L = ls.crd, float(np.float32(ls.box[0]))
+
+    def d(i, j):  # minimum image
+        v = x[i] - x[j]
+        return float(np.linalg.norm(v - L * np.round(v / L)))
+    ls.shake_pair_ids = pairs
+    ls.shake_pair_d = np.array([d(a, b) for a, b in pairs])
+    ls.shake_trio_ids = trios
+    ls.shake_trio_d = np.array([[d(a, b), d(b, c), d(c, a)] for a, b, c in trios])
+    return ls
+
+
+@pytest.mark.parametrize("variant", ["plain", "vcmd", "shake"])
+def test_fused_langevin_step_equals_split_kernels(variant):
+    """The one-kernel Langevin step (SETTLE waters three lanes per water; the
+    chain, its bonds, SHAKE pairs and trios and the VcMD groups in one block)
+    gives bitwise the trajectory, potential terms and VcMD walk of the
+    separate kernels, alone and in a batch.  The excluded-pair, bond and kinetic energies are block sums
     grouped by water instead of by atom index (FP64, then 2^-32 fixed point
     for the first two), so they agree to relative 1e-13 or 64 fixed-point
     quanta."""
+    vcmd = variant == "vcmd"
     lss = [gen.walker_system(n, seed=61 + k, vcmd=vcmd) for k, n in enumerate((4000, 6500))]
     for ls in lss:
         ls.settle_ids, ls.settle_d = ls.shake_trio_ids, ls.shake_trio_d
         ls.shake_trio_ids = np.zeros((0, 3), np.int32)
         ls.shake_trio_d = np.zeros((0, 3))
+        if variant == "shake":
+            _chain_constraints(ls)
     cfg = dict(BASE, **{"integrator": "langevin", "constraint": "shake-settle", "langevin-gamma": 20.0,
                         "temperature": 300.0, "nsgrid-update-intvl": 5, "random-seed": 3, "log-cap": 64,
                         "gap-bug": 0})
     if vcmd:
         cfg.update({"extended-ensemble": "vcmd", "enhance-sigma": 0.5, "enhance-recovery-coef": 1.0})
-    say("fused step", [ls.n_atoms for ls in lss], "vcmd" if vcmd else "")
+    say("fused step", [ls.n_atoms for ls in lss], variant)
     runs = {}
     for split in (1, 0):
         out = []
