@@ -10,9 +10,9 @@
  * reference host build (g++ -std=c++14 -O3, x86-64 SSE, no FP contraction):
  * `real` = double, `real_pw` = float, double literals promote.
  *
- * Parity pins (what checks each function) are listed per function in
- * DESIGN.md and in the oracle test-suite; functions with no pin are marked
- * "parity unpinned" here.
+ * Parity pins (what checks each function) are the oracle test-suite
+ * (tests/test_oracle.py): reference fixtures, reference builds, closed forms
+ * and brute force; functions with no pin are marked "parity unpinned" here.
  */
 #ifndef OG_ORACLE_H
 #define OG_ORACLE_H
