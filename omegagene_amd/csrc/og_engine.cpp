@@ -790,9 +790,16 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
     bool need = false;
     for (og_system *s : x.members) need |= !s->grid_valid;
     if (need) enqueue_rebuild(x);  // a fresh system rebuilds on its first step anyway
-    hipEvent_t e0, e1;
-    OG_HIP(hipEventCreate(&e0));
-    OG_HIP(hipEventCreate(&e1));
+    struct Events {  // destroyed on every return path
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events() {
+            if (a) hipEventDestroy(a);
+            if (b) hipEventDestroy(b);
+        }
+    } ev;
+    OG_HIP(hipEventCreate(&ev.a));
+    OG_HIP(hipEventCreate(&ev.b));
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     double    best     = 1e300;
     int       best_shp = x.pair_shape;
     const int n_shapes = x.members.size() > 1 ? 8 : 4;
@@ -813,8 +820,6 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
         OG_HIP(hipMemsetAsync(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE, x.stream));
     }
     OG_HIP(hipStreamSynchronize(x.stream));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     for (og_system *s : x.members) s->grid_valid = true;
     if (best_shp != x.pair_shape) {
         free_graphs(x);
