@@ -231,8 +231,8 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     lib().og_batch_set_graph_steps(b.h, steps)
     b.run(warmup)
     tune = b.autotune(5)
-    log("walkers: autotuned pair kernel %d waves per block, round robin %s" % (tune["waves_per_block"],
-                                                                              tune["round_robin"]))
+    log("walkers: autotuned pair kernel %d waves per block, block order %s" % (tune["waves_per_block"],
+                                                                              tune["order"]))
     lib().og_thread_sync()
     t0 = time.perf_counter()
     b.run(steps)

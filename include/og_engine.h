@@ -237,12 +237,13 @@ og_status   og_last_error(char *buf, int len);
 og_status   og_set_pair_timing(og_system *sys, int on);
 og_status   og_pair_kernel_time(og_system *sys, double *ms, int *n_launches);
 /* Autotuner (no counterpart in the reference): times the pair kernel over its launch shapes (1, 2, 4 or 8 waves
- * per block; for a batch also blocks in system order or round-robin across systems), `trials` launches each, and
- * keeps the fastest for later runs.  Forces are fixed-point integer sums, so the shape never changes a result. */
+ * per block; for a batch also the block order: 0 system order, 1 round-robin across systems, 2 each system's
+ * blocks on one XCD), `trials` launches each, and keeps the fastest for later runs.  Forces are fixed-point
+ * integer sums, so the shape never changes a result. */
 og_status   og_autotune(og_system *sys, int trials, int *waves, double *pair_ms);
-og_status   og_batch_autotune(og_batch *b, int trials, int *waves, int *round_robin, double *pair_ms);
+og_status   og_batch_autotune(og_batch *b, int trials, int *waves, int *order, double *pair_ms);
 og_status   og_set_pair_shape(og_system *sys, int waves);
-og_status   og_batch_set_pair_shape(og_batch *b, int waves, int round_robin);
+og_status   og_batch_set_pair_shape(og_batch *b, int waves, int order);
 /* 1 when an MD step of this system runs as the fused one-kernel Langevin step
  * (Langevin, SETTLE waters bound to nothing outside the water, at most 4096 other atoms) */
 og_status   og_step_fused(og_system *sys, int *fused);

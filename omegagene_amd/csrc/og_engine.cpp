@@ -163,8 +163,10 @@ void og_set_error(const char *msg) { g_last_error = msg; }
 // ----------------------------------------------------------------- types --
 enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_LSTEP,
        KIND_SCANB,  // blocks of 1024 cells over cap_cells + 1 (multi-block offset scan)
-       KIND_PAIR,   // 8 pair-kernel maps: index log2(waves per block) + 4 * (blocks round-robin across systems)
-       KIND_N = KIND_PAIR + 8 };
+       KIND_PAIR,   // 12 pair-kernel maps: index log2(waves per block) + 4 * order (0 system order,
+                    // 1 round-robin across systems, 2 each system's blocks on one XCD)
+       KIND_N = KIND_PAIR + 12 };
+constexpr int OG_N_XCD = 8;  // MI355X: workgroup i of a launch runs on XCD i mod 8
 constexpr int OG_PAIR_DEFAULT = 2;  // 4 waves per block, blocks in system order
 
 struct og_system {
@@ -289,6 +291,33 @@ og_status exec_prepare(og_exec &x) {
         for (int b = 0; b < most; b++)
             for (int si = 0; si < ns; si++)
                 if (b < cdiv(x.members[si]->h.cap_cells, 1 << lw)) maps[KIND_PAIR + 4 + lw].push_back(make_int2(si, b));
+    }
+    // pair kernel, each system's blocks on one XCD (launch positions p with p mod 8 fixed), so a system's
+    // coordinates and lists stay in that XCD's L2; systems go to the XCD with the fewest blocks so far, largest
+    // first, and once an XCD's list runs out its positions take blocks of the longest remaining list
+    for (int lw = 0; lw < 4; lw++) {
+        std::vector<int> nb(ns), order(ns);
+        for (int si = 0; si < ns; si++) nb[si] = cdiv(x.members[si]->h.cap_cells, 1 << lw), order[si] = si;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nb[a] > nb[b]; });
+        std::vector<std::vector<int2>> lists(OG_N_XCD);
+        std::vector<long long>         load(OG_N_XCD, 0);
+        for (int si : order) {
+            const int xc = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            load[xc] += nb[si];
+            for (int b = 0; b < nb[si]; b++) lists[xc].push_back(make_int2(si, b));
+        }
+        std::vector<size_t> at(OG_N_XCD, 0);
+        size_t              total = 0;
+        for (auto &l : lists) total += l.size();
+        for (size_t p = 0; p < total; p++) {
+            int xc = (int)(p % OG_N_XCD);
+            if (at[xc] >= lists[xc].size()) {
+                size_t most = 0;
+                for (int k = 0; k < OG_N_XCD; k++)
+                    if (lists[k].size() - at[k] > most) most = lists[k].size() - at[k], xc = k;
+            }
+            maps[KIND_PAIR + 8 + lw].push_back(lists[xc][at[xc]++]);
+        }
     }
     // COM-apply map from per-system totals (kept on the system)
     maps[KIND_COMA].clear();
@@ -777,7 +806,8 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
 
 // Autotuner: times the pair kernel of an execution context over its launch
 // shapes (1, 2, 4 or 8 waves per block; for a batch also blocks in system
-// order or round-robin across systems), `trials` launches each after one
+// order, round-robin across systems, or each system on one XCD), `trials`
+// launches each after one
 // warm-up, and keeps the fastest.  The forces are fixed-point integer sums,
 // so every shape gives the same bits and the choice never changes a result.
 // The trial launches add into the fixed-point gradient and energy slots,
@@ -802,7 +832,7 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
     hipEvent_t e0 = ev.a, e1 = ev.b;
     double    best     = 1e300;
     int       best_shp = x.pair_shape;
-    const int n_shapes = x.members.size() > 1 ? 8 : 4;
+    const int n_shapes = x.members.size() > 1 ? 12 : 4;
     for (int shp = 0; shp < n_shapes; shp++) {
         const og_launch a = L(x, KIND_PAIR + shp);
         const int       w = 1 << (shp & 3);
@@ -830,7 +860,7 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
     return check_flags(x);
 }
 
-og_status exec_set_pair_shape(og_exec &x, int waves, int round_robin) {
+og_status exec_set_pair_shape(og_exec &x, int waves, int order) {
     int lw = -1;
     for (int k = 0; k < 4; k++)
         if (waves == 1 << k) lw = k;
@@ -838,7 +868,11 @@ og_status exec_set_pair_shape(og_exec &x, int waves, int round_robin) {
         set_err("pair kernel waves per block must be 1, 2, 4 or 8");
         return OG_E_ARG;
     }
-    const int shp = lw + (round_robin ? 4 : 0);
+    if (order < 0 || order > 2) {
+        set_err("pair kernel block order must be 0 (system order), 1 (round-robin) or 2 (one XCD per system)");
+        return OG_E_ARG;
+    }
+    const int shp = lw + 4 * order;
     if (shp != x.pair_shape) {
         if (x.stream) OG_HIP(hipStreamSynchronize(x.stream));
         free_graphs(x);
@@ -1300,17 +1334,17 @@ og_status og_set_pair_shape(og_system *s, int waves) {
     og_status st = ensure_ctx(s);
     return st != OG_OK ? st : exec_set_pair_shape(*s->ctx, waves, 0);
 }
-og_status og_batch_autotune(og_batch *b, int trials, int *waves, int *round_robin, double *pair_ms) {
+og_status og_batch_autotune(og_batch *b, int trials, int *waves, int *order, double *pair_ms) {
     if (!b) return OG_E_ARG;
     int       shp = OG_PAIR_DEFAULT;
     og_status st  = exec_autotune(b->ctx, trials, &shp, pair_ms);
     if (waves) *waves = 1 << (shp & 3);
-    if (round_robin) *round_robin = shp >> 2;
+    if (order) *order = shp >> 2;
     return st;
 }
-og_status og_batch_set_pair_shape(og_batch *b, int waves, int round_robin) {
+og_status og_batch_set_pair_shape(og_batch *b, int waves, int order) {
     if (!b) return OG_E_ARG;
-    return exec_set_pair_shape(b->ctx, waves, round_robin);
+    return exec_set_pair_shape(b->ctx, waves, order);
 }
 og_status og_batch_set_graph_steps(og_batch *b, int steps) {
     if (!b || steps < 0) return OG_E_ARG;

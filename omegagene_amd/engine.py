@@ -330,13 +330,16 @@ class Batch:
 
     def autotune(self, trials=5):
         """Time the pair kernel over its launch shapes (waves per block, block
-        order across systems) and keep the fastest; results do not depend on it."""
-        w, rr, ms = C.c_int(), C.c_int(), C.c_double()
-        _check(lib().og_batch_autotune(self.h, C.c_int(trials), C.byref(w), C.byref(rr), C.byref(ms)))
-        return {"waves_per_block": w.value, "round_robin": bool(rr.value), "pair_ms": ms.value}
+        order across systems and XCDs) and keep the fastest; results do not
+        depend on it."""
+        w, order, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().og_batch_autotune(self.h, C.c_int(trials), C.byref(w), C.byref(order), C.byref(ms)))
+        return {"waves_per_block": w.value, "order": ("system", "round-robin", "xcd")[order.value],
+                "pair_ms": ms.value}
 
-    def set_pair_shape(self, waves, round_robin=False):
-        _check(lib().og_batch_set_pair_shape(self.h, C.c_int(waves), C.c_int(int(round_robin))))
+    def set_pair_shape(self, waves, order=0):
+        """order: 0 system order, 1 round-robin across systems, 2 each system on one XCD."""
+        _check(lib().og_batch_set_pair_shape(self.h, C.c_int(waves), C.c_int(int(order))))
 
     def run(self, n_steps: int):
         _check(lib().og_batch_run(self.h, C.c_int(n_steps)))

@@ -408,7 +408,7 @@ def test_fused_langevin_step_equals_split_kernels(variant):
 
 def test_pair_launch_shapes_and_autotune_keep_results():
     """The pair kernel's launch shapes (1, 2, 4 or 8 waves per block; blocks in
-    system order or round-robin across a batch) give bitwise the same forces
+    system order, round-robin across a batch or one XCD per system) give bitwise the same forces
     and energies, and autotuning in the middle of a run, alone or in a batch,
     leaves the trajectory bitwise unchanged."""
     ls = gen.build_system(1500, seed=8, n_beads=200)
@@ -437,7 +437,7 @@ def test_pair_launch_shapes_and_autotune_keep_results():
     assert np.array_equal(finals[0], finals[1])
     lss = [gen.walker_system(n, seed=70 + k, vcmd=False) for k, n in enumerate((3000, 5000))]
     outs = []
-    for shape in ((4, False), (2, True), (8, True), "tune"):
+    for shape in ((4, 0), (2, 1), (8, 1), (1, 2), "tune"):
         ss = [System(x, cfg) for x in lss]
         b = Batch()
         for x in ss:
@@ -445,7 +445,7 @@ def test_pair_launch_shapes_and_autotune_keep_results():
         if shape == "tune":
             b.run(4)
             t = b.autotune(3)
-            say("  autotune batch: %d waves per block, round robin %s" % (t["waves_per_block"], t["round_robin"]))
+            say("  autotune batch: %d waves per block, block order %s" % (t["waves_per_block"], t["order"]))
             b.run(11)
         else:
             b.set_pair_shape(*shape)
