@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_cs_work(const og_dev_sys *__rest
     const int         g = blockIdx.x * OG_THREADS + threadIdx.x;
     if (g >= n_atom_array) return;
     const int cap = s.cap_atom_array;
-    for (int k = 0; k < 3; k++) work[3 * g + k] = (double)s.fx[k * cap + g] * OG_FX_INV;
+    og_fx_read(s.fx, cap, og_fx_slot(g), work + 3 * (size_t)g);
 }
 
 // kernel_set_atominfo (cuda_setup.cu:364-379): types, charges and exclusion
@@ -228,7 +228,8 @@ og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_c
     g_cs.max_n_cells       = max_n_cells;
     g_cs.max_n_cell_pairs  = max_n_cell_pairs;
     g_cs.n_columns_alloc   = n_columns;
-    const int cap          = std::max(g_cs.max_n_atom_array, max_n_cells * OG_NAC);
+    // whole groups of four cells: the fixed-point gradient layout (og_fx_slot) permutes within them
+    const int cap          = (std::max(g_cs.max_n_atom_array, max_n_cells * OG_NAC) + 31) / 32 * 32;
     h.cap_atom_array       = cap;
     h.cap_cells            = cap / OG_NAC;
     h.cap_cell_pairs       = max_n_cell_pairs;
@@ -264,7 +265,7 @@ og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_c
     CS_TRY(cs_alloc(&h.cp_head, (size_t)h.cap_cells + 1));
     CS_TRY(cs_alloc(&h.cp_c2, max_n_cell_pairs));
     CS_TRY(cs_alloc(&h.cp_mask, max_n_cell_pairs));
-    CS_TRY(cs_alloc(&h.fx, 3 * (size_t)cap));
+    CS_TRY(cs_alloc(&h.fx, 4 * (size_t)cap));
     CS_TRY(cs_alloc(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE));
     CS_TRY(cs_alloc(&h.energy, OG_E_NTERMS));
     g_cs.have_alloc = true;
@@ -550,7 +551,7 @@ og_status og_pairwise_ljzd(int flg_mod_15mask) {
     CS_REQUIRE(g_cs.have_pairs && g_cs.have_zd, "og_pairwise_ljzd: no cell-pair list (og_enumerate_cell_pairs)");
     og_dev_sys &h = g_cs.h;
     CS_TRY(cs_push_sys());
-    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 3 * (size_t)h.cap_atom_array, g_cs.stream));
+    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 4 * (size_t)h.cap_atom_array, g_cs.stream));
     og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), 0, 4);
     og_launch_energy_reduce(cs_launch(g_cs.map_sys, 1));
     CS_HIP(hipGetLastError());

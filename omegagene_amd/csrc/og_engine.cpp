@@ -557,7 +557,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     const GridGeom &g = s->geom;
     h.n_columns = g.n_columns;
     h.nx = g.nx, h.ny = g.ny, h.nnx = g.nnx, h.nny = g.nny, h.n_uni_z = g.n_uni_z;
-    h.cap_atom_array = n + g.n_columns * OG_NAC;
+    h.cap_atom_array = (n + g.n_columns * OG_NAC + 31) / 32 * 32;  // whole groups of four cells (og_fx_slot)
     h.cap_cells      = h.cap_atom_array / OG_NAC;
     h.cap_cell_pairs   = pair_capacity(g, n, h.cap_cells, p->capacity_factor);
     h.n_types          = d->n_lj_types;
@@ -630,7 +630,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
               A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) && A(&h.cp_stage, (size_t)h.cap_cells * OG_ENUM_STAGE) &&
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_bsum, h.cap_cells / 1024 + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
-              A(&h.fx, 3 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
+              A(&h.fx, 4 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;
     h.n_atom_blocks = cdiv(n, OG_THREADS);
     h.n_ke_parts    = std::max(h.n_atom_blocks, cdiv(h.n_settle, OG_WATERS_PER_BLOCK) + 1);
@@ -646,8 +646,22 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     }
 
     // uploads
-    std::vector<float> lj6f((size_t)h.n_types * h.n_types), lj12f(lj6f.size());
-    for (size_t k = 0; k < lj6f.size(); k++) lj6f[k] = (float)d->lj6[k], lj12f[k] = (float)d->lj12[k];
+    // LJ types are renumbered by frequency (most atoms first, ties by the input number) so the pair kernel's
+    // LDS copy of the leading block of the table covers most pairs; the table entries are the same floats.
+    const int nt = h.n_types;
+    std::vector<int> tcount(nt, 0), torder(nt), tnew(nt), type_new(n);
+    for (int i = 0; i < n; i++) tcount[d->atom_type[i]]++;
+    std::iota(torder.begin(), torder.end(), 0);
+    std::stable_sort(torder.begin(), torder.end(), [&](int a, int b) { return tcount[a] > tcount[b]; });
+    for (int k = 0; k < nt; k++) tnew[torder[k]] = k;
+    for (int i = 0; i < n; i++) type_new[i] = tnew[d->atom_type[i]];
+    std::vector<float> lj6f((size_t)nt * nt), lj12f(lj6f.size());
+    for (int a = 0; a < nt; a++)
+        for (int b = 0; b < nt; b++) {
+            const size_t src = (size_t)torder[a] * nt + torder[b], dst = (size_t)a * nt + b;
+            lj6f[dst]  = (float)d->lj6[src];
+            lj12f[dst] = (float)d->lj12[src];
+        }
     std::vector<int> comp = components(n, d->nb15off);
     // MmSystem::revise_crd_inbox (MmSystem.cpp:808-816): wrap the input into [lb, ub)
     std::vector<double> crd0(d->crd, d->crd + 3 * (size_t)n);
@@ -670,7 +684,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     OG_HIP(hipMemcpy(h.mass, m.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.mass_inv, minv.data(), sizeof(float) * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.charge, q.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    OG_HIP(hipMemcpy(h.atom_type, d->atom_type, sizeof(int) * n, hipMemcpyHostToDevice));
+    OG_HIP(hipMemcpy(h.atom_type, type_new.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.nb15off, d->nb15off, sizeof(int) * (size_t)n * OG_MAX_NB15, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.comp, comp.data(), sizeof(int) * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.lj6, lj6f.data(), sizeof(float) * lj6f.size(), hipMemcpyHostToDevice));
@@ -846,7 +860,7 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
         if (ms / trials < best) best = ms / trials, best_shp = shp;
     }
     for (og_system *s : x.members) {
-        OG_HIP(hipMemsetAsync(s->h.fx, 0, sizeof(long long) * 3 * (size_t)s->h.cap_atom_array, x.stream));
+        OG_HIP(hipMemsetAsync(s->h.fx, 0, sizeof(long long) * 4 * (size_t)s->h.cap_atom_array, x.stream));
         OG_HIP(hipMemsetAsync(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE, x.stream));
     }
     OG_HIP(hipStreamSynchronize(x.stream));
@@ -953,7 +967,7 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     if (grid_geom(g, d->n_atoms, d->box, p->nsgrid_cutoff) != 0) return OG_E_ARG;
     // the allocations of sys_alloc, term by term (the fused step's free-atom list counted at its bound)
     const double n = d->n_atoms, nt = (double)d->n_lj_types * d->n_lj_types;
-    const double cols = g.n_columns, cap = n + cols * OG_NAC, cells = cap / OG_NAC;
+    const double cols = g.n_columns, cap = std::ceil((n + cols * OG_NAC) / 32.0) * 32.0, cells = cap / OG_NAC;
     const double pairs = pair_capacity(g, d->n_atoms, (int)cells, p->capacity_factor);
     const double ex = d->n_excess, nb = d->n_bonds;
     const double trios = p->constraint ? d->n_shake_trios : 0, pc = p->constraint ? d->n_shake_pairs : 0;
@@ -970,7 +984,7 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     b += 2 * (n + 1) * 4 + std::max(2 * ex, 1.0) * 16 + std::max(2 * nb, 1.0) * 16;      // per-atom pair lists
     b += (ng + 1) * 4 + ng_atoms * 4 + (ng + 1) * 4 + 3 * (ng + 1) * 8;
     b += cols * (4 * 6 + 8) + 2 * (cols + 2) * g.n_uni_z * 4;
-    b += cap * (4 + 4 + 24 + 16 + 4 + 4 + 24);                            // tmp_ids, atomids, gcrd, gxyzq, gtype, gcomp, fx
+    b += cap * (4 + 4 + 24 + 16 + 4 + 4 + 32);                            // tmp_ids, atomids, gcrd, gxyzq, gtype, gcomp, fx
     b += cells * (6 * 4 + 4.0 * OG_ENUM_STAGE + 8) + (cells / 1024 + 1) * 8;
     b += pairs * 12;
     b += 8.0 * OG_EACC_SLOTS * OG_EACC_STRIDE + ke_parts * 8 + OG_E_NTERMS * 8 + log_rows * OG_E_NTERMS * 8;

@@ -533,9 +533,9 @@ __device__ __forceinline__ void og_water_lanes(const og_dev_sys &s, const int bl
         x[d]  = s.crd[3 * i + d];
         xp[d] = s.crd_prev[3 * i + d];
         vn[d] = step == 0 ? s.vel_next[3 * i + d] : 0.0;
-        w[d]  = (double)s.fx[d * cap + g] * OG_FX_INV;
         gc[d] = s.gcrd[3 * g + d];
     }
+    og_fx_read(s.fx, cap, og_fx_slot(g), w);
     const float gw = s.gxyzq[g].w;
     int4        ent[2];
     for (int j = 0; j < 2; j++)
@@ -578,7 +578,8 @@ __device__ __forceinline__ void og_water_lanes(const og_dev_sys &s, const int bl
     if (!live) return;
     ke += ka * (double)m;
     for (int d = 0; d < 3; d++) {
-        s.fx[d * cap + g]      = 0;
+        s.fx[d * cap + og_fx_slot(g)] = 0;
+        if (d == 0) s.fx[3 * cap + og_fx_slot(g)] = 0;
         s.work[3 * i + d]      = w[d];
         s.crd_prev2[3 * i + d] = xp[d];
         s.crd_prev[3 * i + d]  = x[d];

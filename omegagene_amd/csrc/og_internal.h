@@ -15,6 +15,14 @@
 #define OG_EMPTY_BIN 0x7fffffff  // uniform-z bin with no cell
 #define OG_FX_SCALE 4294967296.0 // fixed-point force scale, 2^32 units per kcal/mol/A
 #define OG_FX_INV (1.0 / 4294967296.0)
+// j-side (x, y) gradient of the pair kernel: two signed 32-bit fixed-point
+// fields at 2^19 units per kcal/mol/A packed linearly into one 64-bit word
+// (word = x + y * 2^32), so one 64-bit integer atomic adds both and the sums
+// stay exact and order-independent.  Each field holds +-4096 kcal/mol/A; a
+// pair-kernel row beyond +-OG_FXJ_ROW_LIMIT flags the evaluation (an overlap).
+#define OG_FXJ_SCALE 524288.0
+#define OG_FXJ_INV (1.0 / 524288.0)
+#define OG_FXJ_ROW_LIMIT 2048.0f
 #define OG_NO_STEP_END 0x7fffffffffffffffLL  // og_dev_counters::step_end of a system with no segment limit
 
 // energy terms kept per system and per step
@@ -144,7 +152,8 @@ struct og_dev_sys {
     int      *cp_c2;        // [cap_cell_pairs] cell2 | image << 24
     uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
     // ---- forces / energies ----
-    long long *fx;          // [3*cap_atom_array] fixed-point gradient, SoA x|y|z
+    long long *fx;          // [4*cap_atom_array] fixed-point gradient, slot g at og_fx_slot(g): x | y | z
+                            // (2^32 units) | packed j-side (x, y) (og_fxj_pack)
     long long *e_acc;       // [OG_EACC_SLOTS][OG_EACC_STRIDE] energy accumulators, 2^32 fixed point (the pair
                             // count as an integer): vdw, ele, virial, pairs, excess, bond; a cell / block adds to
                             // slot (index % OG_EACC_SLOTS), so no cache line takes every add; zeroed by the reader
@@ -202,6 +211,16 @@ __device__ __forceinline__ float og_dpp_sum8(float x) {
 
 __device__ __forceinline__ long long og_to_fixed(double v) { return __double2ll_rn(v * OG_FX_SCALE); }
 
+// Index of grid slot g in each component of the fixed-point gradient: cell
+// index bits 0 and 1 swapped, so cells c and c + 2 -- the same-parity cells
+// one cell1 takes from a neighbour column -- share a 128-byte line and one
+// memory-side atomic request covers both.  A bijection on every group of
+// four cells (cap_atom_array is a multiple of 32 slots).
+__device__ __forceinline__ int og_fx_slot(int g) {
+    const int c = g >> 3;
+    return ((((c & ~3) | ((c & 1) << 1) | ((c >> 1) & 1))) << 3) | (g & 7);
+}
+
 // Round-to-nearest fixed point without a conversion sequence: adding
 // 1.5*2^52 puts round(d) in the low mantissa bits (valid for |d| < 2^51,
 // i.e. |gradient| < 2^19 kcal/mol/A at the 2^32 scale).
@@ -211,6 +230,27 @@ __device__ __forceinline__ long long og_to_fixed(double v) { return __double2ll_
 __device__ __forceinline__ long long og_fx_fast(double v) {
     const double t = v * OG_FX_SCALE + OG_FX_MAGIC;
     return __double_as_longlong(t) - OG_FX_MAGIC_BITS;
+}
+// (x, y) rounded to 2^-19 and packed as x + y * 2^32 (|x|, |y| < 4096)
+__device__ __forceinline__ unsigned long long og_fxj_pack(double x, double y) {
+    const long long ix = __double_as_longlong(x * OG_FXJ_SCALE + OG_FX_MAGIC) - OG_FX_MAGIC_BITS;
+    const long long iy = __double_as_longlong(y * OG_FXJ_SCALE + OG_FX_MAGIC) - OG_FX_MAGIC_BITS;
+    return (unsigned long long)ix + ((unsigned long long)iy << 32);
+}
+// the two fields of a packed sum (each within +-2^31 units)
+__device__ __forceinline__ void og_fxj_unpack(long long w, double &x, double &y) {
+    const long long ix = (long long)(int)(unsigned)(unsigned long long)w;
+    const long long iy = (long long)((unsigned long long)w - (unsigned long long)ix) >> 32;
+    x = (double)ix * OG_FXJ_INV;
+    y = (double)iy * OG_FXJ_INV;
+}
+// the pair gradient of grid slot g (index f = og_fx_slot(g)) in kcal/mol/A
+__device__ __forceinline__ void og_fx_read(const long long *fx, int cap, int f, double *w) {
+    double jx, jy;
+    og_fxj_unpack(fx[3 * cap + f], jx, jy);
+    w[0] = (double)fx[f] * OG_FX_INV + jx;
+    w[1] = (double)fx[cap + f] * OG_FX_INV + jy;
+    w[2] = (double)fx[2 * cap + f] * OG_FX_INV;
 }
 
 #define OG_GLOBAL(T) __attribute__((address_space(1))) T

@@ -23,6 +23,7 @@
 
 #define OG_CHARGE_COEFF 332.06378f   // CelesteObject.cpp:36
 #define OG_QCAP 128                  // per-wave row queue (two passes of 64)
+#define OG_LJ_LDS_ENTRIES 512        // LJ table entries (whole leading rows) held in LDS per block
 
 typedef float og_f4 __attribute__((ext_vector_type(4)));
 typedef float og_f2 __attribute__((ext_vector_type(2)));
@@ -43,12 +44,23 @@ __device__ __forceinline__ float og_rlane(float v, int k) {
 }
 
 // Cell1 data held by the wave: atom coordinates, charge * 332.06378, LJ row
-// (byte offset of the type's row in the LDS table, or its index in the
-// global table) and the bounding box.
+// offsets (type * n_types, the same in the global table and in the block's
+// LDS copy of its leading rows) and the bounding box.
 struct og_cell1 {
     float    x[OG_NAC], y[OG_NAC], z[OG_NAC], q[OG_NAC];
     unsigned row[OG_NAC];
     float    lo[3], hi[3];
+};
+
+// j-gradient of the last pass, not yet added: its fixed-point atomics are
+// issued after the next pass has read its rows, so they complete while that
+// pass computes.  (A vector-memory wait counts stores and atomics too, so an
+// atomic issued just before a load that is waited on stalls the wave until
+// the memory-side add has completed.)
+struct og_jpend {
+    float x, y, z;
+    int   a;
+    bool  live;
 };
 
 // Fixed-point add of a lane's gradient (x, y, z) at grid slot a (the
@@ -61,14 +73,28 @@ __device__ __forceinline__ void og_add_fx3(OG_GLOBAL(long long) *FX, int cap, in
     if (vz != 0) og_atomic_add_fx_g(FX + 2 * cap + a, vz);
 }
 
+// j-gradient of a row: (x, y) packed into one 64-bit atomic, z in its own
+// (the memory-side atomics cost per element, so two per row instead of three)
+__device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, og_jpend &p) {
+    if (p.live) {
+        const int                f  = og_fx_slot(p.a);
+        const unsigned long long xy = og_fxj_pack(-(double)p.x, -(double)p.y);
+        const long long          vz = og_fx_fast(-(double)p.z);
+        if (xy != 0) og_atomic_add_fx_g(FX + 3 * cap + f, (long long)xy);
+        if (vz != 0) og_atomic_add_fx_g(FX + 2 * cap + f, vz);
+    }
+    p.live = false;
+}
+
 // One pass over up to 64 queued rows (lane = row; lanes >= cnt idle).
-// m8: the row's excluded pairs (bit i set = skip atom i).
-template <int ALPHA, bool ENERGY>
+// m8: the row's excluded pairs (bit i set = skip atom i).  LJ_LDS: the
+// cell1's types are rows of the block's LDS copy of the LJ table.
+template <int ALPHA, bool ENERGY, bool LJ_LDS>
 __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, const og_f4 pj, const unsigned tj,
                                         const unsigned m8, const int a2, const bool live,
-                                        const OG_GLOBAL(og_f2) *__restrict__ LJP,
+                                        const og_f2 *__restrict__ ljl, const OG_GLOBAL(og_f2) *__restrict__ LJP,
                                         float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
-                                        double &ee, double &vir) {
+                                        double &ee, double &vir, og_jpend &pend) {
     const float rc2 = s.cutoff2;
     const float zcore = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
     const float alpha = s.alpha, piewald = s.piewald;
@@ -82,7 +108,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
         npair += __builtin_popcountll(__ballot(ok));
-        const og_f2 lj = LJP[c.row[i] + tj];  // {A6, A12}, one 8-byte load
+        const og_f2 lj = LJ_LDS ? ljl[c.row[i] + tj] : LJP[c.row[i] + tj];  // {A6, A12}
         const float rinv  = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
@@ -117,13 +143,14 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         // finish atom i here: without this the compiler defers the
         // accumulations and keeps all 8 atoms' distances live
         asm volatile("" : "+v"(fjx), "+v"(fjy), "+v"(fjz), "+v"(fi[3 * i]), "+v"(fi[3 * i + 1]), "+v"(fi[3 * i + 2]),
-                     "+v"(px), "+v"(py), "+v"(pz), "+v"(pw), "+s"(npair));
+                     "+v"(px), "+v"(py), "+v"(pz), "+v"(pw));
         if constexpr (ENERGY) asm volatile("" : "+v"(fe), "+v"(fq), "+v"(fv));
     }
-    if (live) {
-        OG_GLOBAL(long long) *FX = (OG_GLOBAL(long long) *)s.fx;
-        og_add_fx3(FX, s.cap_atom_array, a2, -(double)fjx, -(double)fjy, -(double)fjz);
-    }
+    pend.x    = fjx;
+    pend.y    = fjy;
+    pend.z    = fjz;
+    pend.a    = a2;
+    pend.live = live;
     // range check of the converted values; fmaxf drops NaN, the sum keeps it
     vmax = fmaxf(vmax, fmaxf(fabsf(fjx), fmaxf(fabsf(fjy), fabsf(fjz))));
     vsum += fjx + fjy + fjz;
@@ -134,12 +161,14 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
     }
 }
 
-// Pass over queue entries [q0, q0 + cnt).
-template <int ALPHA, bool ENERGY>
+// Pass over queue entries [q0, q0 + cnt): reads the rows, adds the previous
+// pass's j-gradient, then evaluates the rows.
+template <int ALPHA, bool ENERGY, bool LJ_LDS>
 __device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, const og_f4 *qp, const int2 *qa,
-                                        const int q0, const int cnt, const OG_GLOBAL(og_f2) *__restrict__ LJP,
-                                        float (&fi)[3 * OG_NAC], int &npair,
-                                        float &vmax, float &vsum, double &ev, double &ee, double &vir) {
+                                        const int q0, const int cnt, const og_f2 *ljl,
+                                        const OG_GLOBAL(og_f2) *__restrict__ LJP, OG_GLOBAL(long long) *FX,
+                                        float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
+                                        double &ee, double &vir, og_jpend &pend) {
     const int  lane = threadIdx.x & 63;
     const int  slot = (q0 + lane) & (OG_QCAP - 1);
     const bool live = lane < cnt;
@@ -147,9 +176,14 @@ __device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, 
     const int2 ax   = qa[slot];
     // idle lanes (stale queue slots): finite coordinates, LJ column 0, every pair masked
     if (!live) pj = og_f4{0.0f, 0.0f, 0.0f, 0.0f};
-    const unsigned m8 = live ? ((unsigned)ax.y >> 24) : 0xffu;
-    const unsigned tj = live ? ((unsigned)ax.y & 0xffffffu) : 0u;
-    og_rows<ALPHA, ENERGY>(s, c, pj, tj, m8, ax.x, live, LJP, fi, npair, vmax, vsum, ev, ee, vir);
+    const unsigned m8  = live ? ((unsigned)ax.y >> 24) : 0xffu;
+    const unsigned tj  = live ? ((unsigned)ax.y & 0xffffffu) : 0u;
+    // every outstanding load of the filter is complete before the atomics go
+    // out, so nothing in the pass waits on the vector-memory counter
+    // (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima)
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    og_flush_j(FX, s.cap_atom_array, pend);
+    og_rows<ALPHA, ENERGY, LJ_LDS>(s, c, pj, tj, m8, ax.x, live, ljl, LJP, fi, npair, vmax, vsum, ev, ee, vir, pend);
 }
 
 // a <- the kept half of (a, b) summed with the partner lane's other half
@@ -161,10 +195,11 @@ __device__ __forceinline__ float og_keep_add(float a, float b, bool upper, int o
 }
 
 // One cell1 (one wave).  ENERGY = false on steps whose energies are not
-// logged: forces and the pair count only.
-template <int ALPHA, bool ENERGY>
+// logged: forces and the pair count only.  LJ_LDS: the cell1's LJ rows are in
+// the block's LDS copy ljl of the table's leading rows.
+template <int ALPHA, bool ENERGY, bool LJ_LDS>
 __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, const og_f4 *__restrict__ shift,
-                                             og_f4 *qp, int2 *qa) {
+                                             const og_f2 *__restrict__ ljl, og_f4 *qp, int2 *qa) {
     const int lane = threadIdx.x & 63;
     const int jl   = lane & 7;
 
@@ -181,6 +216,19 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     // keep-test radius: the cutoff plus 10 ppm, so float rounding of the box
     // distance can never drop a row that holds an in-range pair
     const float rcb2 = s.cutoff2 * 1.00001f;
+
+    // the first group's cell-pair records load with the cell1 atoms
+    // (lane = 8 * (cell pair in the group) + j atom)
+    unsigned cp = 0u;
+    uint64_t mk = ~0ull;
+    bool     valid = false;
+    if (p0 < p1) {
+        const int t = p0 + (lane >> 3);
+        valid       = t < p1;
+        const int tc = valid ? t : p1 - 1;
+        cp = CP[tc];
+        mk = CM[tc];
+    }
 
     og_cell1 c;
     {
@@ -219,18 +267,22 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     int    npair = 0;                    // in-cutoff unmasked pairs of this cell1 (wave-uniform)
     float  vmax = 0.0f, vsum = 0.0f;     // j-gradient range check
     int    qn = 0, qd = 0;               // rows queued / rows evaluated (wave-uniform)
+    og_jpend pend{0.0f, 0.0f, 0.0f, 0, false};
 
     for (int base = p0; base < p1; base += OG_NAC) {
-        // lane = 8 * (cell pair in this group) + j atom
-        const int      t     = base + (lane >> 3);
-        const bool     valid = t < p1;
-        const int      tc    = valid ? t : p1 - 1;
-        const unsigned cp    = CP[tc];
-        const uint64_t mk    = CM[tc];
-        const int      a2    = (int)(cp & 0xffffffu) * OG_NAC + jl;
-        const og_f4    pj    = X[a2];
-        const int      ty    = TY[a2];
-        const og_f4    sh    = shift[(cp >> 24) & 63u];  // periodic image of the cell pair
+        const int   a2 = (int)(cp & 0xffffffu) * OG_NAC + jl;
+        const og_f4 pj = X[a2];
+        const int   ty = TY[a2];
+        // the next group's records load while this group's atoms are in flight
+        // (past the end: a valid record that the validity flag masks)
+        const int      tn     = base + OG_NAC + (lane >> 3);
+        const bool     validn = tn < p1;
+        const int      tcn    = validn ? tn : p1 - 1;
+        const unsigned cpn    = CP[tcn];
+        const uint64_t mkn    = CM[tcn];
+        // wait for this group's atoms (and types) here, with the next records still in flight
+        asm volatile("" ::"v"(pj.x), "v"(ty));
+        const og_f4    sh = shift[(cp >> 24) & 63u];  // periodic image of the cell pair
         const float    px = pj.x + sh.x, py = pj.y + sh.y, pz = pj.z + sh.z;
         const float    gx = fmaxf(fmaxf(c.lo[0] - px, px - c.hi[0]), 0.0f);
         const float    gy = fmaxf(fmaxf(c.lo[1] - py, py - c.hi[1]), 0.0f);
@@ -238,20 +290,25 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         const unsigned m8 = valid ? (unsigned)(mk >> (8 * jl)) & 0xffu : 0xffu;
         const bool     use = (gx * gx + gy * gy + gz * gz < rcb2) && m8 != 0xffu;
         const uint64_t ub  = __ballot(use);
+        const int2     ent = make_int2(a2, (int)((unsigned)ty | (m8 << 24)));
         if (use) {
             const int rank = __builtin_amdgcn_mbcnt_hi((unsigned)(ub >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)ub, 0u));
             const int slot = (qn + rank) & (OG_QCAP - 1);
-            const unsigned tj = (unsigned)ty;
             qp[slot] = og_f4{px, py, pz, pj.w};
-            qa[slot] = make_int2(a2, (int)(tj | (m8 << 24)));
+            qa[slot] = ent;
         }
         qn += __builtin_popcountll(ub);
         if (qn - qd >= 64) {
-            og_pass<ALPHA, ENERGY>(s, c, qp, qa, qd, 64, LJP, fi, npair, vmax, vsum, ev, ee, vir);
+            og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, 64, ljl, LJP, FX, fi, npair, vmax, vsum, ev, ee, vir, pend);
             qd += 64;
         }
+        cp    = cpn;
+        mk    = mkn;
+        valid = validn;
     }
-    if (qn > qd) og_pass<ALPHA, ENERGY>(s, c, qp, qa, qd, qn - qd, LJP, fi, npair, vmax, vsum, ev, ee, vir);
+    if (qn > qd)
+        og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, qn - qd, ljl, LJP, FX, fi, npair, vmax, vsum, ev, ee, vir, pend);
+    og_flush_j(FX, cap, pend);
 
     // i-gradients: reduce-scatter the 24 partial sums over the wave.  After
     // the xor-32/16/8 steps lane L holds atom L>>3's (x, y, z) summed over 8
@@ -264,11 +321,11 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     for (int k = 0; k < 6; k++) h6[k] = og_keep_add(h12[k], h12[6 + k], b4, 16);
 #pragma unroll
     for (int k = 0; k < 3; k++) h3[k] = og_dpp_sum8(og_keep_add(h6[k], h6[3 + k], b3, 8));
-    if (!(vmax <= (float)OG_FX_LIMIT) || !(fabsf(vsum) < INFINITY)) atomicOr(&s.cnt->flags, OG_F_NAN);
+    if (!(vmax <= OG_FXJ_ROW_LIMIT) || !(fabsf(vsum) < INFINITY)) atomicOr(&s.cnt->flags, OG_F_NAN);
     if (jl == 0) {
         const int a1 = c1 * OG_NAC + (lane >> 3);
         if (!(fabsf(h3[0]) + fabsf(h3[1]) + fabsf(h3[2]) <= (float)OG_FX_LIMIT)) atomicOr(&s.cnt->flags, OG_F_NAN);
-        og_add_fx3(FX, cap, a1, (double)h3[0], (double)h3[1], (double)h3[2]);
+        og_add_fx3(FX, cap, og_fx_slot(a1), (double)h3[0], (double)h3[1], (double)h3[2]);
     }
     if (!ENERGY) return;
     for (int off = 1; off < 64; off <<= 1) {
@@ -285,17 +342,24 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     }
 }
 
-// WAVES waves (one cell each) per block sharing the image table; the block
+// WAVES waves (one cell each) per block sharing the image table and an LDS
+// copy of the leading rows of the LJ table (as many whole rows as fit in
+// OG_LJ_LDS_ENTRIES; the engine numbers types by frequency, so for water the
+// oxygen and hydrogen rows against every type).  A cell1 whose types are all
+// in it reads LJ pairs from LDS, otherwise from the global table.  The block
 // size is a launch shape the autotuner chooses (results do not depend on it).
 template <int ALPHA, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     __shared__ og_f4 shift[64];              // image code -> shift of cell2
+    __shared__ og_f2 ljl[OG_LJ_LDS_ENTRIES];
     __shared__ og_f4 qpos[WAVES][OG_QCAP];  // row queue: shifted x, y, z, charge
     __shared__ int2  qaux[WAVES][OG_QCAP];  // grid slot, LJ column | mask byte << 24
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
     // block-uniform exits (one system per block; this kernel never sets the overflow flag)
     if (og_frozen(s) || (s.cnt->flags & OG_F_CP_OVERFLOW)) return;
+    const int nt   = s.n_types;
+    const int rows = min(nt, OG_LJ_LDS_ENTRIES / nt);  // table rows held in LDS
     if (threadIdx.x < 64) {
         // image bits of MiniCell::add_cell_pair (MiniCell.cpp:1174-1187): neg bit -> -L, pos bit -> +L
         const unsigned img = threadIdx.x;
@@ -303,12 +367,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_pair(const og_dev_sys *__restric
         shift[img] = og_f4{(img & 1u) ? -L0 : ((img & 2u) ? L0 : 0.0f), (img & 4u) ? -L1 : ((img & 8u) ? L1 : 0.0f),
                            (img & 16u) ? -L2 : ((img & 32u) ? L2 : 0.0f), 0.0f};
     }
+    for (int k = threadIdx.x; k < rows * nt; k += 64 * WAVES) ljl[k] = ((const og_f2 *)s.lj_pair)[k];
     __syncthreads();
     const int w  = threadIdx.x >> 6;
     const int c1 = __builtin_amdgcn_readfirstlane(bm.y * WAVES + w);
     if (c1 >= s.cnt->n_cells) return;
-    if (og_energy_step(s)) og_pair_cell<ALPHA, true>(s, c1, shift, qpos[w], qaux[w]);
-    else og_pair_cell<ALPHA, false>(s, c1, shift, qpos[w], qaux[w]);
+    const bool lds = __builtin_amdgcn_readfirstlane(__all(s.gtype[c1 * OG_NAC + (threadIdx.x & 7)] < rows)) != 0;
+    if (og_energy_step(s)) {
+        if (lds) og_pair_cell<ALPHA, true, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        else og_pair_cell<ALPHA, true, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
+    } else {
+        if (lds) og_pair_cell<ALPHA, false, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        else og_pair_cell<ALPHA, false, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
+    }
 }
 
 // Per atom: fixed-point pair gradient -> FP64 "work" with the atom's

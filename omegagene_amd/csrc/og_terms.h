@@ -112,7 +112,7 @@ __device__ __forceinline__ void og_atom_work_calc(const og_dev_sys &s, const int
     double    xi[3];
     for (int k = 0; k < 3; k++) xi[k] = s.crd[3 * i + k];
     const float qi = s.charge[i];
-    for (int k = 0; k < 3; k++) w[k] = (double)s.fx[k * cap + g] * OG_FX_INV;
+    og_fx_read(s.fx, cap, og_fx_slot(g), w);
     for (int eb = x0; eb < x1; eb += OG_TERM_BATCH) {
         const int n = min(OG_TERM_BATCH, x1 - eb);
         int       other[OG_TERM_BATCH];
@@ -185,10 +185,11 @@ __device__ __forceinline__ void og_atom_work_calc(const og_dev_sys &s, const int
 // Stores atom i's work and clears its fixed-point accumulators for the next
 // evaluation.
 __device__ __forceinline__ void og_atom_work_store(const og_dev_sys &s, const int i, const double *w) {
-    const int g   = s.rev[i];
+    const int f   = og_fx_slot(s.rev[i]);
     const int cap = s.cap_atom_array;
+    s.fx[3 * cap + f] = 0;
     for (int k = 0; k < 3; k++) {
-        s.fx[k * cap + g] = 0;
+        s.fx[k * cap + f] = 0;
         s.work[3 * i + k] = w[k];
     }
 }
