@@ -381,3 +381,41 @@ def vcmd_bias(L, atoms_a, atoms_b, crdg, crd, mass, n_states_lo, n_states_hi, vs
           _p(hi, C.c_double), C.c_int(vs), C.c_double(sigma), C.c_double(coef), C.c_double(T), _p(work, C.c_double),
           C.byref(lam), C.byref(inr), cand)
     return e, lam.value, bool(inr.value), (cand[0], cand[1])
+
+
+def ref_normals(seed, skip, n):
+    """orc_ref_normals: the reference's N(0,1) draws (std::mt19937 seeded with
+    seed, a fresh std::normal_distribution per draw, Random.cpp:10-43)."""
+    out = np.zeros(int(n))
+    lib().orc_ref_normals(C.c_int(int(seed)), C.c_longlong(int(skip)), C.c_longlong(int(n)), _p(out, C.c_double))
+    return out
+
+
+JOULE_CAL = 4.184        # CelesteObject.cpp
+GAS_CONST = 8.31451      # CelesteObject.cpp:38
+KINETIC_COEFF = (1e7 / (4.184 * 1e3)) * 0.5
+
+
+def temperature_coeff(d_free):
+    """ThermostatObject::set_temperature_coeff (Thermostat.cpp:13-17)."""
+    return (2.0 * JOULE_CAL * 1e+3) / (GAS_CONST * float(d_free)) * KINETIC_COEFF
+
+
+def thermostat_tau_inv(dt, tau):
+    """SubBox::init_thermostat (SubBox.cpp:1674-1676): tau = --berendsen-tau (fs),
+    replaced by the time step when the step is the smaller."""
+    if dt < tau:
+        tau = dt
+    return 1.0 / tau
+
+
+def thermostat_scaling(dt, tau_inv, temperature, temp_coeff, work, vel, mass, mass_inv):
+    """orc_thermostat_scaling (ThermostatScaling::apply_thermostat): returns (vel_next, scale)."""
+    out = np.zeros_like(np.asarray(vel, np.float64))
+    f = lib().orc_thermostat_scaling
+    f.restype = C.c_double
+    s = f(C.c_int(len(out)), C.c_double(dt), C.c_double(tau_inv), C.c_double(temperature), C.c_double(temp_coeff),
+          _p(np.ascontiguousarray(work, np.float64), C.c_double), _p(np.ascontiguousarray(vel, np.float64), C.c_double),
+          _p(np.ascontiguousarray(mass, np.float32), C.c_float),
+          _p(np.ascontiguousarray(mass_inv, np.float32), C.c_float), _p(out, C.c_double))
+    return out, s

@@ -180,6 +180,19 @@ double orc_kinetic(int n, const float *mass, const double *vel, const double *ve
 void orc_langevin_positions(int n, double dt_half, double gamma, double temperature, const float L[3],
                             const double *crd_prev, const double *crd_prev2, const double *work,
                             const float *mass, const float *mass_inv, const double *zeta, double *crd);
+/* The reference's N(0,1) stream (oracle_rng.cpp): std::mt19937 seeded with
+ * seed, one fresh std::normal_distribution per draw (Random.cpp:10-43);
+ * skips `skip` draws, writes the next n. */
+void orc_ref_normals(int seed, long long skip, long long n, double *out);
+/* ThermostatScaling::apply_thermostat (Thermostat.cpp:57-84) after the
+ * leapfrog velocity update: vel = v(t - dt/2), vel_next overwritten.
+ * tau_inv = 1/tau with tau = --berendsen-tau clamped by the step as
+ * SubBox::init_thermostat (SubBox.cpp:1674-1676); temperature_coeff of
+ * ThermostatObject::set_temperature_coeff (Thermostat.cpp:13-17).  Returns
+ * the scale factor. */
+double orc_thermostat_scaling(int n, double dt, double tau_inv, double temperature, double temperature_coeff,
+                              const double *work, const double *vel, const float *mass, const float *mass_inv,
+                              double *vel_next);
 /* Central-difference velocity (SubBox.cpp:2054-2066). */
 void orc_langevin_velocity(int n, double dt, const float L[3], const double *crd, const double *crd_prev2,
                            double *vel);

@@ -268,6 +268,32 @@ void orc_langevin_positions(int n, double dt_half, double gamma, double T, const
         }
 }
 
+/* ThermostatScaling::apply_thermostat (Thermostat.cpp:57-84): the kinetic
+ * energy of the on-step velocities v(t-dt/2) + dt/2 a(t) sets one scale
+ * factor, then v(t+dt/2) = (2 s - 1) v(t-dt/2) + s dt a(t).  `real` is double,
+ * mass and mass_inv are the float arrays (SubBox.cpp:691). */
+double orc_thermostat_scaling(int n, double dt, double tau_inv, double temperature, double temperature_coeff,
+                              const double *work, const double *vel, const float *mass, const float *mass_inv,
+                              double *vel_next) {
+    double kine_pre = 0.0;
+    for (int i = 0; i < n; i++) {
+        double vel_norm = 0;
+        for (int d = 0; d < 3; d++) {
+            double vel_tmp = vel[i * 3 + d] - 0.5 * (dt * work[i * 3 + d] * FORCE_VEL * mass_inv[i]);
+            vel_norm += vel_tmp * vel_tmp;
+        }
+        kine_pre += mass[i] * vel_norm;
+    }
+    double dt_temperature = kine_pre * temperature_coeff;
+    double scale          = sqrt(1 + dt * tau_inv * (temperature / dt_temperature - 1));
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) {
+            double vel_diff      = dt * -FORCE_VEL * work[i * 3 + d] * mass_inv[i];
+            vel_next[i * 3 + d] = (2.0 * scale - 1.0) * vel[i * 3 + d] + scale * vel_diff;
+        }
+    return scale;
+}
+
 /* SubBox::set_velocities_just_langevin (SubBox.cpp:2054-2066) */
 void orc_langevin_velocity(int n, double dt, const float L[3], const double *crd, const double *crd_prev2,
                            double *vel) {

@@ -8,6 +8,16 @@
       (SubBox.cpp:1445-1454; restart writer WriteTrr.cpp:258-313).
   wons_traj.npz -- 20 leapfrog steps of 1,000 rigid waters with SHAKE; the
       per-step energy terms of the log and the final restart.
+  wons_langevin.npz -- position Langevin (gamma 10/ps, 300 K, seed 1234, no
+      constraints: DynamicsMode.cpp:877) on water + a 20-bead chain, 3 steps
+      plus the reference's extra last step.
+  wons_shake_pair.npz -- 10 leapfrog steps with SHAKE pairs on every third
+      bond of a 40-bead chain and SHAKE trios on the waters.
+  wons_com.npz -- 10 leapfrog steps with SHAKE trios on the waters and
+      --com-motion cancel on the atom groups "solute" and "solvent".
+  wons_thermo.npz -- 10 leapfrog steps with --thermostat scaling (no
+      constraints, so ThermostatScaling::apply_thermostat), --berendsen-tau
+      0.1 ps, heating from 250 K to 300 K over 5 steps.
 
 Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
 Run where the reference tree is mounted:
@@ -55,12 +65,12 @@ def parse_log(text):
     return np.array(rows)
 
 
-def run(ls, cfg, workdir):
+def run(ls, cfg, workdir, extra=""):
     clsf = os.path.join(workdir, "in.cls")
     cfgf = os.path.join(workdir, "in.cfg")
     data = write_cls(ls, clsf)
     with open(cfgf, "w") as f:
-        f.write(gen.cfg_text(cfg))
+        f.write(gen.cfg_text(cfg) + extra)
     r = subprocess.run([EXE, "--cfg", cfgf, "--inp", clsf], cwd=workdir, stdout=subprocess.PIPE,
                        stderr=subprocess.STDOUT, text=True)
     # the reference may fail in its destructors after finishing the run; the
@@ -75,8 +85,49 @@ BASE = {"mode": "md", "integrator": "leapfrog-presto", "cutoff": 12.0, "nsgrid-c
         "com-motion": "none", "print-interval-log": 1, "fn-o-restart": "out.restart", "random-seed": 1}
 
 
+def chain_shake_pairs(ls, every=3):
+    """SHAKE pairs on every `every`-th bond of the chain (disjoint pairs), at the current bond lengths."""
+    b = ls.bond_ids[::every]
+    L = float(ls.box[0])
+    d = ls.crd[b[:, 0]] - ls.crd[b[:, 1]]
+    d -= L * np.round(d / L)
+    ls.shake_pair_ids = b.astype(np.int32)
+    ls.shake_pair_d = np.linalg.norm(d, axis=1)
+    return ls
+
+
+def more(tmp):
+    out = {}
+    ls = gen.build_system(500, seed=41, n_beads=20)
+    cfg = dict(BASE, **{"integrator": "langevin", "langevin-gamma": 10.0, "temperature": 300.0, "random-seed": 1234,
+                        "n-steps": 3, "ele-alpha": 0.0, "constraint": "none"})
+    out["wons_langevin"] = (ls, cfg, "")
+    ls = chain_shake_pairs(gen.build_system(500, seed=43, n_beads=40))
+    cfg = dict(BASE, **{"n-steps": 10, "ele-alpha": 0.0, "constraint": "shake", "const-tolerance": 1e-10,
+                        "const-max-loops": 1000})
+    out["wons_shake_pair"] = (ls, cfg, "")
+    ls = gen.build_system(500, seed=45, n_beads=20)
+    cfg = dict(BASE, **{"n-steps": 10, "ele-alpha": 0.0, "constraint": "shake", "const-tolerance": 1e-10,
+                        "const-max-loops": 1000, "com-motion": "cancel"})
+    out["wons_com"] = (ls, cfg, "--com-cancel-group-name      solute\n--com-cancel-group-name      solvent\n")
+    ls = gen.build_system(500, seed=47, n_beads=20)
+    cfg = dict(BASE, **{"n-steps": 10, "ele-alpha": 0.0, "constraint": "none", "thermostat": "scaling",
+                        "berendsen-tau": 0.1, "temperature": 300.0, "temperature-init": 250.0, "heating-steps": 5})
+    out["wons_thermo"] = (ls, cfg, "")
+    for name, (ls, cfg, extra) in out.items():
+        data, log, rs = run(ls, cfg, tmp, extra)
+        lg = parse_log(log)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), cls=np.frombuffer(data, np.uint8), log=lg,
+                            crd=rs["crd"], vel=rs["vel"], epot=rs["epot"], ekin=rs["ekin"], step=rs["step"],
+                            cfg=np.frombuffer((gen.cfg_text(cfg) + extra).encode(), np.uint8))
+        print(name, "ok", lg.shape, lg[0], lg[-1])
+
+
 def main():
     tmp = tempfile.mkdtemp()
+    if len(sys.argv) > 1 and sys.argv[1] == "more":
+        more(tmp)
+        return
     for tag, alpha in (("a0", 0.0), ("a1", 0.1)):
         ls = gen.build_system(700, seed=77, n_beads=20)
         cfg = dict(BASE, **{"n-steps": 1, "ele-alpha": alpha, "constraint": "none"})

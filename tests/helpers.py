@@ -78,3 +78,108 @@ def ele_scale(ls: LaunchSet, cutoff=12.0, alpha=0.0, nscut=13.0, crd=None):
     lj6, lj12 = ls.lj_tables_f32()
     _, en, _ = o.pairs_fp32(zd, cps, ls.atom_type, lj6, lj12, np.abs(ls.charge).astype(np.float32))
     return float(en[1])
+
+
+def oracle_forces_allpairs(ls: LaunchSet, crd, zd, L, alpha_bug=True):
+    """The reference all-pairs build's force terms (SubBox.cpp:1127-1220 pairs,
+    :1347-1367 excess, :1229-1249 bonds) with the oracle: work [n][3] and the
+    potential pieces (vdw, ele incl. excess and self, bond)."""
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    work, en, _ = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, ls.nb15off_table())
+    ex = ls.excess_pairs()
+    eex = oracle.excess(zd, L, ex, crd, q, work, bug_compat=alpha_bug)
+    eb = oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, crd, work) if len(ls.bond_ids) else 0.0
+    eself, _ = oracle.self_energy(zd, q, ex)
+    return work, en[0], en[1] + eex + eself, eb
+
+
+def cfg_value(npz, key, default=None):
+    """A key of the reference .cfg text stored in a fixture."""
+    for line in bytes(npz["cfg"]).decode().splitlines():
+        parts = line.split()
+        if parts and parts[0] == "--" + key:
+            return parts[1]
+    return default
+
+
+def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-presto", constraint="none",
+              tol=1e-10, max_loops=1000, com_groups=None, thermostat=None, langevin=None, extra_last=False):
+    """The reference step loop with the oracle's functions, in the reference
+    order:
+      leapfrog-presto (DynamicsModePresto::calc_in_each_step, DynamicsMode.cpp:468-560):
+        forces; vel <- vel_next; vel_next += dt a (SubBox.cpp:1445-1454); COM
+        cancel (COMMotion.cpp:30-55); scaling thermostat when unconstrained
+        (SubBox.cpp:1745-1761, Thermostat.cpp:57-84); x += dt vel_next;
+        SHAKE pairs then trios (ConstraintShake.cpp:10-16) and
+        v = dx/dt (SubBox.cpp:1739-1743); PBC wrap; KE of (vel + vel_next)/2;
+      langevin (DynamicsModeLangevin::calc_in_each_step, DynamicsMode.cpp:798-899):
+        forces; x_prev2 <- x_prev <- x; step 0: x += dt v; later steps the
+        position Langevin update (SubBox.cpp:2077-2114) with the reference
+        noise stream, v by central difference (SubBox.cpp:2054-2066) and its
+        KE; no constraints (DynamicsMode.cpp:877); PBC wrap.
+    thermostat: dict(tau_fs, temperature, temperature_init, heating_steps, d_free).
+    langevin: dict(gamma_per_fs, temperature, seed).
+    extra_last: the reference runs one more step after the last output
+    (DynamicsMode.cpp:201-208); its restart then holds x_prev and v of that
+    step for Langevin.  Returns (crd, vel, rows[step, pot, kin, bond, vdw, ele]),
+    crd/vel being what the reference's restart holds."""
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    zd = oracle.zd_setup(cutoff, 0.0)
+    m32 = ls.mass.astype(np.float32)
+    minv = (1.0 / m32.astype(np.float64)).astype(np.float32)
+    vel_next = ls.vel.copy()
+    vel = vel_next.copy()
+    crd_prev = crd.copy()
+    crd_prev2 = crd.copy()
+    rows = []
+    n_total = n_steps + (1 if extra_last else 0)
+    n = ls.n_atoms
+    for step in range(n_total):
+        work, evdw, eele, ebond = oracle_forces_allpairs(ls, crd, zd, L)
+        pot = evdw + eele + ebond
+        if integrator == "leapfrog-presto":
+            vel = vel_next
+            vel_next = oracle.update_velocities(dt, vel, work, minv)
+            if com_groups:
+                vel_next = oracle.cancel_com(com_groups, m32, vel_next)
+            if thermostat and constraint == "none":
+                t = thermostat
+                T = t["temperature"]
+                if step < t["heating_steps"]:
+                    T = t["temperature_init"] + ((t["temperature"] - t["temperature_init"]) /
+                                                 float(t["heating_steps"])) * step
+                vel_next, _ = oracle.thermostat_scaling(dt, oracle.thermostat_tau_inv(dt, t["tau_fs"]), T,
+                                                        oracle.temperature_coeff(t["d_free"]), work, vel, m32, minv)
+            crd_prev = crd
+            crd = oracle.update_coordinates(dt, crd_prev, vel_next)
+            if constraint != "none":
+                if len(ls.shake_pair_ids):
+                    crd = oracle.shake_pair(L, ls.shake_pair_ids, np.asarray(ls.shake_pair_d) ** 2, crd, crd_prev, minv)
+                if len(ls.shake_trio_ids):
+                    crd, nf = oracle.shake_trio(L, ls.shake_trio_ids, np.asarray(ls.shake_trio_d) ** 2, crd, crd_prev,
+                                                minv, tol, max_loops)
+                    assert nf == 0
+                vel_next = oracle.velocity_from_crd(dt, crd, crd_prev)
+            crd = oracle.revise_pbc(L, lb, crd)
+            ke = oracle.kinetic(m32, vel, vel_next)
+            out_crd, out_vel = crd, vel_next
+        else:
+            g = langevin
+            crd_prev2, crd_prev = crd_prev, crd
+            if step == 0:
+                crd = oracle.update_coordinates(dt, crd_prev, vel_next)
+                ke = np.nan
+            else:
+                zeta = oracle.ref_normals(g["seed"], (step - 1) * 3 * n, 3 * n).reshape(n, 3)
+                crd = oracle.langevin_positions(dt / 2, g["gamma_per_fs"], g["temperature"], L, crd_prev, crd_prev2,
+                                                work, m32, minv, zeta)
+                vel = oracle.langevin_velocity(dt, L, crd, crd_prev2)
+                ke = oracle.kinetic(m32, vel, vel)
+            crd = oracle.revise_pbc(L, lb, crd)
+            out_crd, out_vel = crd_prev, vel
+        rows.append([step, pot, ke, ebond, evdw, eele])
+    if integrator == "leapfrog-presto":
+        return out_crd, out_vel, np.array(rows)
+    return out_crd, out_vel, np.array(rows)

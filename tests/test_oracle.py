@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import ala3_launchset, cls_from_fixture, load_npz, oracle_forces, rms_rel, wrap_inbox
+from helpers import ala3_launchset, cls_from_fixture, ele_scale, load_npz, oracle_forces, rms_rel, wrap_inbox
 from omegagene_amd import gen
 
 
@@ -406,3 +406,197 @@ def test_settle_keeps_each_atom_in_its_periodic_frame():
     assert np.max(np.abs(new - cur)) < 0.1
     d = lambda a, b: np.linalg.norm((new[a] - new[b]) - Lf * np.round((new[a] - new[b]) / Lf))
     assert abs(d(0, 1) - dOH) < 1e-12 and abs(d(0, 2) - dOH) < 1e-12 and abs(d(1, 2) - dHH) < 1e-12
+
+
+# ------------------------------------------------ reference runs of the step loop
+# Each fixture is a run of the reference all-pairs build (oracle/_ref/omegagene_wons,
+# compiled from the reference sources by oracle/Makefile; tests/golden/make_wons_fixtures.py):
+# the per-step log and the final restart.  helpers.oracle_md restates the reference
+# step loop with the oracle's functions; the restart must agree exactly.
+
+from helpers import oracle_md  # noqa: E402
+
+
+def _check_run(d, crd, vel, rows, n_rows):
+    assert np.max(np.abs(crd - d["crd"])) <= 1e-12, np.max(np.abs(crd - d["crd"]))
+    assert np.max(np.abs(vel - d["vel"])) <= 1e-12, np.max(np.abs(vel - d["vel"]))
+    log = d["log"]
+    for r, lg in zip(rows[:n_rows], log[:n_rows]):
+        assert fmt(r[4]) == fmt(lg[4])                                      # VDW at print precision
+        assert abs(r[1] - lg[1]) <= 1e-9 * abs(lg[5]) + 1e-7               # potential (printed to 1e-7)
+        if np.isfinite(r[2]):
+            assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]) + 1e-7           # kinetic
+
+
+def test_oracle_langevin_matches_reference_run():
+    """Position Langevin (SubBox.cpp:2077-2114, DynamicsMode.cpp:798-899) with the
+    reference's noise stream (std::mt19937 + a fresh std::normal_distribution per
+    draw, Random.cpp:10-43): pins orc_langevin_positions (friction and noise
+    terms at gamma 10/ps), orc_langevin_velocity and orc_ref_normals."""
+    d = load_npz("wons_langevin.npz")
+    ls = cls_from_fixture(d)
+    crd, vel, rows = oracle_md(ls, 3, integrator="langevin", extra_last=True,
+                               langevin=dict(gamma_per_fs=10.0 / 1000.0, temperature=300.0, seed=1234))
+    _check_run(d, crd, vel, rows, 4)
+    # the noise is live: the same steps without it end elsewhere
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    assert oracle.ref_normals(1234, 0, 6).std() > 0.1
+
+
+def test_oracle_shake_pair_matches_reference_run():
+    """SHAKE pairs on a chain (ConstraintShake.cpp:18-49) with SHAKE trios on the
+    waters, 10 leapfrog steps: pins orc_shake_pair."""
+    d = load_npz("wons_shake_pair.npz")
+    ls = cls_from_fixture(d)
+    assert len(ls.shake_pair_ids) > 0
+    crd, vel, rows = oracle_md(ls, 10, constraint="shake")
+    _check_run(d, crd, vel, rows, 10)
+    L = float(np.float32(ls.box[0]))
+    p = ls.shake_pair_ids
+    v = crd[p[:, 0]] - crd[p[:, 1]]
+    v -= L * np.round(v / L)
+    assert np.max(np.abs(np.linalg.norm(v, axis=1) - ls.shake_pair_d)) < 1e-6
+
+
+def test_oracle_com_cancel_matches_reference_run():
+    """--com-motion cancel on two atom groups (COMMotion.cpp:30-55): pins
+    orc_cancel_com; every group's momentum is zero after the removal."""
+    d = load_npz("wons_com.npz")
+    ls = cls_from_fixture(d)
+    g = dict(ls.groups)
+    groups = [g["solute"], g["solvent"]]
+    crd, vel, rows = oracle_md(ls, 10, constraint="shake", com_groups=groups)
+    _check_run(d, crd, vel, rows, 10)
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=(ls.n_atoms, 3))
+    m32 = ls.mass.astype(np.float32)
+    v2 = oracle.cancel_com(groups, m32, v)
+    for grp in groups:
+        mom = (m32[grp, None].astype(np.float64) * v2[grp]).sum(0)
+        scale = (m32[grp, None].astype(np.float64) * np.abs(v[grp])).sum(0)
+        assert np.all(np.abs(mom) <= 1e-6 * scale)   # float group mass (MmSystem.cpp:572-582)
+
+
+def test_oracle_thermostat_matches_reference_run():
+    """--thermostat scaling without constraints (ThermostatScaling::apply_thermostat,
+    Thermostat.cpp:57-84), --berendsen-tau 0.1 ps clamped to the 2 fs step
+    (SubBox.cpp:1674-1676) and heating 250 -> 300 K over 5 steps
+    (SubBox.cpp:1745-1757): pins orc_thermostat_scaling.  With tau clamped to
+    dt the scale is sqrt(T0/T): the logged temperature is the target."""
+    d = load_npz("wons_thermo.npz")
+    ls = cls_from_fixture(d)
+    th = dict(tau_fs=100.0, temperature=300.0, temperature_init=250.0, heating_steps=5, d_free=ls.d_free(False))
+    crd, vel, rows = oracle_md(ls, 10, thermostat=th)
+    _check_run(d, crd, vel, rows, 10)
+    T = d["log"][:10, 6]
+    assert np.allclose(T, [250, 260, 270, 280, 290, 300, 300, 300, 300, 300], rtol=1e-9)
+
+
+def test_thermostat_scaling_closed_form():
+    """One scaling step: the on-step kinetic temperature of v + dt/2 a is
+    rescaled by sqrt(1 + dt/tau (T0/T - 1)) (Thermostat.cpp:57-84)."""
+    rng = np.random.default_rng(4)
+    n = 64
+    m = rng.uniform(1, 16, n).astype(np.float32)
+    minv = (1.0 / m.astype(np.float64)).astype(np.float32)
+    v = rng.normal(scale=0.01, size=(n, 3))
+    w = rng.normal(size=(n, 3))
+    dt, tau, T0 = 2.0, 8.0, 310.0
+    coeff = oracle.temperature_coeff(3 * n - 3)
+    vn, s = oracle.thermostat_scaling(dt, 1.0 / tau, T0, coeff, w, v, m, minv)
+    a = -4.184e-4 * w * minv[:, None].astype(np.float64)
+    T = coeff * np.sum(m[:, None].astype(np.float64) * (v + 0.5 * dt * a) ** 2)
+    assert s == pytest.approx(np.sqrt(1 + dt / tau * (T0 / T - 1)), rel=1e-13)
+    assert np.allclose(vn, (2 * s - 1) * v + s * dt * a, rtol=1e-12, atol=1e-18)
+    assert oracle.thermostat_tau_inv(2.0, 100.0) == 0.5 and oracle.thermostat_tau_inv(2.0, 1.0) == 1.0
+
+
+@pytest.mark.parametrize("fixture", ["wons_grad_a0", "ala3"])
+def test_celllist_pair_loop_equals_allpairs(fixture):
+    """The FP32 cell-list loop (orc_pairs_celllist, SubBox.cpp:956-1125 over the
+    MiniCell list) against the all-pairs loop pinned by the reference runs
+    (orc_pairs_allpairs_fp32): the same pair count exactly.  The cell list
+    takes image shifts on float grid coordinates where the all-pairs loop takes
+    a float minimum image, so a pair's FP32 terms can differ in their last
+    bits (the vdw totals differ at 4e-8 relative on the first system): energies
+    and gradient agree to the FP32 rounding scale."""
+    if fixture == "ala3":
+        ls, _ = ala3_launchset()
+    else:
+        ls = cls_from_fixture(load_npz(fixture + ".npz"))
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    zd = oracle.zd_setup(12.0, 0.0)
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    tab = ls.nb15off_table()
+    wa, ea, na = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, tab)
+    o = oracle.Oracle(ls.n_atoms, ls.box, 13.0)
+    c = o.build(crd)
+    cps, _ = o.enumerate(tab, gap_bug=False)
+    wg, ec, nc = o.pairs_fp32(zd, cps, ls.atom_type, lj6, lj12, q)
+    wc = np.zeros_like(wa)
+    m = c.atomids >= 0
+    wc[c.atomids[m]] = wg[m]
+    assert nc == na
+    # each pair's FP32 terms agree to a few float ulps: the sums to 2e-7 of the
+    # vdw total and 1e-8 of the sum of the ele term magnitudes
+    assert ec[0] == pytest.approx(ea[0], rel=2e-7)
+    assert abs(ec[1] - ea[1]) <= 1e-8 * ele_scale(ls, crd=crd)
+    err = rms_rel(wc, wa)
+    assert err < 1e-6, err   # measured 5.0e-7 on the 24.5 A box, where most pairs are image pairs
+
+
+def test_excess_fixed_mode_closed_form():
+    """orc_excess with the reference's alpha > 0 energy term fixed (bug_compat 0:
+    -erf(ar)/r instead of ZeroMultipoleSum.cpp:214's -erf(ar)*r) against the
+    closed form per excluded pair, E = cc (-erf(a r)/r + bcoeff r^2) with the
+    float constants, and the bug-compatible mode against its own form."""
+    import math
+    zd = oracle.zd_setup(12.0, 0.1)
+    L = np.array([40.0, 40.0, 40.0], np.float32)
+    rng = np.random.default_rng(6)
+    # excluded pairs at bonded distances (0.9-3 A), one of them across the box edge
+    crd = np.zeros((40, 3))
+    crd[0::2] = rng.uniform(0, 40, (20, 3))
+    u = rng.normal(size=(20, 3))
+    crd[1::2] = crd[0::2] + u / np.linalg.norm(u, axis=1)[:, None] * rng.uniform(0.9, 3.0, 20)[:, None]
+    crd[1] = crd[0] + [39.0, 0.5, 0.0]          # minimum image: 1.1 A apart
+    q = rng.uniform(-0.8, 0.8, 40).astype(np.float32)
+    pairs = np.array([[i, i + 1] for i in range(0, 40, 2)], np.int32)
+    for bug in (0, 1):
+        w = np.zeros((40, 3))
+        e = oracle.excess(zd, L, pairs, crd, q, w, bug_compat=bug)
+        ref, mag = 0.0, 0.0
+        for a, b in pairs:
+            d = crd[a].astype(np.float32) - crd[b].astype(np.float32)
+            d = d - L * np.round(d / L)
+            r = float(np.linalg.norm(d.astype(np.float64)))
+            cc = float(q[a]) * float(q[b]) * 332.06378
+            er = math.erf(0.1 * r)
+            t = cc * ((-er * r if bug else -er / r) + float(zd.bcoeff) * r * r)
+            ref += t
+            mag += abs(t)
+        assert abs(e - ref) <= 2e-6 * mag, (bug, e, ref, mag)
+    # the two modes differ by cc erf(ar) (r - 1/r) summed over the pairs
+    w = np.zeros((40, 3))
+    diff = oracle.excess(zd, L, pairs, crd, q, w, bug_compat=1) - oracle.excess(zd, L, pairs, crd, q, w, bug_compat=0)
+    assert abs(diff) > 1e-3
+
+
+def test_ala3_degrees_of_freedom_and_initial_kinetic_energy():
+    """samples/ala3/log03_md.txt:123-125: d_free 12417 = 3N - 3 - (pairs + 3 trios
+    + 6 quads) + 3 with a position restraint (DynamicsMode.cpp:61-65,
+    Read.cpp:90-94), and the initial kinetic energy 3701.28839 of the .cls
+    velocities (DynamicsMode.cpp:358-372)."""
+    d = load_npz("ala3_step0.npz")
+    _, exp = ala3_launchset()
+    n = len(d["crd"])
+    npair, ntrio, nquad = [int(x) for x in d["n_shake"]]
+    dfree = 3 * n - 3 - (npair + 3 * ntrio + 6 * nquad) + (3 if bool(d["has_pos_rest"]) else 0)
+    assert dfree == int(exp["d_free"]) == 12417
+    m32 = d["mass"].astype(np.float32)
+    ke = oracle.kinetic(m32, d["vel"], d["vel"])
+    assert "%.8e" % ke == "%.8e" % exp["ke_init"]
+    T = ke * 2 * 4184.0 / (8.31451 * dfree)
+    assert T == pytest.approx(exp["t_init"], rel=1e-9)
