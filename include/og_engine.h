@@ -135,7 +135,16 @@ typedef struct {
     double   enhance_recov_coef;   /* --enhance-recovery-coef (VcMD) */
     int      split_step;           /* 1: run a Langevin step as separate kernels even where the fused
                                       one-kernel step applies (same trajectory; for comparison) */
+    int      thermostat;           /* --thermostat: OG_THERMOSTAT_NONE or OG_THERMOSTAT_SCALING (leapfrog-presto,
+                                      no constraints: ThermostatScaling::apply_thermostat, Thermostat.cpp:57-84) */
+    double   berendsen_tau;        /* --berendsen-tau already in fs (Config.cpp:162-164); the step replaces it
+                                      when smaller (SubBox.cpp:1674-1676) */
+    double   temperature_init;     /* --temperature-init (K); < 0: --temperature (Config.cpp:364) */
+    int      heating_steps;        /* --heating-steps: linear ramp temperature_init -> temperature
+                                      (SubBox::update_thermostat, SubBox.cpp:1745-1757) */
 } og_params;
+
+enum { OG_THERMOSTAT_NONE = 0, OG_THERMOSTAT_SCALING = 1 };
 
 void og_params_default(og_params *p);
 
@@ -166,6 +175,16 @@ og_status og_calc_energy_force(og_system *sys, int rebuild, double *work_atom, d
  * DynamicsMode.cpp:468-560 / :798-899), grid rebuilt every nsgrid_update_intvl
  * steps; launched as a captured hipGraph. */
 og_status og_md_run(og_system *sys, int n_steps);
+
+/* The N(0,1) draws zeta [n_atoms*3] (atom-major x, y, z) that the next MD
+ * step's position Langevin update (SubBox.cpp:2077-2114) uses in place of the
+ * engine's counter-based generator; zeta = NULL cancels.  No reference
+ * counterpart: it lets a caller drive the device step with the reference's
+ * own noise stream (std::mt19937 + std::normal_distribution, Random.cpp:10-43)
+ * and compare trajectories bit for bit.  Applies to og_md_run of this system. */
+og_status og_set_langevin_noise(og_system *sys, const double *zeta);
+/* gradient ("work", dE/dx) of the last force evaluation or MD step, [n*3] atom order */
+og_status og_get_work(og_system *sys, double *work_atom);
 
 /* State exchange (host arrays [n*3], atom order). */
 og_status og_get_state(og_system *sys, double *crd, double *vel_next, double *vel_just);

@@ -199,7 +199,7 @@ struct og_exec {
     std::vector<hipEvent_t>  ev;          // pair-kernel timing events
     int                      alpha_mode = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
-    bool                     any_vcmd = false;
+    bool                     any_vcmd = false, any_thermo = false;
     bool                     fused = false;       // every member takes the fused Langevin step
     int                      pair_shape = OG_PAIR_DEFAULT;  // pair-kernel launch shape (KIND_PAIR + pair_shape)
     int                      intvl = 1;
@@ -244,7 +244,7 @@ og_status exec_prepare(og_exec &x) {
     std::vector<og_dev_sys> hs(ns);
     std::vector<int2>       maps[KIND_N];
     x.alpha_mode   = x.members[0]->alpha_mode;
-    x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = false;
+    x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = x.any_thermo = false;
     x.fused        = true;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
@@ -264,6 +264,7 @@ og_status exec_prepare(og_exec &x) {
         x.any_com |= d.com_cancel && d.n_com_groups > 0;
         x.any_constraint |= d.constraint != 0;
         x.any_vcmd |= d.vc_n_states > 0;
+        x.any_thermo |= d.thermostat == OG_THERMOSTAT_SCALING;
         x.fused &= d.integrator == 1 && d.n_water_blocks > 0;
         const int n_bins = (d.n_columns + 1) * d.n_uni_z;
         int       cnt[KIND_N];
@@ -379,6 +380,8 @@ void enqueue_integrate(og_exec &x) {
             og_launch_com_sum(L(x, KIND_SYS));
             og_launch_com_apply(L(x, KIND_COMA));
         }
+        // SubBox::update_thermostat + apply_thermostat (DynamicsMode.cpp:525-532)
+        if (x.any_thermo) og_launch_thermostat(L(x, KIND_ATOM));
         og_launch_positions(L(x, KIND_ATOM));
     }
     if (x.any_langevin) og_launch_langevin_positions(L(x, KIND_ATOM));
@@ -589,6 +592,17 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.tol         = p->const_tolerance;
     h.temperature = p->temperature;
     h.gamma       = p->langevin_gamma;
+    h.thermostat    = p->thermostat;
+    if (h.thermostat == OG_THERMOSTAT_SCALING) {
+        // SubBox::init_thermostat (SubBox.cpp:1660-1679): tau replaced by the step when the step is smaller;
+        // ThermostatObject::set_temperature_coeff (Thermostat.cpp:13-17)
+        double tau = p->berendsen_tau;
+        if (p->time_step < tau) tau = p->time_step;
+        h.th_tau_inv    = 1.0 / tau;
+        h.th_temp_init  = p->temperature_init < 0 ? p->temperature : p->temperature_init;
+        h.heating_steps = p->heating_steps;
+        h.th_temp_coeff = (2.0 * 4.184 * 1e+3) / (8.31451 * (double)d->d_free) * ((1e7 / (4.184 * 1e3)) * 0.5);
+    }
     h.settle_dOH  = d->settle_dOH;
     h.settle_dHH  = d->settle_dHH;
     h.seed_lo     = (uint32_t)p->random_seed;
@@ -634,6 +648,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     if (!ok) return OG_E_DEVICE;
     h.n_atom_blocks = cdiv(n, OG_THREADS);
     h.n_ke_parts    = std::max(h.n_atom_blocks, cdiv(h.n_settle, OG_WATERS_PER_BLOCK) + 1);
+    if (h.thermostat == OG_THERMOSTAT_SCALING && !A(&h.th_part, h.n_atom_blocks)) return OG_E_DEVICE;
     ok = A(&h.e_atom, std::max(h.n_ke_parts, 1)) &&
          A(&h.com_mom, 3 * (size_t)(h.n_com_groups + 1)) && A(&h.energy, OG_E_NTERMS) &&
          A(&h.elog, (size_t)h.log_cap * OG_E_NTERMS) && A(&s->d_cnt, 1) && A(&s->d_self, 1);
@@ -641,7 +656,8 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.cnt = s->d_cnt;
     {
         og_dev_counters c0{};
-        c0.step_end = OG_NO_STEP_END;
+        c0.step_end   = OG_NO_STEP_END;
+        c0.noise_step = -1;
         OG_HIP(hipMemcpy(s->d_cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
     }
 
@@ -810,8 +826,9 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         OG_HIP(hipMemcpy(h.vc_hi, d->vc_max, sizeof(double) * ns, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.vc_qinv, qinv.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
         og_dev_counters c0{};
-        c0.vs       = d->vc_init_state;
-        c0.step_end = OG_NO_STEP_END;
+        c0.vs         = d->vc_init_state;
+        c0.step_end   = OG_NO_STEP_END;
+        c0.noise_step = -1;
         OG_HIP(hipMemcpy(s->d_cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
     }
     OG_HIP(hipMemcpy(s->d_self, &h, sizeof h, hipMemcpyHostToDevice));
@@ -928,6 +945,10 @@ void og_params_default(og_params *p) {
     p->log_stride          = 1;
     p->log_cap             = 1024;
     p->capacity_factor     = 1.0;
+    p->thermostat          = OG_THERMOSTAT_NONE;
+    p->berendsen_tau       = 0.0;   // Config.h:23
+    p->temperature_init    = -1.0;  // Config.h:22
+    p->heating_steps       = 0;     // Config.h:24
 }
 
 const char *og_status_string(og_status s) {
@@ -989,6 +1010,7 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     b += pairs * 12;
     b += 8.0 * OG_EACC_SLOTS * OG_EACC_STRIDE + ke_parts * 8 + OG_E_NTERMS * 8 + log_rows * OG_E_NTERMS * 8;
     b += std::min(n, (double)OG_FREE_MAX) * 4 + vc * 28 + vs * 32;
+    if (p->thermostat == OG_THERMOSTAT_SCALING) b += std::ceil(n / OG_THREADS) * 8;   // thermostat partials
     b += sizeof(og_dev_counters) + sizeof(og_dev_sys);
     *bytes = (uint64_t)(b * 1.02) + 65536;                                // allocation granularity
     return OG_OK;
@@ -1041,6 +1063,21 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
             set_err("SETTLE needs 0 < dHH < 2 dOH");
             return OG_E_ARG;
         }
+    }
+    if (p->thermostat == OG_THERMOSTAT_SCALING) {
+        // the reference switches to ThermostatScaling::apply_thermostat_with_shake under constraints
+        // (DynamicsModePresto::apply_constraint, DynamicsMode.cpp:564-576), which this engine does not run
+        if (p->integrator != OG_INTEGRATOR_LEAPFROG_PRESTO || p->constraint != OG_CONSTRAINT_NONE) {
+            set_err("the scaling thermostat runs with leapfrog-presto and no constraints");
+            return OG_E_ARG;
+        }
+        if (!(p->berendsen_tau > 0.0) || d->d_free <= 0) {
+            set_err("the scaling thermostat needs --berendsen-tau > 0 and positive degrees of freedom");
+            return OG_E_ARG;
+        }
+    } else if (p->thermostat != OG_THERMOSTAT_NONE) {
+        set_err("unknown thermostat %d", p->thermostat);
+        return OG_E_ARG;
     }
     if (p->com_cancel && d->n_com_groups > 0) {
         // groups must be disjoint for the parallel removal
@@ -1154,6 +1191,35 @@ og_status og_get_state(og_system *s, double *crd, double *vel_next, double *vel_
         else
             for (size_t k = 0; k < n; k++) vel_just[k] = (a[k] + b[k]) * 0.5;
     }
+    return OG_OK;
+}
+
+og_status og_set_langevin_noise(og_system *s, const double *zeta) {
+    if (!s) return OG_E_ARG;
+    if (s->ctx && s->ctx->stream) OG_HIP(hipStreamSynchronize(s->ctx->stream));
+    long long step = -1;
+    if (zeta) {
+        const size_t n = 3 * (size_t)s->h.n_atoms;
+        if (!s->h.noise) {
+            double *p = nullptr;
+            OG_HIP(hipMalloc((void **)&p, sizeof(double) * n));
+            s->allocs.push_back(p);
+            s->dev_bytes += sizeof(double) * n;
+            s->h.noise = p;
+            OG_HIP(hipMemcpy(s->d_self, &s->h, sizeof s->h, hipMemcpyHostToDevice));
+            if (s->ctx) s->ctx->dirty = true;  // its descriptor copy carries the pointer
+        }
+        OG_HIP(hipMemcpy(s->h.noise, zeta, sizeof(double) * n, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(&step, &s->d_cnt->step, sizeof step, hipMemcpyDeviceToHost));
+    }
+    OG_HIP(hipMemcpy(&s->d_cnt->noise_step, &step, sizeof step, hipMemcpyHostToDevice));
+    return OG_OK;
+}
+
+og_status og_get_work(og_system *s, double *work_atom) {
+    if (!s || !work_atom) return OG_E_ARG;
+    if (s->ctx && s->ctx->stream) OG_HIP(hipStreamSynchronize(s->ctx->stream));
+    OG_HIP(hipMemcpy(work_atom, s->h.work, sizeof(double) * 3 * (size_t)s->h.n_atoms, hipMemcpyDeviceToHost));
     return OG_OK;
 }
 

@@ -102,6 +102,68 @@ __global__ void __launch_bounds__(OG_THREADS) k_com_apply(const og_dev_sys *__re
     for (int d = 0; d < 3; d++) s.vel_next[3 * a + d] -= s.com_mom[3 * g + d];
 }
 
+// ------------------------------------------------- scaling thermostat --
+// ThermostatScaling::apply_thermostat (Thermostat.cpp:57-84) after the
+// leapfrog velocity update and COM removal, for unconstrained leapfrog runs:
+// the kinetic energy of the on-step velocities vel - dt/2 w FORCE_VEL m^-1
+// sets one factor s = sqrt(1 + dt/tau (T0/T - 1)) and
+// vel_next = (2 s - 1) vel + s dt (-FORCE_VEL) w m^-1.  T0 follows the
+// heating ramp of SubBox::update_thermostat (SubBox.cpp:1745-1757).  Block
+// partials in a fixed tree, then every block sums them in the same order.
+__global__ void __launch_bounds__(OG_THREADS) k_thermo_sum(const og_dev_sys *__restrict__ S,
+                                                           const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s) || s.thermostat != OG_TH_SCALING || s.integrator != 0) return;
+    const int i  = bm.y * OG_THREADS + threadIdx.x;
+    double    ke = 0.0;
+    if (i < s.n_atoms) {
+        double vel_norm = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double vel_tmp =
+                s.vel[3 * i + d] - 0.5 * (s.dt * s.work[3 * i + d] * OG_FORCE_VEL * (double)s.mass_inv[i]);
+            vel_norm += vel_tmp * vel_tmp;
+        }
+        ke = (double)s.mass[i] * vel_norm;
+    }
+    __shared__ double sh[OG_THREADS];
+    sh[threadIdx.x] = ke;
+    __syncthreads();
+    for (int off = OG_THREADS / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) s.th_part[bm.y] = sh[0];
+}
+
+__global__ void __launch_bounds__(OG_THREADS) k_thermo_apply(const og_dev_sys *__restrict__ S,
+                                                             const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s) || s.thermostat != OG_TH_SCALING || s.integrator != 0) return;
+    __shared__ double sh[OG_THREADS];
+    double            t = 0.0;
+    for (int k = threadIdx.x; k < s.n_atom_blocks; k += OG_THREADS) t += s.th_part[k];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    for (int off = OG_THREADS / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+        __syncthreads();
+    }
+    const long long step = s.cnt->step;
+    double          T0   = s.temperature;
+    if (step < s.heating_steps)
+        T0 = s.th_temp_init + ((s.temperature - s.th_temp_init) / (double)s.heating_steps) * (double)step;
+    const double dt_temperature = sh[0] * s.th_temp_coeff;
+    const double scale          = sqrt(1 + s.dt * s.th_tau_inv * (T0 / dt_temperature - 1));
+    const int    i              = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms) return;
+    for (int d = 0; d < 3; d++) {
+        const double vel_diff = s.dt * -OG_FORCE_VEL * s.work[3 * i + d] * (double)s.mass_inv[i];
+        s.vel_next[3 * i + d] = (2.0 * scale - 1.0) * s.vel[3 * i + d] + scale * vel_diff;
+    }
+}
+
 // ---------------------------------------------------------- positions --
 // SubBox::cpy_crd_prev + update_coordinates(dt, crd, vel_next) (SubBox.cpp:1409-1412,1562-1580)
 __global__ void __launch_bounds__(OG_THREADS) k_positions(const og_dev_sys *__restrict__ S,
@@ -133,11 +195,19 @@ __device__ __forceinline__ void og_langevin_core(const og_dev_sys &s, const int 
         for (int d = 0; d < 3; d++) xn[d] = x[d] + vn[d] * s.dt;
         return;
     }
-    const uint4   r  = og_philox(make_uint4((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)i, 0x4c414e47u),
-                                 make_uint2(s.seed_lo, s.seed_hi));
-    const double2 z0 = og_gauss2(r.x, r.y);
-    const double2 z1 = og_gauss2(r.z, r.w);
-    const double  zeta[3] = {z0.x, z0.y, z1.x};
+    double zeta[3];
+    if (s.noise && s.cnt->noise_step == step) {
+        // draws injected by the caller for this step (og_set_langevin_noise)
+        for (int d = 0; d < 3; d++) zeta[d] = s.noise[3 * i + d];
+    } else {
+        const uint4   r  = og_philox(make_uint4((uint32_t)step, (uint32_t)(step >> 32), (uint32_t)i, 0x4c414e47u),
+                                     make_uint2(s.seed_lo, s.seed_hi));
+        const double2 z0 = og_gauss2(r.x, r.y);
+        const double2 z1 = og_gauss2(r.z, r.w);
+        zeta[0] = z0.x;
+        zeta[1] = z0.y;
+        zeta[2] = z1.x;
+    }
     const double  dth     = s.dt * 0.5;
     const double  gm      = s.gamma;
     const double  sig     = sqrt(OG_GAS_CONST * s.temperature * gm * mass / dth * 1e-7);
@@ -797,6 +867,10 @@ void og_launch_com_sum(const og_launch &a) {
 }
 void og_launch_com_apply(const og_launch &a) {
     hipLaunchKernelGGL(k_com_apply, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+}
+void og_launch_thermostat(const og_launch &a) {
+    hipLaunchKernelGGL(k_thermo_sum, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
+    hipLaunchKernelGGL(k_thermo_apply, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);
 }
 void og_launch_positions(const og_launch &a) {
     hipLaunchKernelGGL(k_positions, dim3(a.n_blocks), dim3(OG_THREADS), 0, a.stream, a.sys, a.map);

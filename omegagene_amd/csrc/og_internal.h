@@ -24,6 +24,7 @@
 #define OG_FXJ_INV (1.0 / 524288.0)
 #define OG_FXJ_ROW_LIMIT 2048.0f
 #define OG_NO_STEP_END 0x7fffffffffffffffLL  // og_dev_counters::step_end of a system with no segment limit
+#define OG_TH_SCALING 1  // og_params::thermostat value of the scaling thermostat (OG_THERMOSTAT_SCALING)
 
 // energy terms kept per system and per step
 enum {
@@ -60,6 +61,7 @@ struct og_dev_counters {
     int       vs;        // current VcMD virtual state
     int       force_energy;  // 1: evaluate energies on the next force call whatever the step
     long long step_end;  // the system stands still once step reaches this (segment end)
+    long long noise_step;  // the step whose Langevin update takes its normals from og_dev_sys::noise (-1: none)
 };
 
 struct og_dev_sys {
@@ -92,6 +94,12 @@ struct og_dev_sys {
     float zcore, bcoeff, fcoeff, piewald, alpha;
     // ---- integration (FP64) ----
     double dt, tol, temperature, gamma;
+    // scaling thermostat (leapfrog, no constraints): 1/tau after the clamp by the step, heating ramp,
+    // temperature coefficient 2 * 4184 / (R d_free) * KINETIC_COEFF (Thermostat.cpp:13-17)
+    int    thermostat, heating_steps;
+    double th_tau_inv, th_temp_init, th_temp_coeff;
+    double *th_part;       // [n_atom_blocks] kinetic partials of the thermostat
+    double *noise;         // [n_atoms*3] injected N(0,1) draws (og_set_langevin_noise), or null
     double settle_dOH, settle_dHH;
     double e_self;
     uint32_t seed_lo, seed_hi;
@@ -346,6 +354,7 @@ void og_launch_energy_reduce(const og_launch &sys);
 void og_launch_leapfrog_vel(const og_launch &atoms);
 void og_launch_com_sum(const og_launch &sys);
 void og_launch_com_apply(const og_launch &atoms);
+void og_launch_thermostat(const og_launch &atoms);  // scaling thermostat: kinetic partials, then the rescale
 void og_launch_positions(const og_launch &atoms);
 void og_launch_langevin_positions(const og_launch &atoms);
 void og_launch_constraints(const og_launch &trio, const og_launch &misc);

@@ -23,6 +23,7 @@ TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend", 
 NTERMS = len(TERMS)
 INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
+THERMOSTATS = {"none": 0, "scaling": 1}
 
 
 class EngineError(RuntimeError):
@@ -57,7 +58,8 @@ class Params(C.Structure):
                 ("temperature", C.c_double), ("langevin_gamma", C.c_double), ("random_seed", C.c_uint64),
                 ("com_cancel", C.c_int), ("gap_bug", C.c_int), ("excess_bug", C.c_int), ("log_stride", C.c_int),
                 ("log_cap", C.c_int), ("capacity_factor", C.c_double), ("enhance_sigma", C.c_double),
-                ("enhance_recov_coef", C.c_double), ("split_step", C.c_int)]
+                ("enhance_recov_coef", C.c_double), ("split_step", C.c_int), ("thermostat", C.c_int),
+                ("berendsen_tau", C.c_double), ("temperature_init", C.c_double), ("heating_steps", C.c_int)]
 
 
 def lib():
@@ -104,6 +106,10 @@ def params_from_cfg(cfg: dict) -> Params:
         p.capacity_factor = float(cfg["capacity-factor"])
     p.enhance_sigma = float(g("enhance-sigma", 0.0))                # Config.cpp --enhance-sigma
     p.enhance_recov_coef = float(g("enhance-recovery-coef", 0.0))   # Config.cpp --enhance-recovery-coef
+    p.thermostat = THERMOSTATS[g("thermostat", "none")]             # Config.cpp:133-143
+    p.berendsen_tau = float(g("berendsen-tau", 0.0)) * 1000.0       # ps -> fs (Config.cpp:162-164)
+    p.temperature_init = float(g("temperature-init", -1.0))         # Config.cpp:166-167
+    p.heating_steps = int(g("heating-steps", 0))                    # Config.cpp:169-170
     return p
 
 
@@ -284,6 +290,21 @@ class System:
 
     def set_pair_timing(self, on=True):
         _check(lib().og_set_pair_timing(self.h, C.c_int(int(on))))
+
+    def set_langevin_noise(self, zeta):
+        """N(0,1) draws [n][3] for the next step's Langevin update (None cancels)."""
+        if zeta is None:
+            _check(lib().og_set_langevin_noise(self.h, None))
+            return
+        z = np.ascontiguousarray(zeta, np.float64).reshape(-1)
+        assert z.size == 3 * self.n_atoms
+        _check(lib().og_set_langevin_noise(self.h, z.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def work(self):
+        """Gradient dE/dx of the last evaluation or step, [n][3] atom order."""
+        w = np.zeros((self.n_atoms, 3))
+        _check(lib().og_get_work(self.h, w.ctypes.data_as(C.POINTER(C.c_double))))
+        return w
 
     def autotune(self, trials=5):
         """Time the pair kernel over its launch shapes and keep the fastest

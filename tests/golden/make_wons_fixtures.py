@@ -37,6 +37,9 @@ sys.path.insert(0, ROOT)
 from omegagene_amd import gen  # noqa: E402
 from omegagene_amd.cls import write_cls  # noqa: E402
 
+sys.path.insert(0, os.path.dirname(HERE))
+from helpers import chain_shake_pairs  # noqa: E402
+
 EXE = os.path.join(ROOT, "oracle", "_ref", "omegagene_wons")
 
 
@@ -83,17 +86,6 @@ def run(ls, cfg, workdir, extra=""):
 BASE = {"mode": "md", "integrator": "leapfrog-presto", "cutoff": 12.0, "nsgrid-cutoff": 13.0,
         "nsgrid-update-intvl": 1, "time-step": 2.0, "electrostatic": "zero-dipole", "thermostat": "none",
         "com-motion": "none", "print-interval-log": 1, "fn-o-restart": "out.restart", "random-seed": 1}
-
-
-def chain_shake_pairs(ls, every=3):
-    """SHAKE pairs on every `every`-th bond of the chain (disjoint pairs), at the current bond lengths."""
-    b = ls.bond_ids[::every]
-    L = float(ls.box[0])
-    d = ls.crd[b[:, 0]] - ls.crd[b[:, 1]]
-    d -= L * np.round(d / L)
-    ls.shake_pair_ids = b.astype(np.int32)
-    ls.shake_pair_d = np.linalg.norm(d, axis=1)
-    return ls
 
 
 def more(tmp):

@@ -183,3 +183,14 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
     if integrator == "leapfrog-presto":
         return out_crd, out_vel, np.array(rows)
     return out_crd, out_vel, np.array(rows)
+
+
+def chain_shake_pairs(ls: LaunchSet, every=3):
+    """SHAKE pairs on every `every`-th bond of the chain (disjoint pairs), at the current bond lengths."""
+    b = ls.bond_ids[::every]
+    L = float(ls.box[0])
+    d = ls.crd[b[:, 0]] - ls.crd[b[:, 1]]
+    d -= L * np.round(d / L)
+    ls.shake_pair_ids = b.astype(np.int32)
+    ls.shake_pair_d = np.linalg.norm(d, axis=1)
+    return ls
