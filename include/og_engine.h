@@ -193,6 +193,10 @@ og_status og_get_energies(og_system *sys, double *energies /* [OG_NTERMS] */);
 /* energy log of the last run: rows of OG_NTERMS doubles, one per log_stride steps */
 og_status og_get_energy_log(og_system *sys, double *out, int max_rows, int *n_rows);
 og_status og_get_step(og_system *sys, long long *step);
+/* outcome of the system's last run, batch run or evaluation (an og_status):
+ * in a batch every member reports its own, and a faulted member stands
+ * still while the others go on */
+og_status og_system_status(og_system *sys, int *status);
 /* VcMD state: current virtual state and the visit counts per state
  * (q_raw, Extend.cpp:828-832); counts may be NULL */
 og_status og_get_vcmd(og_system *sys, int *vstate, double *counts, int n_states);

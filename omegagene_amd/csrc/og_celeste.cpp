@@ -577,6 +577,9 @@ og_status og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, i
     og_dev_counters c;
     CS_HIP(hipMemcpy(&c, g_cs.d_cnt, sizeof c, hipMemcpyDeviceToHost));
     if (c.flags & OG_F_NAN) {
+        // reported once: the flag would otherwise stop every later evaluation
+        const int zero = 0;
+        CS_HIP(hipMemcpy(&g_cs.d_cnt->flags, &zero, sizeof(int), hipMemcpyHostToDevice));
         og_set_error("og_memcpy_dtoh_work: non-finite or out-of-range gradient");
         return OG_E_NAN;
     }

@@ -155,6 +155,22 @@ template <class T> hipError_t dalloc(T **p, size_t n) {
 
 int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// MmSystem::revise_crd_inbox (MmSystem.cpp:808-816): coordinates into [lb, ub)
+og_status wrap_inbox(const GridGeom &g, int n, double *crd) {
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) {
+            double &x = crd[3 * (size_t)i + k];
+            if (!std::isfinite(x)) {
+                set_err("non-finite input coordinate (atom %d)", i);
+                return OG_E_NAN;
+            }
+            if (x < (double)g.lb[k] || x >= (double)g.ub[k]) x -= std::floor((x - g.lb[k]) / g.L[k]) * g.L[k];
+            while (x < (double)g.lb[k]) x += g.L[k];
+            while (x >= (double)g.ub[k]) x -= g.L[k];
+        }
+    return OG_OK;
+}
+
 }  // namespace
 
 // error text for the call-point layer (og_celeste.cpp), read by og_last_error
@@ -186,6 +202,7 @@ struct og_system {
     long long           step_end      = OG_NO_STEP_END;  // host mirror of og_dev_counters::step_end
     struct og_exec     *ctx = nullptr;   // private batch-of-one
     uint64_t            dev_bytes = 0;   // device memory allocated for the system
+    og_status           status = OG_OK;  // outcome of the system's last run or evaluation
 };
 
 struct og_exec {
@@ -195,7 +212,7 @@ struct og_exec {
     int                      n_blocks[KIND_N]{};
     hipStream_t              stream = nullptr;
     bool                     dirty  = true;
-    std::map<long long, hipGraphExec_t> graphs;
+    std::map<std::string, hipGraphExec_t> graphs;  // by rebuild pattern: '1' = rebuild before that step
     std::vector<hipEvent_t>  ev;          // pair-kernel timing events
     int                      alpha_mode = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
@@ -390,135 +407,154 @@ void enqueue_integrate(og_exec &x) {
     og_launch_step_close(L(x, KIND_SYS));
 }
 
-// Capture K steps (rebuild at steps whose index within the chunk is a
-// multiple of intvl; chunks start aligned) into one graph.
-og_status get_graph(og_exec &x, int K, hipGraphExec_t *out) {
-    const long long key = K;
-    auto            it  = x.graphs.find(key);
+// Ask for a grid rebuild of this system before its next step whatever its
+// phase (a new system, new coordinates, an explicit rebuild); the step's
+// closing kernel (or the single-point reduction) clears the request.
+og_status request_rebuild(og_system *s, int on = 1) {
+    OG_HIP(hipMemcpy(&s->d_cnt->rebuild_now, &on, sizeof(int), hipMemcpyHostToDevice));
+    return OG_OK;
+}
+
+void enqueue_step(og_exec &x, bool rebuild, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+    if (rebuild) enqueue_rebuild(x);
+    enqueue_forces(x, e0, e1);
+    enqueue_integrate(x);
+}
+
+// Rebuild pattern of the next K steps: step k gets the rebuild kernels when
+// some running member is due then (its own step a multiple of the interval)
+// or, at k = 0, when a rebuild was requested; the kernels skip the members
+// that are not due, so members at different phases share the launches.
+std::string rebuild_pattern(const og_exec &x, int K, long long offset, bool forced) {
+    std::string p((size_t)K, '0');
+    for (const og_system *s : x.members) {
+        const long long st = std::min(s->host_step + offset, s->step_end);
+        if (st >= s->step_end) continue;  // frozen for this chunk
+        for (int k = 0; k < K && st + k < s->step_end; k++)
+            if ((st + k) % x.intvl == 0) p[(size_t)k] = '1';
+    }
+    if (forced && K > 0) p[0] = '1';
+    return p;
+}
+
+// One graph per rebuild pattern (its length is the chunk's step count).
+og_status get_graph(og_exec &x, const std::string &pattern, hipGraphExec_t *out) {
+    auto it = x.graphs.find(pattern);
     if (it != x.graphs.end()) {
         *out = it->second;
         return OG_OK;
     }
+    if (x.graphs.size() >= 32) free_graphs(x);  // bound the cache (serving mixes phases)
     hipGraph_t g;
     OG_HIP(hipStreamBeginCapture(x.stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < K; k++) {
-        if (k % x.intvl == 0) enqueue_rebuild(x);
-        enqueue_forces(x, nullptr, nullptr);
-        enqueue_integrate(x);
-    }
+    for (char c : pattern) enqueue_step(x, c == '1');
     OG_HIP(hipStreamEndCapture(x.stream, &g));
     hipGraphExec_t ge;
     OG_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
     hipGraphDestroy(g);
-    x.graphs[key] = ge;
-    *out          = ge;
+    x.graphs[pattern] = ge;
+    *out              = ge;
     return OG_OK;
 }
 
+// Reads every member's counters: its step, and a status from its flags.  A
+// faulted member stands still on the device (og_frozen); its sticky state is
+// cleared here so it can be run again once the caller has acted on the
+// error, and its grid is rebuilt first.  Returns the first member's error.
 og_status check_flags(og_exec &x) {
+    og_status first = OG_OK;
     for (og_system *s : x.members) {
         og_dev_counters c;
         OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
         s->host_step = c.step;
+        og_status st = OG_OK;
+        char      msg[256];
+        if (c.flags & (OG_F_CP_OVERFLOW | OG_F_CELL_OVERFLOW | OG_F_COL_OVERFLOW)) {
+            st = OG_E_CAPACITY;
+            snprintf(msg, sizeof msg, "device list overflow (flags 0x%x): cells %d, cell pairs cap %d", c.flags,
+                     c.n_cells, s->h.cap_cell_pairs);
+        } else if (c.flags & OG_F_NAN) {
+            st = OG_E_NAN;
+            snprintf(msg, sizeof msg, "non-finite coordinate or force (flags 0x%x) at step %lld", c.flags, c.step);
+        } else if (c.shake_fail) {
+            st = OG_E_NOCONV;
+            snprintf(msg, sizeof msg, "SHAKE did not converge for %d constraint groups", c.shake_fail);
+        }
+        s->status = st;
         // the device flags are sticky: clear them once reported
         if (c.flags & ~OG_F_OUTSIDE_BOX) OG_HIP(hipMemset(&s->d_cnt->flags, 0, sizeof(int)));
-        if (c.flags & (OG_F_CP_OVERFLOW | OG_F_CELL_OVERFLOW | OG_F_COL_OVERFLOW)) {
-            set_err("device list overflow (flags 0x%x): cells %d, cell pairs cap %d", c.flags, c.n_cells,
-                    s->h.cap_cell_pairs);
-            return OG_E_CAPACITY;
+        if (c.shake_fail) OG_HIP(hipMemset(&s->d_cnt->shake_fail, 0, sizeof(int)));
+        if (st == OG_E_CAPACITY || st == OG_E_NAN) {
+            // a step stopped part-way: drop its partial sums, rebuild before the next one
+            OG_HIP(hipMemset(s->h.fx, 0, sizeof(long long) * 4 * (size_t)s->h.cap_atom_array));
+            OG_HIP(hipMemset(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE));
+            s->grid_valid = false;
         }
-        if (c.flags & OG_F_NAN) {
-            set_err("non-finite coordinate or force (flags 0x%x)", c.flags);
-            return OG_E_NAN;
-        }
-        if (c.shake_fail) {
-            set_err("SHAKE did not converge for %d constraint groups", c.shake_fail);
-            OG_HIP(hipMemset(&s->d_cnt->shake_fail, 0, sizeof(int)));
-            return OG_E_NOCONV;
+        if (st != OG_OK && first == OG_OK) {
+            first = st;
+            set_err("%s", msg);
         }
     }
-    return OG_OK;
+    return first;
 }
 
-// Run n steps for every member; chunks of K steps as graph replays.
+// Run n steps for every member as graph replays of up to graph_steps (100 by
+// default) steps; a member at any rebuild phase can join.  With timing on,
+// steps are launched directly with HIP events around each pair launch
+// (events inside a captured graph cannot be timed).
 og_status exec_run(og_exec &x, int n_steps, bool timing) {
     og_status st = exec_prepare(x);
     if (st != OG_OK) return st;
     if (x.members.empty() || n_steps <= 0) return OG_OK;
-    // members whose segment is complete stand still; the others must share the rebuild phase
-    const og_system *lead = nullptr;
-    for (og_system *s : x.members)
-        if (s->host_step < s->step_end) {
-            lead = s;
-            break;
+    bool                     forced = false;
+    std::vector<og_system *> running;
+    for (og_system *s : x.members) {
+        if (s->host_step >= s->step_end) continue;
+        running.push_back(s);
+        if (!s->grid_valid) {  // new system or new coordinates: rebuild before its first step
+            st = request_rebuild(s);
+            if (st != OG_OK) return st;
+            forced = true;
         }
-    if (!lead) return OG_OK;
-    long long step = lead->host_step;
-    for (og_system *s : x.members)
-        if (s->host_step < s->step_end && s->host_step % x.intvl != step % x.intvl) {
-            set_err("batch members are at different rebuild phases");
-            return OG_E_STATE;
-        }
+    }
+    if (running.empty()) return OG_OK;
     for (og_system *s : x.members) {
         s->pair_ms       = 0.0;
         s->pair_launches = 0;
     }
     int done = 0;
-    // unaligned head: direct launches
-    while (done < n_steps && (step % x.intvl) != 0) {
-        enqueue_forces(x, nullptr, nullptr);
-        enqueue_integrate(x);
-        done++;
-        step++;
-    }
     if (timing) {
-        // Pair-kernel timing: HIP events recorded on the run's stream around
-        // every pair launch.  Events recorded inside a captured graph cannot
-        // be timed, so a timed run launches its steps directly.
-        const int n = n_steps - done;
-        for (int k = (int)x.ev.size(); k < 2 * n; k++) {
+        for (int k = (int)x.ev.size(); k < 2 * n_steps; k++) {
             hipEvent_t e;
             OG_HIP(hipEventCreate(&e));
             x.ev.push_back(e);
         }
-        for (int k = 0; k < n; k++) {
-            if (step % x.intvl == 0) enqueue_rebuild(x);
-            enqueue_forces(x, x.ev[2 * k], x.ev[2 * k + 1]);
-            enqueue_integrate(x);
-            step++;
-        }
-        done = n_steps;
+        const std::string p = rebuild_pattern(x, n_steps, 0, forced);
+        for (int k = 0; k < n_steps; k++) enqueue_step(x, p[(size_t)k] == '1', x.ev[2 * k], x.ev[2 * k + 1]);
         OG_HIP(hipStreamSynchronize(x.stream));
         double ms = 0.0;
-        for (int k = 0; k < n; k++) {
+        for (int k = 0; k < n_steps; k++) {
             float t = 0.0f;
             OG_HIP(hipEventElapsedTime(&t, x.ev[2 * k], x.ev[2 * k + 1]));
             ms += t;
         }
         for (og_system *s : x.members) {
             s->pair_ms += ms;
-            s->pair_launches += n;
+            s->pair_launches += n_steps;
         }
+        done = n_steps;
     }
-    const int K = x.graph_steps > 0 ? std::max(x.intvl, x.graph_steps / x.intvl * x.intvl)
-                                    : std::max(x.intvl, std::min(100 / x.intvl, 10) * x.intvl);
-    while (n_steps - done >= K) {
+    const int kmax = x.graph_steps > 0 ? x.graph_steps : 100;
+    while (done < n_steps) {
+        const int      K = std::min(kmax, n_steps - done);
         hipGraphExec_t ge;
-        st = get_graph(x, K, &ge);
+        st = get_graph(x, rebuild_pattern(x, K, done, forced && done == 0), &ge);
         if (st != OG_OK) return st;
         OG_HIP(hipGraphLaunch(ge, x.stream));
         done += K;
-        step += K;
-    }
-    while (done < n_steps) {
-        if (step % x.intvl == 0) enqueue_rebuild(x);
-        enqueue_forces(x, nullptr, nullptr);
-        enqueue_integrate(x);
-        done++;
-        step++;
     }
     OG_HIP(hipStreamSynchronize(x.stream));
-    for (og_system *s : x.members) s->grid_valid = true;
+    for (og_system *s : running) s->grid_valid = true;  // the members that stood still keep their state
     return check_flags(x);
 }
 
@@ -679,19 +715,9 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
             lj12f[dst] = (float)d->lj12[src];
         }
     std::vector<int> comp = components(n, d->nb15off);
-    // MmSystem::revise_crd_inbox (MmSystem.cpp:808-816): wrap the input into [lb, ub)
     std::vector<double> crd0(d->crd, d->crd + 3 * (size_t)n);
-    for (int i = 0; i < n; i++)
-        for (int k = 0; k < 3; k++) {
-            double &x = crd0[3 * (size_t)i + k];
-            if (!std::isfinite(x)) {
-                set_err("non-finite input coordinate (atom %d)", i);
-                return OG_E_NAN;
-            }
-            if (x < (double)g.lb[k] || x >= (double)g.ub[k]) x -= std::floor((x - g.lb[k]) / g.L[k]) * g.L[k];
-            while (x < (double)g.lb[k]) x += g.L[k];
-            while (x >= (double)g.ub[k]) x -= g.L[k];
-        }
+    og_status           wst = wrap_inbox(g, n, crd0.data());
+    if (wst != OG_OK) return wst;
     OG_HIP(hipMemcpy(h.crd, crd0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.crd_prev, crd0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
     OG_HIP(hipMemcpy(h.crd_prev2, crd0.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
@@ -849,8 +875,13 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
     if (x.members.empty()) return OG_E_STATE;
     trials = std::max(trials, 1);
     bool need = false;
-    for (og_system *s : x.members) need |= !s->grid_valid;
-    if (need) enqueue_rebuild(x);  // a fresh system rebuilds on its first step anyway
+    for (og_system *s : x.members)
+        if (!s->grid_valid) {  // a fresh system rebuilds on its first step anyway
+            st = request_rebuild(s);
+            if (st != OG_OK) return st;
+            need = true;
+        }
+    if (need) enqueue_rebuild(x);
     struct Events {  // destroyed on every return path
         hipEvent_t a = nullptr, b = nullptr;
         ~Events() {
@@ -881,6 +912,11 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
         OG_HIP(hipMemsetAsync(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE, x.stream));
     }
     OG_HIP(hipStreamSynchronize(x.stream));
+    if (need)
+        for (og_system *s : x.members) {
+            st = request_rebuild(s, 0);
+            if (st != OG_OK) return st;
+        }
     for (og_system *s : x.members) s->grid_valid = true;
     if (best_shp != x.pair_shape) {
         free_graphs(x);
@@ -1140,8 +1176,12 @@ og_status og_nsgrid_update(og_system *s) {
     if (!s) return OG_E_ARG;
     og_status st = ensure_ctx(s);
     if (st != OG_OK) return st;
+    st = request_rebuild(s);
+    if (st != OG_OK) return st;
     enqueue_rebuild(*s->ctx);
     OG_HIP(hipStreamSynchronize(s->ctx->stream));
+    st = request_rebuild(s, 0);
+    if (st != OG_OK) return st;
     s->grid_valid = true;
     return check_flags(*s->ctx);
 }
@@ -1154,7 +1194,11 @@ og_status og_calc_energy_force(og_system *s, int rebuild, double *work_atom, dou
     // a single-point evaluation always carries its energies
     const int one = 1, zero = 0;
     OG_HIP(hipMemcpy(&s->d_cnt->force_energy, &one, sizeof(int), hipMemcpyHostToDevice));
-    if (rebuild || !s->grid_valid) enqueue_rebuild(x);
+    if (rebuild || !s->grid_valid) {
+        st = request_rebuild(s);  // cleared by the energy reduction
+        if (st != OG_OK) return st;
+        enqueue_rebuild(x);
+    }
     enqueue_forces(x, nullptr, nullptr, true);
     OG_HIP(hipStreamSynchronize(x.stream));
     OG_HIP(hipMemcpy(&s->d_cnt->force_energy, &zero, sizeof(int), hipMemcpyHostToDevice));
@@ -1227,7 +1271,12 @@ og_status og_set_state(og_system *s, const double *crd, const double *vel_next) 
     if (!s) return OG_E_ARG;
     const size_t n = 3 * (size_t)s->h.n_atoms;
     if (crd) {
-        OG_HIP(hipMemcpy(s->h.crd, crd, sizeof(double) * n, hipMemcpyHostToDevice));
+        // into the box as at creation; the grid is rebuilt before the next step
+        std::vector<double> c(crd, crd + n);
+        og_status           st = wrap_inbox(s->geom, s->h.n_atoms, c.data());
+        if (st != OG_OK) return st;
+        if (s->ctx && s->ctx->stream) OG_HIP(hipStreamSynchronize(s->ctx->stream));
+        OG_HIP(hipMemcpy(s->h.crd, c.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         s->grid_valid = false;
     }
     if (vel_next) OG_HIP(hipMemcpy(s->h.vel_next, vel_next, sizeof(double) * n, hipMemcpyHostToDevice));
@@ -1265,6 +1314,12 @@ og_status og_get_step(og_system *s, long long *step) {
     og_dev_counters c;
     OG_HIP(hipMemcpy(&c, s->d_cnt, sizeof c, hipMemcpyDeviceToHost));
     *step = c.step;
+    return OG_OK;
+}
+
+og_status og_system_status(og_system *s, int *status) {
+    if (!s || !status) return OG_E_ARG;
+    *status = (int)s->status;
     return OG_OK;
 }
 

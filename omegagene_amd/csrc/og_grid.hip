@@ -14,7 +14,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_clear(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         t  = bm.y * OG_THREADS + threadIdx.x;
     if (t < s.n_columns) {
         s.n_atoms_col[t] = 0;
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_bin(const og_dev_sys *__res
                                                          const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     const bool        in = i < s.n_atoms;
     int  c[2] = {0, 0};
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_bin(const og_dev_sys *__res
 // exclusive scan to head cell/atom, -1 heads for empty columns.
 __global__ void __launch_bounds__(1024) k_grid_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s  = S[map[blockIdx.x].x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         nc = s.n_columns;
     const int         T  = blockDim.x;
     const int         per = (nc + T - 1) / T;
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_scatter(const og_dev_sys *_
                                                              const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         i  = bm.y * OG_THREADS + threadIdx.x;
     if (s.cnt->n_cells == 0) return;
     const bool in   = i < s.n_atoms;
@@ -179,7 +179,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_grid_sort(const og_dev_sys *__re
                                                           const int2 *__restrict__ map) {
     const int2        bm  = map[blockIdx.x];
     const og_dev_sys &s   = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         col = bm.y;
     __shared__ uint64_t keys[OG_SORT_CAP];
     if (s.cnt->n_cells == 0) return;
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_cells_from_grid(const og_dev_sys
                                                                 const int2 *__restrict__ map) {
     const int2        bm  = map[blockIdx.x];
     const og_dev_sys &s   = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         col = bm.y;
     if (s.head_cell[col] < 0) return;
     og_column_cells(s, col, s.n_cells_z[col]);
@@ -461,7 +461,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_count(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells) return;
     og_enum_cell<1>(s, c1, &s.cp_count[c1]);
@@ -471,7 +471,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_enum_write(const og_dev_sys *__r
                                                            const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         c1 = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + (threadIdx.x >> 6));
     if (c1 >= s.cnt->n_cells) return;
     const int n = s.cp_count[c1];
@@ -495,7 +495,7 @@ __global__ void __launch_bounds__(1024) k_enum_scan_part(const og_dev_sys *__res
                                                          const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int n = s.cnt->n_cells;
     if (bm.y * 1024 > n) return;
     const int k    = bm.y * 1024 + threadIdx.x;
@@ -519,7 +519,7 @@ __global__ void __launch_bounds__(1024) k_enum_scan_add(const og_dev_sys *__rest
                                                         const int2 *__restrict__ map) {
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int n = s.cnt->n_cells;
     if (bm.y * 1024 > n) return;
     const int k    = bm.y * 1024 + threadIdx.x;
@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(1024) k_enum_scan_add(const og_dev_sys *__rest
 
 __global__ void __launch_bounds__(1024) k_enum_scan(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s   = S[map[blockIdx.x].x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int         n   = s.cnt->n_cells;
     const int         T   = blockDim.x;
     const int         per = (n + T - 1) / T;
@@ -596,7 +596,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_masks(const og_dev_sys *__restri
     __shared__ int exq[OG_THREADS / 64][OG_NAC * OG_MAX_NB15];  // (cell2 << 6) | bit, per wave
     const int2        bm = map[blockIdx.x];
     const og_dev_sys &s  = S[bm.x];
-    if (og_frozen(s)) return;
+    if (og_rebuild_skip(s)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int c1   = __builtin_amdgcn_readfirstlane(bm.y * (OG_THREADS / 64) + w);
     if (c1 >= s.cnt->n_cells || s.cnt->n_cell_pairs == 0) return;

@@ -741,7 +741,8 @@ __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys
             double *e = s.elog + (size_t)((stp / s.log_stride) % s.log_cap) * OG_E_NTERMS;
             for (int k = 0; k < OG_E_NTERMS; k++) e[k] = s.energy[k];
         }
-        s.cnt->step = stp + 1;
+        s.cnt->rebuild_now = 0;
+        s.cnt->step        = stp + 1;
     }
 }
 

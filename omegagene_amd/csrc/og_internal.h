@@ -51,6 +51,8 @@ enum {
     OG_F_SHAKE         = 16,
     OG_F_OUTSIDE_BOX   = 32,
 };
+// flags that stop the system (every kernel skips it) until the host has reported and cleared them
+#define OG_F_FATAL (OG_F_CP_OVERFLOW | OG_F_COL_OVERFLOW | OG_F_CELL_OVERFLOW | OG_F_NAN)
 
 struct og_dev_counters {
     int       n_cells;
@@ -62,6 +64,7 @@ struct og_dev_counters {
     int       force_energy;  // 1: evaluate energies on the next force call whatever the step
     long long step_end;  // the system stands still once step reaches this (segment end)
     long long noise_step;  // the step whose Langevin update takes its normals from og_dev_sys::noise (-1: none)
+    int       rebuild_now; // 1: rebuild the grid before the next step whatever its phase (cleared by the step)
 };
 
 struct og_dev_sys {
@@ -197,10 +200,20 @@ enum og_map_kind {
 #define OG_CP_BLOCKS_PER_SYS_MIN 64
 
 // ---- small device helpers ----
-// A system whose segment is complete skips every kernel of the step (each
-// kernel reads the counters before its first write; step only changes in
-// the last kernel of a step, so all kernels of one step agree).
-__device__ __forceinline__ bool og_frozen(const og_dev_sys &s) { return s.cnt->step >= s.cnt->step_end; }
+// A system whose segment is complete, or whose list overflowed or whose
+// forces went non-finite, skips every kernel (each kernel reads the counters
+// before its first write; step only changes in the last kernel of a step, so
+// all kernels of one step agree); a fault never spreads to the other systems
+// of a batch and never integrates on without forces.
+__device__ __forceinline__ bool og_frozen(const og_dev_sys &s) {
+    return s.cnt->step >= s.cnt->step_end || (s.cnt->flags & OG_F_FATAL) != 0;
+}
+// The grid-rebuild kernels run for a system when its own step is a multiple
+// of its rebuild interval (DynamicsMode.cpp:480-486) or a rebuild is forced;
+// systems at other phases in the same launch skip them.
+__device__ __forceinline__ bool og_rebuild_skip(const og_dev_sys &s) {
+    return og_frozen(s) || (s.rebuild_intvl > 0 && (s.cnt->step % s.rebuild_intvl) != 0 && !s.cnt->rebuild_now);
+}
 // Energies (pair, virial) are evaluated on logged steps and on request; the
 // forces and the pair count on every step.
 __device__ __forceinline__ bool og_energy_step(const og_dev_sys &s) {

@@ -89,6 +89,7 @@ struct Walker {
     uint64_t                  bytes = 0;
     int                       n_atoms = 0;
     bool                      busy = false;  // a segment is queued or running
+    bool                      failed = false;  // its last segment faulted; the walker left the device
 };
 
 struct Request {
@@ -204,6 +205,7 @@ og_status og_server_submit_next(og_server *srv, long long walker_id, int n_steps
     if (it == srv->walkers.end()) return fail(OG_E_ARG, "og_server_submit_next: unknown walker");
     Walker &w = it->second;
     if (w.busy) return fail(OG_E_STATE, "og_server_submit_next: the walker has a segment in flight");
+    if (w.failed) return fail(OG_E_STATE, "og_server_submit_next: the walker's last segment faulted");
     if (w.sys && n_steps > w.prm.log_cap)
         return fail(OG_E_ARG, "og_server_submit_next: segment longer than the walker's energy log");
     w.busy = true;
@@ -253,8 +255,9 @@ og_status og_server_pump(og_server *srv, int *n_active, int *n_queued, int *n_do
     int done = 0;
     if (!srv->active.empty()) {
         og_status st = og_batch_run(srv->batch, srv->chunk);
-        if (st != OG_OK) {
-            // a device error or overflow in the shared batch fails every active segment
+        if (st != OG_OK && st != OG_E_CAPACITY && st != OG_E_NAN && st != OG_E_NOCONV) {
+            // a runtime error (or a batch that could not run) leaves no per-walker outcome: every active
+            // segment fails
             for (long long rid : srv->active) {
                 Request &r = srv->requests.at(rid);
                 r.status   = st;
@@ -265,10 +268,28 @@ og_status og_server_pump(og_server *srv, int *n_active, int *n_queued, int *n_do
             srv->active.clear();
             return st;
         }
+        // a walker whose list overflowed or whose forces went non-finite stood still on the device: its
+        // segment fails and it leaves the batch; the other walkers went on and are unaffected
         std::vector<long long> still;
         for (long long rid : srv->active) {
             Request &r = srv->requests.at(rid);
             Walker  &w = srv->walkers.at(r.walker);
+            int       ws = OG_OK;
+            st           = og_system_status(w.sys, &ws);
+            if (st != OG_OK) return st;
+            if (ws != OG_OK) {
+                r.status = (og_status)ws;
+                r.done   = true;
+                r.t_done = now_s() - srv->t0;
+                w.busy   = false;
+                w.failed = true;
+                st       = og_batch_remove(srv->batch, w.sys);
+                if (st != OG_OK) return st;
+                og_system_destroy(w.sys);
+                w.sys = nullptr;
+                srv->used -= w.bytes;
+                continue;
+            }
             long long step = 0;
             st             = og_get_step(w.sys, &step);
             if (st != OG_OK) return st;

@@ -291,6 +291,12 @@ class System:
     def set_pair_timing(self, on=True):
         _check(lib().og_set_pair_timing(self.h, C.c_int(int(on))))
 
+    def status(self):
+        """og_status of the system's last run or evaluation (0 = ok)."""
+        st = C.c_int()
+        _check(lib().og_system_status(self.h, C.byref(st)))
+        return st.value
+
     def set_langevin_noise(self, zeta):
         """N(0,1) draws [n][3] for the next step's Langevin update (None cancels)."""
         if zeta is None:
@@ -361,6 +367,10 @@ class Batch:
     def set_pair_shape(self, waves, order=0):
         """order: 0 system order, 1 round-robin across systems, 2 each system on one XCD."""
         _check(lib().og_batch_set_pair_shape(self.h, C.c_int(waves), C.c_int(int(order))))
+
+    def set_graph_steps(self, steps: int):
+        """Steps per captured hipGraph (0: automatic)."""
+        _check(lib().og_batch_set_graph_steps(self.h, C.c_int(steps)))
 
     def run(self, n_steps: int):
         _check(lib().og_batch_run(self.h, C.c_int(n_steps)))
