@@ -15,9 +15,9 @@
  * gradient ("work") in grid order plus energies {vdw, ele}.
  *
  * What differs, all invisible to the caller's data:
- *  - return values are og_status codes (0 = OK, as the reference's "return 0")
- *    instead of exit(EXIT_FAILURE) on error (cuda_common.h:7-22); the message
- *    is available through og_last_error (og_engine.h);
+ *  - the int return values are og_status codes (0 = OK, as the reference's
+ *    "return 0") instead of exit(EXIT_FAILURE) on error (cuda_common.h:7-22);
+ *    the message is available through og_last_error (og_engine.h);
  *  - forces accumulate in 64-bit fixed point, so the gradient is
  *    deterministic (the reference adds doubles with CAS atomics in
  *    nondeterministic order, cuda_setup.cu:8-16,642-674);
@@ -30,12 +30,16 @@
  * state and is not re-entrant: call it from one host thread.  The handle ABI
  * in og_engine.h is the re-entrant, fully device-resident path.
  * Types: real_pw = float, real_fc = double (CelesteObject.h:11-24);
- * the reference's T*& parameters are T** here.
+ * the reference's T*& parameters are T** here, and otherwise every prototype
+ * is the one its caller declares (tests/abi_check.cpp checks them).
  */
 #ifndef OG_CELESTE_H
 #define OG_CELESTE_H
 
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #include "og_engine.h"
 
@@ -47,76 +51,78 @@ extern "C" {
  * buffers crd [max_n_atom_array*3] float, atomids [max_n_atom_array],
  * work [max_n_atom_array*3] double, energy [2] double.  Must run first: it
  * fixes max_n_atom_array for og_alloc_atom_info. */
-og_status og_hostalloc_atom_info(float **h_crd, int **h_atomids, double **h_work, double **h_energy,
+int og_hostalloc_atom_info(float **h_crd, int **h_atomids, double **h_work, double **h_energy,
                                  int max_n_atom_array);
-/* cuda_hostfree_atom_info cuda_setup.cu:316-324 */
-og_status og_hostfree_atom_info(float *h_crd, int *h_atomids, double *h_work, double *h_energy);
+/* cuda_hostfree_atom_info cuda_setup.cu:316-324, declared by its caller as
+ * (real_pw*, int*, real_fc*&, real_fc*&) (MiniCell.cpp:19): work and energy
+ * by reference, freed and set to null */
+int og_hostfree_atom_info(float *h_crd, int *h_atomids, double **h_work, double **h_energy);
 /* cuda_hostalloc_cell_info cuda_setup.cu:282-287: idx_xy_head_cell [n_columns] */
-og_status og_hostalloc_cell_info(int **h_idx_xy_head_cell, int n_columns);
+int og_hostalloc_cell_info(int **h_idx_xy_head_cell, int n_columns);
 /* cuda_hostfree_cell_info cuda_setup.cu:325-328 */
-og_status og_hostfree_cell_info(int *h_idx_xy_head_cell);
+int og_hostfree_cell_info(int *h_idx_xy_head_cell);
 /* cuda_hostalloc_atom_type_charge cuda_setup.cu:257-264: atom order, n_atoms_system */
-og_status og_hostalloc_atom_type_charge(int **h_atom_type, float **h_charge, int n_atoms_system);
+int og_hostalloc_atom_type_charge(int **h_atom_type, float **h_charge, int n_atoms_system);
 /* cuda_hostfree_atom_type_charge cuda_setup.cu:310-314 */
-og_status og_hostfree_atom_type_charge(int *h_atom_type, float *h_charge);
+int og_hostfree_atom_type_charge(int *h_atom_type, float *h_charge);
 
 /* cuda_alloc_atom_info cuda_setup.cu:42-77 (SubBox.cpp:1688-1690 passes
  * n_columns + 1 as the last argument) */
-og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_cell_pairs, int n_columns);
+int og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_cell_pairs, int n_columns);
 /* cuda_free_atom_info cuda_setup.cu:79-102 */
-og_status og_free_atom_info(void);
-/* cuda_alloc_set_lj_params cuda_setup.cu:205-244: float LJ tables
- * [n_lj_types^2] and the atom-order exclusion table
- * h_nb15off [max_n_atoms_exbox * max_n_nb15off] (-1 padded, MmSystem.cpp:478-525).
- * SubBox.cpp:1693-1694 passes two more ints; C linkage ignores them there
- * and they are accepted and ignored here. */
-og_status og_alloc_set_lj_params(const float *h_lj_6term, const float *h_lj_12term, int n_lj_types,
-                                 const int *h_nb15off, int max_n_nb15off);
+int og_free_atom_info(void);
+/* cuda_alloc_set_lj_params cuda_setup.cu:205-244 as its caller declares it
+ * (SubBox.cpp:27-33, seven arguments; the definition reads five): float LJ
+ * tables [n_lj_types^2] and the atom-order exclusion table
+ * h_nb15off [max_n_atoms_exbox * max_n_nb15off] (-1 padded, MmSystem.cpp:478-525);
+ * max_n_atoms and max_n_atom_array are accepted and not needed. */
+int og_alloc_set_lj_params(float *h_lj_6term, float *h_lj_12term, int n_lj_types, int *h_nb15off,
+                           const int max_n_nb15off, const int max_n_atoms, const int max_n_atom_array);
 /* cuda_free_lj_params cuda_setup.cu:246-255 */
-og_status og_free_lj_params(void);
+int og_free_lj_params(void);
 /* cuda_set_pbc cuda_setup.cu:138-146: L[3], lower_bound[3] (float) */
-og_status og_set_pbc(const float *l, const float *lb);
+int og_set_pbc(float *l, float *lb);
 /* cuda_set_constant cuda_setup.cu:193-203 */
-og_status og_set_constant(float cutoff, float cutoff_pairlist, int n_atomtypes);
+int og_set_constant(float cutoff, float cutoff_pairlist, int n_atomtypes);
 /* cuda_zerodipole_constant cuda_setup.cu:148-153 */
-og_status og_zerodipole_constant(float zcore, float bcoeff, float fcoeff);
+int og_zerodipole_constant(float zcore, float bcoeff, float fcoeff);
 /* cuda_memcpy_htod_atom_info cuda_setup.cu:115-119: atom-order charges and
  * (0-based) LJ types of the n_atoms_system atoms */
-og_status og_memcpy_htod_atom_info(float **h_charge_orig, int **h_atomtype_orig);
+int og_memcpy_htod_atom_info(float **h_charge_orig, int **h_atomtype_orig);
 /* cuda_set_cell_constant cuda_setup.cu:157-189 (per grid rebuild) */
-og_status og_set_cell_constant(int n_cells, int n_atoms_exbox, int n_atom_array, const int *n_cells_xyz,
+int og_set_cell_constant(int n_cells, int n_atoms_exbox, int n_atom_array, const int *n_cells_xyz,
                                int n_columns, const float *l_cell_xyz, const int *n_neighbor_xyz);
 /* cuda_memcpy_htod_atomids cuda_setup.cu:104-110: atomids [n_atom_array]
  * (grid slot -> atom, -1 dummy) and idx_xy_head_cell [n_columns + 1]
  * (-1 for an empty column, MiniCell.cpp:807-810) */
-og_status og_memcpy_htod_atomids(int **h_atomids, int **h_idx_xy_head_cell);
+int og_memcpy_htod_atomids(int **h_atomids, int **h_idx_xy_head_cell);
 /* cuda_set_atominfo cuda_setup.cu:404-425: types, charges and the reverse
  * map into grid order */
-og_status og_set_atominfo(void);
+int og_set_atominfo(void);
 /* cuda_memcpy_htod_crd cuda_setup.cu:124-136: float grid-order xyz [n_atom_array*3] */
-og_status og_memcpy_htod_crd(float **h_crd);
+int og_memcpy_htod_crd(float **h_crd);
 /* cuda_set_crd cuda_setup.cu:427-434: pack xyz + charge into the pair kernel's stream */
-og_status og_set_crd(void);
+int og_set_crd(void);
 /* cuda_enumerate_cell_pairs cuda_setup.cu:1095-1216: cell pairs and their
  * 8x8 exclusion masks from the current grid (idx_atom_cell_xy: column of
  * each grid slot, MiniCell.cpp:351; n_neighbor_col is checked against the
  * cell constants) */
-og_status og_enumerate_cell_pairs(int **h_atomids, int n_cells, int n_neighbor_col, const int *idx_atom_cell_xy);
+int og_enumerate_cell_pairs(int **h_atomids, int n_cells, int n_neighbor_col, const int *idx_atom_cell_xy);
 /* cuda_pairwise_ljzd cuda_setup.cu:777-792: asynchronous; the flag is
  * ignored as in the reference */
-og_status og_pairwise_ljzd(int flg_mod_15mask);
+int og_pairwise_ljzd(const bool flg_mod_15mask);
 /* cuda_pair_sync cuda_setup.cu:797-801 */
-og_status og_pair_sync(void);
+int og_pair_sync(void);
 /* cuda_memcpy_dtoh_work cuda_setup.cu:803-823: work [n_atom_array*3] grid
  * order (dE/dx, kcal/mol/A) and energy [2] = {vdw, ele} */
-og_status og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, int n_atom_array);
+int og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, int n_atom_array);
 
 /* Inspection for parity tests (no reference counterpart): the cell-pair list
  * of the last og_enumerate_cell_pairs as CellPair records
  * {cell_id1, cell_id2, image, pair_mask[2]} (MiniCell.h:17-26) and
  * idx_head_cell_pairs [n_cells + 1]; n_cell_pairs may be queried with
  * cell_pairs5 == NULL. */
-og_status og_get_celeste_cell_pairs(int32_t *cell_pairs5, int *idx_head_cell_pairs, int *n_cell_pairs);
+int og_get_celeste_cell_pairs(int32_t *cell_pairs5, int *idx_head_cell_pairs, int *n_cell_pairs);
 
 #ifdef __cplusplus
 }

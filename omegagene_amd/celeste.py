@@ -66,8 +66,10 @@ class CelesteGpu:
         lj6 = np.ascontiguousarray(lj6, np.float32)
         lj12 = np.ascontiguousarray(lj12, np.float32)
         tab = np.ascontiguousarray(nb15off, np.int32)
+        # SubBox.cpp:1693-1694 passes the two sizes after max_n_nb15off as well
         _check(self.L.og_alloc_set_lj_params(_p(lj6, C.c_float), _p(lj12, C.c_float), C.c_int(lj6.shape[0]),
-                                             _p(tab, C.c_int), C.c_int(max_n_nb15off)))
+                                             _p(tab, C.c_int), C.c_int(max_n_nb15off), C.c_int(tab.shape[0]),
+                                             C.c_int(0)))
 
     def set_pbc(self, L, lb):
         _check(self.L.og_set_pbc(_p(np.ascontiguousarray(L, np.float32), C.c_float),
@@ -117,7 +119,7 @@ class CelesteGpu:
                                               _p(col, C.c_int)))
 
     def pairwise_ljzd(self, flg_mod_15mask=False):
-        _check(self.L.og_pairwise_ljzd(C.c_int(int(flg_mod_15mask))))
+        _check(self.L.og_pairwise_ljzd(C.c_bool(bool(flg_mod_15mask))))
 
     def pair_sync(self):
         _check(self.L.og_pair_sync())
@@ -143,7 +145,9 @@ class CelesteGpu:
         self.L.og_free_lj_params()
         self.L.og_free_atom_info()
         if self.h_crd:
-            _check(self.L.og_hostfree_atom_info(self.h_crd, self.h_atomids, self.h_work, self.h_energy))
+            # work and energy by reference (real_fc *&, MiniCell.cpp:19): freed and nulled
+            _check(self.L.og_hostfree_atom_info(self.h_crd, self.h_atomids, C.byref(self.h_work),
+                                                C.byref(self.h_energy)))
             self.h_crd = _F()
         if self.h_head:
             _check(self.L.og_hostfree_cell_info(self.h_head))

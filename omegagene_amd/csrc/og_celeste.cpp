@@ -173,7 +173,7 @@ std::vector<int> cs_components(int n, const int *tab) {
 
 extern "C" {
 
-og_status og_hostalloc_atom_info(float **h_crd, int **h_atomids, double **h_work, double **h_energy,
+int og_hostalloc_atom_info(float **h_crd, int **h_atomids, double **h_work, double **h_energy,
                                  int max_n_atom_array) {
     if (!h_crd || !h_atomids || !h_work || !h_energy || max_n_atom_array <= 0) return OG_E_ARG;
     g_cs.max_n_atom_array = max_n_atom_array;
@@ -184,27 +184,31 @@ og_status og_hostalloc_atom_info(float **h_crd, int **h_atomids, double **h_work
     return OG_OK;
 }
 
-og_status og_hostfree_atom_info(float *h_crd, int *h_atomids, double *h_work, double *h_energy) {
+int og_hostfree_atom_info(float *h_crd, int *h_atomids, double **h_work, double **h_energy) {
+    // work and energy come by reference (real_fc *&, MiniCell.cpp:19): freed and set to null
+    if (!h_work || !h_energy) return OG_E_ARG;
     CS_HIP(hipHostFree(h_crd));
     CS_HIP(hipHostFree(h_atomids));
-    CS_HIP(hipHostFree(h_work));
-    CS_HIP(hipHostFree(h_energy));
+    CS_HIP(hipHostFree(*h_work));
+    CS_HIP(hipHostFree(*h_energy));
+    *h_work   = nullptr;
+    *h_energy = nullptr;
     return OG_OK;
 }
 
-og_status og_hostalloc_cell_info(int **h_idx_xy_head_cell, int n_columns) {
+int og_hostalloc_cell_info(int **h_idx_xy_head_cell, int n_columns) {
     if (!h_idx_xy_head_cell || n_columns <= 0) return OG_E_ARG;
     // one extra entry: MiniCell writes idx_xy_head_cell[n_columns] (MiniCell.cpp:852)
     CS_HIP(hipHostMalloc((void **)h_idx_xy_head_cell, sizeof(int) * ((size_t)n_columns + 1), hipHostMallocDefault));
     return OG_OK;
 }
 
-og_status og_hostfree_cell_info(int *h_idx_xy_head_cell) {
+int og_hostfree_cell_info(int *h_idx_xy_head_cell) {
     CS_HIP(hipHostFree(h_idx_xy_head_cell));
     return OG_OK;
 }
 
-og_status og_hostalloc_atom_type_charge(int **h_atom_type, float **h_charge, int n_atoms_system) {
+int og_hostalloc_atom_type_charge(int **h_atom_type, float **h_charge, int n_atoms_system) {
     if (!h_atom_type || !h_charge || n_atoms_system <= 0) return OG_E_ARG;
     g_cs.n_atoms_system = n_atoms_system;
     CS_HIP(hipHostMalloc((void **)h_atom_type, sizeof(int) * (size_t)n_atoms_system, hipHostMallocDefault));
@@ -212,13 +216,13 @@ og_status og_hostalloc_atom_type_charge(int **h_atom_type, float **h_charge, int
     return OG_OK;
 }
 
-og_status og_hostfree_atom_type_charge(int *h_atom_type, float *h_charge) {
+int og_hostfree_atom_type_charge(int *h_atom_type, float *h_charge) {
     CS_HIP(hipHostFree(h_atom_type));
     CS_HIP(hipHostFree(h_charge));
     return OG_OK;
 }
 
-og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_cell_pairs, int n_columns) {
+int og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_cell_pairs, int n_columns) {
     CS_REQUIRE(g_cs.max_n_atom_array > 0, "og_alloc_atom_info: call og_hostalloc_atom_info first");
     if (max_n_atoms_exbox <= 0 || max_n_cells <= 0 || max_n_cell_pairs <= 0 || n_columns <= 0) return OG_E_ARG;
     CS_REQUIRE(!g_cs.have_alloc, "og_alloc_atom_info: already allocated (og_free_atom_info first)");
@@ -272,7 +276,7 @@ og_status og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_c
     return OG_OK;
 }
 
-og_status og_free_atom_info(void) {
+int og_free_atom_info(void) {
     if (g_cs.stream) hipStreamSynchronize(g_cs.stream);
     for (void *p : g_cs.allocs) hipFree(p);
     g_cs.allocs.clear();
@@ -287,8 +291,10 @@ og_status og_free_atom_info(void) {
     return OG_OK;
 }
 
-og_status og_alloc_set_lj_params(const float *h_lj_6term, const float *h_lj_12term, int n_lj_types,
-                                 const int *h_nb15off, int max_n_nb15off) {
+int og_alloc_set_lj_params(float *h_lj_6term, float *h_lj_12term, int n_lj_types, int *h_nb15off,
+                           const int max_n_nb15off, const int max_n_atoms, const int max_n_atom_array) {
+    (void)max_n_atoms;       // passed by SubBox.cpp:1693-1694; the sizes come from og_alloc_atom_info
+    (void)max_n_atom_array;
     CS_REQUIRE(g_cs.have_alloc, "og_alloc_set_lj_params: call og_alloc_atom_info first");
     if (!h_lj_6term || !h_lj_12term || n_lj_types <= 0 || !h_nb15off || max_n_nb15off < 0) return OG_E_ARG;
     if (max_n_nb15off > OG_MAX_NB15) {
@@ -330,7 +336,7 @@ og_status og_alloc_set_lj_params(const float *h_lj_6term, const float *h_lj_12te
     return OG_OK;
 }
 
-og_status og_free_lj_params(void) {
+int og_free_lj_params(void) {
     og_dev_sys &h = g_cs.h;
     for (void **p : {(void **)&h.lj6, (void **)&h.lj12, (void **)&h.lj_pair, (void **)&h.nb15off}) {
         if (*p) {
@@ -344,7 +350,7 @@ og_status og_free_lj_params(void) {
     return OG_OK;
 }
 
-og_status og_set_pbc(const float *l, const float *lb) {
+int og_set_pbc(float *l, float *lb) {
     if (!l || !lb) return OG_E_ARG;
     og_dev_sys &h = g_cs.h;
     for (int k = 0; k < 3; k++) {
@@ -356,7 +362,7 @@ og_status og_set_pbc(const float *l, const float *lb) {
     return OG_OK;
 }
 
-og_status og_set_constant(float cutoff, float cutoff_pairlist, int n_atomtypes) {
+int og_set_constant(float cutoff, float cutoff_pairlist, int n_atomtypes) {
     if (!(cutoff > 0.0f) || !(cutoff_pairlist > 0.0f) || n_atomtypes <= 0) return OG_E_ARG;
     og_dev_sys &h     = g_cs.h;
     g_cs.cutoff       = cutoff;
@@ -368,7 +374,7 @@ og_status og_set_constant(float cutoff, float cutoff_pairlist, int n_atomtypes) 
     return OG_OK;
 }
 
-og_status og_zerodipole_constant(float zcore, float bcoeff, float fcoeff) {
+int og_zerodipole_constant(float zcore, float bcoeff, float fcoeff) {
     og_dev_sys &h = g_cs.h;
     h.zcore       = zcore;
     h.bcoeff      = bcoeff;
@@ -379,7 +385,7 @@ og_status og_zerodipole_constant(float zcore, float bcoeff, float fcoeff) {
     return OG_OK;
 }
 
-og_status og_memcpy_htod_atom_info(float **h_charge_orig, int **h_atomtype_orig) {
+int og_memcpy_htod_atom_info(float **h_charge_orig, int **h_atomtype_orig) {
     CS_REQUIRE(g_cs.have_alloc && g_cs.have_lj,
                "og_memcpy_htod_atom_info: call og_alloc_atom_info and og_alloc_set_lj_params first");
     if (!h_charge_orig || !h_atomtype_orig || !*h_charge_orig || !*h_atomtype_orig) return OG_E_ARG;
@@ -396,7 +402,7 @@ og_status og_memcpy_htod_atom_info(float **h_charge_orig, int **h_atomtype_orig)
     return OG_OK;
 }
 
-og_status og_set_cell_constant(int n_cells, int n_atoms_exbox, int n_atom_array, const int *n_cells_xyz,
+int og_set_cell_constant(int n_cells, int n_atoms_exbox, int n_atom_array, const int *n_cells_xyz,
                                int n_columns, const float *l_cell_xyz, const int *n_neighbor_xyz) {
     CS_REQUIRE(g_cs.have_alloc && g_cs.have_pbc && g_cs.have_const,
                "og_set_cell_constant: call og_alloc_atom_info, og_set_pbc and og_set_constant first");
@@ -451,7 +457,7 @@ og_status og_set_cell_constant(int n_cells, int n_atoms_exbox, int n_atom_array,
     return OG_OK;
 }
 
-og_status og_memcpy_htod_atomids(int **h_atomids, int **h_idx_xy_head_cell) {
+int og_memcpy_htod_atomids(int **h_atomids, int **h_idx_xy_head_cell) {
     CS_REQUIRE(g_cs.have_cellconst, "og_memcpy_htod_atomids: call og_set_cell_constant first");
     if (!h_atomids || !h_idx_xy_head_cell || !*h_atomids || !*h_idx_xy_head_cell) return OG_E_ARG;
     og_dev_sys &h  = g_cs.h;
@@ -486,7 +492,7 @@ og_status og_memcpy_htod_atomids(int **h_atomids, int **h_idx_xy_head_cell) {
     return OG_OK;
 }
 
-og_status og_set_atominfo(void) {
+int og_set_atominfo(void) {
     CS_REQUIRE(g_cs.have_ids && g_cs.have_info, "og_set_atominfo: call og_memcpy_htod_atom_info and og_memcpy_htod_atomids first");
     CS_TRY(cs_push_sys());
     const int cap = g_cs.h.cap_atom_array;
@@ -497,14 +503,14 @@ og_status og_set_atominfo(void) {
     return OG_OK;
 }
 
-og_status og_memcpy_htod_crd(float **h_crd) {
+int og_memcpy_htod_crd(float **h_crd) {
     CS_REQUIRE(g_cs.have_cellconst, "og_memcpy_htod_crd: call og_set_cell_constant first");
     if (!h_crd || !*h_crd) return OG_E_ARG;
     CS_HIP(hipMemcpy(g_cs.d_crd_in, *h_crd, sizeof(float) * 3 * g_cs.n_atom_array, hipMemcpyHostToDevice));
     return OG_OK;
 }
 
-og_status og_set_crd(void) {
+int og_set_crd(void) {
     CS_REQUIRE(g_cs.have_cellconst, "og_set_crd: call og_set_cell_constant first");
     CS_TRY(cs_push_sys());
     const int na = g_cs.n_atom_array;
@@ -515,7 +521,7 @@ og_status og_set_crd(void) {
     return OG_OK;
 }
 
-og_status og_enumerate_cell_pairs(int **h_atomids, int n_cells, int n_neighbor_col, const int *idx_atom_cell_xy) {
+int og_enumerate_cell_pairs(int **h_atomids, int n_cells, int n_neighbor_col, const int *idx_atom_cell_xy) {
     (void)h_atomids;
     (void)idx_atom_cell_xy;  // the column of each cell comes from the uploaded column heads
     CS_REQUIRE(g_cs.have_ids && g_cs.have_lj, "og_enumerate_cell_pairs: grid or exclusions not uploaded");
@@ -546,7 +552,7 @@ og_status og_enumerate_cell_pairs(int **h_atomids, int n_cells, int n_neighbor_c
     return OG_OK;
 }
 
-og_status og_pairwise_ljzd(int flg_mod_15mask) {
+int og_pairwise_ljzd(bool flg_mod_15mask) {
     (void)flg_mod_15mask;
     CS_REQUIRE(g_cs.have_pairs && g_cs.have_zd, "og_pairwise_ljzd: no cell-pair list (og_enumerate_cell_pairs)");
     og_dev_sys &h = g_cs.h;
@@ -558,12 +564,12 @@ og_status og_pairwise_ljzd(int flg_mod_15mask) {
     return OG_OK;
 }
 
-og_status og_pair_sync(void) {
+int og_pair_sync(void) {
     if (g_cs.stream) CS_HIP(hipStreamSynchronize(g_cs.stream));
     return OG_OK;
 }
 
-og_status og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, int n_atom_array) {
+int og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, int n_atom_array) {
     (void)n_atoms;
     CS_REQUIRE(g_cs.have_pairs, "og_memcpy_dtoh_work: no pair evaluation");
     if (!h_work || !h_energy || !*h_work || !*h_energy || n_atom_array != g_cs.n_atom_array) return OG_E_ARG;
@@ -586,7 +592,7 @@ og_status og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, i
     return OG_OK;
 }
 
-og_status og_get_celeste_cell_pairs(int32_t *cp5, int *idx_head, int *n_cell_pairs) {
+int og_get_celeste_cell_pairs(int32_t *cp5, int *idx_head, int *n_cell_pairs) {
     CS_REQUIRE(g_cs.have_pairs, "og_get_celeste_cell_pairs: no cell-pair list");
     og_dev_counters c;
     CS_HIP(hipMemcpy(&c, g_cs.d_cnt, sizeof c, hipMemcpyDeviceToHost));

@@ -404,7 +404,7 @@ def exports(header: str = None):
     names = set()
     for p in paths:
         hdr = open(p).read()
-        names |= set(re.findall(r"^\s*(?:og_status|void|const char \*)\s*(og_\w+)\s*\(", hdr, re.M))
+        names |= set(re.findall(r"^\s*(?:og_status|int|void|const char \*)\s*(og_\w+)\s*\(", hdr, re.M))
     return sorted(names)
 
 

@@ -100,3 +100,15 @@ def test_call_point_header_mirrors_reference_abi():
     assert {"og_set_device", "og_thread_sync"} <= set(engine.exports("og_engine.h"))
     lib = ctypes.CDLL(engine.LIB_PATH)
     assert all(hasattr(lib, "og_" + p) for p in points)
+
+
+def test_call_point_prototypes_match_the_reference_declarations():
+    """tests/abi_check.cpp static_asserts each reference call-point prototype
+    (SubBox.cpp:5-78, MiniCell.cpp:7-35) with the og_ prefix and T*& as T**
+    against include/og_celeste.h; it must compile."""
+    import subprocess
+    here = os.path.dirname(os.path.abspath(__file__))
+    inc = os.path.join(os.path.dirname(here), "include")
+    r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I" + inc, os.path.join(here, "abi_check.cpp")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
