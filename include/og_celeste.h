@@ -124,6 +124,39 @@ int og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, int n_a
  * cell_pairs5 == NULL. */
 int og_get_celeste_cell_pairs(int32_t *cell_pairs5, int *idx_head_cell_pairs, int *n_cell_pairs);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): og_pairwise_ljzd followed by og_memcpy_dtoh_work, with
+ * the same arithmetic and bit-identical results, on buffers the caller owns
+ * on the device.  Asynchronous on `stream` (a hipStream_t; NULL = the null
+ * stream): four kernel launches, no allocation, memset or copy.
+ *   d_xyzq      [n_cells*8*4] grid-order float x, y, z, charge (charge 0 in
+ *               empty slots)
+ *   d_atom_type [n_cells*8] LJ type of each grid slot (0 in empty slots)
+ *   d_lj_pair   [n_lj_types^2 * 2] {6-term, 12-term} per type pair, row-major
+ *   d_idx_head_cell_pairs [n_cells+1], d_cell_pair_c2 [n_cell_pairs]
+ *               (cell_id2 | image << 24), d_cell_pair_mask [n_cell_pairs]
+ *               (pair_mask[0] | pair_mask[1] << 32): the CellPair list of
+ *               og_enumerate_cell_pairs (og_get_celeste_cell_pairs) split by field
+ *   l [3]       box lengths (host); cutoff, zcore/bcoeff/fcoeff as
+ *               og_set_constant and og_zerodipole_constant (alpha = 0)
+ *   d_work      [n_cells*8*3] out: FP64 gradient in grid order
+ *   d_energy    [2] out: vdw, ele
+ *   d_flags     [1] out (may be NULL): non-zero when a gradient was
+ *               non-finite or out of the fixed-point range (og_memcpy_dtoh_work
+ *               returns OG_E_NAN then)
+ * Its working memory is a fixed per-GPU device array held until the process
+ * ends; grids above OG_NOALLOC_MAX_ATOM_ARRAY slots (n_cells*8) are rejected
+ * with OG_E_CAPACITY.  Calls that overlap on one GPU share that array and
+ * race: serialize them (one stream, or events). */
+#define OG_NOALLOC_MAX_ATOM_ARRAY 262144
+#define OG_NOALLOC_MAX_DEVICES 64
+int og_pairwise_ljzd_noalloc(const float *d_xyzq, const int *d_atom_type, const float *d_lj_pair, int n_lj_types,
+                             const int *d_idx_head_cell_pairs, const uint32_t *d_cell_pair_c2,
+                             const uint64_t *d_cell_pair_mask, int n_cells, const float *l, float cutoff,
+                             float zcore, float bcoeff, float fcoeff, double *d_work, double *d_energy, int *d_flags,
+                             void *stream);
+
 #ifdef __cplusplus
 }
 #endif

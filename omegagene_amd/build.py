@@ -30,6 +30,7 @@ SOURCES = [
     ("og_engine.cpp", ["-x", "hip"]),
     ("og_celeste.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ("og_server.cpp", ["-x", "hip"]),
+    ("og_noalloc.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -206,3 +206,20 @@ class SubBoxDriver:
 
     def close(self):
         self.gpu.free()
+
+
+OG_NOALLOC_MAX_ATOM_ARRAY = 262144  # include/og_celeste.h
+
+
+def pairwise_ljzd_noalloc(xyzq, atom_type, lj_pair, n_lj_types, idx_head_cell_pairs, cell_pair_c2, cell_pair_mask,
+                          n_cells, L, cutoff, zd, work, energy, flags=0, stream=0):
+    """og_pairwise_ljzd_noalloc: the pair evaluation of pairwise_ljzd +
+    memcpy_dtoh_work on caller-owned device buffers, given as device
+    addresses (ints), asynchronous on ``stream`` (a hipStream_t as int).
+    No counterpart in the reference; it allocates nothing on the device."""
+    lbox = np.ascontiguousarray(L, np.float32)
+    vp = C.c_void_p
+    _check(lib().og_pairwise_ljzd_noalloc(
+        vp(xyzq), vp(atom_type), vp(lj_pair), C.c_int(n_lj_types), vp(idx_head_cell_pairs), vp(cell_pair_c2),
+        vp(cell_pair_mask), C.c_int(n_cells), _p(lbox, C.c_float), C.c_float(cutoff), C.c_float(zd.zcore),
+        C.c_float(zd.bcoeff), C.c_float(zd.fcoeff), vp(work), vp(energy), vp(flags or None), vp(stream or None)))
