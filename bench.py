@@ -36,10 +36,19 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(ls, cfg, budget_s=12.0):
-    """Single-thread oracle force evaluations (grid build + cell-pair list +
-    FP32 LJ/ZD pair loop + excluded pairs) of the same system until
-    budget_s has elapsed; returns pair interactions/s."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_worker(ls, cfg, budget_s):
+    """Oracle force evaluations (grid build + cell-pair list + FP32 LJ/ZD pair
+    loop + excluded pairs) of one system until budget_s has elapsed."""
     import numpy as np
 
     import oracle
@@ -62,8 +71,28 @@ def cpu_baseline(ls, cfg, budget_s=12.0):
         pairs += npair
         if time.perf_counter() - t0 >= budget_s:
             break
-    dt = time.perf_counter() - t0
-    return pairs / dt, n_eval, dt
+    return pairs, n_eval, time.perf_counter() - t0
+
+
+def cpu_baseline(ls, cfg, budget_s=10.0, max_threads=16):
+    """The oracle's cell-list force evaluation of the same system on the
+    host: one thread for budget_s, then one independent evaluation loop per
+    thread on every core this process may use (at most max_threads; the C
+    oracle releases the GIL) for budget_s.  Returns the dict of both."""
+    from concurrent.futures import ThreadPoolExecutor
+    pairs, n_eval, dt = _cpu_worker(ls, cfg, budget_s)
+    single = pairs / dt
+    n_thr = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(n_thr) as ex:
+        res = list(ex.map(lambda _: _cpu_worker(ls, cfg, budget_s), range(n_thr)))
+    multi = sum(p / t for p, _, t in res)
+    n_multi = sum(n for _, n, _ in res)
+    return {"value": multi, "unit": "pair interactions/s", "cores": n_thr, "kind": "port",
+            "sample": "oracle single-point force evaluations (grid + cell-pair list + FP32 LJ/ZD pairs + excluded "
+                      "pairs) of the same 99,200-atom system: %d threads x %.0f s (%d evaluations in total)"
+                      % (n_thr, budget_s, n_multi),
+            "single_thread": {"value": single, "evaluations": n_eval, "seconds": dt},
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0))}
 
 
 # The walker and serving workloads build their cell-pair lists with the z-bin
@@ -195,6 +224,53 @@ def bench_nve(steps=1000, relax=300, gap_bug=1):
     return out
 
 
+def call_point_walker_rate(walkers, cfgs, steps=40):
+    """The walkers one at a time through the reference's own call points
+    (include/og_celeste.h, the plain drop-in path, one system per process as
+    in the reference): per step the grid-order coordinates go up
+    (og_memcpy_htod_crd + og_set_crd), the pair kernel runs (og_pairwise_ljzd
+    + og_pair_sync) and work + energies come back (og_memcpy_dtoh_work);
+    every nsgrid-update-intvl steps the grid is sent again
+    (og_set_cell_constant ... og_enumerate_cell_pairs).  The grid is the
+    engine's own (og_get_grid).  The host side of the reference's step
+    (MiniCell rebuild, integration, constraints) is not run, so this rate is
+    an upper bound for that path."""
+    import numpy as np
+    from omegagene_amd.celeste import SubBoxDriver, ZeroDipole, pbc_from_box
+    from omegagene_amd.engine import System
+    n_done, t_total = 0, 0.0
+    for w, c in zip(walkers, cfgs):
+        s = System(w, c)
+        s.nsgrid_update()
+        ids, gcrd, head = s.grid()
+        geo = s.grid_geometry()
+        n_cells = s.grid_info()["n_cells"]
+        s.close()
+        L, lb = pbc_from_box(w.box)
+        lj6, lj12 = w.lj_tables_f32()
+        drv = SubBoxDriver(w.n_atoms, L, lb, c["cutoff"], c["nsgrid-cutoff"], ZeroDipole(c["cutoff"]), lj6, lj12,
+                           w.nb15off_table(), w.charge.astype(np.float32), w.atom_type, geo,
+                           max_n_atom_array=geo["max_n_cells"] * 8)
+        grid = {"n_cells": n_cells, "atomids": ids, "crd": gcrd.astype(np.float32), "idx_xy_head_cell": head,
+                "column_of_slot": np.zeros(len(ids), np.int32)}
+        intvl = int(c["nsgrid-update-intvl"])
+        try:
+            drv.update_grid(grid)
+            drv.pair_forces()
+            t0 = time.perf_counter()
+            for k in range(steps):
+                if k % intvl == 0:
+                    drv.update_grid(grid)
+                else:
+                    drv.set_coordinates(grid["crd"])
+                drv.pair_forces()
+            t_total += time.perf_counter() - t0
+            n_done += steps
+        finally:
+            drv.close()
+    return n_done / t_total
+
+
 def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     """64 independent water + 200-bead-chain walkers (3,000-30,000 atoms,
     log-uniform), Langevin 300 K, SETTLE, VcMD bias on the chain end-to-end
@@ -223,6 +299,8 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
         s.close()
     seq_rate = seq_steps_done / seq_time
     log("walkers one at a time: %.1f walker-steps/s" % seq_rate)
+    cp_rate = call_point_walker_rate(walkers, cfgs)
+    log("walkers one at a time through the call points: %.1f walker-steps/s (device part only)" % cp_rate)
     # batched
     systems = [System(w, c) for w, c in zip(walkers, cfgs)]
     b = Batch()
@@ -246,7 +324,13 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     return {"workload": "64 walkers, water + 200-bead chain, 3,000-30,000 atoms log-uniform (%d atoms in total), "
                         "Langevin 300 K, SETTLE, VcMD 8 windows, rc 12 A" % n_atoms,
             "batched_walker_steps_per_s": batch_rate, "one_at_a_time_walker_steps_per_s": seq_rate,
-            "speedup": batch_rate / seq_rate, "atom_steps_per_s": n_atoms * steps / wall,
+            "speedup": batch_rate / seq_rate,
+            "one_at_a_time_call_points_walker_steps_per_s": cp_rate,
+            "speedup_vs_call_points": batch_rate / cp_rate,
+            "baselines": "speedup: against the same walkers one at a time on the device-resident single-system "
+                         "path (og_md_run); speedup_vs_call_points: against the device part of the reference's "
+                         "call-point path (og_celeste.h) one walker at a time, host work not counted",
+            "atom_steps_per_s": n_atoms * steps / wall,
             "graph_steps": steps, "timed_steps": steps, "pair_shape": tune}
 
 
@@ -367,15 +451,27 @@ def main():
     tune = s.autotune(5)
     log("autotuned pair kernel: %d waves per block (%.3f ms)" % (tune["waves_per_block"], tune["pair_ms"]))
 
-    s.set_pair_timing(True)
+    # the timed region: the production path (hipGraphs of K steps, no per-launch events)
     lib().og_thread_sync()
     t0 = time.perf_counter()
     s.run(args.steps)
     lib().og_thread_sync()
     wall = time.perf_counter() - t0
-    pair_ms, n_launch = s.pair_time()
     elog = s.energy_log(max_rows=args.steps)
-    log("timed %d steps in %.3f s" % (args.steps, wall))
+    log("timed %d steps in %.3f s (graph path)" % (args.steps, wall))
+    # the pair-kernel time from a separate pass of the same system: steps launched directly with HIP
+    # events around every pair-kernel launch on the engine's stream
+    n_pt = min(args.steps, 500)
+    s.set_pair_timing(True)
+    lib().og_thread_sync()
+    t_pt = time.perf_counter()
+    s.run(n_pt)
+    lib().og_thread_sync()
+    wall_pt = time.perf_counter() - t_pt
+    pair_ms, n_launch = s.pair_time()
+    s.set_pair_timing(False)
+    log("pair-kernel timing pass: %d steps, %.4f ms per pair launch, %.4f ms/step without graphs" %
+        (n_pt, pair_ms / max(n_launch, 1), wall_pt / n_pt * 1e3))
 
     npairs = elog[:, 8]
     pairs_per_step = float(npairs.mean())
@@ -413,10 +509,9 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        v, n_eval, dt = cpu_baseline(ls, cfg)
-        cpu = {"value": v, "unit": "pair interactions/s", "cores": 1, "kind": "port",
-               "sample": "%d single-point force evaluations (grid + cell-pair list + FP32 LJ/ZD pairs + "
-                         "excluded pairs) of the same 99,200-atom system in %.1f s, one thread" % (n_eval, dt)}
+        cpu = cpu_baseline(ls, cfg)
+        log("cpu baseline: %.3e pair interactions/s on %d threads (%.3e on one), %s" %
+            (cpu["value"], cpu["cores"], cpu["single_thread"]["value"], cpu["cpu_model"]))
 
     # HBM bytes per pair-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes
     # (a counter run cannot be nested in this process)
@@ -447,7 +542,8 @@ def main():
         "dtype": "fp32 pairs, int64 fixed-point forces, fp64 integration",
         "data": "synthetic (seeded water lattice + 200-bead chain)",
         "config": {"workload": "33,000 waters + 200-bead chain, 99,200 atoms, 99.7 A box, rc 12 A, list 13 A, "
-                               "rebuild every 10 steps, Langevin 300 K, SETTLE",
+                               "rebuild every 10 steps, Langevin 300 K, SETTLE; water H carries LJ (sigma 0.4 A, "
+                               "eps 0.046) beside the charged chain, so a bead cannot collapse onto an H",
                    "n_atoms": ls.n_atoms, "pairs_per_step": pairs_per_step},
         "ns_per_day": ns_day,
         "temperature_K": T,
@@ -455,7 +551,9 @@ def main():
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "k_pair<%d, %d>" % (alpha_mode, tune["waves_per_block"]), "kernel_ms": t_launch * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR[alpha_mode],
-                     "issued_tflops_counters": issued},
+                     "issued_tflops_counters": issued,
+                     "timing": "HIP events around each pair-kernel launch over %d further steps launched without "
+                               "graphs (%d launches); the headline steps run as hipGraphs" % (n_pt, n_launch)},
         "cpu_baseline": cpu,
         "walkers": walkers,
         "serving": serving,

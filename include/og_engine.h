@@ -208,6 +208,12 @@ og_status og_get_vcmd(og_system *sys, int *vstate, double *counts, int n_states)
  * idx_head_cell_pairs [n_cells+1]. */
 og_status og_get_grid_info(og_system *sys, int *n_cells, int *n_columns, int *n_cell_pairs, int *n_atom_array);
 og_status og_get_grid(og_system *sys, int *atomids, double *grid_crd, int *idx_xy_head_cell);
+/* the MiniCell geometry of the grid (MiniCell::set_grid_xy, MiniCell.cpp:221-290),
+ * in the arguments og_set_cell_constant and og_alloc_atom_info take:
+ * n_cells_xyz {nx, ny, 0}, L_cell_xyz {Lx/nx, Ly/ny, 0}, n_neighbors_xyz
+ * {nnx, nny, 0} and the capacities max_n_cells / max_n_cell_pairs */
+og_status og_get_grid_geometry(og_system *sys, int *n_cells_xyz, float *L_cell_xyz, int *n_neighbors_xyz,
+                               int *max_n_cells, int *max_n_cell_pairs);
 og_status og_get_cell_pairs(og_system *sys, int32_t *cell_pairs5, int *idx_head_cell_pairs);
 /* fixed-point gradient accumulators would be zero after a force call; this
  * returns the last gradient in grid order (reference cuda_memcpy_dtoh_work layout) */

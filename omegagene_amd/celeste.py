@@ -26,6 +26,30 @@ def _p(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 
 
+def pbc_from_box(box):
+    """Box lengths and lower bounds as SubBox hands them to og_set_pbc: float
+    L from the diagonal of the 3x3 box, lb = centre - L/2 (PBC::set_pbc, PBC.cpp:18-37)."""
+    box = np.asarray(box, np.float64)
+    L = np.array([box[0], box[4], box[8]], np.float32)
+    lb = np.array([np.float64(box[9 + i]) - np.float64(np.float32(L[i] * np.float32(0.5))) for i in range(3)],
+                  np.float32)
+    return L, lb
+
+
+class ZeroDipole:
+    """ZD constants at alpha = 0 in the float arithmetic of
+    ZeroMultipoleSum::set_zms_params (ZeroMultipoleSum.cpp:108-190): what
+    SubBox passes to og_zerodipole_constant."""
+
+    def __init__(self, cutoff):
+        c = np.float32(cutoff)
+        c2 = np.float32(c * c)
+        c3 = np.float32(c2 * c)
+        self.fcoeff = np.float32(1.0 / np.float64(c3))
+        self.bcoeff = np.float32(0.5 * np.float64(self.fcoeff))
+        self.zcore = np.float32(np.float32(np.float32(1.0) / c) + np.float32(self.bcoeff * c2))
+
+
 class CelesteGpu:
     """Pinned host buffers and the call points, named as in cuda_setup.cu."""
 

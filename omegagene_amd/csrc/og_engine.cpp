@@ -1350,6 +1350,23 @@ og_status og_get_grid_info(og_system *s, int *n_cells, int *n_columns, int *n_ce
     return OG_OK;
 }
 
+og_status og_get_grid_geometry(og_system *s, int *n_cells_xyz, float *L_cell_xyz, int *n_neighbors_xyz,
+                               int *max_n_cells, int *max_n_cell_pairs) {
+    if (!s || !n_cells_xyz || !L_cell_xyz || !n_neighbors_xyz) return OG_E_ARG;
+    n_cells_xyz[0]     = s->h.nx;
+    n_cells_xyz[1]     = s->h.ny;
+    n_cells_xyz[2]     = 0;
+    L_cell_xyz[0]      = s->h.L_cell[0];
+    L_cell_xyz[1]      = s->h.L_cell[1];
+    L_cell_xyz[2]      = 0.0f;
+    n_neighbors_xyz[0] = s->h.nnx;
+    n_neighbors_xyz[1] = s->h.nny;
+    n_neighbors_xyz[2] = 0;
+    if (max_n_cells) *max_n_cells = s->h.cap_cells;
+    if (max_n_cell_pairs) *max_n_cell_pairs = s->h.cap_cell_pairs;
+    return OG_OK;
+}
+
 og_status og_get_grid(og_system *s, int *atomids, double *grid_crd, int *idx_xy_head_cell) {
     if (!s) return OG_E_ARG;
     og_dev_counters c;

@@ -275,6 +275,17 @@ class System:
                                  head.ctypes.data_as(C.c_void_p)))
         return ids, gcrd, head
 
+    def grid_geometry(self):
+        """MiniCell geometry in the arguments of the call points (og_set_cell_constant, og_alloc_atom_info)."""
+        nc = np.zeros(3, np.int32)
+        lc = np.zeros(3, np.float32)
+        nn = np.zeros(3, np.int32)
+        mc, mp = C.c_int(), C.c_int()
+        _check(lib().og_get_grid_geometry(self.h, nc.ctypes.data_as(C.c_void_p), lc.ctypes.data_as(C.c_void_p),
+                                          nn.ctypes.data_as(C.c_void_p), C.byref(mc), C.byref(mp)))
+        return {"n_cells_xyz": nc, "L_cell_xyz": lc, "n_neighbors_xyz": nn, "n_columns": int(nc[0] * nc[1]),
+                "max_n_cells": mc.value, "max_n_cell_pairs": mp.value}
+
     def cell_pairs(self):
         gi = self.grid_info()
         cp = np.zeros((gi["n_cell_pairs"], 5), np.int32)

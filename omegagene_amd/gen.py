@@ -1,8 +1,10 @@
 """Seeded synthetic systems for the five benchmark workloads, written as
 ``.cls`` launch sets plus ``.cfg`` files with the reference's keys.
 
-Water: rigid 3-site (O-H 0.9572 A, H-O-H 104.52 deg), LJ on O only
-(sigma 3.1507 A, epsilon 0.1521 kcal/mol), charges O -0.834 / H +0.417, on a
+Water: rigid 3-site (O-H 0.9572 A, H-O-H 104.52 deg), LJ on O
+(sigma 3.1507 A, epsilon 0.1521 kcal/mol; in systems with the charged chain
+also a small hydrogen LJ, sigma 0.4 A, epsilon 0.046, so that a bead cannot
+collapse onto a hydrogen), charges O -0.834 / H +0.417, on a
 jittered cubic lattice (+-0.1 spacing) with uniformly random orientations.
 Chain: 200 beads on a seeded self-avoiding walk with 1.53 A bonds, harmonic
 E = k (r - r0)^2 with k = 300 kcal/mol/A^2 (the reference convention has no
@@ -105,8 +107,11 @@ def _chain_walk(rng, n_beads, L, min_sep=3.6, max_tries=400):
 
 
 def build_system(n_waters: int, seed: int, L: Optional[float] = None, n_beads: int = 0, T: float = 300.0,
-                 with_vcmd: bool = False) -> LaunchSet:
-    """Water (+ optional chain) system at 1.0 g/cm^3 unless L is given."""
+                 with_vcmd: bool = False, h_lj: bool = True) -> LaunchSet:
+    """Water (+ optional chain) system at 1.0 g/cm^3 unless L is given.
+    h_lj=False leaves the water hydrogens without LJ (LJ on O only, the
+    plain-water workloads); with a charged chain the hydrogens keep the small
+    CHARMM TIP3P LJ so that a bead cannot collapse onto one."""
     rng = np.random.default_rng(seed)
     total_mass = n_waters * (M_O + 2 * M_H) + n_beads * M_BEAD
     if L is None:
@@ -146,7 +151,8 @@ def build_system(n_waters: int, seed: int, L: Optional[float] = None, n_beads: i
     sig = np.zeros(n_types)
     eps = np.zeros(n_types)
     sig[0], eps[0] = SIG_O, EPS_O
-    sig[1], eps[1] = SIG_H, EPS_H
+    if h_lj:
+        sig[1], eps[1] = SIG_H, EPS_H
     if n_beads:
         sig[2:] = rng.uniform(3.0, 4.0, n_beads)
         eps[2:] = rng.uniform(0.05, 0.2, n_beads)
@@ -276,8 +282,8 @@ def vcmd_windows(ls: LaunchSet, n_windows: int = 8, seed: int = 0) -> VcmdDef:
 # ---- the five workloads ----
 
 def config0(seed: int = 0) -> Tuple[LaunchSet, dict]:
-    """2,000 rigid waters in a 39.1 A box, 12 A, ZD alpha 0, NVE + SHAKE, dt 2 fs."""
-    ls = build_system(2000, seed=1000 + seed, L=39.1)
+    """2,000 rigid waters in a 39.1 A box (LJ on O only), 12 A, ZD alpha 0, NVE + SHAKE, dt 2 fs."""
+    ls = build_system(2000, seed=1000 + seed, L=39.1, h_lj=False)
     cfg = dict(mode="md", integrator="leapfrog-presto", cutoff=12.0, **{"nsgrid-cutoff": 13.0},
                **{"nsgrid-update-intvl": 10}, **{"n-steps": 1000}, **{"time-step": 2.0},
                electrostatic="zero-dipole", **{"ele-alpha": 0.0}, constraint="shake",
@@ -287,7 +293,8 @@ def config0(seed: int = 0) -> Tuple[LaunchSet, dict]:
 
 
 def config1(seed: int = 0) -> Tuple[LaunchSet, dict]:
-    """33,000 waters + 200-bead chain (~99.6 A box), 12 A, Langevin 300 K."""
+    """33,000 waters + 200-bead chain (~99.6 A box), 12 A, Langevin 300 K; water
+    hydrogens carry LJ (sigma 0.4 A, epsilon 0.046 kcal/mol) beside the charged chain."""
     ls = build_system(33000, seed=2000 + seed, n_beads=200)
     cfg = dict(mode="md", integrator="langevin", cutoff=12.0, **{"nsgrid-cutoff": 13.0},
                **{"nsgrid-update-intvl": 10}, **{"n-steps": 10000}, **{"time-step": 2.0},
@@ -299,8 +306,8 @@ def config1(seed: int = 0) -> Tuple[LaunchSet, dict]:
 
 
 def config2(seed: int = 0) -> Tuple[LaunchSet, dict]:
-    """400,000 waters (~228.7 A box), 10 A, ZD alpha 0.1."""
-    ls = build_system(400000, seed=3000 + seed)
+    """400,000 waters (~228.7 A box, LJ on O only), 10 A, ZD alpha 0.1."""
+    ls = build_system(400000, seed=3000 + seed, h_lj=False)
     cfg = dict(mode="md", integrator="leapfrog-presto", cutoff=10.0, **{"nsgrid-cutoff": 11.0},
                **{"nsgrid-update-intvl": 10}, **{"n-steps": 100}, **{"time-step": 2.0},
                electrostatic="zero-dipole", **{"ele-alpha": 0.1}, constraint="shake-settle",
