@@ -117,6 +117,21 @@ int og_pair_sync(void);
  * order (dE/dx, kcal/mol/A) and energy [2] = {vdw, ele} */
 int og_memcpy_dtoh_work(double **h_work, double **h_energy, int n_atoms, int n_atom_array);
 
+/* The second pair form, hydrophobicity-scale LJ + Debye-Hueckel (the
+ * reference's F_HPSCUDA build, SubBox.cpp:1702-1710,1730-1733):
+ * cuda_hps_constant cuda_setup.cu:1254-1258 (--hydrophobicity-scale-epsiron) */
+int og_hps_constant(float epsiron);
+/* cuda_alloc_set_hps_params cuda_setup.cu:1230-1244: float [n_lj_types^2]
+ * switch distance and lambda per type pair (after og_alloc_set_lj_params) */
+int og_alloc_set_hps_params(float *h_hps_cutoff, float *h_hps_lambda, int n_lj_types);
+/* cuda_free_hps_params cuda_setup.cu:1245-1252 */
+int og_free_hps_params(void);
+/* cuda_debye_huckel_constant cuda_setup.cu:1259-1270 */
+int og_debye_huckel_constant(float dielect, float temperature, float ionic_strength);
+/* cuda_pairwise_hps_dh cuda_setup.cu:1478-1493: asynchronous; results through
+ * og_memcpy_dtoh_work as og_pairwise_ljzd (the flag is ignored as in the reference) */
+int og_pairwise_hps_dh(const bool flg_mod_15mask);
+
 /* Inspection for parity tests (no reference counterpart): the cell-pair list
  * of the last og_enumerate_cell_pairs as CellPair records
  * {cell_id1, cell_id2, image, pair_mask[2]} (MiniCell.h:17-26) and

@@ -109,6 +109,11 @@ typedef struct {
     int           vc_n_a, vc_n_b;   /* group sizes */
     const int    *vc_atoms_a;       /* [vc_n_a] */
     const int    *vc_atoms_b;       /* [vc_n_b] */
+    /* the .cls "nbpair hps" section (Read.cpp:322-334): per type pair the HPS
+     * switch distance and hydrophobicity scale lambda; read when nonbond is
+     * OG_NONBOND_HPS (NULL otherwise) */
+    const double *hps_cutoff;       /* [n_types*n_types] */
+    const double *hps_lambda;       /* [n_types*n_types] */
 } og_system_desc;
 
 /* Run parameters: the .cfg keys of the path (Config.cpp:37-367). */
@@ -142,9 +147,22 @@ typedef struct {
     double   temperature_init;     /* --temperature-init (K); < 0: --temperature (Config.cpp:364) */
     int      heating_steps;        /* --heating-steps: linear ramp temperature_init -> temperature
                                       (SubBox::update_thermostat, SubBox.cpp:1745-1757) */
+    /* the second pair form (the reference GPU kernel kernel_pairwise_hps_dh, cuda_setup.cu:1273-1477):
+     * hydrophobicity-scale LJ with Debye-Hueckel electrostatics */
+    int      nonbond;              /* --nonbond: OG_NONBOND_LJ or OG_NONBOND_HPS (hydrophobicity-scale-lj) */
+    double   hps_epsilon;          /* --hydrophobicity-scale-epsiron (kcal/mol) */
+    int      electrostatic;        /* --electrostatic: OG_ELE_ZERO_DIPOLE or OG_ELE_DEBYE_HUCKEL */
+    double   dh_dielectric;        /* --debye-huckel-dielectric */
+    double   dh_ionic_strength;    /* --debye-huckel-ionic-strength */
+    double   dh_temperature;       /* --debye-huckel-temperature (K) */
+    int      dh_grad_fix;          /* 1: Debye-Hueckel gradient -E (1/r + kappa) r_ij / r; 0 (default): the
+                                      reference's -E (1/r + kappa) r_ij (ZeroMultipoleSum.cpp:453,
+                                      cuda_setup.cu:1327), which is r times the derivative */
 } og_params;
 
 enum { OG_THERMOSTAT_NONE = 0, OG_THERMOSTAT_SCALING = 1 };
+enum { OG_NONBOND_LJ = 0, OG_NONBOND_HPS = 1 };
+enum { OG_ELE_ZERO_DIPOLE = 0, OG_ELE_DEBYE_HUCKEL = 1 };
 
 void og_params_default(og_params *p);
 

@@ -145,6 +145,21 @@ class CelesteGpu:
     def pairwise_ljzd(self, flg_mod_15mask=False):
         _check(self.L.og_pairwise_ljzd(C.c_bool(bool(flg_mod_15mask))))
 
+    # ---- the HPS + Debye-Hueckel form (SubBox.cpp:1702-1710,1730-1733) ----
+    def hps_constant(self, eps):
+        _check(self.L.og_hps_constant(C.c_float(eps)))
+
+    def alloc_set_hps_params(self, hps_cutoff, hps_lambda):
+        hc = np.ascontiguousarray(hps_cutoff, np.float32)
+        hl = np.ascontiguousarray(hps_lambda, np.float32)
+        _check(self.L.og_alloc_set_hps_params(_p(hc, C.c_float), _p(hl, C.c_float), C.c_int(hc.shape[0])))
+
+    def debye_huckel_constant(self, dielect, temperature, ionic_strength):
+        _check(self.L.og_debye_huckel_constant(C.c_float(dielect), C.c_float(temperature), C.c_float(ionic_strength)))
+
+    def pairwise_hps_dh(self, flg_mod_15mask=False):
+        _check(self.L.og_pairwise_hps_dh(C.c_bool(bool(flg_mod_15mask))))
+
     def pair_sync(self):
         _check(self.L.og_pair_sync())
 
@@ -222,13 +237,27 @@ class SubBoxDriver:
         self.gpu.memcpy_htod_crd(np.asarray(grid_crd, np.float32))
         self.gpu.set_crd()
 
+    def set_hps_dh(self, hps_cutoff, hps_lambda, hps_eps, dielect, temperature, ionic_strength):
+        """SubBox::gpu_device_setup of the HPS + Debye-Hueckel build (SubBox.cpp:1702-1710): later
+        pair_forces calls evaluate that form (calc_energy_pairwise_cuda, SubBox.cpp:1724-1736)."""
+        g = self.gpu
+        g.alloc_set_hps_params(hps_cutoff, hps_lambda)
+        g.hps_constant(hps_eps)
+        g.debye_huckel_constant(dielect, temperature, ionic_strength)
+        self.hps_dh = True
+
     def pair_forces(self):
         """calc_energy_pairwise_cuda + cuda_pair_sync + cuda_memcpy_dtoh_work."""
+        if getattr(self, "hps_dh", False):
+            self.gpu.pairwise_hps_dh(False)
+            self.gpu.pair_sync()
+            return self.gpu.memcpy_dtoh_work(self.n_atoms, self.n_atom_array)
         self.gpu.pairwise_ljzd(False)
         self.gpu.pair_sync()
         return self.gpu.memcpy_dtoh_work(self.n_atoms, self.n_atom_array)
 
     def close(self):
+        self.gpu.L.og_free_hps_params()
         self.gpu.free()
 
 

@@ -50,6 +50,8 @@ class LaunchSet:
     n_lj_types: int
     lj6: np.ndarray                      # float64[n_types, n_types] (symmetric)
     lj12: np.ndarray
+    hps_cutoff: Optional[np.ndarray] = None  # float64[n_types, n_types]: HPS switch distance (the "nbpair hps" section)
+    hps_lambda: Optional[np.ndarray] = None  # float64[n_types, n_types]: HPS hydrophobicity scale
     bond_ids: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.int32))
     bond_k: np.ndarray = field(default_factory=lambda: np.zeros(0))
     bond_r0: np.ndarray = field(default_factory=lambda: np.zeros(0))
@@ -78,6 +80,13 @@ class LaunchSet:
     def lj_tables_f32(self):
         """The float tables the pair code reads (MmSystem.cpp:313-319, real_pw)."""
         return self.lj6.astype(np.float32), self.lj12.astype(np.float32)
+
+    def hps_tables_f32(self):
+        """HPS cutoff and lambda tables as the pair code reads them (MmSystem.cpp:320-325, real_pw)."""
+        nt = self.n_lj_types
+        hc = self.hps_cutoff if self.hps_cutoff is not None else np.zeros((nt, nt))
+        hl = self.hps_lambda if self.hps_lambda is not None else np.zeros((nt, nt))
+        return hc.astype(np.float32), hl.astype(np.float32)
 
     def nb15off_table(self, max_n: int = MAX_N_NB15OFF) -> np.ndarray:
         """int32[n, 32] partner lists, -1 terminated, in input order (MmSystem.cpp:478-525)."""
@@ -161,8 +170,8 @@ def write_cls(ls: LaunchSet, path=None) -> bytes:
     lj["c"], lj["d"] = ls.lj6[iu], ls.lj12[iu]
     t.write(_p("iii", 8 + lj.nbytes, nt, len(lj)) + lj.tobytes())
     hp = lj.copy()
-    hp["c"] = 0.0
-    hp["d"] = 0.0
+    hp["c"] = ls.hps_cutoff[iu] if ls.hps_cutoff is not None else 0.0
+    hp["d"] = ls.hps_lambda[iu] if ls.hps_lambda is not None else 0.0
     t.write(_p("i", hp.nbytes) + hp.tobytes())
 
     bd = _rec([("a", "<i4"), ("b", "<i4"), ("k", "<f8"), ("r", "<f8")], len(ls.bond_ids))
@@ -320,8 +329,15 @@ def read_cls(src) -> LaunchSet:
         ls.lj6[lj["b"] - 1, lj["a"] - 1] = lj["c"]
         ls.lj12[lj["a"] - 1, lj["b"] - 1] = lj["d"]
         ls.lj12[lj["b"] - 1, lj["a"] - 1] = lj["d"]
+        # "nbpair hps": cutoff and lambda per type pair (Read.cpp:322-334)
         r.i()
-        r.rec([("a", "<i4"), ("b", "<i4"), ("c", "<f8"), ("d", "<f8")], npairs)
+        hp = r.rec([("a", "<i4"), ("b", "<i4"), ("c", "<f8"), ("d", "<f8")], npairs)
+        ls.hps_cutoff = np.zeros((nt, nt))
+        ls.hps_lambda = np.zeros((nt, nt))
+        ls.hps_cutoff[hp["a"] - 1, hp["b"] - 1] = hp["c"]
+        ls.hps_cutoff[hp["b"] - 1, hp["a"] - 1] = hp["c"]
+        ls.hps_lambda[hp["a"] - 1, hp["b"] - 1] = hp["d"]
+        ls.hps_lambda[hp["b"] - 1, hp["a"] - 1] = hp["d"]
         r.i()
         nb = r.i()
         bd = r.rec([("a", "<i4"), ("b", "<i4"), ("k", "<f8"), ("r", "<f8")], nb)

@@ -99,6 +99,8 @@ struct CsCtx {
     bool  have_cellconst = false, have_ids = false, have_info = false, have_pairs = false;
     float cutoff = 0, cutoff_pair = 0;
     std::vector<int> comp_host;  // exclusion components (atom order)
+    std::vector<float> lj6_host, lj12_host;  // the LJ tables as delivered (for the HPS table)
+    bool  have_hps = false, have_dh = false;
     // int2 block maps
     int2 *map_cellw = nullptr, *map_col = nullptr, *map_slot = nullptr, *map_sys = nullptr;
     int   nb_cellw = 0, nb_col = 0, nb_slot = 0;
@@ -314,6 +316,8 @@ int og_alloc_set_lj_params(float *h_lj_6term, float *h_lj_12term, int n_lj_types
         const size_t       nt2 = (size_t)n_lj_types * n_lj_types;
         std::vector<float2> ljp(nt2);
         for (size_t k = 0; k < nt2; k++) ljp[k] = make_float2(h_lj_6term[k], h_lj_12term[k]);
+        g_cs.lj6_host.assign(h_lj_6term, h_lj_6term + nt2);
+        g_cs.lj12_host.assign(h_lj_12term, h_lj_12term + nt2);
         CS_TRY(cs_alloc(&h.lj_pair, nt2));
         CS_HIP(hipMemcpy(h.lj_pair, ljp.data(), sizeof(float2) * nt2, hipMemcpyHostToDevice));
     }
@@ -338,7 +342,7 @@ int og_alloc_set_lj_params(float *h_lj_6term, float *h_lj_12term, int n_lj_types
 
 int og_free_lj_params(void) {
     og_dev_sys &h = g_cs.h;
-    for (void **p : {(void **)&h.lj6, (void **)&h.lj12, (void **)&h.lj_pair, (void **)&h.nb15off}) {
+    for (void **p : {(void **)&h.lj6, (void **)&h.lj12, (void **)&h.lj_pair, (void **)&h.nb15off, (void **)&h.lj_hps}) {
         if (*p) {
             auto it = std::find(g_cs.allocs.begin(), g_cs.allocs.end(), *p);
             if (it != g_cs.allocs.end()) g_cs.allocs.erase(it);
@@ -346,7 +350,56 @@ int og_free_lj_params(void) {
             *p = nullptr;
         }
     }
-    g_cs.have_lj = false;
+    g_cs.have_lj  = false;
+    g_cs.have_hps = false;
+    return OG_OK;
+}
+
+// cuda_alloc_set_hps_params (cuda_setup.cu:1230-1244): the HPS switch distance and lambda per type pair,
+// joined here with the LJ table into the {A6, A12, switch, lambda} entries the pair kernel reads
+int og_alloc_set_hps_params(float *h_hps_cutoff, float *h_hps_lambda, int n_lj_types) {
+    CS_REQUIRE(g_cs.have_lj, "og_alloc_set_hps_params: call og_alloc_set_lj_params first");
+    if (!h_hps_cutoff || !h_hps_lambda || n_lj_types != g_cs.n_types) return OG_E_ARG;
+    og_dev_sys  &h   = g_cs.h;
+    const size_t nt2 = (size_t)n_lj_types * n_lj_types;
+    std::vector<float4> t4(nt2);
+    for (size_t k = 0; k < nt2; k++)
+        t4[k] = make_float4(g_cs.lj6_host[k], g_cs.lj12_host[k], h_hps_cutoff[k], h_hps_lambda[k]);
+    if (!h.lj_hps) CS_TRY(cs_alloc(&h.lj_hps, nt2));
+    CS_HIP(hipMemcpy(h.lj_hps, t4.data(), sizeof(float4) * nt2, hipMemcpyHostToDevice));
+    g_cs.have_hps = true;
+    return OG_OK;
+}
+
+// cuda_free_hps_params (cuda_setup.cu:1245-1252)
+int og_free_hps_params(void) {
+    og_dev_sys &h = g_cs.h;
+    if (h.lj_hps) {
+        auto it = std::find(g_cs.allocs.begin(), g_cs.allocs.end(), (void *)h.lj_hps);
+        if (it != g_cs.allocs.end()) g_cs.allocs.erase(it);
+        hipFree(h.lj_hps);
+        h.lj_hps = nullptr;
+    }
+    g_cs.have_hps = false;
+    return OG_OK;
+}
+
+// cuda_hps_constant (cuda_setup.cu:1254-1258)
+int og_hps_constant(float hps_eps) {
+    g_cs.h.hps_eps = hps_eps;
+    return OG_OK;
+}
+
+// cuda_debye_huckel_constant (cuda_setup.cu:1259-1270): the inverse Debye length in double from the
+// float arguments, kept as float; constants CelesteObject.cpp:32-40
+int og_debye_huckel_constant(float dielect, float temperature, float ionic_strength) {
+    if (!(dielect > 0.0f) || !(temperature > 0.0f) || !(ionic_strength > 0.0f)) return OG_E_ARG;
+    og_dev_sys &h = g_cs.h;
+    h.dh_kappa    = (float)(1.0 / (sqrt(8.85418782e-12 * (double)dielect * 1.380658e-23 * (double)temperature /
+                                        (2 * 6.0221413e23 * 1.60217657e-19 * 1.60217657e-19 * (double)ionic_strength)) *
+                                   1e10));
+    h.dh_diel_inv = (float)(1.0 / (double)dielect);
+    g_cs.have_dh  = true;
     return OG_OK;
 }
 
@@ -558,7 +611,23 @@ int og_pairwise_ljzd(bool flg_mod_15mask) {
     og_dev_sys &h = g_cs.h;
     CS_TRY(cs_push_sys());
     CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 4 * (size_t)h.cap_atom_array, g_cs.stream));
-    og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), 0, 4);
+    og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), OG_FORM_ZD, 4);
+    og_launch_energy_reduce(cs_launch(g_cs.map_sys, 1));
+    CS_HIP(hipGetLastError());
+    return OG_OK;
+}
+
+// cuda_pairwise_hps_dh (cuda_setup.cu:1478-1493): hydrophobicity-scale LJ + Debye-Hueckel on the same
+// cell-pair list, results through og_memcpy_dtoh_work as og_pairwise_ljzd
+int og_pairwise_hps_dh(bool flg_mod_15mask) {
+    (void)flg_mod_15mask;
+    CS_REQUIRE(g_cs.have_pairs, "og_pairwise_hps_dh: no cell-pair list (og_enumerate_cell_pairs)");
+    CS_REQUIRE(g_cs.have_hps && g_cs.have_dh,
+               "og_pairwise_hps_dh: call og_alloc_set_hps_params and og_debye_huckel_constant first");
+    og_dev_sys &h = g_cs.h;
+    CS_TRY(cs_push_sys());
+    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 4 * (size_t)h.cap_atom_array, g_cs.stream));
+    og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), OG_FORM_HPS_DH, 4);
     og_launch_energy_reduce(cs_launch(g_cs.map_sys, 1));
     CS_HIP(hipGetLastError());
     return OG_OK;

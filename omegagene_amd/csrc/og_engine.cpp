@@ -192,7 +192,7 @@ struct og_system {
     og_params           prm{};
     GridGeom            geom{};
     std::vector<void *> allocs;
-    int                 alpha_mode = 0;
+    int                 pair_form = 0;
     int                 d_free     = 0;
     long long           host_step  = 0;  // steps completed (host mirror of cnt->step)
     bool                grid_valid = false;
@@ -214,7 +214,7 @@ struct og_exec {
     bool                     dirty  = true;
     std::map<std::string, hipGraphExec_t> graphs;  // by rebuild pattern: '1' = rebuild before that step
     std::vector<hipEvent_t>  ev;          // pair-kernel timing events
-    int                      alpha_mode = 0;
+    int                      pair_form = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false, any_thermo = false;
     bool                     fused = false;       // every member takes the fused Langevin step
@@ -260,13 +260,13 @@ og_status exec_prepare(og_exec &x) {
     }
     std::vector<og_dev_sys> hs(ns);
     std::vector<int2>       maps[KIND_N];
-    x.alpha_mode   = x.members[0]->alpha_mode;
+    x.pair_form   = x.members[0]->pair_form;
     x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = x.any_thermo = false;
     x.fused        = true;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
         og_system *s = x.members[si];
-        if (s->alpha_mode != x.alpha_mode) {
+        if (s->pair_form != x.pair_form) {
             set_err("systems in one batch must share the electrostatics form (alpha = 0 or > 0)");
             return OG_E_ARG;
         }
@@ -375,7 +375,7 @@ void enqueue_rebuild(og_exec &x) {
 // them in its closing kernel instead
 void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1, bool reduce = false) {
     if (e0) hipEventRecord(e0, x.stream);
-    og_launch_pair(L(x, KIND_PAIR + x.pair_shape), x.alpha_mode, 1 << (x.pair_shape & 3));
+    og_launch_pair(L(x, KIND_PAIR + x.pair_shape), x.pair_form, 1 << (x.pair_shape & 3));
     if (e1) hipEventRecord(e1, x.stream);
     // an MD step of a fused context converts the gradient inside its step kernel
     if (reduce || !x.fused) og_launch_forces_to_work(L(x, KIND_ATOM));
@@ -619,11 +619,26 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.L_cell[0] = g.L_cell[0], h.L_cell[1] = g.L_cell[1];
     h.L_z_uni      = g.L_z_uni;
     h.cutoff_pair2 = g.cutoff_pair2;
-    const ZdConst z = zd_const(p->cutoff, p->ele_alpha);
+    ZdConst z = zd_const(p->cutoff, p->ele_alpha);
     h.cutoff  = z.cutoff;
     h.cutoff2 = z.cutoff * z.cutoff;
     h.zcore = z.zcore, h.bcoeff = z.bcoeff, h.fcoeff = z.fcoeff, h.piewald = z.piewald, h.alpha = z.alpha;
-    s->alpha_mode = z.alpha > 0.0f ? 1 : 0;
+    const bool dh = p->electrostatic == OG_ELE_DEBYE_HUCKEL;
+    s->pair_form  = dh ? OG_FORM_HPS_DH : (z.alpha > 0.0f ? OG_FORM_ZD_ALPHA : OG_FORM_ZD);
+    if (dh) {
+        // ZeroMultipoleSum constructor (ZeroMultipoleSum.cpp:21-24): float members from the double cfg values;
+        // constants CelesteObject.cpp:32-40
+        h.dh_diel_inv = (float)(1.0 / p->dh_dielectric);
+        h.dh_kappa    = (float)(1.0 / (sqrt(8.85418782e-12 * p->dh_dielectric * 1.380658e-23 * p->dh_temperature /
+                                            (2 * 6.0221413e23 * 1.60217657e-19 * 1.60217657e-19 *
+                                             p->dh_ionic_strength)) * 1e10));
+        h.dh_grad_fix = p->dh_grad_fix != 0;
+        h.hps_eps     = p->nonbond == OG_NONBOND_HPS ? (float)p->hps_epsilon : 0.0f;
+        // excluded pairs add nothing under Debye-Hueckel (calc_null, ZeroMultipoleSum.cpp:456-466) and the
+        // self energy has no core term (zmcore = 0 outside the zero-multipole modes, :75-82)
+        h.n_excess = 0;
+        z.zcore    = 0.0f;
+    }
     h.dt          = p->time_step;
     h.tol         = p->const_tolerance;
     h.temperature = p->temperature;
@@ -658,7 +673,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         m[i]    = (float)d->mass[i];
         minv[i] = (float)(1.0 / (double)m[i]);  // SubBox.cpp:691 (real_pw mass_inv = 1.0 / mass)
     }
-    h.e_self = self_energy(z, n, q.data(), d->n_excess, d->excess);
+    h.e_self = self_energy(z, n, q.data(), h.n_excess, d->excess);
 
     const int cap = h.cap_atom_array;
     bool      ok  = A(&h.crd, 3 * n) && A(&h.crd_prev, 3 * n) && A(&h.crd_prev2, 3 * n) && A(&h.vel, 3 * n) &&
@@ -666,6 +681,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.atom_type, n) && A(&h.nb15off, (size_t)n * OG_MAX_NB15) && A(&h.comp, n) &&
               A(&h.lj6, (size_t)h.n_types * h.n_types) && A(&h.lj12, (size_t)h.n_types * h.n_types) &&
               A(&h.lj_pair, (size_t)h.n_types * h.n_types) &&
+              A(&h.lj_hps, s->pair_form == OG_FORM_HPS_DH ? (size_t)h.n_types * h.n_types : 1) &&
               A(&h.excess, 2 * (size_t)h.n_excess) && A(&h.bond_ids, 2 * (size_t)h.n_bonds) &&
               A(&h.bond_k, h.n_bonds) && A(&h.bond_r0, h.n_bonds) && A(&h.trio_ids, 3 * (size_t)h.n_trios) &&
               A(&h.trio_d2, 3 * (size_t)h.n_trios) && A(&h.pairc_ids, 2 * (size_t)h.n_pairs_c) &&
@@ -735,6 +751,19 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         std::vector<float2> ljp(lj6f.size());
         for (size_t k = 0; k < ljp.size(); k++) ljp[k] = make_float2(lj6f[k], lj12f[k]);
         OG_HIP(hipMemcpy(h.lj_pair, ljp.data(), sizeof(float2) * ljp.size(), hipMemcpyHostToDevice));
+    }
+    if (s->pair_form == OG_FORM_HPS_DH) {
+        // {A6, A12, switch distance, lambda} in the renumbered type order; plain LJ (nonbond lennard-jones)
+        // is the HPS form with lambda 1 and switch distance 0
+        const bool          hps = p->nonbond == OG_NONBOND_HPS;
+        std::vector<float4> t4((size_t)nt * nt);
+        for (int a = 0; a < nt; a++)
+            for (int b = 0; b < nt; b++) {
+                const size_t src = (size_t)torder[a] * nt + torder[b], dst = (size_t)a * nt + b;
+                t4[dst] = make_float4(lj6f[dst], lj12f[dst], hps ? (float)d->hps_cutoff[src] : 0.0f,
+                                      hps ? (float)d->hps_lambda[src] : 1.0f);
+            }
+        OG_HIP(hipMemcpy(h.lj_hps, t4.data(), sizeof(float4) * t4.size(), hipMemcpyHostToDevice));
     }
     if (h.n_excess) OG_HIP(hipMemcpy(h.excess, d->excess, sizeof(int) * 2 * h.n_excess, hipMemcpyHostToDevice));
     if (h.n_bonds) {
@@ -898,9 +927,9 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
     for (int shp = 0; shp < n_shapes; shp++) {
         const og_launch a = L(x, KIND_PAIR + shp);
         const int       w = 1 << (shp & 3);
-        og_launch_pair(a, x.alpha_mode, w);
+        og_launch_pair(a, x.pair_form, w);
         OG_HIP(hipEventRecord(e0, x.stream));
-        for (int t = 0; t < trials; t++) og_launch_pair(a, x.alpha_mode, w);
+        for (int t = 0; t < trials; t++) og_launch_pair(a, x.pair_form, w);
         OG_HIP(hipEventRecord(e1, x.stream));
         OG_HIP(hipEventSynchronize(e1));
         float ms = 0.0f;
@@ -985,6 +1014,13 @@ void og_params_default(og_params *p) {
     p->berendsen_tau       = 0.0;   // Config.h:23
     p->temperature_init    = -1.0;  // Config.h:22
     p->heating_steps       = 0;     // Config.h:24
+    p->nonbond             = OG_NONBOND_LJ;       // Config.h:96
+    p->hps_epsilon         = 0.0;                 // Config.h:97
+    p->electrostatic       = OG_ELE_ZERO_DIPOLE;  // Config.h:32
+    p->dh_dielectric       = 0.0;                 // Config.h:92-94
+    p->dh_ionic_strength   = 0.0;
+    p->dh_temperature      = 0.0;
+    p->dh_grad_fix         = 0;
 }
 
 const char *og_status_string(og_status s) {
@@ -1036,7 +1072,7 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     const double log_rows = std::max(1, p->log_cap);
     double b = 0.0;
     b += n * (6 * 24 + 3 * 4 + 4 + 4.0 * OG_MAX_NB15 + 4 + 4 + 4 + 4);   // state, masses, types, table, comp, col, zkey, rev
-    b += nt * (4 + 4 + 8);                                               // lj6, lj12, lj_pair
+    b += nt * (4 + 4 + 8 + (p->electrostatic == OG_ELE_DEBYE_HUCKEL ? 16 : 0));  // lj6, lj12, lj_pair, lj_hps
     b += ex * 8 + nb * (8 + 16) + trios * (12 + 24) + pc * (8 + 8) + settle * 12;
     b += 2 * (n + 1) * 4 + std::max(2 * ex, 1.0) * 16 + std::max(2 * nb, 1.0) * 16;      // per-atom pair lists
     b += (ng + 1) * 4 + ng_atoms * 4 + (ng + 1) * 4 + 3 * (ng + 1) * 8;
@@ -1113,6 +1149,30 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
         }
     } else if (p->thermostat != OG_THERMOSTAT_NONE) {
         set_err("unknown thermostat %d", p->thermostat);
+        return OG_E_ARG;
+    }
+    if (p->electrostatic == OG_ELE_DEBYE_HUCKEL) {
+        if (!(p->dh_dielectric > 0.0 && p->dh_ionic_strength > 0.0 && p->dh_temperature > 0.0)) {
+            set_err("Debye-Hueckel needs positive --debye-huckel-dielectric, -ionic-strength and -temperature");
+            return OG_E_ARG;
+        }
+    } else if (p->electrostatic != OG_ELE_ZERO_DIPOLE) {
+        set_err("unknown electrostatic %d", p->electrostatic);
+        return OG_E_ARG;
+    }
+    if (p->nonbond == OG_NONBOND_HPS) {
+        // the device pairs the HPS switch with Debye-Hueckel only, as the reference's GPU kernel
+        // (kernel_pairwise_hps_dh, cuda_setup.cu:1334-1477)
+        if (p->electrostatic != OG_ELE_DEBYE_HUCKEL) {
+            set_err("hydrophobicity-scale-lj runs with --electrostatic debye-huckel");
+            return OG_E_ARG;
+        }
+        if (!d->hps_cutoff || !d->hps_lambda) {
+            set_err("hydrophobicity-scale-lj needs the HPS cutoff and lambda tables");
+            return OG_E_ARG;
+        }
+    } else if (p->nonbond != OG_NONBOND_LJ) {
+        set_err("unknown nonbond %d", p->nonbond);
         return OG_E_ARG;
     }
     if (p->com_cancel && d->n_com_groups > 0) {

@@ -95,6 +95,11 @@ struct og_dev_sys {
     float cutoff2;
     // ---- zero-dipole constants (ZeroMultipoleSum.cpp:108-190) ----
     float zcore, bcoeff, fcoeff, piewald, alpha;
+    // hydrophobicity-scale LJ + Debye-Hueckel (cal_pair_hps_dh, cuda_setup.cu:1273-1332)
+    float hps_eps;         // --hydrophobicity-scale-epsiron
+    float dh_diel_inv;     // 1 / --debye-huckel-dielectric
+    float dh_kappa;        // inverse Debye length (1/A)
+    int   dh_grad_fix;     // 1: Debye-Hueckel gradient coefficient -E (1/r + kappa) / r instead of the reference's -E (1/r + kappa)
     // ---- integration (FP64) ----
     double dt, tol, temperature, gamma;
     // scaling thermostat (leapfrog, no constraints): 1/tau after the clamp by the step, heating ramp,
@@ -122,6 +127,7 @@ struct og_dev_sys {
     int    *comp;           // exclusion-connectivity component of each atom
     float  *lj6, *lj12;     // [n_types*n_types]
     float2 *lj_pair;        // [n_types*n_types] {A6, A12} interleaved for the pair kernel
+    float4 *lj_hps;         // [n_types*n_types] {A6, A12, HPS switch distance, HPS lambda} (pair form HPS + DH)
     int    *excess;         // [n_excess*2]
     int    *bond_ids;       // [n_bonds*2]
     double *bond_k, *bond_r0;
@@ -360,7 +366,9 @@ void og_launch_enum_scan_blocks(const og_launch &blocks);  // blocks of 1024 cel
 void og_launch_enum_write(const og_launch &cellw);
 void og_launch_masks(const og_launch &cp);
 // forces
-void og_launch_pair(const og_launch &cells, int alpha_mode, int waves);  // waves per block: 1, 2, 4 or 8
+// pair forms of og_launch_pair
+enum { OG_FORM_ZD = 0, OG_FORM_ZD_ALPHA = 1, OG_FORM_HPS_DH = 2 };
+void og_launch_pair(const og_launch &cells, int form, int waves);  // waves per block: 1, 2, 4 or 8
 void og_launch_forces_to_work(const og_launch &atoms);
 void og_launch_energy_reduce(const og_launch &sys);
 // integration

@@ -28,6 +28,14 @@
 typedef float og_f4 __attribute__((ext_vector_type(4)));
 typedef float og_f2 __attribute__((ext_vector_type(2)));
 
+// Pair forms (the FORM template argument): 0 = LJ + zero-dipole at alpha 0,
+// 1 = LJ + zero-dipole at alpha > 0 (cal_pair, cuda_setup.cu:487-521),
+// 2 = hydrophobicity-scale LJ + Debye-Hueckel (cal_pair_hps_dh,
+// cuda_setup.cu:1273-1332).  The LJ table entry is {A6, A12} for forms 0 and
+// 1 and {A6, A12, HPS switch distance, HPS lambda} for form 2.
+template <int FORM> struct og_ljt { typedef og_f2 type; };
+template <> struct og_ljt<2> { typedef og_f4 type; };
+
 // erfc(x) for x >= 0 from Abramowitz & Stegun 7.1.26 (|abs error| <= 1.5e-7),
 // sharing exp(-x^2) with the force term.
 __device__ __forceinline__ float og_erfc_as(float x, float expmx2) {
@@ -89,17 +97,25 @@ __device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, og
 // One pass over up to 64 queued rows (lane = row; lanes >= cnt idle).
 // m8: the row's excluded pairs (bit i set = skip atom i).  LJ_LDS: the
 // cell1's types are rows of the block's LDS copy of the LJ table.
-template <int ALPHA, bool ENERGY, bool LJ_LDS>
+template <int FORM, bool ENERGY, bool LJ_LDS>
 __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, const og_f4 pj, const unsigned tj,
                                         const unsigned m8, const int a2, const bool live,
-                                        const og_f2 *__restrict__ ljl, const OG_GLOBAL(og_f2) *__restrict__ LJP,
+                                        const typename og_ljt<FORM>::type *__restrict__ ljl,
+                                        const OG_GLOBAL(typename og_ljt<FORM>::type) *__restrict__ LJP,
                                         float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
                                         double &ee, double &vir, og_jpend &pend) {
     const float rc2 = s.cutoff2;
     const float zcore = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
     const float alpha = s.alpha, piewald = s.piewald;
+    const float hps_eps = s.hps_eps, kappa = s.dh_kappa;
+    const bool  dh_fix_on = s.dh_grad_fix != 0;
     float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, fe = 0.0f, fq = 0.0f, fv = 0.0f;
     float px = pj.x, py = pj.y, pz = pj.z, pw = pj.w;
+    // the row's eight LJ pairs {A6, A12} load together, ahead of their use
+    typename og_ljt<FORM>::type ljv[OG_NAC];
+    int   npl = 0;  // in-cutoff unmasked pairs of this lane (summed over the wave with the energies)
+#pragma unroll
+    for (int i = 0; i < OG_NAC; i++) ljv[i] = LJ_LDS ? ljl[c.row[i] + tj] : LJP[c.row[i] + tj];
 #pragma unroll
     for (int i = 0; i < OG_NAC; i++) {
         const float dx = c.x[i] - px;
@@ -107,18 +123,28 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         const float dz = c.z[i] - pz;
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
-        npair += __builtin_popcountll(__ballot(ok));
-        const og_f2 lj = LJ_LDS ? ljl[c.row[i] + tj] : LJP[c.row[i] + tj];  // {A6, A12}
+        npl += ok ? 1 : 0;
+        const typename og_ljt<FORM>::type lj = ljv[i];  // {A6, A12(, HPS switch, HPS lambda)}
         const float rinv  = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
         const float t6    = lj.x * r6inv;
         const float t12   = lj.y * r6inv * r6inv;
         const float cc    = c.q[i] * pw;
-        float       g, eel;
-        if (ALPHA == 0) {
+        float       g, eel, elj = t12 - t6;
+        if constexpr (FORM == 0) {
             eel = cc * (rinv - zcore + bcoeff * r2);
             g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv - fcoeff);
+        } else if constexpr (FORM == 2) {
+            // HPS: beyond the pair's switch distance lambda * LJ, inside it LJ + (1 - lambda) eps;
+            // Debye-Hueckel: cc (charges x 332.06378 / eps_r) exp(-kappa r) / r, gradient coefficient
+            // -E (1/r + kappa) as the reference (times 1/r with the gradient fix)
+            const float r   = r2 * rinv;
+            const bool  far = r > lj.z;
+            const float sc  = far ? lj.w : 1.0f;
+            elj             = fmaf(elj, sc, far ? 0.0f : (1.0f - lj.w) * hps_eps);
+            eel             = cc * rinv * __expf(-r * kappa);
+            g = sc * r2inv * (6.0f * t6 - 12.0f * t12) - eel * (rinv + kappa) * (dh_fix_on ? rinv : 1.0f);
         } else {
             const float r   = r2 * rinv;
             const float ar  = alpha * r;
@@ -136,7 +162,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         fi[3 * i + 1]     = fmaf(g, dy, fi[3 * i + 1]);
         fi[3 * i + 2]     = fmaf(g, dz, fi[3 * i + 2]);
         if (ENERGY) {
-            fe += ok ? (t12 - t6) : 0.0f;
+            fe += ok ? elj : 0.0f;
             fq += ok ? eel : 0.0f;
             fv = fmaf(g, r2, fv);
         }
@@ -146,6 +172,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
                      "+v"(px), "+v"(py), "+v"(pz), "+v"(pw));
         if constexpr (ENERGY) asm volatile("" : "+v"(fe), "+v"(fq), "+v"(fv));
     }
+    npair += npl;
     pend.x    = fjx;
     pend.y    = fjy;
     pend.z    = fjz;
@@ -163,10 +190,11 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
 
 // Pass over queue entries [q0, q0 + cnt): reads the rows, adds the previous
 // pass's j-gradient, then evaluates the rows.
-template <int ALPHA, bool ENERGY, bool LJ_LDS>
+template <int FORM, bool ENERGY, bool LJ_LDS>
 __device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, const og_f4 *qp, const int2 *qa,
-                                        const int q0, const int cnt, const og_f2 *ljl,
-                                        const OG_GLOBAL(og_f2) *__restrict__ LJP, OG_GLOBAL(long long) *FX,
+                                        const int q0, const int cnt, const typename og_ljt<FORM>::type *ljl,
+                                        const OG_GLOBAL(typename og_ljt<FORM>::type) *__restrict__ LJP,
+                                        OG_GLOBAL(long long) *FX,
                                         float (&fi)[3 * OG_NAC], int &npair, float &vmax, float &vsum, double &ev,
                                         double &ee, double &vir, og_jpend &pend) {
     const int  lane = threadIdx.x & 63;
@@ -183,7 +211,7 @@ __device__ __forceinline__ void og_pass(const og_dev_sys &s, const og_cell1 &c, 
     // (s_waitcnt vmcnt(0); expcnt and lgkmcnt left at their maxima)
     __builtin_amdgcn_s_waitcnt(0x0F70);
     og_flush_j(FX, s.cap_atom_array, pend);
-    og_rows<ALPHA, ENERGY, LJ_LDS>(s, c, pj, tj, m8, ax.x, live, ljl, LJP, fi, npair, vmax, vsum, ev, ee, vir, pend);
+    og_rows<FORM, ENERGY, LJ_LDS>(s, c, pj, tj, m8, ax.x, live, ljl, LJP, fi, npair, vmax, vsum, ev, ee, vir, pend);
 }
 
 // a <- the kept half of (a, b) summed with the partner lane's other half
@@ -197,15 +225,18 @@ __device__ __forceinline__ float og_keep_add(float a, float b, bool upper, int o
 // One cell1 (one wave).  ENERGY = false on steps whose energies are not
 // logged: forces and the pair count only.  LJ_LDS: the cell1's LJ rows are in
 // the block's LDS copy ljl of the table's leading rows.
-template <int ALPHA, bool ENERGY, bool LJ_LDS>
+template <int FORM, bool ENERGY, bool LJ_LDS>
 __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, const og_f4 *__restrict__ shift,
-                                             const og_f2 *__restrict__ ljl, og_f4 *qp, int2 *qa) {
+                                             const typename og_ljt<FORM>::type *__restrict__ ljl, og_f4 *qp,
+                                             int2 *qa) {
     const int lane = threadIdx.x & 63;
     const int jl   = lane & 7;
 
     const OG_GLOBAL(og_f4) *__restrict__ X      = (const OG_GLOBAL(og_f4) *)s.gxyzq;
     const OG_GLOBAL(int) *__restrict__ TY       = (const OG_GLOBAL(int) *)s.gtype;
-    const OG_GLOBAL(og_f2) *__restrict__ LJP    = (const OG_GLOBAL(og_f2) *)s.lj_pair;
+    typedef typename og_ljt<FORM>::type ljt;
+    const OG_GLOBAL(ljt) *__restrict__ LJP =
+        (const OG_GLOBAL(ljt) *)(FORM == 2 ? (const void *)s.lj_hps : (const void *)s.lj_pair);
     const OG_GLOBAL(unsigned) *__restrict__ CP  = (const OG_GLOBAL(unsigned) *)s.cp_c2;
     const OG_GLOBAL(uint64_t) *__restrict__ CM  = (const OG_GLOBAL(uint64_t) *)s.cp_mask;
     OG_GLOBAL(long long) *__restrict__ FX       = (OG_GLOBAL(long long) *)s.fx;
@@ -234,7 +265,8 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     {
         const og_f4 p  = X[c1 * OG_NAC + jl];
         const int   ty = TY[c1 * OG_NAC + jl];
-        const float qc = p.w * OG_CHARGE_COEFF;
+        // charge x 332.06378 (x 1/eps_r under Debye-Hueckel)
+        const float qc = FORM == 2 ? p.w * OG_CHARGE_COEFF * s.dh_diel_inv : p.w * OG_CHARGE_COEFF;
 #pragma unroll
         for (int i = 0; i < OG_NAC; i++) {
             c.x[i]   = og_rlane(p.x, i);
@@ -264,7 +296,7 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
 #pragma unroll
     for (int k = 0; k < 3 * OG_NAC; k++) fi[k] = 0.0f;
     double ev = 0.0, ee = 0.0, vir = 0.0;
-    int    npair = 0;                    // in-cutoff unmasked pairs of this cell1 (wave-uniform)
+    int    npair = 0;                    // in-cutoff unmasked pairs of this cell1 (per lane until the reduction)
     float  vmax = 0.0f, vsum = 0.0f;     // j-gradient range check
     int    qn = 0, qd = 0;               // rows queued / rows evaluated (wave-uniform)
     og_jpend pend{0.0f, 0.0f, 0.0f, 0, false};
@@ -299,7 +331,7 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         }
         qn += __builtin_popcountll(ub);
         if (qn - qd >= 64) {
-            og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, 64, ljl, LJP, FX, fi, npair, vmax, vsum, ev, ee, vir, pend);
+            og_pass<FORM, ENERGY, LJ_LDS>(s, c, qp, qa, qd, 64, ljl, LJP, FX, fi, npair, vmax, vsum, ev, ee, vir, pend);
             qd += 64;
         }
         cp    = cpn;
@@ -307,7 +339,7 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         valid = validn;
     }
     if (qn > qd)
-        og_pass<ALPHA, ENERGY, LJ_LDS>(s, c, qp, qa, qd, qn - qd, ljl, LJP, FX, fi, npair, vmax, vsum, ev, ee, vir, pend);
+        og_pass<FORM, ENERGY, LJ_LDS>(s, c, qp, qa, qd, qn - qd, ljl, LJP, FX, fi, npair, vmax, vsum, ev, ee, vir, pend);
     og_flush_j(FX, cap, pend);
 
     // i-gradients: reduce-scatter the 24 partial sums over the wave.  After
@@ -332,6 +364,7 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         ev += __shfl_xor(ev, off, 64);
         ee += __shfl_xor(ee, off, 64);
         vir += __shfl_xor(vir, off, 64);
+        npair += __shfl_xor(npair, off, 64);
     }
     if (lane == 0) {
         OG_GLOBAL(long long) *A = (OG_GLOBAL(long long) *)s.e_acc + (c1 % OG_EACC_SLOTS) * OG_EACC_STRIDE;
@@ -348,10 +381,11 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
 // oxygen and hydrogen rows against every type).  A cell1 whose types are all
 // in it reads LJ pairs from LDS, otherwise from the global table.  The block
 // size is a launch shape the autotuner chooses (results do not depend on it).
-template <int ALPHA, int WAVES>
+template <int FORM, int WAVES>
 __global__ void __launch_bounds__(64 * WAVES) k_pair(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    typedef typename og_ljt<FORM>::type ljt;
     __shared__ og_f4 shift[64];              // image code -> shift of cell2
-    __shared__ og_f2 ljl[OG_LJ_LDS_ENTRIES];
+    __shared__ ljt   ljl[OG_LJ_LDS_ENTRIES];
     __shared__ og_f4 qpos[WAVES][OG_QCAP];  // row queue: shifted x, y, z, charge
     __shared__ int2  qaux[WAVES][OG_QCAP];  // grid slot, LJ column | mask byte << 24
     const int2        bm = map[blockIdx.x];
@@ -367,18 +401,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_pair(const og_dev_sys *__restric
         shift[img] = og_f4{(img & 1u) ? -L0 : ((img & 2u) ? L0 : 0.0f), (img & 4u) ? -L1 : ((img & 8u) ? L1 : 0.0f),
                            (img & 16u) ? -L2 : ((img & 32u) ? L2 : 0.0f), 0.0f};
     }
-    for (int k = threadIdx.x; k < rows * nt; k += 64 * WAVES) ljl[k] = ((const og_f2 *)s.lj_pair)[k];
+    const ljt *tab = (const ljt *)(FORM == 2 ? (const void *)s.lj_hps : (const void *)s.lj_pair);
+    for (int k = threadIdx.x; k < rows * nt; k += 64 * WAVES) ljl[k] = tab[k];
     __syncthreads();
     const int w  = threadIdx.x >> 6;
     const int c1 = __builtin_amdgcn_readfirstlane(bm.y * WAVES + w);
     if (c1 >= s.cnt->n_cells) return;
     const bool lds = __builtin_amdgcn_readfirstlane(__all(s.gtype[c1 * OG_NAC + (threadIdx.x & 7)] < rows)) != 0;
     if (og_energy_step(s)) {
-        if (lds) og_pair_cell<ALPHA, true, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
-        else og_pair_cell<ALPHA, true, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        if (lds) og_pair_cell<FORM, true, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        else og_pair_cell<FORM, true, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
     } else {
-        if (lds) og_pair_cell<ALPHA, false, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
-        else og_pair_cell<ALPHA, false, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        if (lds) og_pair_cell<FORM, false, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        else og_pair_cell<FORM, false, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
     }
 }
 
@@ -410,19 +445,21 @@ __global__ void __launch_bounds__(64) k_energy_reduce(const og_dev_sys *__restri
 
 // ------------------------------------------------------------- launchers --
 template <int WAVES>
-static void og_launch_pair_w(const og_launch &a, int alpha_mode) {
-    if (alpha_mode)
+static void og_launch_pair_w(const og_launch &a, int form) {
+    if (form == OG_FORM_HPS_DH)
+        hipLaunchKernelGGL((k_pair<2, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
+    else if (form == OG_FORM_ZD_ALPHA)
         hipLaunchKernelGGL((k_pair<1, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
     else
         hipLaunchKernelGGL((k_pair<0, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
 }
 
-void og_launch_pair(const og_launch &a, int alpha_mode, int waves) {
+void og_launch_pair(const og_launch &a, int form, int waves) {
     switch (waves) {
-    case 1: og_launch_pair_w<1>(a, alpha_mode); break;
-    case 2: og_launch_pair_w<2>(a, alpha_mode); break;
-    case 8: og_launch_pair_w<8>(a, alpha_mode); break;
-    default: og_launch_pair_w<4>(a, alpha_mode); break;
+    case 1: og_launch_pair_w<1>(a, form); break;
+    case 2: og_launch_pair_w<2>(a, form); break;
+    case 8: og_launch_pair_w<8>(a, form); break;
+    default: og_launch_pair_w<4>(a, form); break;
     }
 }
 void og_launch_forces_to_work(const og_launch &a) {

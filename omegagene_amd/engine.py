@@ -24,6 +24,8 @@ NTERMS = len(TERMS)
 INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
 THERMOSTATS = {"none": 0, "scaling": 1}
+NONBONDS = {"lennard-jones": 0, "hydrophobicity-scale-lj": 1}     # Config.cpp:333-339
+ELECTROSTATICS = {"zero-dipole": 0, "debye-huckel": 1}          # Config.cpp:114-128 (the device forms)
 
 
 class EngineError(RuntimeError):
@@ -48,7 +50,8 @@ class Desc(C.Structure):
                 ("vc_n_states", C.c_int), ("vc_min", C.POINTER(C.c_double)), ("vc_max", C.POINTER(C.c_double)),
                 ("vc_q_cano", C.POINTER(C.c_double)), ("vc_init_state", C.c_int), ("vc_interval", C.c_int),
                 ("vc_n_a", C.c_int), ("vc_n_b", C.c_int), ("vc_atoms_a", C.POINTER(C.c_int)),
-                ("vc_atoms_b", C.POINTER(C.c_int))]
+                ("vc_atoms_b", C.POINTER(C.c_int)), ("hps_cutoff", C.POINTER(C.c_double)),
+                ("hps_lambda", C.POINTER(C.c_double))]
 
 
 class Params(C.Structure):
@@ -59,7 +62,10 @@ class Params(C.Structure):
                 ("com_cancel", C.c_int), ("gap_bug", C.c_int), ("excess_bug", C.c_int), ("log_stride", C.c_int),
                 ("log_cap", C.c_int), ("capacity_factor", C.c_double), ("enhance_sigma", C.c_double),
                 ("enhance_recov_coef", C.c_double), ("split_step", C.c_int), ("thermostat", C.c_int),
-                ("berendsen_tau", C.c_double), ("temperature_init", C.c_double), ("heating_steps", C.c_int)]
+                ("berendsen_tau", C.c_double), ("temperature_init", C.c_double), ("heating_steps", C.c_int),
+                ("nonbond", C.c_int), ("hps_epsilon", C.c_double), ("electrostatic", C.c_int),
+                ("dh_dielectric", C.c_double), ("dh_ionic_strength", C.c_double), ("dh_temperature", C.c_double),
+                ("dh_grad_fix", C.c_int)]
 
 
 def lib():
@@ -110,6 +116,14 @@ def params_from_cfg(cfg: dict) -> Params:
     p.berendsen_tau = float(g("berendsen-tau", 0.0)) * 1000.0       # ps -> fs (Config.cpp:162-164)
     p.temperature_init = float(g("temperature-init", -1.0))         # Config.cpp:166-167
     p.heating_steps = int(g("heating-steps", 0))                    # Config.cpp:169-170
+    p.nonbond = NONBONDS[g("nonbond", "lennard-jones")]              # Config.cpp:333-339
+    p.hps_epsilon = float(g("hydrophobicity-scale-epsiron", 0.0))   # Config.cpp:340-342
+    p.electrostatic = ELECTROSTATICS[g("electrostatic", "zero-dipole")]
+    p.dh_dielectric = float(g("debye-huckel-dielectric", 0.0))     # Config.cpp:327-332
+    p.dh_ionic_strength = float(g("debye-huckel-ionic-strength", 0.0))
+    p.dh_temperature = float(g("debye-huckel-temperature", 0.0))
+    if "dh-grad-fix" in cfg:
+        p.dh_grad_fix = int(cfg["dh-grad-fix"])
     return p
 
 
@@ -165,6 +179,9 @@ def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcm
     x.com_group_atoms = d(np.concatenate([np.asarray(g_, np.int32) for g_ in groups]) if groups else
                           np.zeros(1, np.int32), np.int32)
     x.d_free = ls.d_free(constraint != 0)
+    if ls.hps_cutoff is not None:
+        x.hps_cutoff = d(ls.hps_cutoff)
+        x.hps_lambda = d(ls.hps_lambda)
     vc = ls.vcmd if vcmd else None
     if vc is not None:
         # 1-D VcMD on the distance between the mass centres of the dimension's two groups (Read.cpp:705-779)

@@ -70,9 +70,9 @@ def _mimg(d, L):
     return d - L * np.round(d / L)
 
 
-def _chain_walk(rng, n_beads, L, min_sep=3.6, max_tries=400):
-    """Self-avoiding walk (minimum image) with bond 1.53 A and bend <= 70 deg;
-    beads separated by 4+ bonds stay >= min_sep apart."""
+def _chain_walk(rng, n_beads, L, min_sep=3.6, max_tries=400, bond=BOND_R0):
+    """Self-avoiding walk (minimum image) with the given bond (1.53 A) and bend
+    <= 70 deg; beads separated by 4+ bonds stay >= min_sep apart."""
     pos = np.zeros((n_beads, 3))
     pos[0] = rng.uniform(0, L, 3)
     prev_dir = None
@@ -85,7 +85,7 @@ def _chain_walk(rng, n_beads, L, min_sep=3.6, max_tries=400):
             u /= np.linalg.norm(u)
             if prev_dir is not None and np.dot(u, prev_dir) < math.cos(math.radians(70.0)):
                 continue
-            p = pos[i - 1] + BOND_R0 * u
+            p = pos[i - 1] + bond * u
             if i >= 4:
                 d = _mimg(pos[: i - 3] - p, L)
                 if np.min(np.einsum("ij,ij->i", d, d)) < min_sep * min_sep:
@@ -213,6 +213,61 @@ def build_system(n_waters: int, seed: int, L: Optional[float] = None, n_beads: i
     ls.vel = maxwell_velocities(ls, T, rng)
     if with_vcmd and n_beads:
         ls.vcmd = vcmd_windows(ls, n_windows=8, seed=seed)
+    return ls
+
+
+# coarse-grained chains for the HPS + Debye-Hueckel pair form (one bead per
+# residue, as the reference's cg_q8 sample): bond 3.8 A with k = 10
+# kcal/mol/A^2, bead mass 110 amu, HPS epsilon 0.2 kcal/mol
+CG_BOND_R0, CG_BOND_K, CG_MASS, HPS_EPS = 3.8, 10.0, 110.0, 0.2
+
+
+def build_cg_system(n_chains: int, n_beads: int, seed: int, L: float, n_types: int = 20,
+                    T: float = 300.0) -> LaunchSet:
+    """Coarse-grained chains in a cubic box of edge L: per-type sigma
+    ~ U(4.5, 7) A and hydrophobicity ~ U(0, 1); types 0-2 carry +1, types 3-5
+    carry -1, the rest are neutral.  LJ A6 = 4 eps s^6, A12 = 4 eps s^12 at
+    eps = HPS_EPS with s the mean sigma of the pair; the HPS section holds
+    cutoff 2^(1/6) s and lambda the mean hydrophobicity.  Exclusions: bonded
+    neighbours (1-2)."""
+    rng = np.random.default_rng(seed)
+    pos = []
+    for _ in range(n_chains):
+        for _ in range(200):
+            c = _chain_walk(rng, n_beads, L, min_sep=4.5, bond=CG_BOND_R0)
+            if not pos:
+                break
+            prev = np.concatenate(pos, 0)
+            d = _mimg(c[:, None, :] - prev[None], L)
+            if np.min(np.einsum("ijk,ijk->ij", d, d)) >= 4.5 ** 2:
+                break
+        else:
+            raise RuntimeError("chain placement failed")
+        pos.append(c)
+    crd = np.mod(np.concatenate(pos, 0), L)
+    crd[crd >= L] -= L
+    n = len(crd)
+    sig_t = rng.uniform(4.5, 7.0, n_types)
+    lam_t = rng.uniform(0.0, 1.0, n_types)
+    q_t = np.zeros(n_types)
+    q_t[0:3], q_t[3:6] = 1.0, -1.0
+    atom_type = rng.integers(0, n_types, n).astype(np.int32)
+    s_ij = 0.5 * (sig_t[:, None] + sig_t[None, :])
+    lj6 = 4.0 * HPS_EPS * s_ij ** 6
+    lj12 = 4.0 * HPS_EPS * s_ij ** 12
+    box = np.zeros(12)
+    box[0] = box[4] = box[8] = L
+    box[9:12] = 0.5 * L
+    ls = LaunchSet(box=box, crd=crd, vel=np.zeros_like(crd), charge=q_t[atom_type].copy(), mass=np.full(n, CG_MASS),
+                   atom_type=atom_type, n_lj_types=n_types, lj6=lj6, lj12=lj12,
+                   hps_cutoff=2.0 ** (1.0 / 6.0) * s_ij, hps_lambda=0.5 * (lam_t[:, None] + lam_t[None, :]))
+    bi = np.concatenate([np.arange(n_beads - 1) + k * n_beads for k in range(n_chains)]).astype(np.int32)
+    ls.bond_ids = np.stack([bi, bi + 1], 1)
+    ls.bond_k = np.full(len(bi), CG_BOND_K)
+    ls.bond_r0 = np.full(len(bi), CG_BOND_R0)
+    ls.nb15off = ls.bond_ids.copy()
+    ls.groups = [("all", np.arange(n, dtype=np.int32))]
+    ls.vel = maxwell_velocities(ls, T, rng)
     return ls
 
 

@@ -43,6 +43,56 @@ class ZD(C.Structure):
                 ("zcore", C.c_float), ("piewald", C.c_float)]
 
 
+class FF(C.Structure):
+    """orc_ff_t: the HPS / Debye-Hueckel options of the pair form (the two
+    tables are kept alive on the Python object)."""
+    _fields_ = [("hps", C.c_int), ("dh", C.c_int), ("dh_grad_fix", C.c_int), ("hps_eps", C.c_float),
+                ("dielectric_inv", C.c_float), ("debye_inv", C.c_float), ("hps_cutoff", C.POINTER(C.c_float)),
+                ("hps_lambda", C.POINTER(C.c_float))]
+
+
+def ff_setup(hps_cutoff, hps_lambda, hps_eps, dielectric, ionic_strength, temperature, hps=True, dh=True,
+             dh_grad_fix=False) -> FF:
+    """orc_dh_setup + the tables: HPS LJ switch and Debye-Hueckel electrostatics."""
+    ff = FF()
+    ff._hc = np.ascontiguousarray(hps_cutoff, np.float32).reshape(-1)
+    ff._hl = np.ascontiguousarray(hps_lambda, np.float32).reshape(-1)
+    ff.hps, ff.dh, ff.dh_grad_fix = int(hps), int(dh), int(dh_grad_fix)
+    ff.hps_eps = float(np.float32(hps_eps))
+    ff.hps_cutoff = ff._hc.ctypes.data_as(C.POINTER(C.c_float))
+    ff.hps_lambda = ff._hl.ctypes.data_as(C.POINTER(C.c_float))
+    if dh:
+        lib().orc_dh_setup(C.byref(ff), C.c_double(dielectric), C.c_double(ionic_strength), C.c_double(temperature))
+    return ff
+
+
+def pair_ff_fp32(zd, ff, d12, t, a6, a12, q1, q2):
+    d = np.ascontiguousarray(d12, np.float32)
+    ev, ee = C.c_float(), C.c_float()
+    w = np.zeros(3, np.float32)
+    n = lib().orc_pair_ff_fp32(C.byref(zd), C.byref(ff) if ff is not None else None, _p(d, C.c_float), C.c_int(t),
+                               C.c_float(a6), C.c_float(a12), C.c_float(q1), C.c_float(q2), C.byref(ev),
+                               C.byref(ee), _p(w, C.c_float))
+    return n, ev.value, ee.value, w
+
+
+def allpairs_ff(zd, ff, L, crd, atom_type, lj6, lj12, charge, nb15off_table):
+    """orc_pairs_allpairs_ff: the all-pairs loop with the pair form of ff."""
+    n = len(crd)
+    work = np.zeros((n, 3))
+    en = np.zeros(2)
+    f = lib().orc_pairs_allpairs_ff
+    f.restype = C.c_int64
+    npair = f(C.byref(zd), C.byref(ff), _p(np.ascontiguousarray(L, np.float32), C.c_float), C.c_int(n),
+              _p(np.ascontiguousarray(crd, np.float64), C.c_double),
+              _p(np.ascontiguousarray(atom_type, np.int32), C.c_int), C.c_int(lj6.shape[0]),
+              _p(np.ascontiguousarray(lj6, np.float32), C.c_float),
+              _p(np.ascontiguousarray(lj12, np.float32), C.c_float),
+              _p(np.ascontiguousarray(charge, np.float32), C.c_float),
+              _p(np.ascontiguousarray(nb15off_table, np.int32), C.c_int), _p(work, C.c_double), _p(en, C.c_double))
+    return work, en, npair
+
+
 def build():
     """Compile liboracle.so (always) and, where the reference tree is mounted,
     oracle/_ref/ from the reference sources."""
@@ -132,6 +182,20 @@ class Oracle:
         f.restype = C.c_int64
         npair = f(C.byref(zd), C.byref(self.g), C.byref(self.c), cps.ctypes.data_as(C.c_void_p), C.c_int(len(cps)),
                   _p(np.ascontiguousarray(atom_type, np.int32), C.c_int), C.c_int(n_types),
+                  _p(np.ascontiguousarray(lj6, np.float32), C.c_float),
+                  _p(np.ascontiguousarray(lj12, np.float32), C.c_float),
+                  _p(np.ascontiguousarray(charge, np.float32), C.c_float), _p(work, C.c_double), _p(en, C.c_double))
+        return work, en, npair
+
+    def pairs_ff(self, zd, ff, cps, atom_type, lj6, lj12, charge):
+        """orc_pairs_celllist_ff: the cell-list loop with the pair form of ff (grid-order work)."""
+        work = np.zeros((self.c.n_atom_array, 3))
+        en = np.zeros(2)
+        cps = np.ascontiguousarray(cps)
+        f = lib().orc_pairs_celllist_ff
+        f.restype = C.c_int64
+        npair = f(C.byref(zd), C.byref(ff), C.byref(self.g), C.byref(self.c), cps.ctypes.data_as(C.c_void_p),
+                  C.c_int(len(cps)), _p(np.ascontiguousarray(atom_type, np.int32), C.c_int), C.c_int(lj6.shape[0]),
                   _p(np.ascontiguousarray(lj6, np.float32), C.c_float),
                   _p(np.ascontiguousarray(lj12, np.float32), C.c_float),
                   _p(np.ascontiguousarray(charge, np.float32), C.c_float), _p(work, C.c_double), _p(en, C.c_double))

@@ -113,6 +113,34 @@ int orc_excess_pairs(int n_atoms, const int *nb15off, int *out_pairs /* cap*2 */
 int orc_pair_fp32(const orc_zd_t *zd, const float d12[3], float a6, float a12, float q1, float q2,
                   float *e_vdw, float *e_ele, float work[3]);
 
+/* The other pair forms of ForceField::calc_pairwise (ForceField.cpp:333-418):
+ * hydrophobicity-scale LJ (--nonbond hydrophobicity-scale-lj, Config.cpp:333-342)
+ * and Debye-Hueckel electrostatics (--electrostatic debye-huckel,
+ * ZeroMultipoleSum.cpp:443-455, constants :24).  hps_cutoff / hps_lambda are the
+ * [n_types^2] float tables of the .cls "nbpair hps" section (Read.cpp:322-334). */
+typedef struct {
+    int          hps;          /* 1: HPS switch on the LJ term */
+    int          dh;           /* 1: Debye-Hueckel electrostatics (else zero-dipole) */
+    int          dh_grad_fix;  /* 0: the reference's grad_coeff = -E (1/r + 1/lambda_D), :453; 1: that / r */
+    float        hps_eps;      /* --hydrophobicity-scale-epsiron (passed as real_pw, SubBox.cpp:1048) */
+    float        dielectric_inv, debye_inv;  /* real_pw members of ZeroMultipoleSum */
+    const float *hps_cutoff, *hps_lambda;
+} orc_ff_t;
+/* dielectric_inv = 1/eps_r and the inverse Debye length in 1/A from the
+ * double cfg values, as the ZeroMultipoleSum constructor (ZeroMultipoleSum.cpp:21-24) */
+void orc_dh_setup(orc_ff_t *ff, double dielectric, double ionic_strength, double temperature);
+/* one pair with the form of ff (ff == NULL: orc_pair_fp32); t = type pair index */
+int orc_pair_ff_fp32(const orc_zd_t *zd, const orc_ff_t *ff, const float d12[3], int t, float a6, float a12,
+                     float q1, float q2, float *e_vdw, float *e_ele, float work[3]);
+int64_t orc_pairs_celllist_ff(const orc_zd_t *zd, const orc_ff_t *ff, const orc_grid_t *g, const orc_cells_t *c,
+                              const orc_cellpair_t *cps, int n_cp, const int *atom_type, int n_types,
+                              const float *lj6, const float *lj12, const float *charge,
+                              double *work_grid, double energy[2]);
+int64_t orc_pairs_allpairs_ff(const orc_zd_t *zd, const orc_ff_t *ff, const float L[3], int n_atoms,
+                              const double *crd, const int *atom_type, int n_types, const float *lj6,
+                              const float *lj12, const float *charge, const int *nb15off, double *work,
+                              double energy[2]);
+
 /* Pair loop over a cell-pair list (SubBox.cpp:956-1125): FP32 pair terms,
  * FP64 accumulation in list order.  work_grid[n_atom_array*3] (grid order,
  * zeroed here), energy[2] = {vdw, ele}.  Returns in-cutoff pair count. */

@@ -112,6 +112,73 @@ int orc_pair_fp32(const orc_zd_t *zd, const float d12[3], float a6, float a12, f
     return 1;
 }
 
+/* CelesteObject.cpp:32-40 */
+static const double ELEM_CHARGE  = 1.60217657e-19;
+static const double AVOGADRO     = 6.0221413e23;
+static const double PERMITTIVITY = 8.85418782e-12;
+static const double BOLTZMAN     = 1.380658e-23;
+
+void orc_dh_setup(orc_ff_t *ff, double dielectric, double ionic_strength, double temperature) {
+    ff->dielectric_inv = (float)(1.0 / dielectric);
+    ff->debye_inv      = (float)(1.0 / (sqrt(PERMITTIVITY * dielectric * BOLTZMAN * temperature /
+                                             (2 * AVOGADRO * ELEM_CHARGE * ELEM_CHARGE * ionic_strength)) *
+                                        1e10));
+}
+
+/* ForceField::calc_pairwise (ForceField.cpp:333-418) with the HPS switch
+ * (:380-391) and ZeroMultipoleSum::calc_debye_huckel (ZeroMultipoleSum.cpp:443-455):
+ * float locals; exp of a float is the float overload. */
+int orc_pair_ff_fp32(const orc_zd_t *zd, const orc_ff_t *ff, const float d12[3], int t, float a6, float a12,
+                     float q1, float q2, float *e_vdw, float *e_ele, float work[3]) {
+    if (!ff || (!ff->hps && !ff->dh)) return orc_pair_fp32(zd, d12, a6, a12, q1, q2, e_vdw, e_ele, work);
+    float r12_2 = d12[0] * d12[0] + d12[1] * d12[1] + d12[2] * d12[2];
+    float r12   = sqrtf(r12_2);
+    *e_vdw = *e_ele = 0.0f;
+    work[0] = work[1] = work[2] = 0.0f;
+    if (r12 >= zd->cutoff) return 0;
+    float r12_inv    = (float)(1.0 / (double)r12);
+    float r12_2_inv  = (float)(1.0 / (double)r12_2);
+    float r12_3_inv  = r12_inv * r12_2_inv;
+    float r12_6_inv  = r12_3_inv * r12_3_inv;
+    float r12_12_inv = r12_6_inv * r12_6_inv;
+    float term6      = a6 * r12_6_inv;
+    float term12     = a12 * r12_12_inv;
+    float ev         = -term6 + term12;
+    float work_coef  = (float)((double)r12_2_inv * (-12.0 * (double)term12 + 6.0 * (double)term6));
+    if (ff->hps) {
+        const float hc = ff->hps_cutoff[t], hl = ff->hps_lambda[t];
+        if (r12 > hc) {
+            ev *= hl;
+            work_coef *= hl;
+        } else {
+            ev += (1 - hl) * ff->hps_eps;
+        }
+    }
+    *e_vdw   = ev;
+    float cc = (float)((double)(q1 * q2) * CHARGE_COEFF);
+    float g_ele;
+    if (ff->dh) {
+        float ld_exp = expf(-r12 * ff->debye_inv);
+        *e_ele       = cc * r12_inv * ff->dielectric_inv * ld_exp;
+        g_ele        = -*e_ele * (r12_inv + ff->debye_inv);
+        if (ff->dh_grad_fix) g_ele *= r12_inv;
+    } else if (zd->alpha < EPS_REF) {
+        *e_ele = cc * (r12_inv - zd->zcore + zd->bcoeff * r12_2);
+        g_ele  = -cc * (r12_3_inv - zd->fcoeff);
+    } else {
+        float tmp     = r12 * zd->alpha;
+        float errorfc = erfcf(tmp);
+        *e_ele        = cc * (r12_inv * errorfc - zd->zcore + zd->bcoeff * r12_2);
+        g_ele = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff);
+    }
+    for (int d = 0; d < 3; d++) {
+        float wv = work_coef * d12[d];
+        float we = g_ele * d12[d];
+        work[d]  = wv + we;
+    }
+    return 1;
+}
+
 /* PBC::fix_pbc_image on a float coordinate (PBC.cpp:106-121). */
 static void fix_image(float *x, int image, const float L[3]) {
     if (image == 0) return;
@@ -128,6 +195,12 @@ int64_t orc_pairs_celllist(const orc_zd_t *zd, const orc_grid_t *g, const orc_ce
                            const orc_cellpair_t *cps, int n_cp, const int *atom_type, int n_types,
                            const float *lj6, const float *lj12, const float *charge, double *work,
                            double energy[2]) {
+    return orc_pairs_celllist_ff(zd, NULL, g, c, cps, n_cp, atom_type, n_types, lj6, lj12, charge, work, energy);
+}
+int64_t orc_pairs_celllist_ff(const orc_zd_t *zd, const orc_ff_t *ff, const orc_grid_t *g, const orc_cells_t *c,
+                              const orc_cellpair_t *cps, int n_cp, const int *atom_type, int n_types,
+                              const float *lj6, const float *lj12, const float *charge, double *work,
+                              double energy[2]) {
     int64_t n_in = 0;
     memset(work, 0, sizeof(double) * (size_t)c->n_atom_array * 3);
     energy[0] = energy[1] = 0.0;
@@ -147,7 +220,7 @@ int64_t orc_pairs_celllist(const orc_zd_t *zd, const orc_grid_t *g, const orc_ce
                 float d12[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
                 int   t = atom_type[id1] * n_types + atom_type[id2];
                 float ev, ee, w[3];
-                n_in += orc_pair_fp32(zd, d12, lj6[t], lj12[t], charge[id1], charge[id2], &ev, &ee, w);
+                n_in += orc_pair_ff_fp32(zd, ff, d12, t, lj6[t], lj12[t], charge[id1], charge[id2], &ev, &ee, w);
                 energy[0] += ev;
                 energy[1] += ee;
                 for (int d = 0; d < 3; d++) {
@@ -164,6 +237,11 @@ int64_t orc_pairs_celllist(const orc_zd_t *zd, const orc_grid_t *g, const orc_ce
 int64_t orc_pairs_allpairs_fp32(const orc_zd_t *zd, const float L[3], int n, const double *crd,
                                 const int *atom_type, int n_types, const float *lj6, const float *lj12,
                                 const float *charge, const int *nb15off, double *work, double energy[2]) {
+    return orc_pairs_allpairs_ff(zd, NULL, L, n, crd, atom_type, n_types, lj6, lj12, charge, nb15off, work, energy);
+}
+int64_t orc_pairs_allpairs_ff(const orc_zd_t *zd, const orc_ff_t *ff, const float L[3], int n, const double *crd,
+                              const int *atom_type, int n_types, const float *lj6, const float *lj12,
+                              const float *charge, const int *nb15off, double *work, double energy[2]) {
     int64_t n_in = 0;
     float   L_half[3];
     for (int d = 0; d < 3; d++) L_half[d] = (float)(L[d] * 0.5);
@@ -184,7 +262,7 @@ int64_t orc_pairs_allpairs_fp32(const orc_zd_t *zd, const float L[3], int n, con
             float d12[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
             int   t = atom_type[a1] * n_types + atom_type[a2];
             float ev, ee, w[3];
-            n_in += orc_pair_fp32(zd, d12, lj6[t], lj12[t], charge[a1], charge[a2], &ev, &ee, w);
+            n_in += orc_pair_ff_fp32(zd, ff, d12, t, lj6[t], lj12[t], charge[a1], charge[a2], &ev, &ee, w);
             energy[0] += ev;
             energy[1] += ee;
             for (int d = 0; d < 3; d++) {

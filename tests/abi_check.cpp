@@ -48,6 +48,12 @@ SAME(og_hostfree_atom_type_charge, int (*)(int *, real_pw *));
 SAME(og_hostalloc_atom_type_charge, int (*)(int **, real_pw **, const int));
 // SubBox.cpp:57-60 cuda_enumerate_cell_pairs(int *&, const int, const int, const int *)
 SAME(og_enumerate_cell_pairs, int (*)(int **, const int, const int, const int *));
+// SubBox.cpp:62-71 the HPS + Debye-Hueckel call points
+SAME(og_hps_constant, int (*)(real_pw));
+SAME(og_alloc_set_hps_params, int (*)(real_pw *, real_pw *, int));
+SAME(og_free_hps_params, int (*)());
+SAME(og_debye_huckel_constant, int (*)(real_pw, real_pw, real_pw));
+SAME(og_pairwise_hps_dh, int (*)(const bool));
 // MiniCell.cpp:8-12 cuda_hostalloc_atom_info(real_pw *&, int *&, real_fc *&, real_fc *&, int)
 SAME(og_hostalloc_atom_info, int (*)(real_pw **, int **, real_fc **, real_fc **, int));
 // MiniCell.cpp:13-18 cuda_hostalloc_cell_info(int *&, int)

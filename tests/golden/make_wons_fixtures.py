@@ -19,9 +19,18 @@
       constraints, so ThermostatScaling::apply_thermostat), --berendsen-tau
       0.1 ps, heating from 250 K to 300 K over 5 steps.
 
+  wons_hpsdh_grad.npz -- one leapfrog step without constraints of 8 coarse-
+      grained 60-bead chains (omegagene_amd.gen.build_cg_system) with
+      --nonbond hydrophobicity-scale-lj and --electrostatic debye-huckel
+      (cutoff 20 A, list 23 A, dielectric 85, 300 K, ionic strength 150,
+      HPS epsilon 0.2; the cg_q8 sample's settings with a shorter Debye
+      length); the gradient from the restart velocities as above.
+  wons_hpsdh_traj.npz -- 20 leapfrog steps of dt 5 fs of the same form at the
+      sample's ionic strength 0.00015; log and final restart.
+
 Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
 Run where the reference tree is mounted:
-    make -C oracle ref && python tests/golden/make_wons_fixtures.py
+    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps]
 """
 import os
 import re
@@ -115,10 +124,37 @@ def more(tmp):
         print(name, "ok", lg.shape, lg[0], lg[-1])
 
 
+HPS_DH = {"cutoff": 20.0, "nsgrid-cutoff": 23.0, "electrostatic": "debye-huckel", "debye-huckel-dielectric": 85.0,
+          "debye-huckel-temperature": 300.0, "nonbond": "hydrophobicity-scale-lj",
+          "hydrophobicity-scale-epsiron": 0.2, "constraint": "none", "ele-alpha": 0.0}
+
+
+def hps(tmp):
+    ls = gen.build_cg_system(8, 60, seed=3, L=90.0)
+    cfg = dict(BASE, **HPS_DH, **{"n-steps": 1, "time-step": 2.0, "debye-huckel-ionic-strength": 150.0})
+    data, log, rs = run(ls, cfg, tmp)
+    minv = (1.0 / ls.mass.astype(np.float32).astype(np.float64)).astype(np.float32).astype(np.float64)
+    work = -(rs["vel"] - ls.vel) / (2.0 * 4.184e-4 * minv[:, None])
+    np.savez_compressed(os.path.join(HERE, "wons_hpsdh_grad.npz"), cls=np.frombuffer(data, np.uint8), work=work,
+                        log=parse_log(log), crd1=rs["crd"], vel1=rs["vel"],
+                        cfg=np.frombuffer(gen.cfg_text(cfg).encode(), np.uint8))
+    print("hpsdh grad ok", parse_log(log)[0])
+    cfg = dict(BASE, **HPS_DH, **{"n-steps": 20, "time-step": 5.0, "debye-huckel-ionic-strength": 0.00015})
+    data, log, rs = run(ls, cfg, tmp)
+    lg = parse_log(log)
+    np.savez_compressed(os.path.join(HERE, "wons_hpsdh_traj.npz"), cls=np.frombuffer(data, np.uint8), log=lg,
+                        crd=rs["crd"], vel=rs["vel"], epot=rs["epot"], ekin=rs["ekin"], step=rs["step"],
+                        cfg=np.frombuffer(gen.cfg_text(cfg).encode(), np.uint8))
+    print("hpsdh traj ok", lg.shape, lg[0], lg[-1])
+
+
 def main():
     tmp = tempfile.mkdtemp()
     if len(sys.argv) > 1 and sys.argv[1] == "more":
         more(tmp)
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "hps":
+        hps(tmp)
         return
     for tag, alpha in (("a0", 0.0), ("a1", 0.1)):
         ls = gen.build_system(700, seed=77, n_beads=20)

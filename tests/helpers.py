@@ -80,13 +80,22 @@ def ele_scale(ls: LaunchSet, cutoff=12.0, alpha=0.0, nscut=13.0, crd=None):
     return float(en[1])
 
 
-def oracle_forces_allpairs(ls: LaunchSet, crd, zd, L, alpha_bug=True):
+def oracle_forces_allpairs(ls: LaunchSet, crd, zd, L, alpha_bug=True, ff=None):
     """The reference all-pairs build's force terms (SubBox.cpp:1127-1220 pairs,
     :1347-1367 excess, :1229-1249 bonds) with the oracle: work [n][3] and the
-    potential pieces (vdw, ele incl. excess and self, bond)."""
+    potential pieces (vdw, ele incl. excess and self, bond).  ff: the HPS /
+    Debye-Hueckel form (oracle.ff_setup); under Debye-Hueckel the excluded
+    pairs add nothing (calc_null, ZeroMultipoleSum.cpp:456-466) and the self
+    energy is zero (zmcore = 0 outside the zero-multipole modes, :75-82)."""
     lj6, lj12 = ls.lj_tables_f32()
     q = ls.charge.astype(np.float32)
-    work, en, _ = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, ls.nb15off_table())
+    if ff is not None:
+        work, en, _ = oracle.allpairs_ff(zd, ff, L, crd, ls.atom_type, lj6, lj12, q, ls.nb15off_table())
+        if ff.dh:
+            eb = oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, crd, work) if len(ls.bond_ids) else 0.0
+            return work, en[0], en[1], eb
+    else:
+        work, en, _ = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, ls.nb15off_table())
     ex = ls.excess_pairs()
     eex = oracle.excess(zd, L, ex, crd, q, work, bug_compat=alpha_bug)
     eb = oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, crd, work) if len(ls.bond_ids) else 0.0
@@ -103,8 +112,18 @@ def cfg_value(npz, key, default=None):
     return default
 
 
+def hps_dh_form(ls: LaunchSet, npz, dh_grad_fix=False):
+    """oracle.ff_setup from a fixture's .cfg keys (Config.cpp:327-342) and the launch set's HPS tables."""
+    hc, hl = ls.hps_tables_f32()
+    return oracle.ff_setup(hc, hl, float(cfg_value(npz, "hydrophobicity-scale-epsiron")),
+                           float(cfg_value(npz, "debye-huckel-dielectric")),
+                           float(cfg_value(npz, "debye-huckel-ionic-strength")),
+                           float(cfg_value(npz, "debye-huckel-temperature")), dh_grad_fix=dh_grad_fix)
+
+
 def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-presto", constraint="none",
-              tol=1e-10, max_loops=1000, com_groups=None, thermostat=None, langevin=None, extra_last=False):
+              tol=1e-10, max_loops=1000, com_groups=None, thermostat=None, langevin=None, extra_last=False,
+              ff=None):
     """The reference step loop with the oracle's functions, in the reference
     order:
       leapfrog-presto (DynamicsModePresto::calc_in_each_step, DynamicsMode.cpp:468-560):
@@ -118,6 +137,7 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
         position Langevin update (SubBox.cpp:2077-2114) with the reference
         noise stream, v by central difference (SubBox.cpp:2054-2066) and its
         KE; no constraints (DynamicsMode.cpp:877); PBC wrap.
+    ff: the HPS / Debye-Hueckel pair form (oracle.ff_setup), None for zero-dipole.
     thermostat: dict(tau_fs, temperature, temperature_init, heating_steps, d_free).
     langevin: dict(gamma_per_fs, temperature, seed).
     extra_last: the reference runs one more step after the last output
@@ -137,7 +157,7 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
     n_total = n_steps + (1 if extra_last else 0)
     n = ls.n_atoms
     for step in range(n_total):
-        work, evdw, eele, ebond = oracle_forces_allpairs(ls, crd, zd, L)
+        work, evdw, eele, ebond = oracle_forces_allpairs(ls, crd, zd, L, ff=ff)
         pot = evdw + eele + ebond
         if integrator == "leapfrog-presto":
             vel = vel_next

@@ -600,3 +600,73 @@ def test_ala3_degrees_of_freedom_and_initial_kinetic_energy():
     assert "%.8e" % ke == "%.8e" % exp["ke_init"]
     T = ke * 2 * 4184.0 / (8.31451 * dfree)
     assert T == pytest.approx(exp["t_init"], rel=1e-9)
+
+
+# ------------------------------------------------- HPS + Debye-Hueckel pair form
+
+def test_oracle_hpsdh_gradient_matches_reference_one_step():
+    """One leapfrog step of coarse-grained chains with --nonbond
+    hydrophobicity-scale-lj and --electrostatic debye-huckel in the reference
+    all-pairs build: the oracle's gradient to 1e-11 RMS and its VDW / Ele to
+    the printed digits."""
+    from helpers import hps_dh_form, oracle_forces_allpairs
+    d = load_npz("wons_hpsdh_grad.npz")
+    ls = cls_from_fixture(d)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    work, evdw, eele, eb = oracle_forces_allpairs(ls, crd, oracle.zd_setup(20.0, 0.0), L, ff=hps_dh_form(ls, d))
+    ref = d["work"]
+    assert rms_rel(work, ref) < 1e-11
+    assert np.max(np.abs(work - ref)) < 1e-8 * np.max(np.abs(ref))
+    log0 = d["log"][0]
+    assert fmt(evdw) == fmt(log0[4])
+    assert fmt(eele) == fmt(log0[5])
+    assert abs(eb - log0[3]) <= 1e-9 * max(1.0, abs(log0[3]))
+
+
+def test_oracle_hpsdh_md_matches_reference_trajectory():
+    """20 leapfrog steps (dt 5 fs) of the same form at the cg_q8 sample's
+    ionic strength: per-step energies and the final restart."""
+    from helpers import hps_dh_form, oracle_md
+    d = load_npz("wons_hpsdh_traj.npz")
+    ls = cls_from_fixture(d)
+    crd, vel, rows = oracle_md(ls, 20, dt=5.0, cutoff=20.0, ff=hps_dh_form(ls, d))
+    log = d["log"]
+    for r, lg in zip(rows, log[:20]):
+        assert abs(r[1] - lg[1]) <= 1e-9 * (abs(lg[4]) + abs(lg[5]) + abs(lg[3])) + 1e-8, (r, lg)
+        assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]), (r, lg)
+        assert fmt(r[4]) == fmt(lg[4])
+    assert np.max(np.abs(crd - d["crd"])) < 1e-9
+    assert np.max(np.abs(vel - d["vel"])) < 1e-11
+
+
+def test_hpsdh_two_particle_closed_form():
+    """Two beads: E_DH = 332.06378 q1 q2 exp(-kappa r) / (eps_r r) with
+    kappa = sqrt(2 N_A e^2 I / (eps_0 eps_r k_B T)) 1e-10 per A; HPS: beyond
+    the switch distance lambda times LJ, inside it LJ + (1 - lambda) eps.
+    The Debye-Hueckel gradient is the reference's -E (1/r + kappa) d
+    (ZeroMultipoleSum.cpp:453, r times the derivative); dh_grad_fix gives
+    -E (1/r + kappa) d / r, the central difference of E."""
+    eps_r, I, T = 85.0, 150.0, 300.0
+    kappa = np.sqrt(2 * 6.0221413e23 * 1.60217657e-19 ** 2 * I / (8.85418782e-12 * eps_r * 1.380658e-23 * T)) * 1e-10
+    hc = np.array([[6.0]], np.float32)
+    hl = np.array([[0.3]], np.float32)
+    a6, a12 = 4 * 0.2 * 5.5 ** 6, 4 * 0.2 * 5.5 ** 12
+    zd = oracle.zd_setup(20.0, 0.0)
+    for fix in (False, True):
+        ff = oracle.ff_setup(hc, hl, 0.2, eps_r, I, T, dh_grad_fix=fix)
+        assert ff.debye_inv == pytest.approx(kappa, rel=1e-6)
+        for r in (5.0, 6.5, 12.0):
+            d12 = np.array([r, 0.0, 0.0], np.float32)
+            n, ev, ee, w = oracle.pair_ff_fp32(zd, ff, d12, 0, a6, a12, 1.0, -1.0)
+            lj = a12 / r ** 12 - a6 / r ** 6
+            e_ref = -332.06378 * np.exp(-kappa * r) / (eps_r * r)
+            assert ee == pytest.approx(e_ref, rel=2e-6)
+            assert ev == pytest.approx(0.3 * lj if r > 6.0 else lj + 0.7 * 0.2, rel=2e-6, abs=1e-6)
+            # the electrostatic part of the x-gradient
+            glj = (-12 * a12 / r ** 14 + 6 * a6 / r ** 8) * (0.3 if r > 6.0 else 1.0)
+            g_ele = (float(w[0]) - np.float32(glj) * r) / r
+            dE = -e_ref * (1 / r + kappa)   # dE/dr
+            assert g_ele == pytest.approx(dE / r if fix else dE, rel=1e-4)
+        n, _, _, _ = oracle.pair_ff_fp32(zd, ff, np.array([20.0, 0, 0], np.float32), 0, a6, a12, 1.0, -1.0)
+        assert n == 0
