@@ -207,6 +207,10 @@ og_status og_get_work(og_system *sys, double *work_atom);
 /* State exchange (host arrays [n*3], atom order). */
 og_status og_get_state(og_system *sys, double *crd, double *vel_next, double *vel_just);
 og_status og_set_state(og_system *sys, const double *crd, const double *vel_next);
+/* positions before the last step's update [n*3]: what the reference writes to
+ * trajectories (SubBox::copy_crd_prev in DynamicsMode::sub_output,
+ * DynamicsMode.cpp:272-300) and to a Langevin restart (:215-218) */
+og_status og_get_crd_prev(og_system *sys, double *crd_prev);
 og_status og_get_energies(og_system *sys, double *energies /* [OG_NTERMS] */);
 /* energy log of the last run: rows of OG_NTERMS doubles, one per log_stride steps */
 og_status og_get_energy_log(og_system *sys, double *out, int max_rows, int *n_rows);
@@ -259,6 +263,26 @@ og_status og_set_segment(og_system *sys, long long n_steps);
  * advances all admitted segments by chunk_steps in one batch and retires
  * the finished ones; walkers stay resident on the device between segments.
  * Times are seconds on the server's clock: times[3] = {submit, admit, done}. */
+/* ---- on-disk outputs (og_io.cpp) ----
+ * WriteRestart::write_restart (WriteTrr.cpp:258-313): title record (80 bytes,
+ * zero padded), n_atoms twice, step, time, E_total = E_kin + E_pot, E_kin,
+ * E_pot, then crd and vel [n*3] as doubles, each in int-marked records. */
+og_status og_write_restart(const char *path, const char *title, int n_atoms, int n_steps, double time,
+                           double e_potential, double e_kinetic, const double *crd, const double *vel);
+/* WriteTrrPresto::write_trr (WriteTrr.cpp:179-254): one frame (header record
+ * of step and ten floats closed by a 0 marker as the reference writes it, then
+ * float xyz of every atom or of the n_group atoms in group); append = 0
+ * starts the file. */
+og_status og_write_trr_presto_frame(const char *path, int append, int n_atoms, int step, float time,
+                                    float cpu_time, float total_e, float kinetic_e, float temperature,
+                                    float potential_e, float vdw_e, const double *crd, int n_group,
+                                    const int *group);
+/* WriteTrrGromacs::write_trr (WriteTrr.cpp:33-172): GMX_trn_file frame in
+ * double precision, box and crd / vel / force (NULL = left out) in nm. */
+og_status og_write_trr_gromacs_frame(const char *path, int append, int n_atoms, int step, double time, double lx,
+                                     double ly, double lz, const double *crd, const double *vel, const double *force,
+                                     int n_group, const int *group);
+
 typedef struct og_server og_server;
 og_status og_server_create(uint64_t hbm_budget_bytes, int chunk_steps, og_server **out);
 og_status og_server_destroy(og_server *srv);

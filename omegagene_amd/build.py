@@ -31,6 +31,7 @@ SOURCES = [
     ("og_celeste.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ("og_server.cpp", ["-x", "hip"]),
     ("og_noalloc.hip", ["-ffp-contract=off"]),
+    ("og_io.cpp", ["-x", "hip"]),
 ]
 
 

@@ -1298,6 +1298,13 @@ og_status og_get_state(og_system *s, double *crd, double *vel_next, double *vel_
     return OG_OK;
 }
 
+og_status og_get_crd_prev(og_system *s, double *crd_prev) {
+    if (!s || !crd_prev) return OG_E_ARG;
+    if (s->ctx && s->ctx->stream) OG_HIP(hipStreamSynchronize(s->ctx->stream));
+    OG_HIP(hipMemcpy(crd_prev, s->h.crd_prev, sizeof(double) * 3 * (size_t)s->h.n_atoms, hipMemcpyDeviceToHost));
+    return OG_OK;
+}
+
 og_status og_set_langevin_noise(og_system *s, const double *zeta) {
     if (!s) return OG_E_ARG;
     if (s->ctx && s->ctx->stream) OG_HIP(hipStreamSynchronize(s->ctx->stream));

@@ -28,9 +28,14 @@
   wons_hpsdh_traj.npz -- 20 leapfrog steps of dt 5 fs of the same form at the
       sample's ionic strength 0.00015; log and final restart.
 
+  wons_output.npz -- the reference's own output files of a 20-step leapfrog
+      run with SHAKE (500 waters + 20-bead chain): the Presto trajectory
+      (--print-interval-coord 5), the same run's GROMACS .trr, and the final
+      restart, as raw bytes.
+
 Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
 Run where the reference tree is mounted:
-    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps]
+    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out]
 """
 import os
 import re
@@ -148,8 +153,29 @@ def hps(tmp):
     print("hpsdh traj ok", lg.shape, lg[0], lg[-1])
 
 
+def outputs(tmp):
+    ls = gen.build_system(500, seed=49, n_beads=20)
+    cfg = dict(BASE, **{"n-steps": 20, "ele-alpha": 0.0, "constraint": "shake", "const-tolerance": 1e-10,
+                        "const-max-loops": 1000, "print-interval-coord": 5})
+    files = {}
+    for fmt in ("presto", "gromacs"):
+        c = dict(cfg, **{"fn-o-coord": "out.trr", "format-o-coord": fmt})
+        data, log, rs = run(ls, c, tmp)
+        files[fmt] = open(os.path.join(tmp, "out.trr"), "rb").read()
+        files["restart"] = open(os.path.join(tmp, "out.restart"), "rb").read()
+    np.savez_compressed(os.path.join(HERE, "wons_output.npz"), cls=np.frombuffer(data, np.uint8),
+                        log=parse_log(log), trr_presto=np.frombuffer(files["presto"], np.uint8),
+                        trr_gromacs=np.frombuffer(files["gromacs"], np.uint8),
+                        restart=np.frombuffer(files["restart"], np.uint8),
+                        cfg=np.frombuffer(gen.cfg_text(dict(cfg, **{"format-o-coord": "presto"})).encode(), np.uint8))
+    print("outputs ok", len(files["presto"]), len(files["gromacs"]), len(files["restart"]))
+
+
 def main():
     tmp = tempfile.mkdtemp()
+    if len(sys.argv) > 1 and sys.argv[1] == "out":
+        outputs(tmp)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "more":
         more(tmp)
         return
