@@ -275,8 +275,10 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     """64 independent water + 200-bead-chain walkers (3,000-30,000 atoms,
     log-uniform), Langevin 300 K, SETTLE, VcMD bias on the chain end-to-end
     distance (8 windows).  Batched: one og_batch, one 500-step hipGraph per
-    launch.  Baseline: the same walkers one at a time through the
-    single-system path (og_md_run), seq_steps each."""
+    launch, MiniCell column width from the cell-edge autotuner.  Baselines:
+    the same walkers one at a time through the single-system path
+    (og_md_run), seq_steps each, and through the reference's call points."""
+    import numpy as np
     from omegagene_amd import gen
     from omegagene_amd.engine import Batch, System, lib
     sizes = gen.walker_sizes(n_walkers)
@@ -284,6 +286,13 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
     cfgs = [dict(WALKER_CFG, **{"random-seed": 100 + i, "log-cap": steps + warmup}) for i in range(n_walkers)]
     walkers = equilibrate(list(zip(walkers, cfgs)))
     n_atoms = sum(w.n_atoms for w in walkers)
+    # the MiniCell column width, tuned on the median walker (the full z-bin scan lets it vary)
+    from omegagene_amd.engine import autotune_cell_edge
+    mid = int(np.argsort([w.n_atoms for w in walkers])[len(walkers) // 2])
+    edge, edge_ms = autotune_cell_edge(walkers[mid], cfgs[mid], trials=3)
+    cfgs = [dict(c, **{"cell-edge-scale": edge}) for c in cfgs]
+    log("walkers: cell edge x%.2f of the reference width (%.1f us pair kernel on the %d-atom walker)" %
+        (edge, edge_ms * 1e3, walkers[mid].n_atoms))
     log("walkers: %d systems, %d atoms in total (%d-%d)" % (n_walkers, n_atoms, min(sizes), max(sizes)))
     # one at a time (the plain drop-in path)
     seq_time, seq_steps_done = 0.0, 0
@@ -331,7 +340,7 @@ def bench_walkers(n_walkers=64, steps=500, seq_steps=100, warmup=50):
                          "path (og_md_run); speedup_vs_call_points: against the device part of the reference's "
                          "call-point path (og_celeste.h) one walker at a time, host work not counted",
             "atom_steps_per_s": n_atoms * steps / wall,
-            "graph_steps": steps, "timed_steps": steps, "pair_shape": tune}
+            "graph_steps": steps, "timed_steps": steps, "pair_shape": dict(tune, cell_edge_scale=edge)}
 
 
 def bench_serving(n_requests=48, rate=200.0, budget_gb=200.0, chunk=20, seed=7):

@@ -158,6 +158,8 @@ typedef struct {
     int      dh_grad_fix;          /* 1: Debye-Hueckel gradient -E (1/r + kappa) r_ij / r; 0 (default): the
                                       reference's -E (1/r + kappa) r_ij (ZeroMultipoleSum.cpp:453,
                                       cuda_setup.cu:1327), which is r times the derivative */
+    double   cell_edge_scale;      /* MiniCell column width relative to the reference's (1 = the reference
+                                      geometry, required with gap_bug 1); og_autotune_cell_edge picks one */
 } og_params;
 
 enum { OG_THERMOSTAT_NONE = 0, OG_THERMOSTAT_SCALING = 1 };
@@ -314,6 +316,14 @@ og_status   og_pair_kernel_time(og_system *sys, double *ms, int *n_launches);
 og_status   og_autotune(og_system *sys, int trials, int *waves, double *pair_ms);
 og_status   og_batch_autotune(og_batch *b, int trials, int *waves, int *order, double *pair_ms);
 og_status   og_set_pair_shape(og_system *sys, int waves);
+/* Cell-edge autotuner (no counterpart in the reference): for a system on the
+ * full z-bin scan (gap_bug 0) times the pair kernel on grids whose column
+ * width is 0.8, 0.9, 1, 1.12 and 1.25 times the reference's (temporary
+ * systems, each at its best launch shape) and returns the fastest scale for
+ * og_params.cell_edge_scale; the pair set within the cutoff is the same on
+ * every grid.  With gap_bug 1 it returns 1. */
+og_status   og_autotune_cell_edge(const og_system_desc *desc, const og_params *params, int trials,
+                                  double *best_scale, double *pair_ms);
 og_status   og_batch_set_pair_shape(og_batch *b, int waves, int order);
 /* 1 when an MD step of this system runs as the fused one-kernel Langevin step
  * (Langevin, SETTLE waters bound to nothing outside the water, at most 4096 other atoms) */

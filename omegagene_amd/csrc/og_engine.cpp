@@ -52,7 +52,9 @@ struct GridGeom {
 };
 
 #pragma clang fp contract(off)
-int grid_geom(GridGeom &g, int n_atoms, const double *box, double nscut) {
+// edge_scale: the column width relative to MiniCell's (8 atoms per cube of
+// the mean density); 1 reproduces the reference geometry (and list order)
+int grid_geom(GridGeom &g, int n_atoms, const double *box, double nscut, double edge_scale = 1.0) {
     g.L[0] = (float)box[0];
     g.L[1] = (float)box[4];
     g.L[2] = (float)box[8];
@@ -68,6 +70,7 @@ int grid_geom(GridGeom &g, int n_atoms, const double *box, double nscut) {
     g.L_z_uni          = g.L[2] / (float)g.n_uni_z;
     const float volume = g.L[0] * g.L[1] * g.L[2];
     float       w      = powf((float)((float)OG_NAC / ((float)n_atoms / volume)), (float)(1.0 / 3.0));
+    if (edge_scale != 1.0) w = (float)(w * edge_scale);
     if (w > g.cutoff_pair_half) w = g.cutoff_pair_half;
     g.nx        = (int)floorf(g.L[0] / w);
     g.ny        = (int)floorf(g.L[1] / w);
@@ -1021,6 +1024,7 @@ void og_params_default(og_params *p) {
     p->dh_ionic_strength   = 0.0;
     p->dh_temperature      = 0.0;
     p->dh_grad_fix         = 0;
+    p->cell_edge_scale     = 1.0;
 }
 
 const char *og_status_string(og_status s) {
@@ -1057,7 +1061,7 @@ og_status og_thread_sync(void) {
 og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint64_t *bytes) {
     if (!d || !p || !bytes) return OG_E_ARG;
     GridGeom g;
-    if (grid_geom(g, d->n_atoms, d->box, p->nsgrid_cutoff) != 0) return OG_E_ARG;
+    if (grid_geom(g, d->n_atoms, d->box, p->nsgrid_cutoff, p->cell_edge_scale) != 0) return OG_E_ARG;
     // the allocations of sys_alloc, term by term (the fused step's free-atom list counted at its bound)
     const double n = d->n_atoms, nt = (double)d->n_lj_types * d->n_lj_types;
     const double cols = g.n_columns, cap = std::ceil((n + cols * OG_NAC) / 32.0) * 32.0, cells = cap / OG_NAC;
@@ -1096,7 +1100,12 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
     auto s = std::make_unique<og_system>();
     s->prm = *p;
     if (s->prm.nsgrid_update_intvl < 1) s->prm.nsgrid_update_intvl = 1;
-    if (grid_geom(s->geom, d->n_atoms, d->box, p->nsgrid_cutoff) != 0) {
+    if (p->cell_edge_scale != 1.0 && (p->gap_bug || !(p->cell_edge_scale >= 0.5 && p->cell_edge_scale <= 2.0))) {
+        // the reference's list order follows its own geometry: only the full z-bin scan may take another
+        set_err("cell_edge_scale other than 1 needs gap_bug 0 and lies in [0.5, 2]");
+        return OG_E_ARG;
+    }
+    if (grid_geom(s->geom, d->n_atoms, d->box, p->nsgrid_cutoff, p->cell_edge_scale) != 0) {
         set_err("box too small for the pair-list radius (needs at least 2*n_neighbors+1 columns and 2 z bins)");
         return OG_E_ARG;
     }
@@ -1553,6 +1562,34 @@ og_status og_set_pair_shape(og_system *s, int waves) {
     og_status st = ensure_ctx(s);
     return st != OG_OK ? st : exec_set_pair_shape(*s->ctx, waves, 0);
 }
+og_status og_autotune_cell_edge(const og_system_desc *d, const og_params *p, int trials, double *best_scale,
+                                double *pair_ms) {
+    if (!d || !p || !best_scale) return OG_E_ARG;
+    *best_scale = 1.0;
+    if (p->gap_bug) return OG_OK;  // the reference list keeps the reference geometry
+    static const double cand[] = {0.8, 0.9, 1.0, 1.12, 1.25};
+    double best = 1e300;
+    for (double sc : cand) {
+        GridGeom g;
+        if (grid_geom(g, d->n_atoms, d->box, p->nsgrid_cutoff, sc) != 0) continue;
+        og_params q        = *p;
+        q.cell_edge_scale  = sc;
+        q.log_cap          = 1;
+        og_system *s       = nullptr;
+        og_status  st      = og_system_create(d, &q, &s);
+        if (st != OG_OK) return st;
+        int    waves = 0;
+        double ms    = 0.0;
+        st           = og_calc_energy_force(s, 1, nullptr, nullptr);
+        if (st == OG_OK) st = og_autotune(s, trials, &waves, &ms);
+        og_system_destroy(s);
+        if (st != OG_OK) return st;
+        if (ms < best) best = ms, *best_scale = sc;
+    }
+    if (pair_ms) *pair_ms = best;
+    return OG_OK;
+}
+
 og_status og_batch_autotune(og_batch *b, int trials, int *waves, int *order, double *pair_ms) {
     if (!b) return OG_E_ARG;
     int       shp = OG_PAIR_DEFAULT;

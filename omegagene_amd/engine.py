@@ -65,7 +65,7 @@ class Params(C.Structure):
                 ("berendsen_tau", C.c_double), ("temperature_init", C.c_double), ("heating_steps", C.c_int),
                 ("nonbond", C.c_int), ("hps_epsilon", C.c_double), ("electrostatic", C.c_int),
                 ("dh_dielectric", C.c_double), ("dh_ionic_strength", C.c_double), ("dh_temperature", C.c_double),
-                ("dh_grad_fix", C.c_int)]
+                ("dh_grad_fix", C.c_int), ("cell_edge_scale", C.c_double)]
 
 
 def lib():
@@ -124,6 +124,8 @@ def params_from_cfg(cfg: dict) -> Params:
     p.dh_temperature = float(g("debye-huckel-temperature", 0.0))
     if "dh-grad-fix" in cfg:
         p.dh_grad_fix = int(cfg["dh-grad-fix"])
+    if "cell-edge-scale" in cfg:
+        p.cell_edge_scale = float(cfg["cell-edge-scale"])
     return p
 
 
@@ -201,6 +203,19 @@ def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcm
         x.vc_atoms_a = d(ga, np.int32)
         x.vc_atoms_b = d(gb, np.int32)
     return x, k
+
+
+def autotune_cell_edge(ls: LaunchSet, cfg: dict, trials: int = 5, com_groups=None):
+    """og_autotune_cell_edge: the fastest MiniCell column-width scale for this
+    system and configuration (1.0 under the reference list rule, gap-bug 1)
+    and the pair-kernel time it gave (ms)."""
+    _check(lib().og_set_device(C.c_int(0)))
+    p = params_from_cfg(cfg)
+    x, keep = desc_from_launchset(ls, p.constraint, com_groups, vcmd=False)
+    sc, ms = C.c_double(), C.c_double()
+    _check(lib().og_autotune_cell_edge(C.byref(x), C.byref(p), C.c_int(trials), C.byref(sc), C.byref(ms)))
+    del keep
+    return sc.value, ms.value
 
 
 class System:

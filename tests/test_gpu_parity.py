@@ -610,3 +610,38 @@ def test_energy_log_stride_keeps_trajectory_and_logged_energies():
     (c1, l1), (c4, l4) = runs
     assert np.array_equal(c1, c4)
     assert len(l4) == 6 and np.array_equal(l4, l1[::4])
+
+
+def test_cell_edge_scale_keeps_the_pair_set_and_autotuner():
+    """Under the full z-bin scan (gap-bug 0) the MiniCell column width is a
+    free launch parameter: grids 0.8x and 1.25x the reference width list the
+    same in-cutoff pairs (counts within the fused-multiply-add shell) and give
+    the same gradient to FP32 rounding (which atom of a pair is i changes with
+    the cell order); the cell-edge autotuner returns one of its candidates.
+    Under the reference rule (gap-bug 1) another width is refused."""
+    from omegagene_amd.engine import EngineError, autotune_cell_edge
+    ls = gen.walker_system(8000, seed=11, vcmd=False)
+    cfg = dict(BASE, **{"gap-bug": 0})
+    out = {}
+    for sc in (0.8, 1.0, 1.25):
+        s = System(ls, dict(cfg, **{"cell-edge-scale": sc}))
+        w, e = s.energy_force(rebuild=True)
+        out[sc] = (w, e, s.grid_info())
+        s.close()
+    w1, e1, g1 = out[1.0]
+    S = ele_scale(ls)
+    for sc in (0.8, 1.25):
+        w, e, g = out[sc]
+        say("  scale %.2f: %d columns (%d at 1.0), pairs %d / %d, gradient rms rel %.2e" %
+            (sc, g["n_columns"], g1["n_columns"], e["npairs"], e1["npairs"], rms_rel(w, w1)))
+        assert g["n_columns"] != g1["n_columns"]
+        assert abs(int(e["npairs"]) - int(e1["npairs"])) <= fma_shell(e1["npairs"], 12.0)
+        assert rms_rel(w, w1) < 2e-6   # the bar of every FP32 gradient comparison here
+        assert e["vdw"] == pytest.approx(e1["vdw"], rel=2e-6)
+        assert abs(e["ele"] - e1["ele"]) <= 1e-8 * S
+    sc, ms = autotune_cell_edge(ls, cfg, trials=3)
+    say("  cell-edge autotuner: scale %.2f, pair kernel %.1f us" % (sc, ms * 1e3))
+    assert sc in (0.8, 0.9, 1.0, 1.12, 1.25) and ms > 0
+    assert autotune_cell_edge(ls, dict(BASE), trials=1)[0] == 1.0
+    with pytest.raises(EngineError):
+        System(ls, dict(BASE, **{"cell-edge-scale": 0.8}))
