@@ -230,8 +230,10 @@ def test_forces_at_full_bench_size_vs_oracle():
     say("  vs FP64: gradient rms rel %.2e (FP32-faithful oracle %.2e), vdw rel %.2e, ele / sum|q q k| %.2e, "
         "virial rel %.2e" % (r["grad_rms_rel"], r["oracle32_grad_rms_rel"], r["vdw_rel"], r["ele_over_scale"],
                              r["virial_rel"]))
-    # the FP32 rounding of the reference's own arithmetic sets the gradient error (~1e-5 here)
-    assert r["grad_rms_rel"] < 2e-5 and r["vdw_rel"] < 2e-6 and r["ele_over_scale"] < 1e-8
+    # the FP32 rounding of the reference's own arithmetic sets the gradient error (1e-5 to 2e-5 here): the
+    # device is within 10% of the FP32-faithful oracle's own distance from FP64
+    assert r["grad_rms_rel"] <= 1.1 * r["oracle32_grad_rms_rel"] + 1e-6
+    assert r["vdw_rel"] < 2e-6 and r["ele_over_scale"] < 1e-8
     assert r["virial_rel"] < 2e-6
     s.close()
 
@@ -280,8 +282,10 @@ def test_forces_at_1_2m_atoms_alpha_vs_oracle():
     say("  vs FP64: gradient rms rel %.2e (FP32-faithful oracle %.2e), vdw rel %.2e, ele / sum|q q k| %.2e, "
         "virial rel %.2e" % (r["grad_rms_rel"], r["oracle32_grad_rms_rel"], r["vdw_rel"], r["ele_over_scale"],
                              r["virial_rel"]))
-    # the FP32 rounding of the reference's own arithmetic sets the gradient error (~1e-5 here)
-    assert r["grad_rms_rel"] < 2e-5 and r["vdw_rel"] < 2e-6 and r["ele_over_scale"] < 1e-8
+    # the FP32 rounding of the reference's own arithmetic sets the gradient error (1e-5 to 2e-5 here): the
+    # device is within 10% of the FP32-faithful oracle's own distance from FP64
+    assert r["grad_rms_rel"] <= 1.1 * r["oracle32_grad_rms_rel"] + 1e-6
+    assert r["vdw_rel"] < 2e-6 and r["ele_over_scale"] < 1e-8
     assert r["virial_rel"] < 2e-6
 
 

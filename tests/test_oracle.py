@@ -670,3 +670,17 @@ def test_hpsdh_two_particle_closed_form():
             assert g_ele == pytest.approx(dE / r if fix else dE, rel=1e-4)
         n, _, _, _ = oracle.pair_ff_fp32(zd, ff, np.array([20.0, 0, 0], np.float32), 0, a6, a12, 1.0, -1.0)
         assert n == 0
+
+
+def test_oracle_excess_pairs_equal_launch_set_order():
+    """orc_excess_pairs (MmSystem.cpp:700-736) gives the ala3 sample's excluded
+    pairs in the order the host side hands to the engine; with them the self
+    energy reproduces the sample log (test_zd_constants_and_self_energy_kat)."""
+    ls, _ = ala3_launchset()
+    tab = np.ascontiguousarray(ls.nb15off_table(), np.int32)
+    cap = 4 * len(ls.nb15off) + 16
+    out = np.zeros((cap, 2), np.int32)
+    n = oracle.lib().orc_excess_pairs(ctypes.c_int(ls.n_atoms), tab.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_int(cap))
+    assert n == len(ls.excess_pairs()) > 0
+    assert np.array_equal(out[:n], ls.excess_pairs())
