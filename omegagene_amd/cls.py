@@ -81,6 +81,37 @@ class LaunchSet:
         """The float tables the pair code reads (MmSystem.cpp:313-319, real_pw)."""
         return self.lj6.astype(np.float32), self.lj12.astype(np.float32)
 
+    def bonded_tables(self):
+        """Angles, torsions, impropers and 1-4 pairs as flat arrays, with the
+        parameters the reference's MmSystem holds: angle (ids [n,3], k, theta0),
+        torsion / improper (ids [n,4], energy, overlaps, symmetry, phase), and
+        1-4 (ids [n,2], types [n,2], coeff_vdw, coeff_ele).  Read.cpp:438 passes
+        the file's two 1-4 coefficients to set_nb14_param (MmSystem.cpp:395-407)
+        in the other order, so the LJ scaling is the file's second value and the
+        Coulomb scaling the first, as here."""
+        def tors(rows):
+            if not rows:
+                return (np.zeros((0, 4), np.int32), np.zeros(0), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                        np.zeros(0))
+            a = np.array([r[:4] for r in rows], np.int32)
+            return (a, np.array([r[4] for r in rows], float), np.array([r[5] for r in rows], np.int32),
+                    np.array([r[6] for r in rows], np.int32), np.array([r[7] for r in rows], float))
+        ang = (np.array([r[:3] for r in self.angles], np.int32).reshape(-1, 3),
+               np.array([r[3] for r in self.angles], float), np.array([r[4] for r in self.angles], float))
+        nb = (np.array([r[:2] for r in self.nb14], np.int32).reshape(-1, 2),
+              np.array([r[2:4] for r in self.nb14], np.int32).reshape(-1, 2),
+              np.array([r[5] for r in self.nb14], float), np.array([r[4] for r in self.nb14], float))
+        return {"angle": ang, "torsion": tors(self.torsions), "impro": tors(self.impros), "nb14": nb}
+
+    def shake_quad_tables(self):
+        """Four-atom SHAKE groups: ids [n,4] and squared distances [n,6] of the
+        pairs 01 12 20 30 23 13 (Read.cpp:499-516 squares them)."""
+        if not self.shake_quads:
+            return np.zeros((0, 4), np.int32), np.zeros((0, 6))
+        ids = np.array([q[:4] for q in self.shake_quads], np.int32)
+        d = np.array([q[4:10] for q in self.shake_quads], float)
+        return ids, d * d
+
     def hps_tables_f32(self):
         """HPS cutoff and lambda tables as the pair code reads them (MmSystem.cpp:320-325, real_pw)."""
         nt = self.n_lj_types

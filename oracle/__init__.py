@@ -483,3 +483,47 @@ def thermostat_scaling(dt, tau_inv, temperature, temp_coeff, work, vel, mass, ma
           _p(np.ascontiguousarray(mass, np.float32), C.c_float),
           _p(np.ascontiguousarray(mass_inv, np.float32), C.c_float), _p(out, C.c_double))
     return out, s
+
+
+def bonded(ls, crd, work):
+    """orc_angles + orc_torsions (torsions, impropers) + orc_nb14 on atom-order
+    crd, adding into work; returns the five energies."""
+    t = ls.bonded_tables()
+    L = np.ascontiguousarray(pbc_from_box(ls.box)[0], np.float32)
+    crd = np.ascontiguousarray(crd, np.float64)
+    lf = lib()
+    lf.orc_angles.restype = C.c_double
+    lf.orc_torsions.restype = C.c_double
+    ids, k, th = t["angle"]
+    ea = lf.orc_angles(_p(L, C.c_float), C.c_int(len(ids)), _p(np.ascontiguousarray(ids), C.c_int),
+                       _p(np.ascontiguousarray(k), C.c_double), _p(np.ascontiguousarray(th), C.c_double),
+                       _p(crd, C.c_double), _p(work, C.c_double))
+    et = []
+    for key in ("torsion", "impro"):
+        ids, e, ov, sy, ph = t[key]
+        et.append(lf.orc_torsions(_p(L, C.c_float), C.c_int(len(ids)), _p(np.ascontiguousarray(ids), C.c_int),
+                                  _p(np.ascontiguousarray(e), C.c_double), _p(np.ascontiguousarray(ov), C.c_int),
+                                  _p(np.ascontiguousarray(sy), C.c_int), _p(np.ascontiguousarray(ph), C.c_double),
+                                  _p(crd, C.c_double), _p(work, C.c_double)))
+    ids, ty, cv, ce = t["nb14"]
+    lj6, lj12 = ls.lj_tables_f32()
+    ev, ee = C.c_double(), C.c_double()
+    lf.orc_nb14(_p(L, C.c_float), C.c_int(len(ids)), _p(np.ascontiguousarray(ids), C.c_int),
+                _p(np.ascontiguousarray(ty), C.c_int), _p(np.ascontiguousarray(cv), C.c_double),
+                _p(np.ascontiguousarray(ce), C.c_double), _p(np.ascontiguousarray(lj6), C.c_float),
+                _p(np.ascontiguousarray(lj12), C.c_float), C.c_int(lj6.shape[0]),
+                _p(np.ascontiguousarray(ls.charge, np.float32), C.c_float), _p(crd, C.c_double),
+                _p(work, C.c_double), C.byref(ev), C.byref(ee))
+    return {"angle": ea, "torsion": et[0], "impro": et[1], "vdw14": ev.value, "ele14": ee.value}
+
+
+def shake_quad(L, ids, dist2, crd, crd_prev, mass_inv, tol, max_loops):
+    """orc_shake_quad on a copy of crd; returns (crd, n_not_converged)."""
+    out = np.ascontiguousarray(crd, np.float64).copy()
+    n = lib().orc_shake_quad(_p(np.ascontiguousarray(L, np.float32), C.c_float), C.c_int(len(ids)),
+                             _p(np.ascontiguousarray(ids, np.int32), C.c_int),
+                             _p(np.ascontiguousarray(dist2, np.float64), C.c_double), _p(out, C.c_double),
+                             _p(np.ascontiguousarray(crd_prev, np.float64), C.c_double),
+                             _p(np.ascontiguousarray(mass_inv, np.float32), C.c_float), C.c_double(tol),
+                             C.c_int(max_loops))
+    return out, n

@@ -236,4 +236,21 @@ double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const
 #ifdef __cplusplus
 }
 #endif
+/* ---- bonded terms beyond bonds and four-atom SHAKE (oracle_bonded.c) ----
+ * Angles (SubBox.cpp:1250-1270, ForceField.cpp:138-177): ids [3n], E = k (theta - theta0)^2. */
+double orc_angles(const float L[3], int n, const int *ids, const double *k_eps, const double *theta0,
+                  const double *crd, double *work);
+/* Torsions and impropers (SubBox.cpp:1272-1321, ForceField.cpp:179-299): ids [4n]. */
+double orc_torsions(const float L[3], int n, const int *ids, const double *ene, const int *overlaps,
+                    const int *symmetry, const double *phase, const double *crd, double *work);
+/* 1-4 pairs (SubBox.cpp:1323-1346, ForceField.cpp:301-331): ids [2n], the entry's
+ * own types [2n], scaling of the LJ and Coulomb terms. */
+void orc_nb14(const float L[3], int n, const int *ids, const int *types, const double *coeff_vdw,
+              const double *coeff_ele, const float *lj6, const float *lj12, int n_types, const float *charge,
+              const double *crd, double *work, double *e_vdw, double *e_ele);
+/* SHAKE on four-atom groups (ConstraintShake.cpp:151-276): ids [4n], squared
+ * distances [6n] of pairs 01 12 20 30 23 13.  Returns the groups not converged. */
+int orc_shake_quad(const float L[3], int n, const int *ids, const double *dist2, double *crd, const double *crd_prev,
+                   const float *mass_inv, double tol, int max_loops);
+
 #endif

@@ -28,6 +28,12 @@
   wons_hpsdh_traj.npz -- 20 leapfrog steps of dt 5 fs of the same form at the
       sample's ionic strength 0.00015; log and final restart.
 
+  wons_ala3_grad.npz -- one leapfrog step without constraints of the ala3
+      sample's own launch set (samples/ala3/md.inp.cls: angles, torsions,
+      impropers, 1-4 pairs); the gradient from the restart velocities.
+  wons_ala3_traj.npz -- 10 leapfrog steps of the same launch set with SHAKE
+      (pairs, trios and the four-atom groups, tolerance 1e-10) and COM cancel
+      on the solute and solvent groups, as the sample runs it; log and restart.
   wons_output.npz -- the reference's own output files of a 20-step leapfrog
       run with SHAKE (500 waters + 20-bead chain): the Presto trajectory
       (--print-interval-coord 5), the same run's GROMACS .trr, and the final
@@ -35,7 +41,7 @@
 
 Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
 Run where the reference tree is mounted:
-    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out]
+    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out|ala3]
 """
 import os
 import re
@@ -85,7 +91,11 @@ def parse_log(text):
 def run(ls, cfg, workdir, extra=""):
     clsf = os.path.join(workdir, "in.cls")
     cfgf = os.path.join(workdir, "in.cfg")
-    data = write_cls(ls, clsf)
+    if isinstance(ls, bytes):  # a launch set file as it is
+        data = ls
+        open(clsf, "wb").write(data)
+    else:
+        data = write_cls(ls, clsf)
     with open(cfgf, "w") as f:
         f.write(gen.cfg_text(cfg) + extra)
     r = subprocess.run([EXE, "--cfg", cfgf, "--inp", clsf], cwd=workdir, stdout=subprocess.PIPE,
@@ -171,8 +181,37 @@ def outputs(tmp):
     print("outputs ok", len(files["presto"]), len(files["gromacs"]), len(files["restart"]))
 
 
+def ala3(tmp):
+    data = open("/root/reference/samples/ala3/md.inp.cls", "rb").read()
+    base = dict(BASE, **{"ele-alpha": 0.0, "const-tolerance": 1e-10, "const-max-loops": 1000})
+    cfg = dict(base, **{"n-steps": 1, "constraint": "none"})
+    from omegagene_amd.cls import read_cls
+    ls = read_cls(data)
+    _, log, rs = run(data, cfg, tmp)
+    minv = (1.0 / ls.mass.astype(np.float32).astype(np.float64)).astype(np.float32).astype(np.float64)
+    work = -(rs["vel"] - ls.vel) / (2.0 * 4.184e-4 * minv[:, None])
+    np.savez_compressed(os.path.join(HERE, "wons_ala3_grad.npz"), cls=np.frombuffer(data, np.uint8), work=work,
+                        log=parse_log(log), crd1=rs["crd"], vel1=rs["vel"],
+                        cfg=np.frombuffer(gen.cfg_text(cfg).encode(), np.uint8))
+    print("ala3 grad ok", parse_log(log)[0])
+    extra = "--com-cancel-group-name      solute\n--com-cancel-group-name      solvent\n"
+    cfg = dict(base, **{"n-steps": 10, "constraint": "shake", "com-motion": "cancel"})
+    _, log, rs = run(data, cfg, tmp, extra)
+    lg = parse_log(log)
+    np.savez_compressed(os.path.join(HERE, "wons_ala3_traj.npz"), cls=np.frombuffer(data, np.uint8), log=lg,
+                        crd=rs["crd"], vel=rs["vel"], epot=rs["epot"], ekin=rs["ekin"], step=rs["step"],
+                        cfg=np.frombuffer((gen.cfg_text(cfg) + extra).encode(), np.uint8), raw_log=np.frombuffer(
+                            "\n".join(l for l in log.splitlines() if l[:5] in ("Step:", "Total", "Poten", "Bond:",
+                                                                               "Torsi", "14-VD", "VDW: ", "Tempe")
+                                       ).encode(), np.uint8))
+    print("ala3 traj ok", lg.shape, lg[0], lg[-1])
+
+
 def main():
     tmp = tempfile.mkdtemp()
+    if len(sys.argv) > 1 and sys.argv[1] == "ala3":
+        ala3(tmp)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "out":
         outputs(tmp)
         return

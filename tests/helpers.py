@@ -159,6 +159,9 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
     for step in range(n_total):
         work, evdw, eele, ebond = oracle_forces_allpairs(ls, crd, zd, L, ff=ff)
         pot = evdw + eele + ebond
+        if len(ls.angles) or len(ls.torsions) or len(ls.impros) or len(ls.nb14):
+            # angles, torsions, impropers, 1-4 (SubBox.cpp:1250-1346)
+            pot += sum(oracle.bonded(ls, crd, work).values())
         if integrator == "leapfrog-presto":
             vel = vel_next
             vel_next = oracle.update_velocities(dt, vel, work, minv)
@@ -180,6 +183,10 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
                 if len(ls.shake_trio_ids):
                     crd, nf = oracle.shake_trio(L, ls.shake_trio_ids, np.asarray(ls.shake_trio_d) ** 2, crd, crd_prev,
                                                 minv, tol, max_loops)
+                    assert nf == 0
+                if len(ls.shake_quads):
+                    qids, qd2 = ls.shake_quad_tables()
+                    crd, nf = oracle.shake_quad(L, qids, qd2, crd, crd_prev, minv, tol, max_loops)
                     assert nf == 0
                 vel_next = oracle.velocity_from_crd(dt, crd, crd_prev)
             crd = oracle.revise_pbc(L, lb, crd)

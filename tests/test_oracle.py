@@ -684,3 +684,92 @@ def test_oracle_excess_pairs_equal_launch_set_order():
                                       out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_int(cap))
     assert n == len(ls.excess_pairs()) > 0
     assert np.array_equal(out[:n], ls.excess_pairs())
+
+
+# ------------------------------ angles, torsions, impropers, 1-4 and SHAKE quad
+
+def _raw_log_terms(d):
+    """Angle, Torsion, Improper, 14-VDW, 14-Ele per step from a fixture's log lines."""
+    out = []
+    for ln in bytes(d["raw_log"]).decode().splitlines():
+        f = ln.split()
+        if ln.startswith("Bond:"):
+            out.append({"bond": float(f[1]), "angle": float(f[3])})
+        elif ln.startswith("Torsion:"):
+            out[-1].update(torsion=float(f[1]), impro=float(f[3]))
+        elif ln.startswith("14-VDW:"):
+            out[-1].update(vdw14=float(f[1]), ele14=float(f[3]))
+    return out
+
+
+def test_ala3_bonded_terms_match_log():
+    """The ala3 sample's own launch set at step 0: angle, torsion, improper,
+    1-4 VDW and 1-4 Ele energies to the printed digits of the sample log
+    (samples/ala3/log03_md.txt:131-133; the same values head the reference
+    run in wons_ala3_traj.npz)."""
+    d = load_npz("wons_ala3_traj.npz")
+    ls = cls_from_fixture(d)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    e = oracle.bonded(ls, crd, np.zeros((ls.n_atoms, 3)))
+    ref = _raw_log_terms(d)[0]
+    assert (ref["angle"], ref["torsion"], ref["impro"]) == (4.1796133554, 28.320798304, 0.064242037627)
+    for k in ("angle", "torsion", "impro", "vdw14", "ele14"):
+        assert fmt(e[k]) == fmt(ref[k]), k
+
+
+def test_oracle_ala3_gradient_matches_reference_one_step():
+    """One leapfrog step of the ala3 launch set without constraints in the
+    reference all-pairs build: the whole gradient (pairs, excluded pairs,
+    bonds, angles, torsions, impropers, 1-4) to 1e-11 RMS."""
+    from helpers import oracle_forces_allpairs
+    d = load_npz("wons_ala3_grad.npz")
+    ls = cls_from_fixture(d)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    work, evdw, eele, eb = oracle_forces_allpairs(ls, crd, oracle.zd_setup(12.0, 0.0), L)
+    oracle.bonded(ls, crd, work)
+    ref = d["work"]
+    assert rms_rel(work, ref) < 1e-11
+    assert np.max(np.abs(work - ref)) < 1e-8 * np.max(np.abs(ref))
+
+
+def test_oracle_ala3_md_with_quads_matches_reference_run():
+    """10 leapfrog steps of ala3 with SHAKE pairs, trios and four-atom groups
+    and COM cancel on the solute and solvent groups: per-step bonded terms,
+    potential and kinetic energy, and the final restart."""
+    from helpers import oracle_md
+    d = load_npz("wons_ala3_traj.npz")
+    ls = cls_from_fixture(d)
+    assert len(ls.shake_quads) == 9
+    groups = [g for name, g in ls.groups if name in ("solute", "solvent")]
+    crd, vel, rows = oracle_md(ls, 10, constraint="shake", com_groups=groups)
+    log = d["log"]
+    for r, lg in zip(rows, log[:10]):
+        assert abs(r[1] - lg[1]) <= 2e-9 * abs(lg[5]) + 1e-8, (r, lg)
+        assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]), (r, lg)
+    assert np.max(np.abs(crd - d["crd"])) < 1e-9
+    assert np.max(np.abs(vel - d["vel"])) < 1e-11
+
+
+def test_shake_quad_restores_distances():
+    """Closed check: after orc_shake_quad every one of the six distances of
+    each group is at its target to the tolerance, and a group already at its
+    targets does not move."""
+    d = load_npz("wons_ala3_traj.npz")
+    ls = cls_from_fixture(d)
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    ids, d2 = ls.shake_quad_tables()
+    minv = (1.0 / ls.mass.astype(np.float32).astype(np.float64)).astype(np.float32)
+    rng = np.random.default_rng(5)
+    prev = ls.crd.copy()
+    cur = prev + rng.normal(scale=0.02, size=prev.shape)
+    out, nf = oracle.shake_quad(L, ids, d2, cur, prev, minv, 1e-12, 1000)
+    assert nf == 0
+    pairs = [(0, 1), (1, 2), (2, 0), (3, 0), (2, 3), (1, 3)]
+    for q in range(len(ids)):
+        for p, (i, j) in enumerate(pairs):
+            v = out[ids[q, i]] - out[ids[q, j]]
+            assert abs(v @ v - d2[q, p]) <= 1e-10 * d2[q, p]
+    out2, _ = oracle.shake_quad(L, ids, d2, out, out, minv, 1e-12, 1000)
+    assert np.max(np.abs(out2 - out)) < 1e-12
