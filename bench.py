@@ -183,7 +183,7 @@ def bench_nve(steps=1000, relax=300, gap_bug=1):
     fluctuation is the RMS residual about that line."""
     import numpy as np
     from omegagene_amd import gen
-    from omegagene_amd.engine import EngineError, System
+    from omegagene_amd.engine import EngineError, System, potential
     out = []
     cases = [("6,000 atoms (2,000 waters), rc 12 A, alpha 0, SHAKE", gen.config0(), False),
              ("99,200 atoms (33,000 waters + 200-bead chain), rc 12 A, alpha 0, SETTLE", gen.config1(), True),
@@ -207,7 +207,7 @@ def bench_nve(steps=1000, relax=300, gap_bug=1):
             out.append({"workload": name, "n_atoms": int(ls.n_atoms), "list_rule": rule, "error": str(e)})
             log("nve %s (%s): %s" % (name, rule, e))
             continue
-        tot = el[:, 0] + el[:, 1] + el[:, 2] + el[:, 3] + el[:, 4] + el[:, 5] + el[:, 7]
+        tot = potential(el) + el[:, 5] + el[:, 7]
         t_ns = np.arange(len(tot)) * cfg["time-step"] * 1e-6
         slope, icpt = np.polyfit(t_ns, tot, 1)
         resid = tot - (slope * t_ns + icpt)

@@ -59,6 +59,11 @@ enum {
     OG_TERM_NPAIRS,     /* in-cutoff unmasked atom pairs of the last evaluation (an exact integer) */
     OG_TERM_LAMBDA,     /* VcMD reaction coordinate: distance between the two group mass centres (A) */
     OG_TERM_VSTATE,     /* VcMD virtual state in force during the step (0-based) */
+    OG_TERM_ANGLE,      /* harmonic angles                     (MmSystem::pote_angle) */
+    OG_TERM_TORSION,    /* torsions                            (MmSystem::pote_torsion) */
+    OG_TERM_IMPRO,      /* improper torsions                   (MmSystem::pote_impro) */
+    OG_TERM_VDW14,      /* scaled 1-4 LJ                       (MmSystem::pote_14vdw) */
+    OG_TERM_ELE14,      /* scaled 1-4 Coulomb                  (MmSystem::pote_14ele) */
     OG_NTERMS
 };
 
@@ -114,6 +119,33 @@ typedef struct {
      * OG_NONBOND_HPS (NULL otherwise) */
     const double *hps_cutoff;       /* [n_types*n_types] */
     const double *hps_lambda;       /* [n_types*n_types] */
+    /* bonded terms beyond bonds, in FP64 (MmSystem's angle / torsion / impro /
+     * nb14 tables, Read.cpp:336-459, MmSystem.cpp:349-407): */
+    int           n_angles;         /* E = k (theta - theta0)^2, theta at the middle atom */
+    const int    *angle_ids;        /* [n_angles*3] */
+    const double *angle_k;
+    const double *angle_theta0;     /* radians */
+    int           n_torsions;       /* E = V (1 + cos(n phi - phase)) / overlaps */
+    const int    *torsion_ids;      /* [n_torsions*4] */
+    const double *torsion_ene;      /* V */
+    const int    *torsion_overlaps;
+    const int    *torsion_symmetry; /* n, 0..10 */
+    const double *torsion_phase;    /* radians */
+    int           n_impros;         /* improper torsions, same form */
+    const int    *impro_ids;
+    const double *impro_ene;
+    const int    *impro_overlaps;
+    const int    *impro_symmetry;
+    const double *impro_phase;
+    int           n_nb14;           /* scaled 1-4 LJ + Coulomb pairs */
+    const int    *nb14_ids;         /* [n_nb14*2] */
+    const int    *nb14_types;       /* [n_nb14*2] 0-based LJ types of the entry (the table row used) */
+    const double *nb14_coeff_vdw;   /* LJ scale of the pair */
+    const double *nb14_coeff_ele;   /* Coulomb scale of the pair */
+    /* SHAKE on four-atom groups (ConstraintShake::shake_quad, ConstraintShake.cpp:151-276) */
+    int           n_shake_quads;
+    const int    *shake_quad_ids;   /* [n*4] */
+    const double *shake_quad_d2;    /* [n*6] squared distances of the pairs 01 12 20 30 23 13 */
 } og_system_desc;
 
 /* Run parameters: the .cfg keys of the path (Config.cpp:37-367). */

@@ -27,6 +27,7 @@ SOURCES = [
     ("og_grid.hip", ["-ffp-contract=off"]),
     ("og_pair.hip", []),
     ("og_integrate.hip", ["-ffp-contract=off"]),
+    ("og_bonded.hip", ["-ffp-contract=off"]),
     ("og_engine.cpp", ["-x", "hip"]),
     ("og_celeste.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ("og_server.cpp", ["-x", "hip"]),

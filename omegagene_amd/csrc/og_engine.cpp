@@ -181,6 +181,8 @@ void og_set_error(const char *msg) { g_last_error = msg; }
 
 // ----------------------------------------------------------------- types --
 enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND_TRIO, KIND_COMA, KIND_LSTEP,
+       KIND_BTERM,  // OG_THREADS threads per block over the bonded terms (angles, torsions, impropers, 1-4)
+       KIND_QUAD,   // OG_QUAD_THREADS threads per block over the four-atom SHAKE groups
        KIND_SCANB,  // blocks of 1024 cells over cap_cells + 1 (multi-block offset scan)
        KIND_PAIR,   // 12 pair-kernel maps: index log2(waves per block) + 4 * order (0 system order,
                     // 1 round-robin across systems, 2 each system's blocks on one XCD)
@@ -220,6 +222,7 @@ struct og_exec {
     int                      pair_form = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false, any_thermo = false;
+    bool                     any_bonded = false, any_quad = false;
     bool                     fused = false;       // every member takes the fused Langevin step
     int                      pair_shape = OG_PAIR_DEFAULT;  // pair-kernel launch shape (KIND_PAIR + pair_shape)
     int                      intvl = 1;
@@ -265,6 +268,7 @@ og_status exec_prepare(og_exec &x) {
     std::vector<int2>       maps[KIND_N];
     x.pair_form   = x.members[0]->pair_form;
     x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = x.any_thermo = false;
+    x.any_bonded = x.any_quad = false;
     x.fused        = true;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
@@ -285,6 +289,8 @@ og_status exec_prepare(og_exec &x) {
         x.any_constraint |= d.constraint != 0;
         x.any_vcmd |= d.vc_n_states > 0;
         x.any_thermo |= d.thermostat == OG_THERMOSTAT_SCALING;
+        x.any_bonded |= d.n_bterms > 0;
+        x.any_quad |= d.constraint != 0 && d.n_quads > 0;
         x.fused &= d.integrator == 1 && d.n_water_blocks > 0;
         const int n_bins = (d.n_columns + 1) * d.n_uni_z;
         int       cnt[KIND_N];
@@ -297,6 +303,8 @@ og_status exec_prepare(og_exec &x) {
         cnt[KIND_TRIO]  = cdiv(std::max(d.n_trios, d.n_settle), OG_TRIO_THREADS);
         cnt[KIND_COMA] = 0;  // filled below
         cnt[KIND_LSTEP] = d.n_water_blocks ? d.n_water_blocks + 1 : 0;  // free-atom block + water blocks
+        cnt[KIND_BTERM] = cdiv(d.n_bterms, OG_THREADS);
+        cnt[KIND_QUAD]  = d.constraint ? cdiv(d.n_quads, OG_QUAD_THREADS) : 0;
         cnt[KIND_SCANB] = d.cap_cells / 1024 + 1;
         for (int k = 0; k < KIND_PAIR; k++)
             for (int b = 0; b < cnt[k]; b++) maps[k].push_back(make_int2(si, b));
@@ -380,6 +388,9 @@ void enqueue_forces(og_exec &x, hipEvent_t e0, hipEvent_t e1, bool reduce = fals
     if (e0) hipEventRecord(e0, x.stream);
     og_launch_pair(L(x, KIND_PAIR + x.pair_shape), x.pair_form, 1 << (x.pair_shape & 3));
     if (e1) hipEventRecord(e1, x.stream);
+    // angles, torsions, impropers, 1-4 pairs: each atom's shares, summed with its pair gradient below (or in
+    // the fused step kernel)
+    if (x.any_bonded) og_launch_bonded(L(x, KIND_BTERM));
     // an MD step of a fused context converts the gradient inside its step kernel
     if (reduce || !x.fused) og_launch_forces_to_work(L(x, KIND_ATOM));
     if (reduce) og_launch_energy_reduce(L(x, KIND_SYS));
@@ -406,6 +417,8 @@ void enqueue_integrate(og_exec &x) {
     }
     if (x.any_langevin) og_launch_langevin_positions(L(x, KIND_ATOM));
     if (x.any_constraint) og_launch_constraints(L(x, KIND_TRIO), L(x, KIND_MISC));
+    // four-atom groups after pairs and trios (ConstraintShake::apply_constraint, ConstraintShake.cpp:10-16)
+    if (x.any_quad) og_launch_shake_quad(L(x, KIND_QUAD));
     og_launch_finish(L(x, KIND_ATOM));
     og_launch_step_close(L(x, KIND_SYS));
 }
@@ -608,6 +621,12 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.n_pairs_c        = p->constraint ? d->n_shake_pairs : 0;
     h.n_trios          = p->constraint ? d->n_shake_trios : 0;
     h.n_settle         = p->constraint == OG_CONSTRAINT_SHAKE_SETTLE ? d->n_settle : 0;
+    h.n_quads          = p->constraint ? d->n_shake_quads : 0;
+    h.n_angles         = d->n_angles;
+    h.n_torsions       = d->n_torsions;
+    h.n_impros         = d->n_impros;
+    h.n_nb14           = d->n_nb14;
+    h.n_bterms         = h.n_angles + h.n_torsions + h.n_impros + h.n_nb14;
     h.n_com_groups     = p->com_cancel ? d->n_com_groups : 0;
     h.integrator       = p->integrator;
     h.constraint       = p->constraint;
@@ -701,6 +720,20 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_bsum, h.cap_cells / 1024 + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
               A(&h.fx, 4 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;
+    const int n_tor     = h.n_torsions + h.n_impros;
+    const int n_contrib = 3 * h.n_angles + 4 * n_tor + 2 * h.n_nb14;
+    if (h.n_bterms) {
+        ok = A(&h.ang_ids, std::max(3 * (size_t)h.n_angles, (size_t)1)) &&
+             A(&h.ang_par, std::max((size_t)h.n_angles, (size_t)1)) &&
+             A(&h.tor_ids, std::max(4 * (size_t)n_tor, (size_t)1)) && A(&h.tor_par, std::max((size_t)n_tor, (size_t)1)) &&
+             A(&h.nb14_ids, std::max(2 * (size_t)h.n_nb14, (size_t)1)) &&
+             A(&h.nb14_tp, std::max((size_t)h.n_nb14, (size_t)1)) &&
+             A(&h.nb14_coef, std::max((size_t)h.n_nb14, (size_t)1)) && A(&h.bcontrib, 3 * (size_t)n_contrib) &&
+             A(&h.bc_off, (size_t)n + 1) && A(&h.bc_at, (size_t)n_contrib);
+        if (!ok) return OG_E_DEVICE;
+    }
+    if (h.n_quads && !(A(&h.quad_ids, 4 * (size_t)h.n_quads) && A(&h.quad_d2, 6 * (size_t)h.n_quads)))
+        return OG_E_DEVICE;
     h.n_atom_blocks = cdiv(n, OG_THREADS);
     h.n_ke_parts    = std::max(h.n_atom_blocks, cdiv(h.n_settle, OG_WATERS_PER_BLOCK) + 1);
     if (h.thermostat == OG_THERMOSTAT_SCALING && !A(&h.th_part, h.n_atom_blocks)) return OG_E_DEVICE;
@@ -799,6 +832,59 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         st = csr(d->bond_ids, h.n_bonds, &h.bond_off, &h.bond_at);
         if (st != OG_OK) return st;
     }
+    if (h.n_bterms) {
+        // term tables; 1-4 type pairs in the renumbered LJ table; per-atom contribution lists in slot order
+        std::vector<int>     ids4(4 * (size_t)n_tor), tp(h.n_nb14), off(n + 1, 0), at(n_contrib);
+        std::vector<double2> ap(h.n_angles), cf(h.n_nb14);
+        std::vector<double4> tpar(n_tor);
+        for (int t = 0; t < h.n_angles; t++) ap[t] = make_double2(d->angle_k[t], d->angle_theta0[t]);
+        for (int t = 0; t < n_tor; t++) {
+            const bool    im  = t >= h.n_torsions;
+            const int     u   = im ? t - h.n_torsions : t;
+            const int    *src = im ? d->impro_ids : d->torsion_ids;
+            for (int k = 0; k < 4; k++) ids4[4 * (size_t)t + k] = src[4 * (size_t)u + k];
+            tpar[t] = im ? make_double4(d->impro_ene[u], (double)d->impro_overlaps[u], (double)d->impro_symmetry[u],
+                                        d->impro_phase[u])
+                         : make_double4(d->torsion_ene[u], (double)d->torsion_overlaps[u],
+                                        (double)d->torsion_symmetry[u], d->torsion_phase[u]);
+        }
+        for (int t = 0; t < h.n_nb14; t++) {
+            tp[t] = tnew[d->nb14_types[2 * t]] * nt + tnew[d->nb14_types[2 * t + 1]];
+            cf[t] = make_double2(d->nb14_coeff_vdw[t], d->nb14_coeff_ele[t]);
+        }
+        // slot -> atom, in slot order
+        std::vector<int> slot_atom;
+        slot_atom.reserve(n_contrib);
+        for (int t = 0; t < h.n_angles; t++)
+            for (int k = 0; k < 3; k++) slot_atom.push_back(d->angle_ids[3 * (size_t)t + k]);
+        for (int t = 0; t < n_tor; t++)
+            for (int k = 0; k < 4; k++) slot_atom.push_back(ids4[4 * (size_t)t + k]);
+        for (int t = 0; t < h.n_nb14; t++)
+            for (int k = 0; k < 2; k++) slot_atom.push_back(d->nb14_ids[2 * (size_t)t + k]);
+        for (int c = 0; c < n_contrib; c++) off[slot_atom[c] + 1]++;
+        for (int i = 0; i < n; i++) off[i + 1] += off[i];
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int c = 0; c < n_contrib; c++) at[fill[slot_atom[c]]++] = c;
+        if (h.n_angles) {
+            OG_HIP(hipMemcpy(h.ang_ids, d->angle_ids, sizeof(int) * 3 * h.n_angles, hipMemcpyHostToDevice));
+            OG_HIP(hipMemcpy(h.ang_par, ap.data(), sizeof(double2) * ap.size(), hipMemcpyHostToDevice));
+        }
+        if (n_tor) {
+            OG_HIP(hipMemcpy(h.tor_ids, ids4.data(), sizeof(int) * ids4.size(), hipMemcpyHostToDevice));
+            OG_HIP(hipMemcpy(h.tor_par, tpar.data(), sizeof(double4) * tpar.size(), hipMemcpyHostToDevice));
+        }
+        if (h.n_nb14) {
+            OG_HIP(hipMemcpy(h.nb14_ids, d->nb14_ids, sizeof(int) * 2 * h.n_nb14, hipMemcpyHostToDevice));
+            OG_HIP(hipMemcpy(h.nb14_tp, tp.data(), sizeof(int) * tp.size(), hipMemcpyHostToDevice));
+            OG_HIP(hipMemcpy(h.nb14_coef, cf.data(), sizeof(double2) * cf.size(), hipMemcpyHostToDevice));
+        }
+        OG_HIP(hipMemcpy(h.bc_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.bc_at, at.data(), sizeof(int) * at.size(), hipMemcpyHostToDevice));
+    }
+    if (h.n_quads) {
+        OG_HIP(hipMemcpy(h.quad_ids, d->shake_quad_ids, sizeof(int) * 4 * h.n_quads, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.quad_d2, d->shake_quad_d2, sizeof(double) * 6 * h.n_quads, hipMemcpyHostToDevice));
+    }
     if (h.n_trios) {
         OG_HIP(hipMemcpy(h.trio_ids, d->shake_trio_ids, sizeof(int) * 3 * h.n_trios, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.trio_d2, d->shake_trio_d2, sizeof(double) * 3 * h.n_trios, hipMemcpyHostToDevice));
@@ -831,7 +917,9 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         for (int t = 0; t < h.n_excess && fuse; t++) fuse = water[d->excess[2 * t]] == water[d->excess[2 * t + 1]];
         fuse = fuse && outside(d->bond_ids, 2LL * h.n_bonds) && outside(d->shake_pair_ids, 2LL * h.n_pairs_c) &&
                outside(d->shake_trio_ids, 3LL * h.n_trios) && outside(d->vc_atoms_a, h.vc_na) &&
-               outside(d->vc_atoms_b, h.vc_nb);
+               outside(d->vc_atoms_b, h.vc_nb) && outside(d->angle_ids, 3LL * h.n_angles) &&
+               outside(d->torsion_ids, 4LL * h.n_torsions) && outside(d->impro_ids, 4LL * h.n_impros) &&
+               outside(d->nb14_ids, 2LL * h.n_nb14) && h.n_quads == 0;  // the step kernel runs no four-atom SHAKE
         std::vector<int> free_ids;
         for (int i = 0; i < n && fuse; i++)
             if (water[i] < 0) free_ids.push_back(i);
@@ -1086,6 +1174,13 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     b += pairs * 12;
     b += 8.0 * OG_EACC_SLOTS * OG_EACC_STRIDE + ke_parts * 8 + OG_E_NTERMS * 8 + log_rows * OG_E_NTERMS * 8;
     b += std::min(n, (double)OG_FREE_MAX) * 4 + vc * 28 + vs * 32;
+    {
+        const double na = d->n_angles, ntr = (double)d->n_torsions + d->n_impros, n14 = d->n_nb14;
+        const double nq = p->constraint ? d->n_shake_quads : 0;
+        const double contrib = 3 * na + 4 * ntr + 2 * n14;
+        if (na + ntr + n14 > 0) b += na * (12 + 16) + ntr * (16 + 32) + n14 * (8 + 4 + 16) + contrib * (24 + 4) + (n + 1) * 4;
+        b += nq * (16 + 48);
+    }
     if (p->thermostat == OG_THERMOSTAT_SCALING) b += std::ceil(n / OG_THREADS) * 8;   // thermostat partials
     b += sizeof(og_dev_counters) + sizeof(og_dev_sys);
     *bytes = (uint64_t)(b * 1.02) + 65536;                                // allocation granularity
@@ -1135,10 +1230,54 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
             (d->n_bonds && bad_ids(d->bond_ids, 2LL * d->n_bonds)) ||
             (p->constraint && d->n_shake_pairs && bad_ids(d->shake_pair_ids, 2LL * d->n_shake_pairs)) ||
             (p->constraint && d->n_shake_trios && bad_ids(d->shake_trio_ids, 3LL * d->n_shake_trios)) ||
-            (p->constraint == OG_CONSTRAINT_SHAKE_SETTLE && d->n_settle && bad_ids(d->settle_ids, 3LL * d->n_settle))) {
+            (p->constraint == OG_CONSTRAINT_SHAKE_SETTLE && d->n_settle && bad_ids(d->settle_ids, 3LL * d->n_settle)) ||
+            (p->constraint && d->n_shake_quads && bad_ids(d->shake_quad_ids, 4LL * d->n_shake_quads))) {
             set_err("an atom index in the excess/bond/constraint lists is out of range");
             return OG_E_ARG;
         }
+        if (d->n_angles < 0 || d->n_torsions < 0 || d->n_impros < 0 || d->n_nb14 < 0 || d->n_shake_quads < 0 ||
+            (d->n_angles && (!d->angle_ids || !d->angle_k || !d->angle_theta0)) ||
+            (d->n_torsions && (!d->torsion_ids || !d->torsion_ene || !d->torsion_overlaps || !d->torsion_symmetry ||
+                               !d->torsion_phase)) ||
+            (d->n_impros && (!d->impro_ids || !d->impro_ene || !d->impro_overlaps || !d->impro_symmetry ||
+                             !d->impro_phase)) ||
+            (d->n_nb14 && (!d->nb14_ids || !d->nb14_types || !d->nb14_coeff_vdw || !d->nb14_coeff_ele)) ||
+            (p->constraint && d->n_shake_quads && !d->shake_quad_d2)) {
+            set_err("og_system_create: a bonded-term count without its tables");
+            return OG_E_ARG;
+        }
+        if ((d->n_angles && bad_ids(d->angle_ids, 3LL * d->n_angles)) ||
+            (d->n_torsions && bad_ids(d->torsion_ids, 4LL * d->n_torsions)) ||
+            (d->n_impros && bad_ids(d->impro_ids, 4LL * d->n_impros)) ||
+            (d->n_nb14 && bad_ids(d->nb14_ids, 2LL * d->n_nb14))) {
+            set_err("an atom index in the angle/torsion/improper/1-4 lists is out of range");
+            return OG_E_ARG;
+        }
+        // torsion multiplicity indexes the reference's 0..10 table (ForceField.cpp:7-17); a term with
+        // V > 1e-10 divides by its overlap count
+        for (int pass = 0; pass < 2; pass++) {
+            const int     nt  = pass ? d->n_impros : d->n_torsions;
+            const int    *sym = pass ? d->impro_symmetry : d->torsion_symmetry;
+            const int    *ov  = pass ? d->impro_overlaps : d->torsion_overlaps;
+            const double *ene = pass ? d->impro_ene : d->torsion_ene;
+            for (int t = 0; t < nt; t++)
+                if (sym[t] < 0 || sym[t] > 10 || (ene[t] > 1e-10 && ov[t] == 0)) {
+                    set_err("%s %d: multiplicity outside 0..10 or zero overlap count", pass ? "improper" : "torsion", t);
+                    return OG_E_ARG;
+                }
+        }
+        for (int t = 0; t < d->n_nb14; t++)
+            if (d->nb14_types[2 * t] < 0 || d->nb14_types[2 * t] >= d->n_lj_types || d->nb14_types[2 * t + 1] < 0 ||
+                d->nb14_types[2 * t + 1] >= d->n_lj_types) {
+                set_err("1-4 pair %d: LJ type out of range", t);
+                return OG_E_ARG;
+            }
+        if (p->constraint)
+            for (long long k = 0; k < 6LL * d->n_shake_quads; k++)
+                if (!(d->shake_quad_d2[k] > 0.0)) {
+                    set_err("four-atom SHAKE group %lld: non-positive target distance", k / 6);
+                    return OG_E_ARG;
+                }
         if (p->constraint == OG_CONSTRAINT_SHAKE_SETTLE && d->n_settle &&
             !(d->settle_dOH > 0.0 && d->settle_dHH > 0.0 && d->settle_dHH < 2.0 * d->settle_dOH)) {
             set_err("SETTLE needs 0 < dHH < 2 dOH");

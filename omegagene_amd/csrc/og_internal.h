@@ -39,6 +39,11 @@ enum {
     OG_E_NPAIRS,     // in-cutoff unmasked atom pairs evaluated (exact count, stored as double)
     OG_E_LAMBDA,     // VcMD reaction coordinate
     OG_E_VSTATE,     // VcMD virtual state during the step
+    OG_E_ANGLE,      // harmonic angles
+    OG_E_TORSION,    // torsions
+    OG_E_IMPRO,      // improper torsions
+    OG_E_VDW14,      // scaled 1-4 LJ
+    OG_E_ELE14,      // scaled 1-4 Coulomb
     OG_E_NTERMS
 };
 
@@ -74,6 +79,9 @@ struct og_dev_sys {
     int cap_atom_array, cap_cells, cap_cell_pairs;
     int n_types;
     int n_excess, n_bonds, n_trios, n_pairs_c, n_settle;
+    int n_angles, n_torsions, n_impros, n_nb14;  // bonded terms beyond bonds (og_bonded.hip)
+    int n_bterms;          // n_angles + n_torsions + n_impros + n_nb14: one thread each
+    int n_quads;           // SHAKE four-atom groups
     int n_free;            // atoms outside SETTLE waters (fused Langevin step)
     int n_water_blocks;    // fused Langevin step: blocks of OG_WATERS_PER_BLOCK waters; 0 = not fusable
     int n_com_groups;
@@ -133,6 +141,22 @@ struct og_dev_sys {
     double *bond_k, *bond_r0;
     int    *exc_off, *bond_off;  // per-atom CSR offsets of the excess pairs / bonds: [n_atoms+1]
     int4   *exc_at, *bond_at;    // entries {a, b, pair index, 0}: [2*n_excess], [2*n_bonds]
+    // bonded terms: each term writes the gradient share of each of its atoms
+    // (its "contributions") to bcontrib; an atom sums its shares through the
+    // per-atom list bc_off / bc_at in ascending slot order.  Slots: angle t
+    // 3t..3t+2, then torsions (4 each), impropers (4 each), 1-4 pairs (2 each).
+    int     *ang_ids;       // [n_angles*3]
+    double2 *ang_par;       // {k, theta0}
+    int     *tor_ids;       // [(n_torsions+n_impros)*4], torsions then impropers
+    double4 *tor_par;       // {V, overlaps, symmetry, phase}
+    int     *nb14_ids;      // [n_nb14*2]
+    int     *nb14_tp;       // index of the entry's type pair in lj6 / lj12
+    double2 *nb14_coef;     // {LJ scale, Coulomb scale}
+    double  *bcontrib;      // [n_contrib*3]
+    int     *bc_off;        // [n_atoms+1]
+    int     *bc_at;         // [n_contrib] contribution slots of each atom
+    int    *quad_ids;       // [n_quads*4]
+    double *quad_d2;        // [n_quads*6]
     int    *trio_ids;       // [n_trios*3]
     double *trio_d2;        // [n_trios*3]
     int    *pairc_ids;      // [n_pairs_c*2]
@@ -199,6 +223,7 @@ enum og_map_kind {
 
 #define OG_THREADS 256
 #define OG_TRIO_THREADS 128
+#define OG_QUAD_THREADS 64
 #define OG_WATER_THREADS 256  // fused Langevin step: threads per block
 #define OG_WATERS_PER_WAVE 21  // fused Langevin step: three lanes per water
 #define OG_WATERS_PER_BLOCK (OG_WATERS_PER_WAVE * (OG_WATER_THREADS / 64))
@@ -283,9 +308,10 @@ __device__ __forceinline__ void og_fx_read(const long long *fx, int cap, int f, 
 #define OG_GLOBAL(T) __attribute__((address_space(1))) T
 
 #define OG_RED_THREADS 1024  // per-system reduction block
-enum { OG_EACC_VDW, OG_EACC_ELE, OG_EACC_VIR, OG_EACC_PAIRS, OG_EACC_EXCESS, OG_EACC_BOND, OG_EACC };
+enum { OG_EACC_VDW, OG_EACC_ELE, OG_EACC_VIR, OG_EACC_PAIRS, OG_EACC_EXCESS, OG_EACC_BOND, OG_EACC_ANGLE,
+       OG_EACC_TORSION, OG_EACC_IMPRO, OG_EACC_VDW14, OG_EACC_ELE14, OG_EACC };
 #define OG_EACC_SLOTS 256   // accumulator slots per system
-#define OG_EACC_STRIDE 8    // long longs per slot (one 64-byte line)
+#define OG_EACC_STRIDE 16   // long longs per slot (two 64-byte lines: pair terms, then the others)
 
 // Pair, excluded-pair and bond energies of one system into s.energy from
 // the fixed-point accumulator slots (integer adds: the same bits whatever the
@@ -319,6 +345,11 @@ __device__ __forceinline__ void og_take_pair_energies(const og_dev_sys &s, long 
         s.energy[OG_E_NPAIRS] = (double)t[OG_EACC_PAIRS];
         s.energy[OG_E_EXCESS] = (double)t[OG_EACC_EXCESS] * OG_FX_INV;
         s.energy[OG_E_BOND]   = (double)t[OG_EACC_BOND] * OG_FX_INV;
+        s.energy[OG_E_ANGLE]   = (double)t[OG_EACC_ANGLE] * OG_FX_INV;
+        s.energy[OG_E_TORSION] = (double)t[OG_EACC_TORSION] * OG_FX_INV;
+        s.energy[OG_E_IMPRO]   = (double)t[OG_EACC_IMPRO] * OG_FX_INV;
+        s.energy[OG_E_VDW14]   = (double)t[OG_EACC_VDW14] * OG_FX_INV;
+        s.energy[OG_E_ELE14]   = (double)t[OG_EACC_ELE14] * OG_FX_INV;
         s.energy[OG_E_SELF]   = s.e_self;
     }
     __syncthreads();
@@ -379,6 +410,9 @@ void og_launch_thermostat(const og_launch &atoms);  // scaling thermostat: kinet
 void og_launch_positions(const og_launch &atoms);
 void og_launch_langevin_positions(const og_launch &atoms);
 void og_launch_constraints(const og_launch &trio, const og_launch &misc);
+// bonded terms beyond bonds and four-atom SHAKE (og_bonded.hip)
+void og_launch_bonded(const og_launch &terms);  // OG_THREADS threads per block over n_bterms
+void og_launch_shake_quad(const og_launch &quads);  // OG_QUAD_THREADS threads per block over n_quads
 void og_launch_langevin_step(const og_launch &a);
 void og_launch_finish(const og_launch &atoms);
 void og_launch_step_close(const og_launch &sys);  // energies of the step, log entry, step counter

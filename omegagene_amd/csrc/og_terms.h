@@ -180,6 +180,15 @@ __device__ __forceinline__ void og_atom_work_calc(const og_dev_sys &s, const int
                 }
             }
     }
+    // angles, torsions, impropers and 1-4 pairs: the atom's shares, written
+    // by k_bonded earlier in the evaluation, in ascending slot order
+    if (s.n_bterms) {
+        const int c0 = s.bc_off[i], c1 = s.bc_off[i + 1];
+        for (int e = c0; e < c1; e++) {
+            const double *c = s.bcontrib + 3 * (size_t)s.bc_at[e];
+            for (int k = 0; k < 3; k++) w[k] += c[k];
+        }
+    }
 }
 
 // Stores atom i's work and clears its fixed-point accumulators for the next

@@ -19,8 +19,20 @@ LIB_PATH = os.path.join(_HERE, "libog_engine.so")
 _LIB = None
 
 OG_OK, OG_E_ARG, OG_E_CAPACITY, OG_E_NAN, OG_E_NOCONV, OG_E_DEVICE, OG_E_STATE = range(7)
-TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend", "npairs", "lambda", "vstate"]
+TERMS = ["vdw", "ele", "excess", "self", "bond", "kinetic", "virial", "extend", "npairs", "lambda", "vstate",
+         "angle", "torsion", "impro", "vdw14", "ele14"]
 NTERMS = len(TERMS)
+# the potential-energy terms (MmSystem's pote_* sum the reference prints as "Potential", DynamicsMode.cpp:272-300)
+POTENTIAL_TERMS = ["vdw", "ele", "excess", "self", "bond", "angle", "torsion", "impro", "vdw14", "ele14"]
+_POT_COLS = [TERMS.index(t) for t in POTENTIAL_TERMS]
+
+
+def potential(rows):
+    """Potential energy of energy-log rows (or one row / an energies dict)."""
+    if isinstance(rows, dict):
+        return sum(rows[t] for t in POTENTIAL_TERMS)
+    r = np.asarray(rows)
+    return r[..., _POT_COLS].sum(axis=-1)
 INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
 THERMOSTATS = {"none": 0, "scaling": 1}
@@ -51,7 +63,19 @@ class Desc(C.Structure):
                 ("vc_q_cano", C.POINTER(C.c_double)), ("vc_init_state", C.c_int), ("vc_interval", C.c_int),
                 ("vc_n_a", C.c_int), ("vc_n_b", C.c_int), ("vc_atoms_a", C.POINTER(C.c_int)),
                 ("vc_atoms_b", C.POINTER(C.c_int)), ("hps_cutoff", C.POINTER(C.c_double)),
-                ("hps_lambda", C.POINTER(C.c_double))]
+                ("hps_lambda", C.POINTER(C.c_double)),
+                ("n_angles", C.c_int), ("angle_ids", C.POINTER(C.c_int)), ("angle_k", C.POINTER(C.c_double)),
+                ("angle_theta0", C.POINTER(C.c_double)),
+                ("n_torsions", C.c_int), ("torsion_ids", C.POINTER(C.c_int)), ("torsion_ene", C.POINTER(C.c_double)),
+                ("torsion_overlaps", C.POINTER(C.c_int)), ("torsion_symmetry", C.POINTER(C.c_int)),
+                ("torsion_phase", C.POINTER(C.c_double)),
+                ("n_impros", C.c_int), ("impro_ids", C.POINTER(C.c_int)), ("impro_ene", C.POINTER(C.c_double)),
+                ("impro_overlaps", C.POINTER(C.c_int)), ("impro_symmetry", C.POINTER(C.c_int)),
+                ("impro_phase", C.POINTER(C.c_double)),
+                ("n_nb14", C.c_int), ("nb14_ids", C.POINTER(C.c_int)), ("nb14_types", C.POINTER(C.c_int)),
+                ("nb14_coeff_vdw", C.POINTER(C.c_double)), ("nb14_coeff_ele", C.POINTER(C.c_double)),
+                ("n_shake_quads", C.c_int), ("shake_quad_ids", C.POINTER(C.c_int)),
+                ("shake_quad_d2", C.POINTER(C.c_double))]
 
 
 class Params(C.Structure):
@@ -184,6 +208,25 @@ def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcm
     if ls.hps_cutoff is not None:
         x.hps_cutoff = d(ls.hps_cutoff)
         x.hps_lambda = d(ls.hps_lambda)
+    # angles, torsions, impropers, 1-4 pairs and four-atom SHAKE groups of the launch set
+    bt = ls.bonded_tables()
+    ids, kk, th = bt["angle"]
+    if len(ids):
+        x.n_angles, x.angle_ids, x.angle_k, x.angle_theta0 = len(ids), d(ids, np.int32), d(kk), d(th)
+    for key in ("torsion", "impro"):
+        ids, e, ov, sy, ph = bt[key]
+        if len(ids):
+            setattr(x, "n_%ss" % key, len(ids))
+            for name, arr, dt in (("ids", ids, np.int32), ("ene", e, np.float64), ("overlaps", ov, np.int32),
+                                  ("symmetry", sy, np.int32), ("phase", ph, np.float64)):
+                setattr(x, "%s_%s" % (key, name), d(arr, dt))
+    ids, ty, cv, ce = bt["nb14"]
+    if len(ids):
+        x.n_nb14, x.nb14_ids, x.nb14_types = len(ids), d(ids, np.int32), d(ty, np.int32)
+        x.nb14_coeff_vdw, x.nb14_coeff_ele = d(cv), d(ce)
+    qids, qd2 = ls.shake_quad_tables()
+    if len(qids):
+        x.n_shake_quads, x.shake_quad_ids, x.shake_quad_d2 = len(qids), d(qids, np.int32), d(qd2)
     vc = ls.vcmd if vcmd else None
     if vc is not None:
         # 1-D VcMD on the distance between the mass centres of the dimension's two groups (Read.cpp:705-779)

@@ -18,7 +18,7 @@ import time
 
 import numpy as np
 
-from .engine import _check, lib
+from .engine import _check, lib, potential
 
 KINETIC_TO_K = 2.0 * 4.184 * 1e3 / 8.31451  # temperature_coeff * d_free (DynamicsMode.cpp:67)
 TITLE = "omegagene_amd (MI355X)"
@@ -89,8 +89,7 @@ def run_md(system, ls, cfg: dict, n_steps: int):
 
     def last_energies():
         row = system.energy_log(max_rows=1)[-1]
-        pot = row[0] + row[1] + row[2] + row[3] + row[4]
-        return pot, row[5], row[0], row[7]
+        return float(potential(row)), row[5], row[0], row[7]
 
     def frame(k):
         crd = _crd_prev(system)
