@@ -39,8 +39,16 @@ class Cells(C.Structure):
 
 
 class ZD(C.Structure):
+    """orc_zd_t: zero-multipole constants of the selected order."""
     _fields_ = [("cutoff", C.c_float), ("alpha", C.c_float), ("fcoeff", C.c_float), ("bcoeff", C.c_float),
-                ("zcore", C.c_float), ("piewald", C.c_float)]
+                ("zcore", C.c_float), ("piewald", C.c_float), ("mode", C.c_int), ("zo_grad_bug", C.c_int),
+                ("zmcore", C.c_float)] + [(k, C.c_float) for k in (
+                    "zq2", "zq4", "zq22", "zq44", "zo2", "zo4", "zo6", "zo22", "zo44", "zo66",
+                    "zh2", "zh4", "zh6", "zh8", "zh22", "zh44", "zh66", "zh88")]
+
+
+# --electrostatic orders of ZD.mode (Config.cpp:114-128)
+ZM_MODES = {"zero-dipole": 0, "zero-quadrupole": 1, "zero-octupole": 2, "zero-hexadecapole": 3}
 
 
 class FF(C.Structure):
@@ -217,9 +225,11 @@ class Oracle:
         return work, en, npair
 
 
-def zd_setup(cutoff, alpha) -> ZD:
+def zd_setup(cutoff, alpha, mode=0, zo_grad_bug=True) -> ZD:
+    """orc_zm_setup: the constants of zero-multipole order `mode` (ZM_MODES)."""
     z = ZD()
-    lib().orc_zd_setup(C.byref(z), C.c_double(cutoff), C.c_double(alpha))
+    lib().orc_zm_setup(C.byref(z), C.c_double(cutoff), C.c_double(alpha), C.c_int(int(mode)),
+                       C.c_int(int(zo_grad_bug)))
     return z
 
 

@@ -92,12 +92,24 @@ typedef struct {
 int orc_enumerate_cell_pairs(const orc_grid_t *g, const orc_cells_t *c, const int *nb15off,
                              int gap_bug, orc_cellpair_t *out, int cap, int *idx_head /* [n_cells+1] */);
 
-/* Zero-dipole constants (ZeroMultipoleSum.cpp:108-190; float members). */
+/* Zero-multipole constants (ZeroMultipoleSum.cpp:108-190; float members).
+ * orc_zd_setup fills the dipole members (mode ORC_ZM_DIPOLE); orc_zm_setup
+ * every order's and selects one (--electrostatic zero-dipole / -quadrupole /
+ * -octupole / -hexadecapole, Config.cpp:114-128). */
+enum { ORC_ZM_DIPOLE = 0, ORC_ZM_QUADRUPOLE = 1, ORC_ZM_OCTUPOLE = 2, ORC_ZM_HEXADECAPOLE = 3 };
 typedef struct {
     float cutoff, alpha;
     float fcoeff, bcoeff, zcore, piewald;
+    int   mode;         /* ORC_ZM_* */
+    int   zo_grad_bug;  /* 1: the octupole alpha > 0 pair gradient takes the hexadecapole coefficients
+                           (ZeroMultipoleSum.cpp:404-406, the reference's form) */
+    float zmcore;       /* the selected order's core constant (cal_self_energy :72-80) */
+    float zq2, zq4, zq22, zq44;
+    float zo2, zo4, zo6, zo22, zo44, zo66;
+    float zh2, zh4, zh6, zh8, zh22, zh44, zh66, zh88;
 } orc_zd_t;
 void orc_zd_setup(orc_zd_t *zd, double cutoff, double alpha);
+void orc_zm_setup(orc_zd_t *zd, double cutoff, double alpha, int mode, int zo_grad_bug);
 
 /* Self energy (ZeroMultipoleSum.cpp:43-106). Returns the sum of per-atom terms
  * (d_self_mon, stored in a float member) and writes the per-atom FP64 terms. */

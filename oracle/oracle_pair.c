@@ -16,9 +16,15 @@ static const double CHARGE_COEFF = 332.06378; /* CelesteObject.cpp:36 */
 static const double PI_REF       = 3.14159265; /* CelesteObject.cpp:26 */
 static const double EPS_REF      = 1e-10;      /* CelesteObject.cpp:31 */
 
-/* ZeroMultipoleSum::set_zms_params, dipole members only (ZeroMultipoleSum.cpp:108-190).
- * All members are real_pw (float). */
-void orc_zd_setup(orc_zd_t *zd, double cutoff, double alpha) {
+void orc_zd_setup(orc_zd_t *zd, double cutoff, double alpha) { orc_zm_setup(zd, cutoff, alpha, ORC_ZM_DIPOLE, 1); }
+
+/* ZeroMultipoleSum::set_zms_params (ZeroMultipoleSum.cpp:108-190).  Every
+ * member is real_pw (float); the double literals promote their expressions,
+ * which round to float on assignment. */
+void orc_zm_setup(orc_zd_t *zd, double cutoff, double alpha, int mode, int zo_grad_bug) {
+    memset(zd, 0, sizeof *zd);
+    zd->mode        = mode;
+    zd->zo_grad_bug = zo_grad_bug;
     zd->cutoff      = (float)cutoff;
     zd->alpha       = (float)alpha;
     float cutoff_2  = zd->cutoff * zd->cutoff;
@@ -37,6 +43,116 @@ void orc_zd_setup(orc_zd_t *zd, double cutoff, double alpha) {
         zd->bcoeff    = (float)(0.5 * zd->fcoeff);
     }
     zd->zcore = errorfc / zd->cutoff + zd->bcoeff * cutoff_2;
+    /* the higher orders: derivative coefficients of erfc(a r)/r at the cutoff, then the polynomial
+     * coefficients and cores */
+    const float c = zd->cutoff, c4 = cutoff_2 * cutoff_2, c5 = cutoff_3 * cutoff_2, c6 = cutoff_3 * cutoff_3,
+                c7 = cutoff_3 * c4;
+    float hzqrc, d1, d2, d3, d4;
+    if (zd->alpha < EPS_REF) {
+        hzqrc = (float)(2.0 / cutoff_3);
+        d1    = (float)(1.0 / cutoff_2);
+        d2    = (float)(2.0 / cutoff_3);
+        d3    = (float)(6.0 / c4);
+        d4    = (float)(24.0 / c5);
+    } else {
+        float root_pi = (float)sqrt(PI_REF);
+        float t3      = tmp * tmp * tmp;
+        float t5      = t3 * tmp * tmp;
+        float t7      = t5 * tmp * tmp;
+        hzqrc         = (float)((errorfc + expterm * (tmp + t3) * 2.0 / root_pi) * 2.0 / cutoff_3);
+        d1            = (float)((errorfc + expterm * tmp * 2.0 / root_pi) / cutoff_2);
+        d2            = (float)((errorfc + expterm * (tmp + t3) * 2.0 / root_pi) * 2.0 / cutoff_3);
+        d3 = (float)((errorfc + expterm * (tmp + 2.0 / 3.0 * t3 + 2.0 / 3.0 * t5) * 2.0 / root_pi) * 6.0 / c4);
+        /* the reference's series (ZeroMultipoleSum.cpp:152-154), its "1/3 + t7" term as written */
+        d4 = (float)((errorfc + expterm * (tmp + 2.0 / 3.0 * t3 + 1.0 / 6.0 * t5 + 1.0 / 3.0 + t7) * 2.0 / root_pi) *
+                     24.0 / c5);
+    }
+    zd->zq2  = (float)(-3.0 / 4.0 * zd->fcoeff - hzqrc / 4.0);
+    zd->zq4  = (float)(zd->fcoeff / 8.0 / cutoff_2 + hzqrc / 8.0 / cutoff_2);
+    zd->zq22 = (float)(2.0 * zd->zq2);
+    zd->zq44 = (float)(4.0 * zd->zq4);
+    zd->zo2  = (float)(-15.0 / 16.0 * d1 / c - 7.0 / 16.0 * d2 - 1.0 / 16.0 * d3 * c);
+    zd->zo4  = (float)(5.0 / 16.0 * d1 / cutoff_3 + 5.0 / 16.0 * d2 / cutoff_2 + 1.0 / 16.0 * d3 / c);
+    zd->zo6  = (float)(-1.0 / 16.0 * d1 / c5 - 1.0 / 16.0 * d2 / c4 - 1.0 / 48.0 * d3 / cutoff_3);
+    zd->zo22 = (float)(2.0 * zd->zo2);
+    zd->zo44 = (float)(4.0 * zd->zo4);
+    zd->zo66 = (float)(6.0 * zd->zo6);
+    zd->zh2  = (float)(-35.0 / 32.0 * d1 / c - 19.0 / 32.0 * d2 - 1.0 / 96.0 * d4 * cutoff_2 - 1.0 / 8.0 * d3 * c);
+    zd->zh4  = (float)(1.0 / 64.0 * d4 + 35.0 / 64.0 * d1 / cutoff_3 + 35.0 / 64.0 * d2 / cutoff_2 +
+                      5.0 / 32.0 * d3 / c);
+    zd->zh6  = (float)(-7.0 / 32.0 * d1 / c5 - 7.0 / 32.0 * d2 / c4 - 1.0 / 12.0 * d3 / cutoff_3 -
+                      1.0 / 96.0 * d4 / cutoff_2);
+    zd->zh8  = (float)(5.0 / 128.0 * d2 / c6 + 5.0 / 128.0 * d1 / c7 + 1.0 / 64.0 * d3 / c5 + 1.0 / 384.0 * d4 / c4);
+    zd->zh22 = (float)(2.0 * zd->zh2);
+    zd->zh44 = (float)(4.0 * zd->zh4);
+    zd->zh66 = (float)(6.0 * zd->zh6);
+    zd->zh88 = (float)(8.0 * zd->zh8);
+    switch (mode) {
+    case ORC_ZM_QUADRUPOLE: zd->zmcore = errorfc / c - zd->zq2 * cutoff_2 - zd->zq4 * c4; break;
+    case ORC_ZM_OCTUPOLE: zd->zmcore = errorfc / c - zd->zo2 * cutoff_2 - zd->zo4 * c4 - zd->zo6 * c6; break;
+    case ORC_ZM_HEXADECAPOLE:
+        zd->zmcore = errorfc / c - zd->zh2 * cutoff_2 - zd->zh4 * c4 - zd->zh6 * c6 - zd->zh8 * c6 * cutoff_2;
+        break;
+    default: zd->zmcore = zd->zcore; break;
+    }
+}
+
+/* The electrostatic term of ForceField::calc_pairwise for the selected order:
+ * calc_zero02pole(_alpha0) ... calc_zero16pole(_alpha0) (ZeroMultipoleSum.cpp:309-442),
+ * float locals. */
+static void zm_pair(const orc_zd_t *zd, float r12, float r12_2, float r12_inv, float r12_2_inv, float r12_3_inv,
+                    float cc, float *e, float *g) {
+    const int   a0    = zd->alpha < EPS_REF;
+    const float r12_4 = r12_2 * r12_2;
+    const float r12_6 = r12_4 * r12_2;
+    const float r12_8 = r12_4 * r12_4;
+    float       tmp = 0.0f, errorfc = 1.0f;
+    if (!a0) {
+        tmp     = r12 * zd->alpha;
+        errorfc = erfcf(tmp);
+    }
+    switch (zd->mode) {
+    case ORC_ZM_QUADRUPOLE:
+        if (a0) {
+            *e = cc * (r12_inv - zd->zmcore - zd->zq2 * r12_2 - zd->zq4 * r12_4);
+            *g = -cc * (r12_3_inv + zd->zq22 + zd->zq44 * r12_2);
+        } else {
+            *e = cc * (r12_inv * errorfc - zd->zmcore - zd->zq2 * r12_2 - zd->zq4 * r12_4);
+            *g = -cc * (r12_3_inv * errorfc + zd->piewald * expf(-tmp * tmp) * r12_2_inv + zd->zq22 + zd->zq44 * r12_2);
+        }
+        break;
+    case ORC_ZM_OCTUPOLE:
+        if (a0) {
+            *e = cc * (r12_inv - zd->zmcore - zd->zo2 * r12_2 - zd->zo4 * r12_4 - zd->zo6 * r12_6);
+            *g = -cc * (r12_3_inv + zd->zo22 + zd->zo44 * r12_2 + zd->zo66 * r12_4);
+        } else {
+            const float g2 = zd->zo_grad_bug ? zd->zh22 : zd->zo22, g4 = zd->zo_grad_bug ? zd->zh44 : zd->zo44,
+                        g6 = zd->zo_grad_bug ? zd->zh66 : zd->zo66;
+            *e = cc * (r12_inv * errorfc - zd->zmcore - zd->zo2 * r12_2 - zd->zo4 * r12_4 - zd->zo6 * r12_6);
+            *g = -cc * (r12_3_inv * errorfc + zd->piewald * expf(-tmp * tmp) * r12_2_inv + g2 + g4 * r12_2 + g6 * r12_4);
+        }
+        break;
+    case ORC_ZM_HEXADECAPOLE:
+        if (a0) {
+            *e = cc * (r12_inv - zd->zmcore - zd->zh2 * r12_2 - zd->zh4 * r12_4 - zd->zh6 * r12_6 - zd->zh8 * r12_8);
+            *g = -cc * (r12_3_inv + zd->zh22 + zd->zh44 * r12_2 + zd->zh66 * r12_4 + zd->zh88 * r12_6);
+        } else {
+            *e = cc * (r12_inv * errorfc - zd->zmcore - zd->zh2 * r12_2 - zd->zh4 * r12_4 - zd->zh6 * r12_6 -
+                       zd->zh8 * r12_8);
+            *g = -cc * (r12_3_inv * errorfc + zd->piewald * expf(-tmp * tmp) * r12_2_inv + zd->zh22 +
+                        zd->zh44 * r12_2 + zd->zh66 * r12_4 + zd->zh88 * r12_6);
+        }
+        break;
+    default:
+        if (a0) {
+            *e = cc * (r12_inv - zd->zcore + zd->bcoeff * r12_2);
+            *g = -cc * (r12_3_inv - zd->fcoeff);
+        } else {
+            *e = cc * (r12_inv * errorfc - zd->zcore + zd->bcoeff * r12_2);
+            *g = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff);
+        }
+        break;
+    }
 }
 
 /* ZeroMultipoleSum::cal_self_energy (ZeroMultipoleSum.cpp:43-106), dipole case. */
@@ -46,8 +162,8 @@ double orc_self_energy(const orc_zd_t *zd, int n, const float *q, int n_excess, 
         dwork[ex[2 * i]] += q[ex[2 * i + 1]];
         dwork[ex[2 * i + 1]] += q[ex[2 * i]];
     }
-    double zmcore = zd->zcore;
-    float  mon    = 0.0f; /* d_self_mon is a real_pw member */
+    double zmcore = zd->zmcore; /* the selected order's core (ZeroMultipoleSum.cpp:72-80) */
+    float  mon    = 0.0f;       /* d_self_mon is a real_pw member */
     for (int i = 0; i < n; i++) {
         float  pq = -zd->piewald * q[i] * q[i];
         double e  = ((double)pq - zmcore * q[i] * (q[i] + dwork[i])) * 0.5 * CHARGE_COEFF;
@@ -95,15 +211,7 @@ int orc_pair_fp32(const orc_zd_t *zd, const float d12[3], float a6, float a12, f
     float work_coef  = (float)((double)r12_2_inv * (-12.0 * (double)term12 + 6.0 * (double)term6));
     float cc         = (float)((double)(q1 * q2) * CHARGE_COEFF);
     float g_ele;
-    if (zd->alpha < EPS_REF) {
-        *e_ele = cc * (r12_inv - zd->zcore + zd->bcoeff * r12_2);
-        g_ele  = -cc * (r12_3_inv - zd->fcoeff);
-    } else {
-        float tmp     = r12 * zd->alpha;
-        float errorfc = erfcf(tmp);
-        *e_ele        = cc * (r12_inv * errorfc - zd->zcore + zd->bcoeff * r12_2);
-        g_ele = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff);
-    }
+    zm_pair(zd, r12, r12_2, r12_inv, r12_2_inv, r12_3_inv, cc, e_ele, &g_ele);
     for (int d = 0; d < 3; d++) {
         float wv = work_coef * d12[d];
         float we = g_ele * d12[d];
@@ -162,14 +270,8 @@ int orc_pair_ff_fp32(const orc_zd_t *zd, const orc_ff_t *ff, const float d12[3],
         *e_ele       = cc * r12_inv * ff->dielectric_inv * ld_exp;
         g_ele        = -*e_ele * (r12_inv + ff->debye_inv);
         if (ff->dh_grad_fix) g_ele *= r12_inv;
-    } else if (zd->alpha < EPS_REF) {
-        *e_ele = cc * (r12_inv - zd->zcore + zd->bcoeff * r12_2);
-        g_ele  = -cc * (r12_3_inv - zd->fcoeff);
     } else {
-        float tmp     = r12 * zd->alpha;
-        float errorfc = erfcf(tmp);
-        *e_ele        = cc * (r12_inv * errorfc - zd->zcore + zd->bcoeff * r12_2);
-        g_ele = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff);
+        zm_pair(zd, r12, r12_2, r12_inv, r12_2_inv, r12_3_inv, cc, e_ele, &g_ele);
     }
     for (int d = 0; d < 3; d++) {
         float wv = work_coef * d12[d];
@@ -371,6 +473,72 @@ static void minim_image_d(double d[3], const double *x1, const double *x2, const
     }
 }
 
+/* The excluded-pair term of the selected order: calc_zero02pole_excess(_alpha0) ...
+ * calc_zero16pole_excess(_alpha0) (ZeroMultipoleSum.cpp:191-307).  bug_compat
+ * keeps the reference's forms where they differ from the order's polynomial:
+ * the dipole alpha > 0 energy -erf(ar) r (:214), the octupole r^5 in place of
+ * r^6 (:255, :271), the hexadecapole alpha = 0 gradient c66 r^2 and -1/r^3
+ * (:288) and alpha > 0 energy without the r^6 and r^8 terms (:304). */
+static void excess_term(const orc_zd_t *zd, float r12, float r12_2, float r12_inv, float r12_2_inv, float r12_3_inv,
+                        float cc, int bug, float *ene, float *gc) {
+    const int a0 = zd->alpha < EPS_REF;
+    float     tmp = 0.0f, errorfc = 1.0f, errorfn = 0.0f, pe = 0.0f;
+    if (!a0) {
+        tmp     = zd->alpha * r12;
+        errorfc = erfcf(tmp);
+        errorfn = (float)(1.0 - (double)errorfc);
+        pe      = zd->piewald * expf(-tmp * tmp) * r12_2_inv;
+    }
+    switch (zd->mode) {
+    case ORC_ZM_QUADRUPOLE:
+        if (a0) {
+            *ene = cc * (-zd->zq2 * r12_2 - zd->zq4 * r12_2 * r12_2);
+            *gc  = -cc * (zd->zq22 + zd->zq44 * r12_2);
+        } else {
+            *ene = cc * (-errorfn * r12_inv - zd->zq2 * r12_2 - zd->zq4 * r12_2 * r12_2);
+            *gc  = -cc * (errorfc * r12_3_inv + pe + zd->zq22 + zd->zq44 * r12_2 - r12_3_inv);
+        }
+        break;
+    case ORC_ZM_OCTUPOLE:
+        if (a0) {
+            const double r12_4 = r12_2 * r12_2; /* a double local in the reference (:254) */
+            *ene = (float)(cc * (-zd->zo2 * r12_2 - zd->zo4 * r12_4 -
+                                 (bug ? zd->zo6 * r12_4 * r12 : zd->zo6 * r12_4 * r12_2)));
+            *gc  = (float)(-cc * (zd->zo22 + zd->zo44 * r12_2 + zd->zo66 * r12_4));
+        } else {
+            const float r12_4 = r12_2 * r12_2;
+            *ene = cc * (-errorfn * r12_inv - zd->zo2 * r12_2 - zd->zo4 * r12_4 -
+                         (bug ? zd->zo6 * r12_4 * r12 : zd->zo6 * r12_4 * r12_2));
+            *gc  = -cc * (errorfc * r12_3_inv + pe + zd->zo22 + zd->zo44 * r12_2 + zd->zo66 * r12_4 - r12_3_inv);
+        }
+        break;
+    case ORC_ZM_HEXADECAPOLE: {
+        const float r12_4 = r12_2 * r12_2, r12_6 = r12_4 * r12_2, r12_8 = r12_4 * r12_4;
+        if (a0) {
+            *ene = cc * (-zd->zh2 * r12_2 - zd->zh4 * r12_4 - zd->zh6 * r12_6 - zd->zh8 * r12_8);
+            if (bug) *gc = -cc * (zd->zh22 + zd->zh44 * r12_2 + zd->zh66 * r12_2 + zd->zh88 * r12_6 - r12_3_inv);
+            else *gc = -cc * (zd->zh22 + zd->zh44 * r12_2 + zd->zh66 * r12_4 + zd->zh88 * r12_6);
+        } else {
+            if (bug) *ene = cc * (-errorfn * r12_inv - zd->zh2 * r12_2 - zd->zh4 * r12_4);
+            else *ene = cc * (-errorfn * r12_inv - zd->zh2 * r12_2 - zd->zh4 * r12_4 - zd->zh6 * r12_6 - zd->zh8 * r12_8);
+            *gc = -cc * (errorfc * r12_3_inv + pe + zd->zh22 + zd->zh44 * r12_2 + zd->zh66 * r12_4 + zd->zh88 * r12_6 -
+                         r12_3_inv);
+        }
+        break;
+    }
+    default:
+        if (a0) {
+            *ene = cc * zd->bcoeff * r12_2;
+            *gc  = cc * zd->fcoeff;
+        } else {
+            if (bug) *ene = cc * (-errorfn * r12 + zd->bcoeff * r12_2);
+            else *ene = cc * (-errorfn * r12_inv + zd->bcoeff * r12_2);
+            *gc = -cc * (errorfc * r12_3_inv + pe - zd->fcoeff - r12_3_inv);
+        }
+        break;
+    }
+}
+
 /* SubBox::calc_energy_ele_excess (SubBox.cpp:1347-1367) + ForceField::calc_zms_excess
  * (ForceField.cpp:420-437) + calc_zero02pole_excess(_alpha0) (ZeroMultipoleSum.cpp:191-217). */
 double orc_excess(const orc_zd_t *zd, const float L[3], int n_excess, const int *pairs, const double *crd,
@@ -389,17 +557,7 @@ double orc_excess(const orc_zd_t *zd, const float L[3], int n_excess, const int 
         float r12_3_inv = r12_2_inv * r12_inv;
         float cc        = (float)((double)(q[a] * q[b]) * CHARGE_COEFF);
         float ene, gc;
-        if (zd->alpha < EPS_REF) {
-            ene = cc * zd->bcoeff * r12_2;
-            gc  = cc * zd->fcoeff;
-        } else {
-            float tmp     = zd->alpha * r12;
-            float errorfc = erfcf(tmp);
-            float errorfn = (float)(1.0 - (double)errorfc);
-            if (bug_compat) ene = cc * (-errorfn * r12 + zd->bcoeff * r12_2);
-            else ene = cc * (-errorfn * r12_inv + zd->bcoeff * r12_2);
-            gc = -cc * (errorfc * r12_3_inv + zd->piewald * expf(-tmp * tmp) * r12_2_inv - zd->fcoeff - r12_3_inv);
-        }
+        excess_term(zd, r12, r12_2, r12_inv, r12_2_inv, r12_3_inv, cc, bug_compat, &ene, &gc);
         e_sum += ene;
         for (int d = 0; d < 3; d++) {
             float w = gc * d12[d];

@@ -34,6 +34,11 @@
   wons_ala3_traj.npz -- 10 leapfrog steps of the same launch set with SHAKE
       (pairs, trios and the four-atom groups, tolerance 1e-10) and COM cancel
       on the solute and solvent groups, as the sample runs it; log and restart.
+  wons_zm.npz -- one leapfrog step without constraints of the wons_grad
+      system under --electrostatic zero-quadrupole, zero-octupole and
+      zero-hexadecapole at --ele-alpha 0 and 0.1 (gradient from the restart
+      velocities, log), and 10 leapfrog steps with SHAKE under
+      zero-quadrupole at alpha 0 (log and final restart).
   wons_output.npz -- the reference's own output files of a 20-step leapfrog
       run with SHAKE (500 waters + 20-bead chain): the Presto trajectory
       (--print-interval-coord 5), the same run's GROMACS .trr, and the final
@@ -41,7 +46,7 @@
 
 Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
 Run where the reference tree is mounted:
-    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out|ala3]
+    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out|ala3|zm]
 """
 import os
 import re
@@ -207,8 +212,36 @@ def ala3(tmp):
     print("ala3 traj ok", lg.shape, lg[0], lg[-1])
 
 
+ZM = {"q": "zero-quadrupole", "o": "zero-octupole", "h": "zero-hexadecapole"}
+
+
+def zm(tmp):
+    ls = gen.build_system(700, seed=77, n_beads=20)
+    minv = (1.0 / ls.mass.astype(np.float32).astype(np.float64)).astype(np.float32).astype(np.float64)
+    out = {}
+    for tag, ele in ZM.items():
+        for atag, alpha in (("0", 0.0), ("1", 0.1)):
+            cfg = dict(BASE, **{"n-steps": 1, "ele-alpha": alpha, "constraint": "none", "electrostatic": ele})
+            data, log, rs = run(ls, cfg, tmp)
+            out["work_%s%s" % (tag, atag)] = -(rs["vel"] - ls.vel) / (2.0 * 4.184e-4 * minv[:, None])
+            out["log_%s%s" % (tag, atag)] = parse_log(log)
+            print(ele, alpha, parse_log(log)[0])
+    tl = gen.build_system(1000, seed=3, n_beads=0)
+    cfg = dict(BASE, **{"n-steps": 10, "ele-alpha": 0.0, "constraint": "shake", "const-tolerance": 1e-10,
+                        "const-max-loops": 1000, "electrostatic": "zero-quadrupole"})
+    tdata, log, rs = run(tl, cfg, tmp)
+    lg = parse_log(log)
+    print("quadrupole traj", lg.shape, lg[0], lg[-1])
+    np.savez_compressed(os.path.join(HERE, "wons_zm.npz"), cls=np.frombuffer(data, np.uint8), traj_cls=np.frombuffer(
+        tdata, np.uint8), traj_log=lg, traj_crd=rs["crd"], traj_vel=rs["vel"],
+        traj_cfg=np.frombuffer(gen.cfg_text(cfg).encode(), np.uint8), **out)
+
+
 def main():
     tmp = tempfile.mkdtemp()
+    if len(sys.argv) > 1 and sys.argv[1] == "zm":
+        zm(tmp)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "ala3":
         ala3(tmp)
         return

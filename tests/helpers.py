@@ -123,7 +123,7 @@ def hps_dh_form(ls: LaunchSet, npz, dh_grad_fix=False):
 
 def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-presto", constraint="none",
               tol=1e-10, max_loops=1000, com_groups=None, thermostat=None, langevin=None, extra_last=False,
-              ff=None):
+              ff=None, zd=None):
     """The reference step loop with the oracle's functions, in the reference
     order:
       leapfrog-presto (DynamicsModePresto::calc_in_each_step, DynamicsMode.cpp:468-560):
@@ -137,7 +137,8 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
         position Langevin update (SubBox.cpp:2077-2114) with the reference
         noise stream, v by central difference (SubBox.cpp:2054-2066) and its
         KE; no constraints (DynamicsMode.cpp:877); PBC wrap.
-    ff: the HPS / Debye-Hueckel pair form (oracle.ff_setup), None for zero-dipole.
+    ff: the HPS / Debye-Hueckel pair form (oracle.ff_setup), None for zero-multipole.
+    zd: the zero-multipole constants (oracle.zd_setup); None: zero-dipole at alpha 0.
     thermostat: dict(tau_fs, temperature, temperature_init, heating_steps, d_free).
     langevin: dict(gamma_per_fs, temperature, seed).
     extra_last: the reference runs one more step after the last output
@@ -146,7 +147,7 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
     crd/vel being what the reference's restart holds."""
     L, lb, ub = oracle.pbc_from_box(ls.box)
     crd = wrap_inbox(ls.crd, L, lb, ub)
-    zd = oracle.zd_setup(cutoff, 0.0)
+    zd = oracle.zd_setup(cutoff, 0.0) if zd is None else zd
     m32 = ls.mass.astype(np.float32)
     minv = (1.0 / m32.astype(np.float64)).astype(np.float32)
     vel_next = ls.vel.copy()

@@ -773,3 +773,146 @@ def test_shake_quad_restores_distances():
             assert abs(v @ v - d2[q, p]) <= 1e-10 * d2[q, p]
     out2, _ = oracle.shake_quad(L, ids, d2, out, out, minv, 1e-12, 1000)
     assert np.max(np.abs(out2 - out)) < 1e-12
+
+
+# ------------------------------------- zero-quadrupole, -octupole, -hexadecapole
+
+ZM_NAMES = {"q": "zero-quadrupole", "o": "zero-octupole", "h": "zero-hexadecapole"}
+ZM_CASES = [(t, a) for t in "qoh" for a in "01"]
+
+
+def _zm_poly(zd):
+    """(core, {power: coefficient}) of the selected order's pair energy
+    1/r (erfc) - core - sum c_k r^k."""
+    m = zd.mode
+    c = {0: {2: -zd.bcoeff}, 1: {2: zd.zq2, 4: zd.zq4}, 2: {2: zd.zo2, 4: zd.zo4, 6: zd.zo6},
+         3: {2: zd.zh2, 4: zd.zh4, 6: zd.zh6, 8: zd.zh8}}[m]
+    return zd.zmcore, c
+
+
+@pytest.mark.parametrize("tag,atag", ZM_CASES)
+def test_oracle_zero_multipole_gradient_matches_reference_one_step(tag, atag):
+    """orc_zm_setup / the pair, excess and self terms of every higher order at
+    alpha 0 and 0.1 against one step of the reference all-pairs build
+    (wons_zm.npz): gradient to 1e-11 RMS, VDW and Ele to the printed digits."""
+    d = load_npz("wons_zm.npz")
+    ls = cls_from_fixture(d)
+    alpha = 0.1 if atag == "1" else 0.0
+    zd = oracle.zd_setup(12.0, alpha, oracle.ZM_MODES[ZM_NAMES[tag]])
+    L, lb, ub = oracle.pbc_from_box(ls.box)
+    crd = wrap_inbox(ls.crd, L, lb, ub)
+    lj6, lj12 = ls.lj_tables_f32()
+    q = ls.charge.astype(np.float32)
+    work, en, _ = oracle.allpairs_fp32(zd, L, crd, ls.atom_type, lj6, lj12, q, ls.nb15off_table())
+    ex = ls.excess_pairs()
+    eb = oracle.bonds(L, ls.bond_ids, ls.bond_k, ls.bond_r0, crd, work)
+    eex = oracle.excess(zd, L, ex, crd, q, work, bug_compat=True)
+    eself, _ = oracle.self_energy(zd, q, ex)
+    ref = d["work_" + tag + atag]
+    assert rms_rel(work, ref) < 1e-11
+    assert np.max(np.abs(work - ref)) < 1e-8 * np.max(np.abs(ref))
+    log0 = d["log_" + tag + atag][0]
+    assert fmt(en[0]) == fmt(log0[4])
+    assert fmt(en[1] + eex + eself) == fmt(log0[5])
+    assert abs(eb - log0[3]) <= 1e-9 * max(1.0, abs(log0[3]))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("alpha", [0.0, 0.1])
+def test_zero_multipole_constants_closed_form(mode, alpha):
+    """The order-l zero-multipole pair energy erfc(ar)/r - core - sum c_k r^k
+    and its first l derivatives vanish at the cutoff (l = 1 dipole ... 4
+    hexadecapole), to the float rounding of the constants; the one exception
+    is the hexadecapole's 4th derivative at alpha > 0, which carries the
+    reference's series for the 4th derivative coefficient
+    (ZeroMultipoleSum.cpp:152-154) and is checked to miss."""
+    sp = pytest.importorskip("sympy")
+    zd = oracle.zd_setup(12.0, alpha, mode)
+    core, c = _zm_poly(zd)
+    r = sp.symbols("r")
+    f = sp.erfc(alpha * r) / r
+    E = f - core - sum(v * r ** k for k, v in c.items())
+    for n in range(mode + 2):
+        val = float(sp.diff(E, r, n).subs(r, 12.0))
+        scale = abs(float(sp.diff(f, r, n).subs(r, 12.0)))
+        if mode == 3 and alpha > 0 and n == 4:
+            assert abs(val) > 0.1 * scale
+        else:
+            assert abs(val) < 3e-6 * scale, (n, val, scale)
+    # the next derivative does not vanish (the order is not higher than stated)
+    n = mode + 2
+    assert abs(float(sp.diff(E, r, n).subs(r, 12.0))) > 0.1 * abs(float(sp.diff(f, r, n).subs(r, 12.0)))
+
+
+def _fd_check(efun, r, h=2e-2):
+    """(g, dE/dr / r by central difference): the gradient coefficient of a
+    pair term against its energy."""
+    e1, _ = efun(r + h)
+    e0, _ = efun(r - h)
+    _, g = efun(r)
+    return g, (e1 - e0) / (2 * h) / r
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("alpha", [0.0, 0.1])
+def test_zero_multipole_pair_and_excess_forms_consistent(mode, alpha):
+    """Pair term: the gradient coefficient equals dE/dr / r for every order
+    (with the octupole alpha > 0 gradient on its own coefficients), and the
+    reference's octupole alpha > 0 gradient, which takes the hexadecapole
+    coefficients (ZeroMultipoleSum.cpp:405-406), does not.  Excluded-pair term
+    in the fixed forms: consistent for every order; the reference forms miss
+    where they drop or misplace a power (octupole r^5, hexadecapole)."""
+    L = np.array([60.0, 60.0, 60.0], np.float32)
+    q = np.array([0.8, -0.6], np.float32)
+
+    def pair(zd):
+        def e(r):
+            n, ev, ee, w = oracle.pair_fp32(zd, np.array([r, 0.0, 0.0]), 0.0, 0.0, q[0], q[1])
+            return ee, w[0] / r
+        return e
+
+    def excess(zd, bug):
+        def e(r):
+            crd = np.array([[10.0, 10.0, 10.0], [10.0 + r, 10.0, 10.0]])
+            w = np.zeros((2, 3))
+            ene = oracle.excess(zd, L, np.array([[0, 1]], np.int32), crd, q, w, bug_compat=bug)
+            # work is dE/dx_a = g * (x_a - x_b) = -g r along x
+            return ene, -w[0, 0] / r
+        return e
+
+    for r in (4.0, 7.5, 10.5):
+        zd = oracle.zd_setup(12.0, alpha, mode, zo_grad_bug=False)
+        g, fd = _fd_check(pair(zd), r)
+        # float32 energies: the difference quotient carries ~1e-6 / h of rounding
+        assert g == pytest.approx(fd, rel=2e-3, abs=2e-5), (r, g, fd)
+    if mode == 2 and alpha > 0:
+        zd = oracle.zd_setup(12.0, alpha, mode, zo_grad_bug=True)
+        g, fd = _fd_check(pair(zd), 7.5)
+        assert abs(g - fd) > 1e-2 * abs(fd)
+    for r in (1.0, 1.6, 2.5):
+        zd = oracle.zd_setup(12.0, alpha, mode)
+        g, fd = _fd_check(excess(zd, False), r)
+        assert g == pytest.approx(fd, rel=2e-3, abs=2e-5), (r, g, fd)
+    ref_slip = (mode == 2) or (mode == 3) or (mode == 0 and alpha > 0)
+    if ref_slip:
+        # at a separation where the misplaced powers weigh (excluded pairs sit at 1-3 A, where the octupole and
+        # hexadecapole slips change the terms by less than 1e-4)
+        g, fd = _fd_check(excess(oracle.zd_setup(12.0, alpha, mode), True), 9.0)
+        assert abs(g - fd) > 1e-2 * abs(fd), (g, fd)
+
+
+def test_oracle_zero_quadrupole_md_matches_reference_run():
+    """10 leapfrog steps with SHAKE under zero-quadrupole at alpha 0: per-step
+    potential and kinetic energy and the final restart (wons_zm.npz)."""
+    from helpers import oracle_md
+    d = load_npz("wons_zm.npz")
+    from omegagene_amd.cls import read_cls
+    ls = read_cls(bytes(d["traj_cls"]))
+    zd = oracle.zd_setup(12.0, 0.0, oracle.ZM_MODES["zero-quadrupole"])
+    crd, vel, rows = oracle_md(ls, 10, constraint="shake", zd=zd)
+    log = d["traj_log"]
+    for r, lg in zip(rows, log[:10]):
+        assert abs(r[1] - lg[1]) <= 2e-9 * abs(lg[5]) + 1e-8, (r, lg)
+        assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]), (r, lg)
+    assert np.max(np.abs(crd - d["traj_crd"])) < 1e-9
+    assert np.max(np.abs(vel - d["traj_vel"])) < 1e-11
