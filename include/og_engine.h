@@ -164,7 +164,10 @@ typedef struct {
     uint64_t random_seed;          /* --random-seed */
     int      com_cancel;           /* --com-motion cancel */
     int      gap_bug;              /* 1: reproduce the uniform-z scan stop at the first empty bin (MiniCell.cpp:1063-1077) */
-    int      excess_bug;           /* 1: alpha>0 excess energy uses -erf(ar)*r as ZeroMultipoleSum.cpp:214 */
+    int      excess_bug;           /* 1: the reference's excluded-pair forms: the zero-dipole alpha>0 energy
+                                      -erf(ar)*r (ZeroMultipoleSum.cpp:214), the zero-octupole r^5 for r^6
+                                      (:255,:271), the zero-hexadecapole alpha 0 gradient (:288) and alpha>0
+                                      energy (:304) */
     int      log_stride;           /* energies kept every log_stride steps */
     int      log_cap;              /* entries kept in the device energy log */
     double   capacity_factor;      /* cell-pair list capacity multiplier (1.0 default) */
@@ -183,7 +186,8 @@ typedef struct {
      * hydrophobicity-scale LJ with Debye-Hueckel electrostatics */
     int      nonbond;              /* --nonbond: OG_NONBOND_LJ or OG_NONBOND_HPS (hydrophobicity-scale-lj) */
     double   hps_epsilon;          /* --hydrophobicity-scale-epsiron (kcal/mol) */
-    int      electrostatic;        /* --electrostatic: OG_ELE_ZERO_DIPOLE or OG_ELE_DEBYE_HUCKEL */
+    int      electrostatic;        /* --electrostatic: OG_ELE_ZERO_DIPOLE, _QUADRUPOLE, _OCTUPOLE, _HEXADECAPOLE or
+                                      OG_ELE_DEBYE_HUCKEL */
     double   dh_dielectric;        /* --debye-huckel-dielectric */
     double   dh_ionic_strength;    /* --debye-huckel-ionic-strength */
     double   dh_temperature;       /* --debye-huckel-temperature (K) */
@@ -192,11 +196,16 @@ typedef struct {
                                       cuda_setup.cu:1327), which is r times the derivative */
     double   cell_edge_scale;      /* MiniCell column width relative to the reference's (1 = the reference
                                       geometry, required with gap_bug 1); og_autotune_cell_edge picks one */
+    int      zo_grad_bug;          /* 1 (default): the zero-octupole alpha > 0 pair gradient takes the
+                                      zero-hexadecapole coefficients, as ZeroMultipoleSum.cpp:404-406;
+                                      0: its own.  (excess_bug likewise keeps the reference's excluded-pair
+                                      forms of every order, ZeroMultipoleSum.cpp:203-307.) */
 } og_params;
 
 enum { OG_THERMOSTAT_NONE = 0, OG_THERMOSTAT_SCALING = 1 };
 enum { OG_NONBOND_LJ = 0, OG_NONBOND_HPS = 1 };
-enum { OG_ELE_ZERO_DIPOLE = 0, OG_ELE_DEBYE_HUCKEL = 1 };
+enum { OG_ELE_ZERO_DIPOLE = 0, OG_ELE_DEBYE_HUCKEL = 1, OG_ELE_ZERO_QUADRUPOLE = 2, OG_ELE_ZERO_OCTUPOLE = 3,
+       OG_ELE_ZERO_HEXADECAPOLE = 4 };
 
 void og_params_default(og_params *p);
 

@@ -83,9 +83,11 @@ int grid_geom(GridGeom &g, int n_atoms, const double *box, double nscut, double 
     return 0;
 }
 
-// ZeroMultipoleSum::set_zms_params, dipole members (ZeroMultipoleSum.cpp:108-190), float members.
+// ZeroMultipoleSum::set_zms_params (ZeroMultipoleSum.cpp:108-190), float members.
 struct ZdConst {
     float cutoff, alpha, fcoeff, bcoeff, zcore, piewald;
+    // higher orders: polynomial coefficients c_k of r^(2k+2) and the cores, index 0 quadrupole ... 2 hexadecapole
+    float zc[3][4], zcc[3][4], zmcore[3];
 };
 ZdConst zd_const(double cutoff, double alpha) {
     ZdConst z;
@@ -107,18 +109,54 @@ ZdConst zd_const(double cutoff, double alpha) {
         z.bcoeff      = (float)(0.5 * z.fcoeff);
     }
     z.zcore = errorfc / z.cutoff + z.bcoeff * c2;
+    // derivative coefficients of erfc(a r)/r at the cutoff; the expressions keep the reference's double
+    // promotions (each rounds to a float member on assignment), including its 4th-derivative series
+    const float c = z.cutoff, c4 = c2 * c2, c5 = c3 * c2, c6 = c3 * c3, c7 = c3 * c4;
+    float       hz, d1, d2, d3, d4;
+    if (z.alpha < EPS_REF) {
+        hz = (float)(2.0 / c3);
+        d1 = (float)(1.0 / c2);
+        d2 = (float)(2.0 / c3);
+        d3 = (float)(6.0 / c4);
+        d4 = (float)(24.0 / c5);
+    } else {
+        const float rp = (float)sqrt(PI_REF);
+        const float t3 = tmp * tmp * tmp, t5 = t3 * tmp * tmp, t7 = t5 * tmp * tmp;
+        hz = (float)((errorfc + expterm * (tmp + t3) * 2.0 / rp) * 2.0 / c3);
+        d1 = (float)((errorfc + expterm * tmp * 2.0 / rp) / c2);
+        d2 = (float)((errorfc + expterm * (tmp + t3) * 2.0 / rp) * 2.0 / c3);
+        d3 = (float)((errorfc + expterm * (tmp + 2.0 / 3.0 * t3 + 2.0 / 3.0 * t5) * 2.0 / rp) * 6.0 / c4);
+        d4 = (float)((errorfc + expterm * (tmp + 2.0 / 3.0 * t3 + 1.0 / 6.0 * t5 + 1.0 / 3.0 + t7) * 2.0 / rp) * 24.0 / c5);
+    }
+    float(&q)[4] = z.zc[0], (&o)[4] = z.zc[1], (&h)[4] = z.zc[2];
+    q[0] = (float)(-3.0 / 4.0 * z.fcoeff - hz / 4.0);
+    q[1] = (float)(z.fcoeff / 8.0 / c2 + hz / 8.0 / c2);
+    q[2] = q[3] = 0.0f;
+    o[0] = (float)(-15.0 / 16.0 * d1 / c - 7.0 / 16.0 * d2 - 1.0 / 16.0 * d3 * c);
+    o[1] = (float)(5.0 / 16.0 * d1 / c3 + 5.0 / 16.0 * d2 / c2 + 1.0 / 16.0 * d3 / c);
+    o[2] = (float)(-1.0 / 16.0 * d1 / c5 - 1.0 / 16.0 * d2 / c4 - 1.0 / 48.0 * d3 / c3);
+    o[3] = 0.0f;
+    h[0] = (float)(-35.0 / 32.0 * d1 / c - 19.0 / 32.0 * d2 - 1.0 / 96.0 * d4 * c2 - 1.0 / 8.0 * d3 * c);
+    h[1] = (float)(1.0 / 64.0 * d4 + 35.0 / 64.0 * d1 / c3 + 35.0 / 64.0 * d2 / c2 + 5.0 / 32.0 * d3 / c);
+    h[2] = (float)(-7.0 / 32.0 * d1 / c5 - 7.0 / 32.0 * d2 / c4 - 1.0 / 12.0 * d3 / c3 - 1.0 / 96.0 * d4 / c2);
+    h[3] = (float)(5.0 / 128.0 * d2 / c6 + 5.0 / 128.0 * d1 / c7 + 1.0 / 64.0 * d3 / c5 + 1.0 / 384.0 * d4 / c4);
+    for (int m = 0; m < 3; m++)
+        for (int k = 0; k < 4; k++) z.zcc[m][k] = (float)((2.0 * (k + 1)) * z.zc[m][k]);
+    z.zmcore[0] = errorfc / c - q[0] * c2 - q[1] * c4;
+    z.zmcore[1] = errorfc / c - o[0] * c2 - o[1] * c4 - o[2] * c6;
+    z.zmcore[2] = errorfc / c - h[0] * c2 - h[1] * c4 - h[2] * c6 - h[3] * c6 * c2;
     return z;
 }
 
 // ZeroMultipoleSum::cal_self_energy (ZeroMultipoleSum.cpp:43-106): per-atom FP64
 // terms summed into a float member (d_self_mon).
-double self_energy(const ZdConst &z, int n, const float *q, int n_ex, const int *ex) {
+double self_energy(const ZdConst &z, float zmcore_f, int n, const float *q, int n_ex, const int *ex) {
     std::vector<double> dwork(n, 0.0);
     for (int i = 0; i < n_ex; i++) {
         dwork[ex[2 * i]] += q[ex[2 * i + 1]];
         dwork[ex[2 * i + 1]] += q[ex[2 * i]];
     }
-    const double zmcore = z.zcore;
+    const double zmcore = zmcore_f;  // the order's core (ZeroMultipoleSum.cpp:72-80)
     float        mon    = 0.0f;
     for (int i = 0; i < n; i++) {
         const float  pq = -z.piewald * q[i] * q[i];
@@ -645,8 +683,26 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.cutoff  = z.cutoff;
     h.cutoff2 = z.cutoff * z.cutoff;
     h.zcore = z.zcore, h.bcoeff = z.bcoeff, h.fcoeff = z.fcoeff, h.piewald = z.piewald, h.alpha = z.alpha;
-    const bool dh = p->electrostatic == OG_ELE_DEBYE_HUCKEL;
-    s->pair_form  = dh ? OG_FORM_HPS_DH : (z.alpha > 0.0f ? OG_FORM_ZD_ALPHA : OG_FORM_ZD);
+    const bool dh    = p->electrostatic == OG_ELE_DEBYE_HUCKEL;
+    const int  order = p->electrostatic == OG_ELE_ZERO_QUADRUPOLE  ? 1
+                       : p->electrostatic == OG_ELE_ZERO_OCTUPOLE  ? 2
+                       : p->electrostatic == OG_ELE_ZERO_HEXADECAPOLE ? 3
+                                                                    : 0;
+    const bool alpha_on = z.alpha > 0.0f;
+    s->pair_form = dh ? OG_FORM_HPS_DH
+                      : order ? (alpha_on ? OG_FORM_ZM_ALPHA : OG_FORM_ZM) : (alpha_on ? OG_FORM_ZD_ALPHA : OG_FORM_ZD);
+    float zmcore = z.zcore;
+    h.zm_order   = order;
+    if (order) {
+        zmcore    = z.zmcore[order - 1];
+        h.zm_core = zmcore;
+        for (int k = 0; k < 4; k++) {
+            h.zm_c[k]  = z.zc[order - 1][k];
+            h.zm_cc[k] = z.zcc[order - 1][k];
+            // ZeroMultipoleSum::calc_zero08pole (:392-408) takes the hexadecapole coefficients for its gradient
+            h.zm_pg[k] = (order == 2 && alpha_on && p->zo_grad_bug) ? (k < 3 ? z.zcc[2][k] : 0.0f) : h.zm_cc[k];
+        }
+    }
     if (dh) {
         // ZeroMultipoleSum constructor (ZeroMultipoleSum.cpp:21-24): float members from the double cfg values;
         // constants CelesteObject.cpp:32-40
@@ -660,6 +716,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         // self energy has no core term (zmcore = 0 outside the zero-multipole modes, :75-82)
         h.n_excess = 0;
         z.zcore    = 0.0f;
+        zmcore     = 0.0f;
     }
     h.dt          = p->time_step;
     h.tol         = p->const_tolerance;
@@ -695,7 +752,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         m[i]    = (float)d->mass[i];
         minv[i] = (float)(1.0 / (double)m[i]);  // SubBox.cpp:691 (real_pw mass_inv = 1.0 / mass)
     }
-    h.e_self = self_energy(z, n, q.data(), h.n_excess, d->excess);
+    h.e_self = self_energy(z, zmcore, n, q.data(), h.n_excess, d->excess);
 
     const int cap = h.cap_atom_array;
     bool      ok  = A(&h.crd, 3 * n) && A(&h.crd_prev, 3 * n) && A(&h.crd_prev2, 3 * n) && A(&h.vel, 3 * n) &&
@@ -1098,6 +1155,7 @@ void og_params_default(og_params *p) {
     p->com_cancel          = 0;
     p->gap_bug             = 1;
     p->excess_bug          = 1;
+    p->zo_grad_bug         = 1;
     p->log_stride          = 1;
     p->log_cap             = 1024;
     p->capacity_factor     = 1.0;
@@ -1304,7 +1362,7 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
             set_err("Debye-Hueckel needs positive --debye-huckel-dielectric, -ionic-strength and -temperature");
             return OG_E_ARG;
         }
-    } else if (p->electrostatic != OG_ELE_ZERO_DIPOLE) {
+    } else if (p->electrostatic < OG_ELE_ZERO_DIPOLE || p->electrostatic > OG_ELE_ZERO_HEXADECAPOLE) {
         set_err("unknown electrostatic %d", p->electrostatic);
         return OG_E_ARG;
     }

@@ -103,6 +103,14 @@ struct og_dev_sys {
     float cutoff2;
     // ---- zero-dipole constants (ZeroMultipoleSum.cpp:108-190) ----
     float zcore, bcoeff, fcoeff, piewald, alpha;
+    // higher zero-multipole orders (zero-quadrupole/-octupole/-hexadecapole, ZeroMultipoleSum.cpp:108-190,
+    // 218-442): zm_order 0 = dipole (the constants above), 1..3 = quadrupole..hexadecapole.  Pair energy
+    // cc (1/r [erfc] - zm_core - sum_k zm_c[k] r^(2k+2)), gradient coefficient
+    // -cc (1/r^3 [erfc] [+ piewald exp(-a^2 r^2) / r^2] + sum_k zm_pg[k] r^(2k)); zm_cc[k] = (2k+2) zm_c[k]
+    // as the reference's float members (its excluded-pair terms use these; zm_pg differs from zm_cc only
+    // where the reference's octupole alpha > 0 pair gradient takes the hexadecapole coefficients)
+    int   zm_order;
+    float zm_core, zm_c[4], zm_cc[4], zm_pg[4];
     // hydrophobicity-scale LJ + Debye-Hueckel (cal_pair_hps_dh, cuda_setup.cu:1273-1332)
     float hps_eps;         // --hydrophobicity-scale-epsiron
     float dh_diel_inv;     // 1 / --debye-huckel-dielectric
@@ -398,7 +406,7 @@ void og_launch_enum_write(const og_launch &cellw);
 void og_launch_masks(const og_launch &cp);
 // forces
 // pair forms of og_launch_pair
-enum { OG_FORM_ZD = 0, OG_FORM_ZD_ALPHA = 1, OG_FORM_HPS_DH = 2 };
+enum { OG_FORM_ZD = 0, OG_FORM_ZD_ALPHA = 1, OG_FORM_HPS_DH = 2, OG_FORM_ZM = 3, OG_FORM_ZM_ALPHA = 4 };
 void og_launch_pair(const og_launch &cells, int form, int waves);  // waves per block: 1, 2, 4 or 8
 void og_launch_forces_to_work(const og_launch &atoms);
 void og_launch_energy_reduce(const og_launch &sys);

@@ -31,7 +31,9 @@ typedef float og_f2 __attribute__((ext_vector_type(2)));
 // Pair forms (the FORM template argument): 0 = LJ + zero-dipole at alpha 0,
 // 1 = LJ + zero-dipole at alpha > 0 (cal_pair, cuda_setup.cu:487-521),
 // 2 = hydrophobicity-scale LJ + Debye-Hueckel (cal_pair_hps_dh,
-// cuda_setup.cu:1273-1332).  The LJ table entry is {A6, A12} for forms 0 and
+// cuda_setup.cu:1273-1332), 3 / 4 = LJ + zero-quadrupole, -octupole or
+// -hexadecapole at alpha 0 / > 0 (the reference's host forms,
+// ZeroMultipoleSum.cpp:347-442).  The LJ table entry is {A6, A12} for forms 0 and
 // 1 and {A6, A12, HPS switch distance, HPS lambda} for form 2.
 template <int FORM> struct og_ljt { typedef og_f2 type; };
 template <> struct og_ljt<2> { typedef og_f4 type; };
@@ -108,6 +110,8 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
     const float zcore = s.zcore, bcoeff = s.bcoeff, fcoeff = s.fcoeff;
     const float alpha = s.alpha, piewald = s.piewald;
     const float hps_eps = s.hps_eps, kappa = s.dh_kappa;
+    const float zmc = s.zm_core, ze0 = s.zm_c[0], ze1 = s.zm_c[1], ze2 = s.zm_c[2], ze3 = s.zm_c[3];
+    const float zg0 = s.zm_pg[0], zg1 = s.zm_pg[1], zg2 = s.zm_pg[2], zg3 = s.zm_pg[3];
     const bool  dh_fix_on = s.dh_grad_fix != 0;
     float fjx = 0.0f, fjy = 0.0f, fjz = 0.0f, fe = 0.0f, fq = 0.0f, fv = 0.0f;
     float px = pj.x, py = pj.y, pz = pj.z, pw = pj.w;
@@ -145,6 +149,21 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
             elj             = fmaf(elj, sc, far ? 0.0f : (1.0f - lj.w) * hps_eps);
             eel             = cc * rinv * __expf(-r * kappa);
             g = sc * r2inv * (6.0f * t6 - 12.0f * t12) - eel * (rinv + kappa) * (dh_fix_on ? rinv : 1.0f);
+        } else if constexpr (FORM == 3 || FORM == 4) {
+            // zero-quadrupole/-octupole/-hexadecapole: the order's polynomials in r^2 by Horner's rule
+            const float pe = r2 * fmaf(r2, fmaf(r2, fmaf(r2, ze3, ze2), ze1), ze0);
+            const float pg = fmaf(r2, fmaf(r2, fmaf(r2, zg3, zg2), zg1), zg0);
+            if constexpr (FORM == 3) {
+                eel = cc * (rinv - zmc - pe);
+                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv + pg);
+            } else {
+                const float r   = r2 * rinv;
+                const float ar  = alpha * r;
+                const float ex  = __expf(-ar * ar);
+                const float erc = og_erfc_as(ar, ex);
+                eel = cc * (rinv * erc - zmc - pe);
+                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (erc * rinv * r2inv + piewald * ex * r2inv + pg);
+            }
         } else {
             const float r   = r2 * rinv;
             const float ar  = alpha * r;
@@ -448,6 +467,10 @@ template <int WAVES>
 static void og_launch_pair_w(const og_launch &a, int form) {
     if (form == OG_FORM_HPS_DH)
         hipLaunchKernelGGL((k_pair<2, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
+    else if (form == OG_FORM_ZM)
+        hipLaunchKernelGGL((k_pair<3, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
+    else if (form == OG_FORM_ZM_ALPHA)
+        hipLaunchKernelGGL((k_pair<4, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
     else if (form == OG_FORM_ZD_ALPHA)
         hipLaunchKernelGGL((k_pair<1, WAVES>), dim3(a.n_blocks), dim3(64 * WAVES), 0, a.stream, a.sys, a.map);
     else

@@ -28,6 +28,49 @@ __device__ __forceinline__ void og_block_sum2(double &a, double &b, double *sh) 
     }
 }
 
+// The excluded-pair term of the zero-quadrupole, -octupole and -hexadecapole
+// orders (calc_zero04pole_excess ... calc_zero16pole_excess(_alpha0),
+// ZeroMultipoleSum.cpp:218-307), float as the reference; excess_bug keeps the
+// reference's forms where they misplace or drop a power (:255, :271, :288,
+// :304).  w = gradient coefficient times d; returns the energy.
+// Written without branches on the order: every form is evaluated (the unused
+// coefficients are zeros, which leave a float sum's bits unchanged) and the
+// order's result selected.  (An if/else-if chain over the order here was
+// compiled by this toolchain with the hexadecapole alpha > 0 gradient scaled
+// by a register never assigned on that path.)
+__device__ __forceinline__ float og_excess_zm(const og_dev_sys &s, float r, float r2, float rinv, float r2inv,
+                                              float r3inv, float cc, const float *d, float *w) {
+#pragma clang fp contract(off)
+    const float c2 = s.zm_c[0], c4 = s.zm_c[1], c6 = s.zm_c[2], c8 = s.zm_c[3];
+    const float g2 = s.zm_cc[0], g4 = s.zm_cc[1], g6 = s.zm_cc[2], g8 = s.zm_cc[3];
+    const int   ord = s.zm_order;
+    const bool  bug = s.excess_bug != 0;
+    const float r4 = r2 * r2, r6 = r4 * r2, r8 = r4 * r4;
+    // alpha = 0
+    const float  e1a = cc * (-c2 * r2 - c4 * r2 * r2);
+    const float  g1a = -cc * (g2 + g4 * r2);
+    const double r4d = (double)r4;  // a double local in the reference's octupole (:254)
+    const float  e2a = (float)(cc * (-c2 * r2 - c4 * r4d - (bug ? c6 * r4d * r : c6 * r4d * r2)));
+    const float  g2a = (float)(-cc * (g2 + g4 * r2 + g6 * r4d));
+    const float  e3a = cc * (-c2 * r2 - c4 * r4 - c6 * r6 - c8 * r8);
+    const float  g3a = bug ? -cc * (g2 + g4 * r2 + g6 * r2 + g8 * r6 - r3inv) : -cc * (g2 + g4 * r2 + g6 * r4 + g8 * r6);
+    // alpha > 0 (the gradient takes the same form in every order)
+    const float x     = s.alpha * r;
+    const float erfcv = erfcf(x);
+    const float erfv  = (float)(1.0 - (double)erfcv);
+    const float pe    = s.piewald * expf(-x * x) * r2inv;
+    const float e1b   = cc * (-erfv * rinv - c2 * r2 - c4 * r2 * r2);
+    const float e2b   = cc * (-erfv * rinv - c2 * r2 - c4 * r4 - (bug ? c6 * r4 * r : c6 * r4 * r2));
+    const float e3b   = bug ? cc * (-erfv * rinv - c2 * r2 - c4 * r4)
+                            : cc * (-erfv * rinv - c2 * r2 - c4 * r4 - c6 * r6 - c8 * r8);
+    const float gb    = -cc * (erfcv * r3inv + pe + g2 + g4 * r2 + g6 * r4 + g8 * r6 - r3inv);
+    const bool  a0    = s.alpha == 0.0f;
+    const float ene   = a0 ? (ord == 1 ? e1a : ord == 2 ? e2a : e3a) : (ord == 1 ? e1b : ord == 2 ? e2b : e3b);
+    const float gc    = a0 ? (ord == 1 ? g1a : ord == 2 ? g2a : g3a) : gb;
+    for (int k = 0; k < 3; k++) w[k] = gc * d[k];
+    return ene;
+}
+
 // Excluded pair (a, b), a < b in the excess list, from the atoms' FP64
 // coordinates and charges: gradient w added to a and subtracted from b,
 // energy returned, in the reference float arithmetic
@@ -52,6 +95,7 @@ __device__ __forceinline__ float og_excess_pair(const og_dev_sys &s, const doubl
     const float r3inv = r2inv * rinv;
     const float cc    = (float)((double)(qa * qb) * 332.06378);
     float       ene, gc;
+    if (s.zm_order) return og_excess_zm(s, r, r2, rinv, r2inv, r3inv, cc, d, w);
     if (s.alpha == 0.0f) {
         ene = cc * s.bcoeff * r2;
         gc  = cc * s.fcoeff;

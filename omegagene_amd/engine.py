@@ -37,7 +37,8 @@ INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
 THERMOSTATS = {"none": 0, "scaling": 1}
 NONBONDS = {"lennard-jones": 0, "hydrophobicity-scale-lj": 1}     # Config.cpp:333-339
-ELECTROSTATICS = {"zero-dipole": 0, "debye-huckel": 1}          # Config.cpp:114-128 (the device forms)
+ELECTROSTATICS = {"zero-dipole": 0, "debye-huckel": 1, "zero-quadrupole": 2, "zero-octupole": 3,
+                  "zero-hexadecapole": 4}                           # Config.cpp:114-128
 
 
 class EngineError(RuntimeError):
@@ -89,7 +90,7 @@ class Params(C.Structure):
                 ("berendsen_tau", C.c_double), ("temperature_init", C.c_double), ("heating_steps", C.c_int),
                 ("nonbond", C.c_int), ("hps_epsilon", C.c_double), ("electrostatic", C.c_int),
                 ("dh_dielectric", C.c_double), ("dh_ionic_strength", C.c_double), ("dh_temperature", C.c_double),
-                ("dh_grad_fix", C.c_int), ("cell_edge_scale", C.c_double)]
+                ("dh_grad_fix", C.c_int), ("cell_edge_scale", C.c_double), ("zo_grad_bug", C.c_int)]
 
 
 def lib():
@@ -150,6 +151,8 @@ def params_from_cfg(cfg: dict) -> Params:
         p.dh_grad_fix = int(cfg["dh-grad-fix"])
     if "cell-edge-scale" in cfg:
         p.cell_edge_scale = float(cfg["cell-edge-scale"])
+    if "zo-grad-bug" in cfg:
+        p.zo_grad_bug = int(cfg["zo-grad-bug"])
     return p
 
 
