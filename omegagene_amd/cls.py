@@ -40,6 +40,28 @@ class VcmdDef:
 
 
 @dataclass
+class VmcmdDef:
+    """V-McMD / V-AUS virtual-state block (Read.cpp:521-561)."""
+    interval: int
+    temperature: float
+    # per state: (order, lambda_low, lambda_high, prob_down, prob_up, [order + 1 polynomial coefficients],
+    # alpha_low, alpha_high)
+    states: List[Tuple[int, float, float, float, float, List[float], float, float]]
+    init_vs: int                             # 0-based
+    seed: int
+
+
+@dataclass
+class GroupCoordDef:
+    """Group-coordinate block for restarting V-AUS / VcMD (Read.cpp:646-703):
+    the enhanced atom groups and their unwrapped coordinates."""
+    header: str
+    aus_type: int
+    group_ids: List[int]                     # indices into the group table, 1-based as on disk
+    crd: List[np.ndarray]                    # per group float64[n_atoms_in_group, 3]
+
+
+@dataclass
 class LaunchSet:
     box: np.ndarray                      # float64[12] = {Lx,0,0, 0,Ly,0, 0,0,Lz, cx,cy,cz}
     crd: np.ndarray                      # float64[n,3]  (Angstrom)
@@ -71,6 +93,8 @@ class LaunchSet:
     has_dist_rest: bool = True
     has_pos_rest: bool = False
     vcmd: Optional[VcmdDef] = None
+    vmcmd: Optional[VmcmdDef] = None
+    group_coord: Optional[GroupCoordDef] = None
     pote: Optional[float] = None
 
     @property
@@ -276,12 +300,34 @@ def write_cls(ls: LaunchSet, path=None) -> bytes:
             vb.write(b"".join(_p("i", s + 1) for s in st) + _p("d", q))
         v = vb.getvalue()
     pote = _p("d", ls.pote) if ls.pote is not None else b""
+    ext = b""
+    if ls.vmcmd is not None:
+        mc = ls.vmcmd
+        eb = io.BytesIO()
+        eb.write(_p("iid", len(mc.states), mc.interval, mc.temperature))
+        for order, lo, hi, pdn, pup, poly, alo, ahi in mc.states:
+            eb.write(_p("i", order) + _p("dddd", lo, hi, pdn, pup))
+            eb.write(_p("%dd" % (order + 1), *poly[:order + 1]))
+            eb.write(_p("dd", alo, ahi))
+        eb.write(_p("ii", mc.init_vs + 1, mc.seed))
+        ext = eb.getvalue()
+    gcs = b""
+    if ls.group_coord is not None:
+        gc = ls.group_coord
+        gb = io.BytesIO()
+        hb = gc.header.encode() + b"\0"
+        gb.write(_p("i", len(hb)) + hb + _p("ii", gc.aus_type, len(gc.group_ids)))
+        gb.write(_p("%di" % len(gc.group_ids), *gc.group_ids))
+        gb.write(_p("%di" % len(gc.crd), *[len(c) for c in gc.crd]))
+        for c in gc.crd:
+            gb.write(np.ascontiguousarray(c, "<f8").tobytes())
+        gcs = gb.getvalue()
 
     ver = LS_VERSION.encode() + b"\0"
-    sizes = [len(box), len(crd), len(vel), len(topol), len(shake), len(settle), 0, len(groups), len(dist),
-             len(pos), len(v), 0, len(pote)]
+    sizes = [len(box), len(crd), len(vel), len(topol), len(shake), len(settle), len(ext), len(groups), len(dist),
+             len(pos), len(v), len(gcs), len(pote)]
     out = (_p("i", MAGIC) + _p("i", len(ver)) + ver + _p("13i", *sizes) + box + crd + vel + topol + shake
-           + settle + groups + dist + pos + v + pote)
+           + settle + ext + groups + dist + pos + v + gcs + pote)
     if path is not None:
         with open(path, "wb") as f:
             f.write(out)
@@ -407,7 +453,20 @@ def read_cls(src) -> LaunchSet:
     if s_set:
         _, _, ls.settle_ids, ls.settle_d, _ = cst()
     if s_ext:
-        r.o += s_ext  # V-McMD block (Read.cpp:521-561): not on this path
+        # V-McMD / V-AUS block (Read.cpp:521-561)
+        end = r.o + s_ext
+        n_vs, interval = r.get("ii")
+        temperature = r.get("d")[0]
+        states = []
+        for _ in range(n_vs):
+            order = r.i()
+            lo, hi, pdn, pup = r.get("dddd")
+            poly = list(r.get("%dd" % (order + 1)))
+            alo, ahi = r.get("dd")
+            states.append((order, lo, hi, pdn, pup, poly, alo, ahi))
+        init, seed = r.get("ii")
+        ls.vmcmd = VmcmdDef(interval, temperature, states, init - 1, seed)
+        r.o = end
     if s_grp:
         ng = r.i()
         names, counts = [], []
@@ -440,7 +499,14 @@ def read_cls(src) -> LaunchSet:
             q.append((st, r.get("d")[0]))
         ls.vcmd = VcmdDef(interval, dims, init_vs, seed, q)
     if s_gc:
-        r.o += s_gc
+        # group coordinates of the enhanced groups (Read.cpp:646-703)
+        end = r.o + s_gc
+        header = r.cstr(r.i())
+        aus_type, ng = r.get("ii")
+        gids = list(r.get("%di" % ng))
+        counts = list(r.get("%di" % ng))
+        ls.group_coord = GroupCoordDef(header, aus_type, gids, [r.arr("<f8", 3 * c).reshape(c, 3) for c in counts])
+        r.o = end
     if s_pote:
         ls.pote = r.get("d")[0]
     return ls

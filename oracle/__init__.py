@@ -537,3 +537,43 @@ def shake_quad(L, ids, dist2, crd, crd_prev, mass_inv, tol, max_loops):
                              _p(np.ascontiguousarray(mass_inv, np.float32), C.c_float), C.c_double(tol),
                              C.c_int(max_loops))
     return out, n
+
+
+def _group_minv(m32, ids):
+    """Float group mass accumulated in order and its float inverse (MmSystem.cpp:572-582)."""
+    s = np.float32(0.0)
+    for i in ids:
+        s = np.float32(s + m32[i])
+    return np.float32(1.0 / np.float64(s))
+
+
+def vs_transition(lo, hi, p_down, p_up, vs, lam, u_up, u_down):
+    """orc_vs_transition: the V-McMD / V-AUS state after a transition trial."""
+    a = [np.ascontiguousarray(x, np.float64) for x in (lo, hi, p_down, p_up)]
+    return lib().orc_vs_transition(C.c_int(len(a[0])), *[_p(x, C.c_double) for x in a], C.c_int(vs),
+                                   C.c_double(lam), C.c_double(u_up), C.c_double(u_down))
+
+
+def vmcmd_factor(poly, lo, hi, T, lam):
+    """orc_vmcmd_factor: the V-McMD gradient factor of one state."""
+    f = lib().orc_vmcmd_factor
+    f.restype = C.c_double
+    p = np.ascontiguousarray(poly, np.float64)
+    return f(_p(p, C.c_double), C.c_int(len(p) - 1), C.c_double(lo), C.c_double(hi), C.c_double(T), C.c_double(lam))
+
+
+def vaus_bias(L, atoms_a, atoms_b, crdg, crd, mass, poly, lo, hi, sigma, coef, T, work):
+    """orc_vaus_bias: V-AUS bias of one state into work (in place), crdg updated
+    in place; returns lambda."""
+    f = lib().orc_vaus_bias
+    f.restype = C.c_double
+    m32 = np.ascontiguousarray(mass, np.float32)
+    a = np.ascontiguousarray(atoms_a, np.int32)
+    b = np.ascontiguousarray(atoms_b, np.int32)
+    p = np.ascontiguousarray(poly, np.float64)
+    assert crdg.dtype == np.float64 and crdg.flags.c_contiguous and work.dtype == np.float64
+    return f(_p(np.ascontiguousarray(L, np.float32), C.c_float), C.c_int(len(a)), _p(a, C.c_int), C.c_int(len(b)),
+             _p(b, C.c_int), _p(crdg, C.c_double), _p(np.ascontiguousarray(crd, np.float64), C.c_double),
+             _p(m32, C.c_float), C.c_float(_group_minv(m32, a)), C.c_float(_group_minv(m32, b)), _p(p, C.c_double),
+             C.c_int(len(p) - 1), C.c_double(lo), C.c_double(hi), C.c_double(sigma), C.c_double(coef), C.c_double(T),
+             _p(work, C.c_double))

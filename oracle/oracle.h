@@ -245,6 +245,17 @@ double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const
                      const double *lo, const double *hi, int vs, double sigma, double coef, double temperature,
                      double *work, double *lambda_out, int *in_range, int cand[2]);
 
+/* V-McMD / V-AUS (Extend.cpp:126-540): the virtual-state move with the two
+ * uniform draws given, the V-McMD gradient factor, and the V-AUS bias (adds
+ * into work, returns lambda; crdg as for orc_vcmd_bias). */
+int    orc_vs_transition(int n_states, const double *lo, const double *hi, const double *p_down, const double *p_up,
+                         int vs, double lambda, double u_up, double u_down);
+double orc_vmcmd_factor(const double *poly, int order, double lo, double hi, double temperature, double lambda);
+double orc_vaus_bias(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
+                     const double *crd, const float *mass, float minv_a, float minv_b, const double *poly,
+                     int order, double lo, double hi, double sigma, double recov_coef, double temperature,
+                     double *work);
+
 #ifdef __cplusplus
 }
 #endif

@@ -316,10 +316,12 @@ void orc_langevin_velocity(int n, double dt, const float L[3], const double *crd
  * Returns the bias energy coef*excess^2 and writes lambda, whether lambda is
  * inside the current state's range, and the transition candidates
  * cand[0] = vs, cand[1] = neighbour or -1. */
-double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
-                     const double *crd, const float *mass, float minv_a, float minv_b, int n_states,
-                     const double *lo, const double *hi, int vs, double sigma, double coef, double temperature,
-                     double *work, double *lambda_out, int *in_range, int cand[2]) {
+/* Group coordinates unwrapped against their previous values, the two mass
+ * centres and their minimum-image difference (ExtendedVcMD::set_crd_centers
+ * Extend.cpp:753-797 and ExtendedVAUS::set_crd_centers :366-413 are the same
+ * loop); returns |diff|. */
+static double group_diff(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
+                         const double *crd, const float *mass, float minv_a, float minv_b, double diff[3]) {
     double c[2][3] = {{0, 0, 0}, {0, 0, 0}};
     for (int g = 0; g < 2; g++) {
         const int  n   = g == 0 ? na : nb;
@@ -330,13 +332,13 @@ double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const
             for (int d = 0; d < 3; d++) {
                 const float Lh   = L[d] * 0.5f;
                 double      tmp  = crd[3 * a + d];
-                double      diff = tmp - cg[3 * i + d];
-                while (diff > Lh) {
-                    diff -= L[d];
+                double      dif  = tmp - cg[3 * i + d];
+                while (dif > Lh) {
+                    dif -= L[d];
                     tmp -= L[d];
                 }
-                while (-diff > Lh) {
-                    diff += L[d];
+                while (-dif > Lh) {
+                    dif += L[d];
                     tmp += L[d];
                 }
                 cg[3 * i + d] = tmp;
@@ -344,7 +346,6 @@ double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const
             }
         }
     }
-    double diff[3];
     for (int d = 0; d < 3; d++) {
         c[0][d] *= minv_a;
         c[1][d] *= minv_b;
@@ -355,7 +356,15 @@ double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const
         v       = v - L[d] * (float)(int)(v * Lhi);
         diff[d] = v;
     }
-    const double lam = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+    return sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+}
+
+double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
+                     const double *crd, const float *mass, float minv_a, float minv_b, int n_states,
+                     const double *lo, const double *hi, int vs, double sigma, double coef, double temperature,
+                     double *work, double *lambda_out, int *in_range, int cand[2]) {
+    double       diff[3];
+    const double lam = group_diff(L, na, atoms_a, nb, atoms_b, crdg, crd, mass, minv_a, minv_b, diff);
     double       u[3];
     for (int d = 0; d < 3; d++) u[d] = diff[d] / lam;
     double ex = 0.0, rec = 0.0;
@@ -390,4 +399,88 @@ double orc_vcmd_bias(const float L[3], int na, const int *atoms_a, int nb, const
         cand[1] = 1;
     }
     return rec * ex;
+}
+
+/* ---- V-McMD / V-AUS virtual states --------------------------------------
+ * ExtendedVMcMD::trial_transition + set_current_vstate (Extend.cpp:143-176):
+ * from state vs, a move up is tried first, then a move down; a move needs
+ * lambda inside both ranges (closed intervals) and its uniform draw above
+ * 1 - the state's probability for that direction.  u_up / u_down are the
+ * draws of the two trials.  Returns the new state. */
+static int vs_trial(int n, const double *lo, const double *hi, const double *p_down, const double *p_up, int src,
+                    int rel, double lambda, double u) {
+    if (!(lambda >= lo[src] && lambda <= hi[src])) return src;
+    if (src == 0 && rel == -1) return src;
+    if (src == n - 1 && rel == 1) return src;
+    const int dst = src + rel;
+    if (lambda >= lo[dst] && lambda <= hi[dst]) {
+        const double p = rel == -1 ? p_down[src] : p_up[src];
+        if (u > 1.0 - p) return dst;
+    }
+    return src;
+}
+int orc_vs_transition(int n, const double *lo, const double *hi, const double *p_down, const double *p_up, int vs,
+                      double lambda, double u_up, double u_down) {
+    const int up = vs_trial(n, lo, hi, p_down, p_up, vs, 1, lambda, u_up);
+    if (up != vs) return up;
+    return vs_trial(n, lo, hi, p_down, p_up, vs, -1, lambda, u_down);
+}
+
+/* ExtendedVMcMD::scale_force (Extend.cpp:177-200): the factor the whole
+ * gradient is multiplied by, k_B T (kcal/mol) times the state's polynomial
+ * d ln P / d lambda at lambda clamped to the state's range (set_temperature,
+ * :119-122). */
+double orc_vmcmd_factor(const double *poly, int order, double lo, double hi, double temperature, double lambda) {
+    double param = lambda;
+    if (lambda <= lo) param = lo;
+    else if (lambda >= hi) param = hi;
+    double d_ln_p = poly[0], t = 1.0;
+    for (int i = 1; i < order + 1; i++) {
+        t *= param;
+        d_ln_p += poly[i] * t;
+    }
+    const double const_k = (8.31451 / 4.184) * 1e-3 * temperature;
+    return const_k * d_ln_p;
+}
+
+/* ExtendedVAUS::cal_struct_parameters + scale_force (Extend.cpp:427-540) for
+ * one pair of groups: lambda = distance between the mass centres; outside
+ * [lo - sigma, hi + sigma] a harmonic recovery term; the force
+ * k_B T (d ln P(lambda clamped) + recovery) along the unit vector, weighted by
+ * m_i / M_group, + on group a and - on group b.  Returns lambda. */
+double orc_vaus_bias(const float L[3], int na, const int *atoms_a, int nb, const int *atoms_b, double *crdg,
+                     const double *crd, const float *mass, float minv_a, float minv_b, const double *poly,
+                     int order, double lo, double hi, double sigma, double recov_coef, double temperature,
+                     double *work) {
+    double       diff[3];
+    const double lam = group_diff(L, na, atoms_a, nb, atoms_b, crdg, crd, mass, minv_a, minv_b, diff);
+    double       u[3];
+    for (int d = 0; d < 3; d++) u[d] = diff[d] / lam;
+    double param = lam, recovery = 0.0;
+    if (param <= lo) {
+        if (param <= lo - sigma) recovery = recov_coef * (param - (lo - sigma));
+        param = lo;
+    } else if (param >= hi) {
+        if (param >= hi + sigma) recovery = recov_coef * (param - (hi + sigma));
+        param = hi;
+    }
+    double t = 1.0, d_ln_p = poly[0];
+    for (int i = 1; i <= order; i++) {
+        t *= param;
+        d_ln_p += poly[i] * t;
+    }
+    const double const_k = (8.31451 / 4.184) * 1e-3 * temperature;
+    const double dew     = const_k * (d_ln_p + recovery);
+    double       dir     = 1.0;
+    for (int g = 0; g < 2; g++) {
+        const int   n    = g == 0 ? na : nb;
+        const int  *ids  = g == 0 ? atoms_a : atoms_b;
+        const float minv = g == 0 ? minv_a : minv_b;
+        double      bias[3];
+        for (int d = 0; d < 3; d++) bias[d] = dew * u[d] * (double)minv;
+        for (int i = 0; i < n; i++)
+            for (int d = 0; d < 3; d++) work[3 * ids[i] + d] += dir * bias[d] * (double)mass[ids[i]];
+        dir *= -1.0;
+    }
+    return lam;
 }

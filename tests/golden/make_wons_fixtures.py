@@ -39,6 +39,13 @@
       zero-hexadecapole at --ele-alpha 0 and 0.1 (gradient from the restart
       velocities, log), and 10 leapfrog steps with SHAKE under
       zero-quadrupole at alpha 0 (log and final restart).
+  wons_ext.npz -- 20 leapfrog steps with SHAKE on the waters of a water +
+      20-bead-chain system under --extended-ensemble v-mcmd (three virtual
+      states over the potential energy, transitions every 2 steps with
+      probabilities 0 or 1 so the walk does not depend on the random stream)
+      and the same under v-aus on the distance between the chain's end
+      groups (group coordinates in the launch set); logs, lambda files,
+      virtual-state logs and final restarts.
   wons_output.npz -- the reference's own output files of a 20-step leapfrog
       run with SHAKE (500 waters + 20-bead chain): the Presto trajectory
       (--print-interval-coord 5), the same run's GROMACS .trr, and the final
@@ -46,7 +53,7 @@
 
 Inputs are the .cls bytes written by omegagene_amd.gen (stored in the fixture).
 Run where the reference tree is mounted:
-    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out|ala3|zm]
+    make -C oracle ref && python tests/golden/make_wons_fixtures.py [more|hps|out|ala3|zm|ext]
 """
 import os
 import re
@@ -237,8 +244,81 @@ def zm(tmp):
         traj_cfg=np.frombuffer(gen.cfg_text(cfg).encode(), np.uint8), **out)
 
 
+def _read_text(path):
+    return open(path).read() if os.path.exists(path) else ""
+
+
+def ext(tmp):
+    from omegagene_amd.cls import GroupCoordDef, VmcmdDef
+    sys.path.insert(0, os.path.dirname(HERE))
+    import oracle
+    from helpers import oracle_md
+    out = {}
+    files = {"fn-o-vmcmd-log": "vs.log", "fn-o-extended-lambda": "lambda.out", "format-o-extended-lambda": "ascii",
+             "print-interval-extended-lambda": 1}
+    base = dict(BASE, **{"n-steps": 20, "ele-alpha": 0.0, "constraint": "shake", "const-tolerance": 1e-10,
+                         "const-max-loops": 1000, "temperature": 300.0})
+    # V-McMD: lambda = potential energy; states over the unbiased run's range
+    ls = gen.build_system(500, seed=51, n_beads=20)
+    _, _, rows = oracle_md(ls, 20, constraint="shake")
+    e_lo, e_hi = float(rows[:, 1].min()), float(rows[:, 1].max())
+    span = e_hi - e_lo
+    kT = (8.31451 / 4.184) * 1e-3 * 300.0
+    cuts = [e_lo - 0.2 * span, e_lo + 0.35 * span, e_lo + 0.65 * span, e_hi + 0.2 * span]
+    states = []
+    for k in range(3):
+        lo, hi = cuts[k] - 0.1 * span, cuts[k + 1] + 0.1 * span
+        c1 = 2e-5 * (k + 1)
+        c0 = (1.0 + 0.01 * k) / kT - c1 * 0.5 * (lo + hi)
+        poly = [c0, c1] + ([1e-10] if k == 1 else [])
+        states.append((len(poly) - 1, lo, hi, 0.0 if k == 0 else 1.0, 0.0 if k == 2 else 1.0, poly, 0.0, 0.0))
+    init = int(np.searchsorted(cuts, rows[0, 1]) - 1)
+    ls.vmcmd = VmcmdDef(2, 300.0, states, min(max(init, 0), 2), 5)
+    cfg = dict(base, **{"extended-ensemble": "v-mcmd"}, **files)
+    data, log, rs = run(ls, cfg, tmp)
+    out["mcmd"] = (data, log, rs, cfg, _read_text(os.path.join(tmp, "lambda.out")),
+                   _read_text(os.path.join(tmp, "vs.log")))
+    # V-AUS: lambda = distance between the mass centres of the chain's end groups
+    ls = gen.build_system(500, seed=53, n_beads=20, with_vcmd=True)
+    ls.vcmd = None
+    names = [nm for nm, _ in ls.groups]
+    ga, gb = dict(ls.groups)["end_a"], dict(ls.groups)["end_b"]
+    m = ls.mass
+    lam0 = float(np.linalg.norm((ls.crd[ga] * m[ga, None]).sum(0) / m[ga].sum() -
+                                (ls.crd[gb] * m[gb, None]).sum(0) / m[gb].sum()))
+    ls.group_coord = GroupCoordDef("V-AUS restart", 0, [names.index("end_a") + 1, names.index("end_b") + 1],
+                                   [ls.crd[ga].copy(), ls.crd[gb].copy()])
+    cfg = dict(base, **{"extended-ensemble": "v-aus", "enhance-sigma": 0.5, "enhance-recovery-coef": 2.0,
+                        "fn-o-aus-restart": "aus.restart"}, **files)
+    # "aus": lambda starts in the overlap of states 1 and 2 (moves every trial);
+    # "aus_rec": state 0 ends 1.5 A below lambda (recovery force, no move)
+    for tag, cuts, init in (("aus", [lam0 - 8.0, lam0 - 3.0, lam0 - 0.2, lam0 + 6.0], 1),
+                            ("aus_rec", [lam0 - 9.0, lam0 - 2.1, lam0 + 2.0, lam0 + 8.0], 0)):
+        states = []
+        for k in range(3):
+            states.append((1, cuts[k] - 0.6, cuts[k + 1] + 0.6, 0.0 if k == 0 else 1.0, 0.0 if k == 2 else 1.0,
+                           [0.4 - 0.3 * k, -0.05 * (k + 1)], 0.0, 0.0))
+        ls.vmcmd = VmcmdDef(2, 300.0, states, init, 5)
+        data, log, rs = run(ls, cfg, tmp)
+        out[tag] = (data, log, rs, cfg, _read_text(os.path.join(tmp, "lambda.out")),
+                    _read_text(os.path.join(tmp, "vs.log")))
+    arrs = {}
+    for tag, (data, log, rs, cfg, lam_txt, vs_txt) in out.items():
+        lg = parse_log(log)
+        lam = np.array([float(x) for x in lam_txt.split()])
+        vs = np.array([[int(x) for x in ln.split()] for ln in vs_txt.splitlines() if ln.strip()], np.int64)
+        arrs.update({tag + "_cls": np.frombuffer(data, np.uint8), tag + "_log": lg, tag + "_crd": rs["crd"],
+                     tag + "_vel": rs["vel"], tag + "_lambda": lam, tag + "_vs": vs.reshape(-1, 2),
+                     tag + "_cfg": np.frombuffer(gen.cfg_text(cfg).encode(), np.uint8)})
+        print(tag, lg.shape, lg[0], lg[-1], "lambda", lam[:3], "vs", vs[:5].tolist())
+    np.savez_compressed(os.path.join(HERE, "wons_ext.npz"), **arrs)
+
+
 def main():
     tmp = tempfile.mkdtemp()
+    if len(sys.argv) > 1 and sys.argv[1] == "ext":
+        ext(tmp)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "zm":
         zm(tmp)
         return

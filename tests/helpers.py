@@ -123,7 +123,7 @@ def hps_dh_form(ls: LaunchSet, npz, dh_grad_fix=False):
 
 def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-presto", constraint="none",
               tol=1e-10, max_loops=1000, com_groups=None, thermostat=None, langevin=None, extra_last=False,
-              ff=None, zd=None):
+              ff=None, zd=None, ext=None, ext_out=None):
     """The reference step loop with the oracle's functions, in the reference
     order:
       leapfrog-presto (DynamicsModePresto::calc_in_each_step, DynamicsMode.cpp:468-560):
@@ -139,6 +139,13 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
         KE; no constraints (DynamicsMode.cpp:877); PBC wrap.
     ff: the HPS / Debye-Hueckel pair form (oracle.ff_setup), None for zero-multipole.
     zd: the zero-multipole constants (oracle.zd_setup); None: zero-dipole at alpha 0.
+    ext: V-McMD / V-AUS on the gradient before the velocity update
+    (DynamicsMode.cpp:504-507, Extend.cpp:126-540): dict(mode "v-mcmd" or
+    "v-aus", vmcmd VmcmdDef, and for V-AUS groups (a, b), crdg [unwrapped
+    group coordinates, updated], sigma, recov); transitions every interval
+    steps with both uniform draws 0.5 (the probabilities in {0, 1} of the
+    fixtures make the walk independent of the draws); ext_out collects
+    (lambda, state) per step.
     thermostat: dict(tau_fs, temperature, temperature_init, heating_steps, d_free).
     langevin: dict(gamma_per_fs, temperature, seed).
     extra_last: the reference runs one more step after the last output
@@ -163,6 +170,30 @@ def oracle_md(ls: LaunchSet, n_steps, dt=2.0, cutoff=12.0, integrator="leapfrog-
         if len(ls.angles) or len(ls.torsions) or len(ls.impros) or len(ls.nb14):
             # angles, torsions, impropers, 1-4 (SubBox.cpp:1250-1346)
             pot += sum(oracle.bonded(ls, crd, work).values())
+        if ext is not None:
+            mc = ext["vmcmd"]
+            st = mc.states
+            lo = [x[1] for x in st]
+            hi = [x[2] for x in st]
+            if "vs" not in ext:
+                ext["vs"] = mc.init_vs
+            if ext["mode"] == "v-mcmd":
+                lam = pot
+            else:
+                m_ = ls.mass
+                lam = oracle.vaus_bias(L, ext["groups"][0], ext["groups"][1], ext["crdg"], crd, m_, [0.0], 0.0, 0.0,
+                                       0.0, 0.0, mc.temperature, np.zeros_like(work))
+            if step > 0 and step % mc.interval == 0:
+                ext["vs"] = oracle.vs_transition(lo, hi, [x[3] for x in st], [x[4] for x in st], ext["vs"], lam,
+                                                 0.5, 0.5)
+            vs = ext["vs"]
+            if ext["mode"] == "v-mcmd":
+                work = work * oracle.vmcmd_factor(st[vs][5], lo[vs], hi[vs], mc.temperature, lam)
+            else:
+                oracle.vaus_bias(L, ext["groups"][0], ext["groups"][1], ext["crdg"], crd, ls.mass, st[vs][5], lo[vs],
+                                 hi[vs], ext["sigma"], ext["recov"], mc.temperature, work)
+            if ext_out is not None:
+                ext_out.append((lam, vs))
         if integrator == "leapfrog-presto":
             vel = vel_next
             vel_next = oracle.update_velocities(dt, vel, work, minv)

@@ -916,3 +916,69 @@ def test_oracle_zero_quadrupole_md_matches_reference_run():
         assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]), (r, lg)
     assert np.max(np.abs(crd - d["traj_crd"])) < 1e-9
     assert np.max(np.abs(vel - d["traj_vel"])) < 1e-11
+
+
+# ------------------------------------------------------------ V-McMD, V-AUS
+
+def _ext_setup(d, tag):
+    from omegagene_amd.cls import read_cls
+    ls = read_cls(bytes(d[tag + "_cls"]))
+    ext = {"mode": "v-mcmd" if tag == "mcmd" else "v-aus", "vmcmd": ls.vmcmd}
+    if tag != "mcmd":
+        gc = ls.group_coord
+        names = [nm for nm, _ in ls.groups]
+        ga = np.asarray(ls.groups[gc.group_ids[0] - 1][1], np.int32)
+        gb = np.asarray(ls.groups[gc.group_ids[1] - 1][1], np.int32)
+        assert names[gc.group_ids[0] - 1] == "end_a"
+        ext.update(groups=(ga, gb), crdg=np.ascontiguousarray(np.concatenate(gc.crd), np.float64), sigma=0.5,
+                   recov=2.0)
+    return ls, ext
+
+
+@pytest.mark.parametrize("tag", ["mcmd", "aus", "aus_rec"])
+def test_oracle_vmcmd_vaus_match_reference_runs(tag):
+    """20 leapfrog steps with SHAKE under V-McMD (lambda = potential energy,
+    the whole gradient scaled) and V-AUS (lambda = end-to-end mass-centre
+    distance; state moves every trial, or a recovery force with no move)
+    against the reference runs (wons_ext.npz): lambda per step to the printed
+    11 digits, the virtual-state log, per-step energies and the restart."""
+    from helpers import oracle_md
+    d = load_npz("wons_ext.npz")
+    ls, ext = _ext_setup(d, tag)
+    crd, vel, rows = oracle_md(ls, 20, constraint="shake", ext=ext)
+    # lambda of the reference's extra last step (step 20) too
+    got = []
+    ls, ext = _ext_setup(d, tag)
+    oracle_md(ls, 21, constraint="shake", ext=ext, ext_out=got)
+    lam_ref = d[tag + "_lambda"]
+    assert len(lam_ref) == 20
+    for k in range(1, 21):   # the reference writes lambda for steps 1..n (Extend.cpp:136)
+        assert "%.10e" % got[k][0] == "%.10e" % lam_ref[k - 1], (k, got[k][0], lam_ref[k - 1])
+    vs_ref = d[tag + "_vs"]
+    for step, vs1 in vs_ref:
+        assert got[step][1] == vs1 - 1, (step, got[step][1], vs1)
+    n_states = len(set(v for _, v in got))
+    assert n_states == 1 if tag == "aus_rec" else n_states >= 2
+    log = d[tag + "_log"]
+    for r, lg in zip(rows, log[:20]):
+        assert abs(r[1] - lg[1]) <= 2e-9 * abs(lg[5]) + 1e-8, (r, lg)
+        assert abs(r[2] - lg[2]) <= 1e-9 * abs(lg[2]), (r, lg)
+    assert np.max(np.abs(crd - d[tag + "_crd"])) < 1e-9
+    assert np.max(np.abs(vel - d[tag + "_vel"])) < 1e-11
+
+
+def test_vs_transition_rule():
+    """orc_vs_transition on its own: up is tried first, a move needs lambda in
+    both closed ranges and the draw above 1 - p; the edges never move out."""
+    lo, hi = [0.0, 4.0, 8.0], [5.0, 9.0, 12.0]
+    pdn, pup = [0.0, 0.3, 1.0], [0.6, 0.7, 0.0]
+    T = oracle.vs_transition
+    assert T(lo, hi, pdn, pup, 0, 4.5, 0.5, 0.99) == 1        # 0.5 > 1 - 0.6
+    assert T(lo, hi, pdn, pup, 0, 4.5, 0.3, 0.99) == 0        # 0.3 <= 0.4; down impossible at 0
+    assert T(lo, hi, pdn, pup, 1, 4.5, 0.99, 0.8) == 0        # lambda not in state 2; down: 0.8 > 1 - 0.3
+    assert T(lo, hi, pdn, pup, 1, 4.5, 0.99, 0.5) == 1        # 0.5 <= 0.7: stays
+    assert T(lo, hi, pdn, pup, 1, 5.0, 0.99, 0.71) == 0       # closed range: 5.0 is in state 0
+    assert T(lo, hi, pdn, pup, 1, 8.5, 0.31, 0.99) == 2       # up first
+    assert T(lo, hi, pdn, pup, 1, 9.5, 0.99, 0.99) == 1       # lambda outside the source's range
+    assert T(lo, hi, pdn, pup, 2, 8.5, 0.99, 0.5) == 1        # p_down 1
+    assert T(lo, hi, pdn, pup, 2, 8.5, 0.99, 0.0) == 2        # the draw must exceed 1 - p
