@@ -146,7 +146,28 @@ typedef struct {
     int           n_shake_quads;
     const int    *shake_quad_ids;   /* [n*4] */
     const double *shake_quad_d2;    /* [n*6] squared distances of the pairs 01 12 20 30 23 13 */
+    /* V-McMD / V-AUS virtual states (ExtendedVMcMD / ExtendedVAUS, Extend.cpp:126-540; input block
+     * Read.cpp:521-561); mc_mode 0 disables them.  V-McMD: lambda = the potential energy, the whole
+     * gradient multiplied by k_B T d ln P / d lambda.  V-AUS: lambda = the distance between the mass
+     * centres of the groups vc_atoms_a and vc_atoms_b (vc_n_states 0), a force along it on those groups,
+     * with og_params::enhance_sigma / enhance_recov_coef for the recovery outside a state's range. */
+    int           mc_mode;          /* 0, OG_EXT_VMCMD or OG_EXT_VAUS */
+    int           mc_n_states;
+    int           mc_interval;      /* steps between transition trials */
+    double        mc_temperature;   /* K, for k_B T (ExtendedVMcMD::set_temperature) */
+    const int    *mc_order;         /* [n_states] order of each state's polynomial, 0..OG_MC_MAX_ORDER */
+    const double *mc_poly;          /* [n_states * (OG_MC_MAX_ORDER + 1)] d ln P / d lambda, constant term first */
+    const double *mc_lo, *mc_hi;    /* [n_states] lambda range of each state (closed) */
+    const double *mc_prob_down;     /* [n_states] probability of a move down, when lambda allows it */
+    const double *mc_prob_up;       /* [n_states] probability of a move up */
+    int           mc_init_state;    /* 0-based */
+    /* unwrapped coordinates of the bias groups at the start, group a then group b ([3*(vc_n_a+vc_n_b)]; the
+     * group-coordinate section, Read.cpp:646-703); NULL: the atoms' own coordinates.  VcMD and V-AUS. */
+    const double *vc_crd_init;
 } og_system_desc;
+
+enum { OG_EXT_VMCMD = 2, OG_EXT_VAUS = 3 };
+#define OG_MC_MAX_ORDER 8
 
 /* Run parameters: the .cfg keys of the path (Config.cpp:37-367). */
 typedef struct {

@@ -228,6 +228,9 @@ enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND
 constexpr int OG_N_XCD = 8;  // MI355X: workgroup i of a launch runs on XCD i mod 8
 constexpr int OG_PAIR_DEFAULT = 2;  // 4 waves per block, blocks in system order
 
+static_assert(OG_MC_VMCMD == OG_EXT_VMCMD && OG_MC_VAUS == OG_EXT_VAUS && OG_MC_NPOLY == OG_MC_MAX_ORDER + 1,
+              "device extended-ensemble constants follow og_engine.h");
+
 struct og_system {
     og_dev_sys          h{};           // host copy of the descriptor (device pointers)
     og_dev_sys         *d_self = nullptr;
@@ -260,6 +263,7 @@ struct og_exec {
     int                      pair_form = 0;
     bool                     any_langevin = false, any_leapfrog = false, any_com = false, any_constraint = false;
     bool                     any_vcmd = false, any_thermo = false;
+    bool                     any_vmcmd = false;  // a member runs V-McMD
     bool                     any_bonded = false, any_quad = false;
     bool                     fused = false;       // every member takes the fused Langevin step
     int                      pair_shape = OG_PAIR_DEFAULT;  // pair-kernel launch shape (KIND_PAIR + pair_shape)
@@ -306,7 +310,7 @@ og_status exec_prepare(og_exec &x) {
     std::vector<int2>       maps[KIND_N];
     x.pair_form   = x.members[0]->pair_form;
     x.any_langevin = x.any_leapfrog = x.any_com = x.any_constraint = x.any_vcmd = x.any_thermo = false;
-    x.any_bonded = x.any_quad = false;
+    x.any_bonded = x.any_quad = x.any_vmcmd = false;
     x.fused        = true;
     x.intvl        = x.members[0]->prm.nsgrid_update_intvl;
     for (int si = 0; si < ns; si++) {
@@ -325,7 +329,9 @@ og_status exec_prepare(og_exec &x) {
         x.any_leapfrog |= d.integrator == 0;
         x.any_com |= d.com_cancel && d.n_com_groups > 0;
         x.any_constraint |= d.constraint != 0;
-        x.any_vcmd |= d.vc_n_states > 0;
+        x.any_vcmd |= d.vc_n_states > 0 || d.mc_mode == OG_MC_VAUS;
+        x.any_vmcmd |= d.mc_mode == OG_MC_VMCMD;
+        x.fused &= d.mc_mode != OG_MC_VMCMD;  // V-McMD scales the whole gradient by the step's potential energy
         x.any_thermo |= d.thermostat == OG_THERMOSTAT_SCALING;
         x.any_bonded |= d.n_bterms > 0;
         x.any_quad |= d.constraint != 0 && d.n_quads > 0;
@@ -443,6 +449,7 @@ void enqueue_integrate(og_exec &x) {
     // the extended-ensemble bias acts on the gradient of the step
     // (DynamicsMode.cpp:504-512, :824-834)
     if (x.any_vcmd) og_launch_vcmd(L(x, KIND_SYS));
+    if (x.any_vmcmd) og_launch_vmcmd(L(x, KIND_SYS), L(x, KIND_ATOM));
     if (x.any_leapfrog) {
         og_launch_leapfrog_vel(L(x, KIND_ATOM));
         if (x.any_com) {
@@ -739,8 +746,14 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.seed_hi     = (uint32_t)(p->random_seed >> 32) ^ 0x9E3779B9u;
     h.vc_n_states = d->vc_n_states;
     h.vc_interval = std::max(1, d->vc_interval);
-    h.vc_na       = d->vc_n_states ? d->vc_n_a : 0;
-    h.vc_nb       = d->vc_n_states ? d->vc_n_b : 0;
+    const bool groups_on = d->vc_n_states > 0 || d->mc_mode == OG_EXT_VAUS;  // VcMD or V-AUS groups
+    h.vc_na       = groups_on ? d->vc_n_a : 0;
+    h.vc_nb       = groups_on ? d->vc_n_b : 0;
+    h.mc_mode     = d->mc_mode;
+    h.mc_n_states = d->mc_mode ? d->mc_n_states : 0;
+    h.mc_interval = std::max(1, d->mc_interval);
+    // ExtendedVMcMD::set_temperature (Extend.cpp:119-122) with the input section's temperature
+    h.mc_constk = (8.31451 / 4.184) * 1e-3 * (double)d->mc_temperature;
     h.vc_sigma    = p->enhance_sigma;
     h.vc_coef     = p->enhance_recov_coef;
     // ExtendedVcMD::set_temperature (Extend.cpp:573-576): k_B T in kcal/mol
@@ -1001,21 +1014,30 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         }
         OG_HIP(hipMemcpy(h.com_minv, gminv.data(), sizeof(float) * ng, hipMemcpyHostToDevice));
     }
-    if (h.vc_n_states) {
-        const int ns = h.vc_n_states, ng = h.vc_na + h.vc_nb;
-        ok = A(&h.vc_atoms, ng) && A(&h.vc_crdg, 3 * (size_t)ng) && A(&h.vc_lo, ns) && A(&h.vc_hi, ns) &&
-             A(&h.vc_qinv, ns) && A(&h.vc_hist, ns);
+    if (groups_on) {
+        // VcMD / V-AUS groups: atoms, unwrapped coordinates (the launch set's group coordinates when given,
+        // else the atoms' own), float group masses (MmSystem.cpp:572-582)
+        const int ng = h.vc_na + h.vc_nb;
+        ok = A(&h.vc_atoms, ng) && A(&h.vc_crdg, 3 * (size_t)ng);
         if (!ok) return OG_E_DEVICE;
         std::vector<int>    ids(d->vc_atoms_a, d->vc_atoms_a + h.vc_na);
         ids.insert(ids.end(), d->vc_atoms_b, d->vc_atoms_b + h.vc_nb);
         std::vector<double> cg(3 * (size_t)ng);
-        float               ma = 0.0f, mb = 0.0f;  // float group masses (MmSystem.cpp:572-582)
+        float               ma = 0.0f, mb = 0.0f;
         for (int k = 0; k < ng; k++) {
-            for (int j = 0; j < 3; j++) cg[3 * k + j] = crd0[3 * (size_t)ids[k] + j];
+            for (int j = 0; j < 3; j++)
+                cg[3 * k + j] = d->vc_crd_init ? d->vc_crd_init[3 * (size_t)k + j] : crd0[3 * (size_t)ids[k] + j];
             (k < h.vc_na ? ma : mb) += m[ids[k]];
         }
         h.vc_minv_a = (double)(float)(1.0 / (double)ma);
         h.vc_minv_b = (double)(float)(1.0 / (double)mb);
+        OG_HIP(hipMemcpy(h.vc_atoms, ids.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.vc_crdg, cg.data(), sizeof(double) * cg.size(), hipMemcpyHostToDevice));
+    }
+    if (h.vc_n_states) {
+        const int ns = h.vc_n_states;
+        ok = A(&h.vc_lo, ns) && A(&h.vc_hi, ns) && A(&h.vc_qinv, ns) && A(&h.vc_hist, ns);
+        if (!ok) return OG_E_DEVICE;
         // ExtendedVcMD::set_default_q_cano (Extend.cpp:548-566): unset weights take the smallest positive one
         double qmin = 1e10;
         for (int k = 0; k < ns; k++)
@@ -1023,15 +1045,28 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
         if (qmin == 1e10) qmin = 1.0;
         std::vector<double> qinv(ns);
         for (int k = 0; k < ns; k++) qinv[k] = 1.0 / (d->vc_q_cano[k] > 0 ? d->vc_q_cano[k] : qmin);
-        OG_HIP(hipMemcpy(h.vc_atoms, ids.data(), sizeof(int) * ng, hipMemcpyHostToDevice));
-        OG_HIP(hipMemcpy(h.vc_crdg, cg.data(), sizeof(double) * cg.size(), hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.vc_lo, d->vc_min, sizeof(double) * ns, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.vc_hi, d->vc_max, sizeof(double) * ns, hipMemcpyHostToDevice));
         OG_HIP(hipMemcpy(h.vc_qinv, qinv.data(), sizeof(double) * ns, hipMemcpyHostToDevice));
+    }
+    if (h.mc_n_states) {
+        const int ns = h.mc_n_states;
+        ok = A(&h.mc_order, ns) && A(&h.mc_poly, (size_t)ns * OG_MC_NPOLY) && A(&h.mc_lo, ns) && A(&h.mc_hi, ns) &&
+             A(&h.mc_pdown, ns) && A(&h.mc_pup, ns);
+        if (!ok) return OG_E_DEVICE;
+        OG_HIP(hipMemcpy(h.mc_order, d->mc_order, sizeof(int) * ns, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.mc_poly, d->mc_poly, sizeof(double) * ns * OG_MC_NPOLY, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.mc_lo, d->mc_lo, sizeof(double) * ns, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.mc_hi, d->mc_hi, sizeof(double) * ns, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.mc_pdown, d->mc_prob_down, sizeof(double) * ns, hipMemcpyHostToDevice));
+        OG_HIP(hipMemcpy(h.mc_pup, d->mc_prob_up, sizeof(double) * ns, hipMemcpyHostToDevice));
+    }
+    if (h.vc_n_states || h.mc_n_states) {
         og_dev_counters c0{};
-        c0.vs         = d->vc_init_state;
+        c0.vs         = h.vc_n_states ? d->vc_init_state : d->mc_init_state;
         c0.step_end   = OG_NO_STEP_END;
         c0.noise_step = -1;
+        c0.mc_scale   = 1.0;
         OG_HIP(hipMemcpy(s->d_cnt, &c0, sizeof c0, hipMemcpyHostToDevice));
     }
     OG_HIP(hipMemcpy(s->d_self, &h, sizeof h, hipMemcpyHostToDevice));
@@ -1218,7 +1253,8 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     const double ng = p->com_cancel ? d->n_com_groups : 0;
     const double ng_atoms = ng > 0 ? d->com_group_off[d->n_com_groups] : 1;
     const double ke_parts = std::max(std::ceil(n / OG_THREADS), std::floor(settle / OG_WATERS_PER_BLOCK) + 2);
-    const double vc = d->vc_n_states ? d->vc_n_a + d->vc_n_b : 0, vs = d->vc_n_states;
+    const double vc = (d->vc_n_states || d->mc_mode == OG_EXT_VAUS) ? d->vc_n_a + d->vc_n_b : 0, vs = d->vc_n_states;
+    const double mcs = d->mc_mode ? d->mc_n_states : 0;
     const double log_rows = std::max(1, p->log_cap);
     double b = 0.0;
     b += n * (6 * 24 + 3 * 4 + 4 + 4.0 * OG_MAX_NB15 + 4 + 4 + 4 + 4);   // state, masses, types, table, comp, col, zkey, rev
@@ -1231,7 +1267,7 @@ og_status og_system_footprint(const og_system_desc *d, const og_params *p, uint6
     b += cells * (6 * 4 + 4.0 * OG_ENUM_STAGE + 8) + (cells / 1024 + 1) * 8;
     b += pairs * 12;
     b += 8.0 * OG_EACC_SLOTS * OG_EACC_STRIDE + ke_parts * 8 + OG_E_NTERMS * 8 + log_rows * OG_E_NTERMS * 8;
-    b += std::min(n, (double)OG_FREE_MAX) * 4 + vc * 28 + vs * 32;
+    b += std::min(n, (double)OG_FREE_MAX) * 4 + vc * 28 + vs * 32 + mcs * (4 + 8.0 * OG_MC_NPOLY + 32);
     {
         const double na = d->n_angles, ntr = (double)d->n_torsions + d->n_impros, n14 = d->n_nb14;
         const double nq = p->constraint ? d->n_shake_quads : 0;
@@ -1415,6 +1451,37 @@ og_status og_system_create(const og_system_desc *d, const og_params *p, og_syste
                 set_err("VcMD state %d: empty lambda range", k);
                 return OG_E_ARG;
             }
+    }
+    if (d->mc_mode) {
+        // V-McMD / V-AUS: one of the two, not beside VcMD; states with orders, ranges and probabilities;
+        // V-AUS needs two disjoint non-empty groups
+        const int ns = d->mc_n_states;
+        if ((d->mc_mode != OG_EXT_VMCMD && d->mc_mode != OG_EXT_VAUS) || d->vc_n_states || ns <= 0 ||
+            !d->mc_order || !d->mc_poly || !d->mc_lo || !d->mc_hi || !d->mc_prob_down || !d->mc_prob_up ||
+            d->mc_init_state < 0 || d->mc_init_state >= ns || d->mc_interval <= 0 || !(d->mc_temperature > 0.0)) {
+            set_err("V-McMD/V-AUS: mode, states, initial state, interval or temperature missing or out of range");
+            return OG_E_ARG;
+        }
+        for (int k = 0; k < ns; k++)
+            if (d->mc_order[k] < 0 || d->mc_order[k] > OG_MC_MAX_ORDER || !(d->mc_lo[k] <= d->mc_hi[k])) {
+                set_err("V-McMD/V-AUS state %d: order outside 0..%d or empty lambda range", k, OG_MC_MAX_ORDER);
+                return OG_E_ARG;
+            }
+        if (d->mc_mode == OG_EXT_VAUS) {
+            if (d->vc_n_a <= 0 || d->vc_n_b <= 0 || !d->vc_atoms_a || !d->vc_atoms_b) {
+                set_err("V-AUS needs its two groups (vc_atoms_a, vc_atoms_b)");
+                return OG_E_ARG;
+            }
+            std::vector<char> seen(d->n_atoms, 0);
+            for (int k = 0; k < d->vc_n_a + d->vc_n_b; k++) {
+                const int a = k < d->vc_n_a ? d->vc_atoms_a[k] : d->vc_atoms_b[k - d->vc_n_a];
+                if (a < 0 || a >= d->n_atoms || seen[a]) {
+                    set_err("V-AUS groups must be disjoint sets of valid atoms");
+                    return OG_E_ARG;
+                }
+                seen[a] = 1;
+            }
+        }
     }
     s->d_free = d->d_free;
     og_status st = sys_alloc(s.get(), d, p);

@@ -674,7 +674,7 @@ __global__ void __launch_bounds__(OG_WATER_THREADS) k_langevin_step(const og_dev
         const int nf = s.n_free;
         for (int k = tid; k < nf; k += OG_WATER_THREADS) og_atom_work(s, s.free_ids[k], e_ex, e_b);
         __syncthreads();
-        if (s.vc_n_states) {
+        if (s.vc_n_states || s.mc_mode == OG_MC_VAUS) {
             __shared__ double bias[3];
             __shared__ int    active;
             og_vcmd_block(s, bias, active);
@@ -762,7 +762,11 @@ __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys
 //    visit while lambda is in its range, and every vc_interval steps moves
 //    to a neighbouring state whose range holds lambda with probability
 //    proportional to 1/q_cano (uniform from Philox keyed by seed and step).
+// V-AUS (mc_mode OG_MC_VAUS, ExtendedVAUS, Extend.cpp:366-540) takes the same
+// groups and lambda; its state moves first (og_mc_transition) and its force is
+// og_vaus_lane0's.
 // Called by every thread of a block (any size); bias and active live in LDS.
+__device__ void og_vaus_lane0(const og_dev_sys &s, const double *diff, double lam, double *bias, int &active);
 __device__ void og_vcmd_block(const og_dev_sys &s, double *bias, int &active) {
     const int lane = threadIdx.x, nt = blockDim.x;
     const int n    = s.vc_na + s.vc_nb;
@@ -797,6 +801,9 @@ __device__ void og_vcmd_block(const og_dev_sys &s, double *bias, int &active) {
         double diff[3];
         for (int d = 0; d < 3; d++) diff[d] = og_mimg(ca[d] * s.vc_minv_a - cb[d] * s.vc_minv_b, s.L[d]);
         const double lam = sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);
+        if (s.mc_mode == OG_MC_VAUS) {
+            og_vaus_lane0(s, diff, lam, bias, active);
+        } else {
         const int    vs  = s.cnt->vs;
         const double lo = s.vc_lo[vs], hi = s.vc_hi[vs];
         double       ex = 0.0, rec = 0.0;
@@ -834,6 +841,7 @@ __device__ void og_vcmd_block(const og_dev_sys &s, double *bias, int &active) {
                 s.cnt->vs        = u * (w0 + w1) < w0 ? cand[0] : cand[1];
             }
         }
+        }
     }
     __syncthreads();
     if (!active) return;
@@ -847,10 +855,121 @@ __device__ void og_vcmd_block(const og_dev_sys &s, double *bias, int &active) {
     }
 }
 
+// ------------------------------------------------------- V-McMD, V-AUS --
+// ExtendedVMcMD::trial_transition + set_current_vstate (Extend.cpp:143-176):
+// a move up is tried first, then down; a move needs lambda inside both
+// (closed) ranges and its uniform draw above 1 - the state's probability
+// for that direction.  The two draws come from Philox keyed by seed and step.
+__device__ int og_mc_trial(const og_dev_sys &s, int src, int rel, double lam, double u) {
+    if (!(lam >= s.mc_lo[src] && lam <= s.mc_hi[src])) return src;
+    if (src == 0 && rel == -1) return src;
+    if (src == s.mc_n_states - 1 && rel == 1) return src;
+    const int dst = src + rel;
+    if (lam >= s.mc_lo[dst] && lam <= s.mc_hi[dst]) {
+        const double p = rel == -1 ? s.mc_pdown[src] : s.mc_pup[src];
+        if (u > 1.0 - p) return dst;
+    }
+    return src;
+}
+__device__ int og_mc_transition(const og_dev_sys &s, int vs, double lam, long long step) {
+    const uint4  r    = og_philox(make_uint4((uint32_t)step, (uint32_t)(step >> 32), 1u, 0x4D43u),
+                                  make_uint2(s.seed_lo, s.seed_hi));
+    const double u_up = ((double)r.x + 0.5) * 2.3283064365386963e-10;
+    const double u_dn = ((double)r.y + 0.5) * 2.3283064365386963e-10;
+    const int    up   = og_mc_trial(s, vs, 1, lam, u_up);
+    return up != vs ? up : og_mc_trial(s, vs, -1, lam, u_dn);
+}
+// the state's d ln P / d lambda at lambda clamped to its range
+__device__ double og_mc_dlnp(const og_dev_sys &s, int vs, double param) {
+    const double *c = s.mc_poly + (size_t)vs * OG_MC_NPOLY;
+    double        d = c[0], t = 1.0;
+    for (int i = 1; i <= s.mc_order[vs]; i++) {
+        t *= param;
+        d += c[i] * t;
+    }
+    return d;
+}
+
+// V-AUS on lane 0 of the group block (ExtendedVAUS::scale_force,
+// Extend.cpp:460-540): the state moves first (apply_bias, :126-141); outside
+// [lo - sigma, hi + sigma] a harmonic recovery, and k_B T (d ln P(lambda
+// clamped) + recovery) along the unit vector.  The reference reports no
+// bias energy for V-AUS.
+__device__ void og_vaus_lane0(const og_dev_sys &s, const double *diff, double lam, double *bias, int &active) {
+    const long long step = s.cnt->step;
+    int             vs   = s.cnt->vs;
+    if (step > 0 && step % s.mc_interval == 0) {
+        vs        = og_mc_transition(s, vs, lam, step);
+        s.cnt->vs = vs;
+    }
+    const double lo = s.mc_lo[vs], hi = s.mc_hi[vs];
+    double       param = lam, recovery = 0.0;
+    if (param <= lo) {
+        if (param <= lo - s.vc_sigma) recovery = s.vc_coef * (param - (lo - s.vc_sigma));
+        param = lo;
+    } else if (param >= hi) {
+        if (param >= hi + s.vc_sigma) recovery = s.vc_coef * (param - (hi + s.vc_sigma));
+        param = hi;
+    }
+    const double dew = s.mc_constk * (og_mc_dlnp(s, vs, param) + recovery);
+    for (int d = 0; d < 3; d++) bias[d] = diff[d] / lam;
+    for (int d = 0; d < 3; d++) bias[d] = dew * bias[d];
+    active                = 1;
+    s.energy[OG_E_EXTEND] = 0.0;
+    s.energy[OG_E_LAMBDA] = lam;
+    s.energy[OG_E_VSTATE] = (double)vs;
+}
+
+// V-McMD (ExtendedVMcMD::apply_bias + scale_force, Extend.cpp:126-200), one
+// wave per system after the gradient is complete: lambda = the potential
+// energy of the step from the fixed-point accumulators (read, not cleared:
+// the step's closing kernel takes them), the state move every mc_interval
+// steps, then the factor k_B T d ln P(lambda clamped) that k_vmcmd_scale
+// applies to every atom's gradient.
+__global__ void __launch_bounds__(64) k_vmcmd(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
+    const og_dev_sys &s = S[map[blockIdx.x].x];
+    if (og_frozen(s) || s.mc_mode != OG_MC_VMCMD) return;
+    long long v = 0;
+    for (int slot = threadIdx.x; slot < OG_EACC_SLOTS; slot += 64) {
+        const long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
+        v += a[OG_EACC_VDW] + a[OG_EACC_ELE] + a[OG_EACC_EXCESS] + a[OG_EACC_BOND] + a[OG_EACC_ANGLE] +
+             a[OG_EACC_TORSION] + a[OG_EACC_IMPRO] + a[OG_EACC_VDW14] + a[OG_EACC_ELE14];
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (threadIdx.x != 0) return;
+    const double    lam  = (double)v * OG_FX_INV + s.e_self;
+    const long long step = s.cnt->step;
+    int             vs   = s.cnt->vs;
+    if (step > 0 && step % s.mc_interval == 0) {
+        vs        = og_mc_transition(s, vs, lam, step);
+        s.cnt->vs = vs;
+    }
+    const double lo = s.mc_lo[vs], hi = s.mc_hi[vs];
+    const double param = lam <= lo ? lo : (lam >= hi ? hi : lam);
+    s.cnt->mc_scale       = s.mc_constk * og_mc_dlnp(s, vs, param);
+    s.energy[OG_E_EXTEND] = 0.0;
+    s.energy[OG_E_LAMBDA] = lam;
+    s.energy[OG_E_VSTATE] = (double)vs;
+}
+__global__ void __launch_bounds__(OG_THREADS) k_vmcmd_scale(const og_dev_sys *__restrict__ S,
+                                                            const int2 *__restrict__ map) {
+    const int2        bm = map[blockIdx.x];
+    const og_dev_sys &s  = S[bm.x];
+    if (og_frozen(s) || s.mc_mode != OG_MC_VMCMD) return;
+    const int i = bm.y * OG_THREADS + threadIdx.x;
+    if (i >= s.n_atoms) return;
+    const double f = s.cnt->mc_scale;
+    for (int k = 0; k < 3; k++) s.work[3 * i + k] *= f;
+}
+void og_launch_vmcmd(const og_launch &sys, const og_launch &atoms) {
+    hipLaunchKernelGGL(k_vmcmd, dim3(sys.n_blocks), dim3(64), 0, sys.stream, sys.sys, sys.map);
+    hipLaunchKernelGGL(k_vmcmd_scale, dim3(atoms.n_blocks), dim3(OG_THREADS), 0, atoms.stream, atoms.sys, atoms.map);
+}
+
 __global__ void __launch_bounds__(64) k_vcmd(const og_dev_sys *__restrict__ S, const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
     if (og_frozen(s)) return;
-    if (s.vc_n_states == 0) return;
+    if (s.vc_n_states == 0 && s.mc_mode != OG_MC_VAUS) return;
     __shared__ double bias[3];
     __shared__ int    active;
     og_vcmd_block(s, bias, active);

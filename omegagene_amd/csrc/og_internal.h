@@ -25,6 +25,9 @@
 #define OG_FXJ_ROW_LIMIT 2048.0f
 #define OG_NO_STEP_END 0x7fffffffffffffffLL  // og_dev_counters::step_end of a system with no segment limit
 #define OG_TH_SCALING 1  // og_params::thermostat value of the scaling thermostat (OG_THERMOSTAT_SCALING)
+// og_dev_sys::mc_mode values (the public OG_EXT_VMCMD / OG_EXT_VAUS) and coefficients per state (OG_MC_MAX_ORDER + 1)
+enum { OG_MC_VMCMD = 2, OG_MC_VAUS = 3 };
+#define OG_MC_NPOLY 9
 
 // energy terms kept per system and per step
 enum {
@@ -70,6 +73,7 @@ struct og_dev_counters {
     long long step_end;  // the system stands still once step reaches this (segment end)
     long long noise_step;  // the step whose Langevin update takes its normals from og_dev_sys::noise (-1: none)
     int       rebuild_now; // 1: rebuild the grid before the next step whatever its phase (cleared by the step)
+    double    mc_scale;    // V-McMD: the gradient factor of the current step
 };
 
 struct og_dev_sys {
@@ -135,6 +139,12 @@ struct og_dev_sys {
     double *vc_crdg;                   // [3*(vc_na+vc_nb)] unwrapped group coordinates
     double *vc_lo, *vc_hi, *vc_qinv;   // [n_states] ranges and 1/q_cano
     double *vc_hist;                   // [n_states] visit counts (q_raw)
+    // ---- V-McMD / V-AUS (mc_mode OG_EXT_VMCMD / OG_EXT_VAUS; V-AUS takes its groups from vc_*) ----
+    int     mc_mode, mc_n_states, mc_interval;
+    double  mc_constk;                 // k_B T in kcal/mol
+    int    *mc_order;                  // [n_states]
+    double *mc_poly;                   // [n_states * OG_MC_NPOLY]
+    double *mc_lo, *mc_hi, *mc_pdown, *mc_pup;  // [n_states]
     // ---- atom-order arrays ----
     double *crd, *crd_prev, *crd_prev2, *vel, *vel_next, *work;
     float  *mass, *mass_inv, *charge;
@@ -256,7 +266,8 @@ __device__ __forceinline__ bool og_rebuild_skip(const og_dev_sys &s) {
 // Energies (pair, virial) are evaluated on logged steps and on request; the
 // forces and the pair count on every step.
 __device__ __forceinline__ bool og_energy_step(const og_dev_sys &s) {
-    return s.cnt->force_energy || (s.cnt->step % s.log_stride) == 0;
+    // V-McMD needs the potential energy of every step (its lambda)
+    return s.cnt->force_energy || (s.cnt->step % s.log_stride) == 0 || s.mc_mode == OG_MC_VMCMD;
 }
 
 __device__ __forceinline__ int og_lane() { return __lane_id(); }
@@ -424,5 +435,6 @@ void og_launch_shake_quad(const og_launch &quads);  // OG_QUAD_THREADS threads p
 void og_launch_langevin_step(const og_launch &a);
 void og_launch_finish(const og_launch &atoms);
 void og_launch_step_close(const og_launch &sys);  // energies of the step, log entry, step counter
-void og_launch_vcmd(const og_launch &sys);
+void og_launch_vcmd(const og_launch &sys);  // VcMD and V-AUS group bias
+void og_launch_vmcmd(const og_launch &sys, const og_launch &atoms);  // V-McMD: lambda, state, gradient factor
 void og_launch_refresh_grid(const og_launch &slots);

@@ -76,7 +76,27 @@ class Desc(C.Structure):
                 ("n_nb14", C.c_int), ("nb14_ids", C.POINTER(C.c_int)), ("nb14_types", C.POINTER(C.c_int)),
                 ("nb14_coeff_vdw", C.POINTER(C.c_double)), ("nb14_coeff_ele", C.POINTER(C.c_double)),
                 ("n_shake_quads", C.c_int), ("shake_quad_ids", C.POINTER(C.c_int)),
-                ("shake_quad_d2", C.POINTER(C.c_double))]
+                ("shake_quad_d2", C.POINTER(C.c_double)),
+                ("mc_mode", C.c_int), ("mc_n_states", C.c_int), ("mc_interval", C.c_int),
+                ("mc_temperature", C.c_double), ("mc_order", C.POINTER(C.c_int)), ("mc_poly", C.POINTER(C.c_double)),
+                ("mc_lo", C.POINTER(C.c_double)), ("mc_hi", C.POINTER(C.c_double)),
+                ("mc_prob_down", C.POINTER(C.c_double)), ("mc_prob_up", C.POINTER(C.c_double)),
+                ("mc_init_state", C.c_int), ("vc_crd_init", C.POINTER(C.c_double))]
+
+
+OG_EXT_VMCMD, OG_EXT_VAUS = 2, 3
+OG_MC_MAX_ORDER = 8
+
+
+def ext_mode(cfg: dict, ls: LaunchSet) -> str:
+    """The --extended-ensemble of cfg (Config.cpp:145-156) when the launch set
+    carries its block: "vcmd", "v-mcmd", "v-aus" or "none"."""
+    m = cfg.get("extended-ensemble", "none")
+    if m == "vcmd" and ls.vcmd is not None:
+        return m
+    if m in ("v-mcmd", "v-aus") and ls.vmcmd is not None and (m == "v-mcmd" or ls.group_coord is not None):
+        return m
+    return "none"
 
 
 class Params(C.Structure):
@@ -160,7 +180,11 @@ class _Keep:
     """Holds the numpy arrays a Desc points into."""
 
 
-def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcmd: bool = False):
+def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcmd: bool = False, ext: str = "none"):
+    """og_system_desc of a launch set; ext (or vcmd=True for "vcmd") selects
+    the extended ensemble whose block the launch set carries."""
+    if vcmd:
+        ext = "vcmd"
     k = _Keep()
 
     def d(a, dt=np.float64):
@@ -230,7 +254,7 @@ def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcm
     qids, qd2 = ls.shake_quad_tables()
     if len(qids):
         x.n_shake_quads, x.shake_quad_ids, x.shake_quad_d2 = len(qids), d(qids, np.int32), d(qd2)
-    vc = ls.vcmd if vcmd else None
+    vc = ls.vcmd if ext == "vcmd" else None
     if vc is not None:
         # 1-D VcMD on the distance between the mass centres of the dimension's two groups (Read.cpp:705-779)
         gids, _names, ranges = vc.dims[0]
@@ -248,6 +272,33 @@ def desc_from_launchset(ls: LaunchSet, constraint: int = 1, com_groups=None, vcm
         x.vc_n_a, x.vc_n_b = len(ga), len(gb)
         x.vc_atoms_a = d(ga, np.int32)
         x.vc_atoms_b = d(gb, np.int32)
+        gc = ls.group_coord
+        if gc is not None and list(gc.group_ids[:2]) == list(gids[:2]):
+            x.vc_crd_init = d(np.concatenate(gc.crd[:2]))   # the launch set's unwrapped group coordinates
+    if ext in ("v-mcmd", "v-aus"):
+        # V-McMD / V-AUS block (Read.cpp:521-561); V-AUS groups from the group-coordinate block (:646-703)
+        mc = ls.vmcmd
+        ns = len(mc.states)
+        poly = np.zeros((ns, OG_MC_MAX_ORDER + 1))
+        for i, st in enumerate(mc.states):
+            if st[0] > OG_MC_MAX_ORDER:
+                raise ValueError("virtual state %d: polynomial order %d above %d" % (i, st[0], OG_MC_MAX_ORDER))
+            poly[i, :st[0] + 1] = st[5][:st[0] + 1]
+        x.mc_mode = OG_EXT_VMCMD if ext == "v-mcmd" else OG_EXT_VAUS
+        x.mc_n_states, x.mc_interval, x.mc_temperature = ns, int(mc.interval), float(mc.temperature)
+        x.mc_order = d(np.array([st[0] for st in mc.states]), np.int32)
+        x.mc_poly = d(poly)
+        x.mc_lo, x.mc_hi = d(np.array([st[1] for st in mc.states])), d(np.array([st[2] for st in mc.states]))
+        x.mc_prob_down = d(np.array([st[3] for st in mc.states]))
+        x.mc_prob_up = d(np.array([st[4] for st in mc.states]))
+        x.mc_init_state = int(mc.init_vs)
+        if ext == "v-aus":
+            gc = ls.group_coord
+            ga = np.asarray(ls.groups[gc.group_ids[0] - 1][1], np.int32)
+            gb = np.asarray(ls.groups[gc.group_ids[1] - 1][1], np.int32)
+            x.vc_n_a, x.vc_n_b = len(ga), len(gb)
+            x.vc_atoms_a, x.vc_atoms_b = d(ga, np.int32), d(gb, np.int32)
+            x.vc_crd_init = d(np.concatenate(gc.crd[:2]))
     return x, k
 
 
@@ -271,8 +322,7 @@ class System:
         L = lib()
         _check(L.og_set_device(C.c_int(device)))
         self.params = params_from_cfg(cfg)
-        vcmd = cfg.get("extended-ensemble", "none") == "vcmd" and ls.vcmd is not None
-        self.desc, self._keep = desc_from_launchset(ls, self.params.constraint, com_groups, vcmd=vcmd)
+        self.desc, self._keep = desc_from_launchset(ls, self.params.constraint, com_groups, ext=ext_mode(cfg, ls))
         self.n_atoms = ls.n_atoms
         self.d_free = self.desc.d_free
         self.h = C.c_void_p()
@@ -478,7 +528,7 @@ class Batch:
 
 def footprint(ls: LaunchSet, cfg: dict) -> int:
     p = params_from_cfg(cfg)
-    d, keep = desc_from_launchset(ls, p.constraint)
+    d, keep = desc_from_launchset(ls, p.constraint, ext=ext_mode(cfg, ls))
     b = C.c_uint64()
     _check(lib().og_system_footprint(C.byref(d), C.byref(p), C.byref(b)))
     return int(b.value)
@@ -512,8 +562,7 @@ class Server:
 
     def submit_new(self, ls: LaunchSet, cfg: dict, n_steps: int):
         p = params_from_cfg(cfg)
-        vcmd = cfg.get("extended-ensemble", "none") == "vcmd" and ls.vcmd is not None
-        d, keep = desc_from_launchset(ls, p.constraint, vcmd=vcmd)
+        d, keep = desc_from_launchset(ls, p.constraint, ext=ext_mode(cfg, ls))
         rid, wid = C.c_longlong(), C.c_longlong()
         _check(lib().og_server_submit_new(self.h, C.byref(d), C.byref(p), C.c_int(n_steps), C.byref(rid),
                                           C.byref(wid)))
