@@ -315,7 +315,8 @@ int og_alloc_set_lj_params(float *h_lj_6term, float *h_lj_12term, int n_lj_types
     {
         const size_t       nt2 = (size_t)n_lj_types * n_lj_types;
         std::vector<float2> ljp(nt2);
-        for (size_t k = 0; k < nt2; k++) ljp[k] = make_float2(h_lj_6term[k], h_lj_12term[k]);
+        // the pair kernel's scaled entries {6 A6, 12 A12}
+        for (size_t k = 0; k < nt2; k++) ljp[k] = make_float2(6.0f * h_lj_6term[k], 12.0f * h_lj_12term[k]);
         g_cs.lj6_host.assign(h_lj_6term, h_lj_6term + nt2);
         g_cs.lj12_host.assign(h_lj_12term, h_lj_12term + nt2);
         CS_TRY(cs_alloc(&h.lj_pair, nt2));
@@ -356,7 +357,7 @@ int og_free_lj_params(void) {
 }
 
 // cuda_alloc_set_hps_params (cuda_setup.cu:1230-1244): the HPS switch distance and lambda per type pair,
-// joined here with the LJ table into the {A6, A12, switch, lambda} entries the pair kernel reads
+// joined here with the LJ table into the {6 A6, 12 A12, switch, lambda} entries the pair kernel reads
 int og_alloc_set_hps_params(float *h_hps_cutoff, float *h_hps_lambda, int n_lj_types) {
     CS_REQUIRE(g_cs.have_lj, "og_alloc_set_hps_params: call og_alloc_set_lj_params first");
     if (!h_hps_cutoff || !h_hps_lambda || n_lj_types != g_cs.n_types) return OG_E_ARG;
@@ -364,7 +365,7 @@ int og_alloc_set_hps_params(float *h_hps_cutoff, float *h_hps_lambda, int n_lj_t
     const size_t nt2 = (size_t)n_lj_types * n_lj_types;
     std::vector<float4> t4(nt2);
     for (size_t k = 0; k < nt2; k++)
-        t4[k] = make_float4(g_cs.lj6_host[k], g_cs.lj12_host[k], h_hps_cutoff[k], h_hps_lambda[k]);
+        t4[k] = make_float4(6.0f * g_cs.lj6_host[k], 12.0f * g_cs.lj12_host[k], h_hps_cutoff[k], h_hps_lambda[k]);
     if (!h.lj_hps) CS_TRY(cs_alloc(&h.lj_hps, nt2));
     CS_HIP(hipMemcpy(h.lj_hps, t4.data(), sizeof(float4) * nt2, hipMemcpyHostToDevice));
     g_cs.have_hps = true;

@@ -855,18 +855,19 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     OG_HIP(hipMemcpy(h.lj12, lj12f.data(), sizeof(float) * lj12f.size(), hipMemcpyHostToDevice));
     {
         std::vector<float2> ljp(lj6f.size());
-        for (size_t k = 0; k < ljp.size(); k++) ljp[k] = make_float2(lj6f[k], lj12f[k]);
+        // the pair kernel's scaled entries {6 A6, 12 A12}
+        for (size_t k = 0; k < ljp.size(); k++) ljp[k] = make_float2(6.0f * lj6f[k], 12.0f * lj12f[k]);
         OG_HIP(hipMemcpy(h.lj_pair, ljp.data(), sizeof(float2) * ljp.size(), hipMemcpyHostToDevice));
     }
     if (s->pair_form == OG_FORM_HPS_DH) {
-        // {A6, A12, switch distance, lambda} in the renumbered type order; plain LJ (nonbond lennard-jones)
+        // {6 A6, 12 A12, switch distance, lambda} in the renumbered type order; plain LJ (nonbond lennard-jones)
         // is the HPS form with lambda 1 and switch distance 0
         const bool          hps = p->nonbond == OG_NONBOND_HPS;
         std::vector<float4> t4((size_t)nt * nt);
         for (int a = 0; a < nt; a++)
             for (int b = 0; b < nt; b++) {
                 const size_t src = (size_t)torder[a] * nt + torder[b], dst = (size_t)a * nt + b;
-                t4[dst] = make_float4(lj6f[dst], lj12f[dst], hps ? (float)d->hps_cutoff[src] : 0.0f,
+                t4[dst] = make_float4(6.0f * lj6f[dst], 12.0f * lj12f[dst], hps ? (float)d->hps_cutoff[src] : 0.0f,
                                       hps ? (float)d->hps_lambda[src] : 1.0f);
             }
         OG_HIP(hipMemcpy(h.lj_hps, t4.data(), sizeof(float4) * t4.size(), hipMemcpyHostToDevice));

@@ -152,8 +152,10 @@ struct og_dev_sys {
     int    *nb15off;        // [n_atoms*32], -1 terminated
     int    *comp;           // exclusion-connectivity component of each atom
     float  *lj6, *lj12;     // [n_types*n_types]
-    float2 *lj_pair;        // [n_types*n_types] {A6, A12} interleaved for the pair kernel
-    float4 *lj_hps;         // [n_types*n_types] {A6, A12, HPS switch distance, HPS lambda} (pair form HPS + DH)
+    float2 *lj_pair;        // [n_types*n_types] {6 A6, 12 A12} interleaved for the pair kernel (the gradient's
+                            // coefficients; lj_unscaled: {A6, A12} as the reference's table)
+    float4 *lj_hps;         // [n_types*n_types] {6 A6, 12 A12, HPS switch distance, HPS lambda} (pair form HPS + DH)
+    int     lj_unscaled;    // 1: lj_pair holds {A6, A12} (a caller's table), scaled by the pair kernel
     int    *excess;         // [n_excess*2]
     int    *bond_ids;       // [n_bonds*2]
     double *bond_k, *bond_r0;
@@ -306,6 +308,14 @@ __device__ __forceinline__ long long og_fx_fast(double v) {
 __device__ __forceinline__ unsigned long long og_fxj_pack(double x, double y) {
     const long long ix = __double_as_longlong(x * OG_FXJ_SCALE + OG_FX_MAGIC) - OG_FX_MAGIC_BITS;
     const long long iy = __double_as_longlong(y * OG_FXJ_SCALE + OG_FX_MAGIC) - OG_FX_MAGIC_BITS;
+    return (unsigned long long)ix + ((unsigned long long)iy << 32);
+}
+// The same word from float inputs (|x|, |y| below 4096): x * 2^19 is exact in
+// float and rintf rounds half to even like the FP64 form, so the result is
+// bit-identical to og_fxj_pack((double)x, (double)y), in FP32 instructions.
+__device__ __forceinline__ unsigned long long og_fxj_pack_f(float x, float y) {
+    const long long ix = (long long)(int)__builtin_rintf(x * 524288.0f);
+    const long long iy = (long long)(int)__builtin_rintf(y * 524288.0f);
     return (unsigned long long)ix + ((unsigned long long)iy << 32);
 }
 // the two fields of a packed sum (each within +-2^31 units)

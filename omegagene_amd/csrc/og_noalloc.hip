@@ -147,6 +147,7 @@ extern "C" int og_pairwise_ljzd_noalloc(const float *d_xyzq, const int *d_atom_t
     h.gxyzq    = (float4 *)d_xyzq;
     h.gtype    = (int *)d_atom_type;
     h.lj_pair  = (float2 *)d_lj_pair;
+    h.lj_unscaled = 1;  // the caller's table is the reference's {A6, A12}
     h.cp_head  = (int *)d_idx_head_cell_pairs;
     h.cp_c2    = (int *)d_cell_pair_c2;
     h.cp_mask  = (uint64_t *)d_cell_pair_mask;

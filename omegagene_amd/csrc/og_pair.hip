@@ -88,7 +88,7 @@ __device__ __forceinline__ void og_add_fx3(OG_GLOBAL(long long) *FX, int cap, in
 __device__ __forceinline__ void og_flush_j(OG_GLOBAL(long long) *FX, int cap, og_jpend &p) {
     if (p.live) {
         const int                f  = og_fx_slot(p.a);
-        const unsigned long long xy = og_fxj_pack(-(double)p.x, -(double)p.y);
+        const unsigned long long xy = og_fxj_pack_f(-p.x, -p.y);
         const long long          vz = og_fx_fast(-(double)p.z);
         if (xy != 0) og_atomic_add_fx_g(FX + 3 * cap + f, (long long)xy);
         if (vz != 0) og_atomic_add_fx_g(FX + 2 * cap + f, vz);
@@ -120,6 +120,14 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
     int   npl = 0;  // in-cutoff unmasked pairs of this lane (summed over the wave with the energies)
 #pragma unroll
     for (int i = 0; i < OG_NAC; i++) ljv[i] = LJ_LDS ? ljl[c.row[i] + tj] : LJP[c.row[i] + tj];
+    // the tables hold {6 A6, 12 A12}; a caller's table in the reference's {A6, A12} is scaled here
+    if (s.lj_unscaled) {
+#pragma unroll
+        for (int i = 0; i < OG_NAC; i++) {
+            ljv[i].x *= 6.0f;
+            ljv[i].y *= 12.0f;
+        }
+    }
 #pragma unroll
     for (int i = 0; i < OG_NAC; i++) {
         const float dx = c.x[i] - px;
@@ -128,17 +136,18 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         const float r2 = dx * dx + dy * dy + dz * dz;
         const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
         npl += ok ? 1 : 0;
-        const typename og_ljt<FORM>::type lj = ljv[i];  // {A6, A12(, HPS switch, HPS lambda)}
+        const typename og_ljt<FORM>::type lj = ljv[i];  // {6 A6, 12 A12(, HPS switch, HPS lambda)}
         const float rinv  = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
-        const float t6    = lj.x * r6inv;
-        const float t12   = lj.y * r6inv * r6inv;
+        // r^2 times the LJ gradient coefficient, 6 A6 r^-6 - 12 A12 r^-12, and (energy steps) the LJ energy
+        const float ljf   = r6inv * fmaf(-lj.y, r6inv, lj.x);
         const float cc    = c.q[i] * pw;
-        float       g, eel, elj = t12 - t6;
+        float       g, eel, elj = 0.0f;
+        if constexpr (ENERGY) elj = r6inv * fmaf(lj.y * (1.0f / 12.0f), r6inv, lj.x * (-1.0f / 6.0f));
         if constexpr (FORM == 0) {
             eel = cc * (rinv - zcore + bcoeff * r2);
-            g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv - fcoeff);
+            g   = r2inv * ljf - cc * (rinv * r2inv - fcoeff);
         } else if constexpr (FORM == 2) {
             // HPS: beyond the pair's switch distance lambda * LJ, inside it LJ + (1 - lambda) eps;
             // Debye-Hueckel: cc (charges x 332.06378 / eps_r) exp(-kappa r) / r, gradient coefficient
@@ -148,21 +157,21 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
             const float sc  = far ? lj.w : 1.0f;
             elj             = fmaf(elj, sc, far ? 0.0f : (1.0f - lj.w) * hps_eps);
             eel             = cc * rinv * __expf(-r * kappa);
-            g = sc * r2inv * (6.0f * t6 - 12.0f * t12) - eel * (rinv + kappa) * (dh_fix_on ? rinv : 1.0f);
+            g = sc * r2inv * ljf - eel * (rinv + kappa) * (dh_fix_on ? rinv : 1.0f);
         } else if constexpr (FORM == 3 || FORM == 4) {
             // zero-quadrupole/-octupole/-hexadecapole: the order's polynomials in r^2 by Horner's rule
             const float pe = r2 * fmaf(r2, fmaf(r2, fmaf(r2, ze3, ze2), ze1), ze0);
             const float pg = fmaf(r2, fmaf(r2, fmaf(r2, zg3, zg2), zg1), zg0);
             if constexpr (FORM == 3) {
                 eel = cc * (rinv - zmc - pe);
-                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (rinv * r2inv + pg);
+                g   = r2inv * ljf - cc * (rinv * r2inv + pg);
             } else {
                 const float r   = r2 * rinv;
                 const float ar  = alpha * r;
                 const float ex  = __expf(-ar * ar);
                 const float erc = og_erfc_as(ar, ex);
                 eel = cc * (rinv * erc - zmc - pe);
-                g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (erc * rinv * r2inv + piewald * ex * r2inv + pg);
+                g   = r2inv * ljf - cc * (erc * rinv * r2inv + piewald * ex * r2inv + pg);
             }
         } else {
             const float r   = r2 * rinv;
@@ -170,7 +179,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
             const float ex  = __expf(-ar * ar);
             const float erc = og_erfc_as(ar, ex);
             eel = cc * (rinv * erc - zcore + bcoeff * r2);
-            g   = r2inv * (6.0f * t6 - 12.0f * t12) - cc * (erc * rinv * r2inv + piewald * ex * r2inv - fcoeff);
+            g   = r2inv * ljf - cc * (erc * rinv * r2inv + piewald * ex * r2inv - fcoeff);
         }
         // lanes without a pair (masked, out of range, idle) contribute exact zeros
         g                 = ok ? g : 0.0f;
