@@ -503,7 +503,7 @@ def main():
         stride = int(cfg.get("print-interval-log", 1000))
         s2 = System(ls2, dict(cfg, **{"log-stride": stride, "log-cap": max(args.steps // stride, 1)}))
         s2.run(args.warmup)
-        s2.set_pair_shape(tune["waves_per_block"])
+        s2.set_pair_shape(tune["waves_per_block"], tune["order"])
         lib().og_thread_sync()
         t1 = time.perf_counter()
         s2.run(args.steps)

@@ -378,6 +378,10 @@ og_status   og_pair_kernel_time(og_system *sys, double *ms, int *n_launches);
 og_status   og_autotune(og_system *sys, int trials, int *waves, double *pair_ms);
 og_status   og_batch_autotune(og_batch *b, int trials, int *waves, int *order, double *pair_ms);
 og_status   og_set_pair_shape(og_system *sys, int waves);
+/* The pair kernel's launch shape of a system: waves per block and block order (0 system order, 3 the
+ * block list in eight contiguous parts, one per XCD); og_autotune may choose either order. */
+og_status   og_get_pair_shape(og_system *sys, int *waves, int *order);
+og_status   og_set_pair_shape_order(og_system *sys, int waves, int order);
 /* Cell-edge autotuner (no counterpart in the reference): for a system on the
  * full z-bin scan (gap_bug 0) times the pair kernel on grids whose column
  * width is 0.8, 0.9, 1, 1.12 and 1.25 times the reference's (temporary

@@ -222,9 +222,10 @@ enum { KIND_ATOM = 0, KIND_SLOT, KIND_CELLW, KIND_COL, KIND_SYS, KIND_MISC, KIND
        KIND_BTERM,  // OG_THREADS threads per block over the bonded terms (angles, torsions, impropers, 1-4)
        KIND_QUAD,   // OG_QUAD_THREADS threads per block over the four-atom SHAKE groups
        KIND_SCANB,  // blocks of 1024 cells over cap_cells + 1 (multi-block offset scan)
-       KIND_PAIR,   // 12 pair-kernel maps: index log2(waves per block) + 4 * order (0 system order,
-                    // 1 round-robin across systems, 2 each system's blocks on one XCD)
-       KIND_N = KIND_PAIR + 12 };
+       KIND_PAIR,   // 16 pair-kernel maps: index log2(waves per block) + 4 * order (0 system order,
+                    // 1 round-robin across systems, 2 each system's blocks on one XCD, 3 the system-order
+                    // list cut into eight contiguous parts, one per XCD)
+       KIND_N = KIND_PAIR + 16 };
 constexpr int OG_N_XCD = 8;  // MI355X: workgroup i of a launch runs on XCD i mod 8
 constexpr int OG_PAIR_DEFAULT = 2;  // 4 waves per block, blocks in system order
 
@@ -391,6 +392,18 @@ og_status exec_prepare(og_exec &x) {
             }
             maps[KIND_PAIR + 8 + lw].push_back(lists[xc][at[xc]++]);
         }
+    }
+    // pair kernel, the system-order list in eight contiguous parts, part k at the launch positions of XCD k:
+    // neighbouring cells (and so the atoms their rows share) stay on one XCD's L2
+    for (int lw = 0; lw < 4; lw++) {
+        const std::vector<int2> &all = maps[KIND_PAIR + lw];
+        const size_t             T   = all.size();
+        size_t                   start[OG_N_XCD + 1];
+        start[0] = 0;
+        for (int k = 0; k < OG_N_XCD; k++) start[k + 1] = start[k] + T / OG_N_XCD + ((size_t)k < T % OG_N_XCD);
+        for (size_t i = 0; i < T / OG_N_XCD + 1; i++)
+            for (int k = 0; k < OG_N_XCD; k++)
+                if (start[k] + i < start[k + 1]) maps[KIND_PAIR + 12 + lw].push_back(all[start[k] + i]);
     }
     // COM-apply map from per-system totals (kept on the system)
     maps[KIND_COMA].clear();
@@ -1107,8 +1120,9 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
     hipEvent_t e0 = ev.a, e1 = ev.b;
     double    best     = 1e300;
     int       best_shp = x.pair_shape;
-    const int n_shapes = x.members.size() > 1 ? 12 : 4;
-    for (int shp = 0; shp < n_shapes; shp++) {
+    // shapes: waves per block in each block order (for one system only orders 0 and 3 differ)
+    for (int shp = 0; shp < 16; shp++) {
+        if (x.members.size() == 1 && shp >= 4 && shp < 12) continue;
         const og_launch a = L(x, KIND_PAIR + shp);
         const int       w = 1 << (shp & 3);
         og_launch_pair(a, x.pair_form, w);
@@ -1148,8 +1162,9 @@ og_status exec_set_pair_shape(og_exec &x, int waves, int order) {
         set_err("pair kernel waves per block must be 1, 2, 4 or 8");
         return OG_E_ARG;
     }
-    if (order < 0 || order > 2) {
-        set_err("pair kernel block order must be 0 (system order), 1 (round-robin) or 2 (one XCD per system)");
+    if (order < 0 || order > 3) {
+        set_err("pair kernel block order must be 0 (system order), 1 (round-robin), 2 (one XCD per system) or 3 "
+                "(contiguous part per XCD)");
         return OG_E_ARG;
     }
     const int shp = lw + 4 * order;
@@ -1826,6 +1841,19 @@ og_status og_set_pair_shape(og_system *s, int waves) {
     if (!s) return OG_E_ARG;
     og_status st = ensure_ctx(s);
     return st != OG_OK ? st : exec_set_pair_shape(*s->ctx, waves, 0);
+}
+og_status og_set_pair_shape_order(og_system *s, int waves, int order) {
+    if (!s) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    return st != OG_OK ? st : exec_set_pair_shape(*s->ctx, waves, order);
+}
+og_status og_get_pair_shape(og_system *s, int *waves, int *order) {
+    if (!s || !waves || !order) return OG_E_ARG;
+    og_status st = ensure_ctx(s);
+    if (st != OG_OK) return st;
+    *waves = 1 << (s->ctx->pair_shape & 3);
+    *order = s->ctx->pair_shape >> 2;
+    return OG_OK;
 }
 og_status og_autotune_cell_edge(const og_system_desc *d, const og_params *p, int trials, double *best_scale,
                                 double *pair_ms) {

@@ -36,6 +36,8 @@ def potential(rows):
 INTEGRATORS = {"leapfrog-presto": 0, "langevin": 1}
 CONSTRAINTS = {"none": 0, "shake": 1, "shake-settle": 2}
 THERMOSTATS = {"none": 0, "scaling": 1}
+# pair-kernel block orders (og_set_pair_shape_order / og_batch_set_pair_shape)
+ORDERS = ("system", "round-robin", "xcd", "xcd-parts")
 NONBONDS = {"lennard-jones": 0, "hydrophobicity-scale-lj": 1}     # Config.cpp:333-339
 ELECTROSTATICS = {"zero-dipole": 0, "debye-huckel": 1, "zero-quadrupole": 2, "zero-octupole": 3,
                   "zero-hexadecapole": 4}                           # Config.cpp:114-128
@@ -454,12 +456,13 @@ class System:
     def autotune(self, trials=5):
         """Time the pair kernel over its launch shapes and keep the fastest
         (results do not depend on it); returns the chosen shape and time."""
-        w, ms = C.c_int(), C.c_double()
+        w, ms, order = C.c_int(), C.c_double(), C.c_int()
         _check(lib().og_autotune(self.h, C.c_int(trials), C.byref(w), C.byref(ms)))
-        return {"waves_per_block": w.value, "pair_ms": ms.value}
+        _check(lib().og_get_pair_shape(self.h, C.byref(w), C.byref(order)))
+        return {"waves_per_block": w.value, "order": ORDERS[order.value], "pair_ms": ms.value}
 
-    def set_pair_shape(self, waves):
-        _check(lib().og_set_pair_shape(self.h, C.c_int(waves)))
+    def set_pair_shape(self, waves, order="system"):
+        _check(lib().og_set_pair_shape_order(self.h, C.c_int(waves), C.c_int(ORDERS.index(order))))
 
     def device_bytes(self):
         b = C.c_uint64()
@@ -500,12 +503,13 @@ class Batch:
         depend on it."""
         w, order, ms = C.c_int(), C.c_int(), C.c_double()
         _check(lib().og_batch_autotune(self.h, C.c_int(trials), C.byref(w), C.byref(order), C.byref(ms)))
-        return {"waves_per_block": w.value, "order": ("system", "round-robin", "xcd")[order.value],
-                "pair_ms": ms.value}
+        return {"waves_per_block": w.value, "order": ORDERS[order.value], "pair_ms": ms.value}
 
     def set_pair_shape(self, waves, order=0):
-        """order: 0 system order, 1 round-robin across systems, 2 each system on one XCD."""
-        _check(lib().og_batch_set_pair_shape(self.h, C.c_int(waves), C.c_int(int(order))))
+        """order: 0 / "system" system order, 1 / "round-robin" across systems, 2 / "xcd" each system on one
+        XCD, 3 / "xcd-parts" the list in eight contiguous parts, one per XCD."""
+        o = ORDERS.index(order) if isinstance(order, str) else int(order)
+        _check(lib().og_batch_set_pair_shape(self.h, C.c_int(waves), C.c_int(o)))
 
     def set_graph_steps(self, steps: int):
         """Steps per captured hipGraph (0: automatic)."""

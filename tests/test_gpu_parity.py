@@ -499,28 +499,30 @@ def test_fused_langevin_step_equals_split_kernels(variant):
 
 def test_pair_launch_shapes_and_autotune_keep_results():
     """The pair kernel's launch shapes (1, 2, 4 or 8 waves per block; blocks in
-    system order, round-robin across a batch or one XCD per system) give bitwise the same forces
+    system order, round-robin across a batch, one XCD per system or the list in
+    eight contiguous parts, one per XCD) give bitwise the same forces
     and energies, and autotuning in the middle of a run, alone or in a batch,
     leaves the trajectory bitwise unchanged."""
     ls = gen.build_system(1500, seed=8, n_beads=200)
     cfg = dict(BASE, **{"nsgrid-update-intvl": 5})
     say("pair launch shapes", ls.n_atoms)
     ref = None
-    for w in (1, 2, 4, 8):
+    for w, order in ((1, "system"), (2, "system"), (4, "system"), (8, "system"), (4, "xcd-parts"), (1, "xcd-parts")):
         s = System(ls, cfg)
-        s.set_pair_shape(w)
+        s.set_pair_shape(w, order)
         wk, e = s.energy_force(rebuild=True)
         s.close()
         if ref is None:
             ref = (wk, e)
-        assert np.array_equal(wk, ref[0]) and all(e[k] == ref[1][k] for k in e), w
+        assert np.array_equal(wk, ref[0]) and all(e[k] == ref[1][k] for k in e), (w, order)
     finals = []
     for tune in (False, True):
         s = System(ls, cfg)
         s.run(12)
         if tune:
             t = s.autotune(3)
-            say("  autotune alone: %d waves per block, %.3f ms" % (t["waves_per_block"], t["pair_ms"]))
+            say("  autotune alone: %d waves per block, order %s, %.3f ms" % (t["waves_per_block"], t["order"],
+                                                                            t["pair_ms"]))
             assert t["waves_per_block"] in (1, 2, 4, 8)
         s.run(13)
         finals.append(s.state()[0])
@@ -528,7 +530,7 @@ def test_pair_launch_shapes_and_autotune_keep_results():
     assert np.array_equal(finals[0], finals[1])
     lss = [gen.walker_system(n, seed=70 + k, vcmd=False) for k, n in enumerate((3000, 5000))]
     outs = []
-    for shape in ((4, 0), (2, 1), (8, 1), (1, 2), "tune"):
+    for shape in ((4, 0), (2, 1), (8, 1), (1, 2), (4, 3), "tune"):
         ss = [System(x, cfg) for x in lss]
         b = Batch()
         for x in ss:
