@@ -559,6 +559,49 @@ __global__ void __launch_bounds__(OG_THREADS) k_finish(const og_dev_sys *__restr
     og_ke_partial(s, bm.y, ke, sh);
 }
 
+#define OG_STEP_THREADS 256
+
+// End of an MD step for one system, by one block of OG_STEP_THREADS threads:
+// pair/excluded-pair/bond energies (on logged or requested steps), kinetic
+// energy from the per-block partials, the energy log entry, and the step
+// counter.  c: the system's counters as read at the start of the block.
+__device__ __forceinline__ void og_close_step(const og_dev_sys &s, const og_dev_counters &c) {
+    // the kinetic partials are read first (loads issued four strides at a
+    // time, added in stride order) so they overlap the counter reads
+    double t = 0.0;
+    for (int k = threadIdx.x; k < s.n_ke_parts; k += 4 * OG_STEP_THREADS) {
+        double v[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = k + j * OG_STEP_THREADS < s.n_ke_parts ? s.e_atom[k + j * OG_STEP_THREADS] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (k + j * OG_STEP_THREADS < s.n_ke_parts) t += v[j];
+    }
+    if (og_frozen_c(c)) return;
+    __shared__ double    sh[OG_STEP_THREADS / 64];
+    __shared__ long long she[16 * OG_EACC];
+    if (og_energy_step_c(s, c)) og_take_pair_energies(s, she);
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double ke = 0.0;
+        for (int w = 0; w < OG_STEP_THREADS / 64; w++) ke += sh[w];
+        s.energy[OG_E_KIN]  = ke * OG_KINETIC_COEFF;
+        const long long stp = c.step;
+        if (s.log_cap > 0 && (stp % s.log_stride) == 0) {
+            // all terms loaded before the first store: a term-by-term copy
+            // would wait one memory round trip per term (the row may alias)
+            double v[OG_E_NTERMS];
+            for (int k = 0; k < OG_E_NTERMS; k++) v[k] = s.energy[k];
+            double *e = s.elog + (size_t)((stp / s.log_stride) % s.log_cap) * OG_E_NTERMS;
+            for (int k = 0; k < OG_E_NTERMS; k++) e[k] = v[k];
+        }
+        s.cnt->rebuild_now = 0;
+        s.cnt->step        = stp + 1;
+    }
+}
+
 // ------------------------------------------------- fused Langevin step --
 // One launch for the whole integration step of a Langevin system whose
 // SETTLE waters interact with nothing but their own atoms through exclusions
@@ -706,44 +749,14 @@ __global__ void __launch_bounds__(OG_WATER_THREADS) k_langevin_step(const og_dev
     }
 }
 
-#define OG_STEP_THREADS 256
-
-// End of an MD step, one block per system: pair/excluded-pair/bond energies
-// (on logged or requested steps), kinetic energy from the per-block partials,
-// the energy log entry, and the step counter.
+// One block per system, the step's last kernel.
 __global__ void __launch_bounds__(OG_STEP_THREADS) k_step_close(const og_dev_sys *__restrict__ S,
                                                                const int2 *__restrict__ map) {
     const og_dev_sys &s = S[map[blockIdx.x].x];
-    // the kinetic partials are read first (loads issued four strides at a
-    // time, added in stride order) so they overlap the counter reads
-    double t = 0.0;
-    for (int k = threadIdx.x; k < s.n_ke_parts; k += 4 * OG_STEP_THREADS) {
-        double v[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) v[j] = k + j * OG_STEP_THREADS < s.n_ke_parts ? s.e_atom[k + j * OG_STEP_THREADS] : 0.0;
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (k + j * OG_STEP_THREADS < s.n_ke_parts) t += v[j];
-    }
-    if (og_frozen(s)) return;
-    __shared__ double    sh[OG_STEP_THREADS / 64];
-    __shared__ long long she[16 * OG_EACC];
-    if (og_energy_step(s)) og_take_pair_energies(s, she);
-    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double ke = 0.0;
-        for (int w = 0; w < OG_STEP_THREADS / 64; w++) ke += sh[w];
-        s.energy[OG_E_KIN]  = ke * OG_KINETIC_COEFF;
-        const long long stp = s.cnt->step;
-        if (s.log_cap > 0 && (stp % s.log_stride) == 0) {
-            double *e = s.elog + (size_t)((stp / s.log_stride) % s.log_cap) * OG_E_NTERMS;
-            for (int k = 0; k < OG_E_NTERMS; k++) e[k] = s.energy[k];
-        }
-        s.cnt->rebuild_now = 0;
-        s.cnt->step        = stp + 1;
-    }
+    // one batch of loads for the counters this kernel reads (a single block
+    // runs per system, so every dependent round trip adds to the step)
+    const og_dev_counters c = *s.cnt;
+    og_close_step(s, c);
 }
 
 // ----------------------------------------------------------------- VcMD --

@@ -256,9 +256,10 @@ enum og_map_kind {
 // before its first write; step only changes in the last kernel of a step, so
 // all kernels of one step agree); a fault never spreads to the other systems
 // of a batch and never integrates on without forces.
-__device__ __forceinline__ bool og_frozen(const og_dev_sys &s) {
-    return s.cnt->step >= s.cnt->step_end || (s.cnt->flags & OG_F_FATAL) != 0;
+__device__ __forceinline__ bool og_frozen_c(const og_dev_counters &c) {
+    return c.step >= c.step_end || (c.flags & OG_F_FATAL) != 0;
 }
+__device__ __forceinline__ bool og_frozen(const og_dev_sys &s) { return og_frozen_c(*s.cnt); }
 // The grid-rebuild kernels run for a system when its own step is a multiple
 // of its rebuild interval (DynamicsMode.cpp:480-486) or a rebuild is forced;
 // systems at other phases in the same launch skip them.
@@ -267,10 +268,11 @@ __device__ __forceinline__ bool og_rebuild_skip(const og_dev_sys &s) {
 }
 // Energies (pair, virial) are evaluated on logged steps and on request; the
 // forces and the pair count on every step.
-__device__ __forceinline__ bool og_energy_step(const og_dev_sys &s) {
+__device__ __forceinline__ bool og_energy_step_c(const og_dev_sys &s, const og_dev_counters &c) {
     // V-McMD needs the potential energy of every step (its lambda)
-    return s.cnt->force_energy || (s.cnt->step % s.log_stride) == 0 || s.mc_mode == OG_MC_VMCMD;
+    return c.force_energy || (c.step % s.log_stride) == 0 || s.mc_mode == OG_MC_VMCMD;
 }
+__device__ __forceinline__ bool og_energy_step(const og_dev_sys &s) { return og_energy_step_c(s, *s.cnt); }
 
 __device__ __forceinline__ int og_lane() { return __lane_id(); }
 
