@@ -20,16 +20,9 @@ constexpr double kPiRef       = 3.14159265; // CelesteObject.cpp:26 (the referen
 constexpr double kEps         = 1e-10;      // CelesteObject.cpp:31
 constexpr double kEps3        = 1e-30;      // CelesteObject.cpp:31
 
-// PBC::diff_crd_minim_image, double overload (PBC.cpp:62-82), with PBC's float box lengths
+// minimum-image difference xa - xb (og_mimg per axis)
 __device__ __forceinline__ void mimg3(const og_dev_sys &s, const double *xa, const double *xb, double *d) {
-    for (int k = 0; k < 3; k++) {
-        const float Lh  = s.L[k] * 0.5f;
-        const float Lhi = 1.0f / Lh;
-        double      v   = xa[k] - xb[k];
-        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-        d[k]            = v;
-    }
+    for (int k = 0; k < 3; k++) d[k] = og_mimg(xa[k] - xb[k], s.L[k]);
 }
 
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {

@@ -195,6 +195,8 @@ template <class T> hipError_t dalloc(T **p, size_t n) {
 }
 
 int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
+// pair-kernel work units at the cell capacity (cells plus the split cells' second halves)
+int pair_units(const og_dev_sys &h) { return h.cap_cells + og_pair_split_cells(h.pair_split, h.cap_cells); }
 
 // MmSystem::revise_crd_inbox (MmSystem.cpp:808-816): coordinates into [lb, ub)
 og_status wrap_inbox(const GridGeom &g, int n, double *crd) {
@@ -354,24 +356,24 @@ og_status exec_prepare(og_exec &x) {
         for (int k = 0; k < KIND_PAIR; k++)
             for (int b = 0; b < cnt[k]; b++) maps[k].push_back(make_int2(si, b));
         for (int lw = 0; lw < 4; lw++) {  // pair kernel, blocks in system order
-            const int nb = cdiv(d.cap_cells, 1 << lw);
+            const int nb = cdiv(pair_units(d), 1 << lw);
             for (int b = 0; b < nb; b++) maps[KIND_PAIR + lw].push_back(make_int2(si, b));
         }
     }
     // pair kernel, blocks round-robin across the systems of a batch
     for (int lw = 0; lw < 4; lw++) {
         int most = 0;
-        for (int si = 0; si < ns; si++) most = std::max(most, cdiv(x.members[si]->h.cap_cells, 1 << lw));
+        for (int si = 0; si < ns; si++) most = std::max(most, cdiv(pair_units(x.members[si]->h), 1 << lw));
         for (int b = 0; b < most; b++)
             for (int si = 0; si < ns; si++)
-                if (b < cdiv(x.members[si]->h.cap_cells, 1 << lw)) maps[KIND_PAIR + 4 + lw].push_back(make_int2(si, b));
+                if (b < cdiv(pair_units(x.members[si]->h), 1 << lw)) maps[KIND_PAIR + 4 + lw].push_back(make_int2(si, b));
     }
     // pair kernel, each system's blocks on one XCD (launch positions p with p mod 8 fixed), so a system's
     // coordinates and lists stay in that XCD's L2; systems go to the XCD with the fewest blocks so far, largest
     // first, and once an XCD's list runs out its positions take blocks of the longest remaining list
     for (int lw = 0; lw < 4; lw++) {
         std::vector<int> nb(ns), order(ns);
-        for (int si = 0; si < ns; si++) nb[si] = cdiv(x.members[si]->h.cap_cells, 1 << lw), order[si] = si;
+        for (int si = 0; si < ns; si++) nb[si] = cdiv(pair_units(x.members[si]->h), 1 << lw), order[si] = si;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return nb[a] > nb[b]; });
         std::vector<std::vector<int2>> lists(OG_N_XCD);
         std::vector<long long>         load(OG_N_XCD, 0);
@@ -672,6 +674,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
     h.nx = g.nx, h.ny = g.ny, h.nnx = g.nnx, h.nny = g.nny, h.n_uni_z = g.n_uni_z;
     h.cap_atom_array = (n + g.n_columns * OG_NAC + 31) / 32 * 32;  // whole groups of four cells (og_fx_slot)
     h.cap_cells      = h.cap_atom_array / OG_NAC;
+    h.pair_split     = OG_PAIR_SPLIT;
     h.cap_cell_pairs   = pair_capacity(g, n, h.cap_cells, p->capacity_factor);
     h.n_types          = d->n_lj_types;
     h.n_excess         = d->n_excess;

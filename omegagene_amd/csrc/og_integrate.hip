@@ -35,15 +35,6 @@ __device__ __forceinline__ double2 og_gauss2(uint32_t a, uint32_t b) {
     return make_double2(r * cs, r * sn);
 }
 
-// PBC::diff_crd_minim_image, double overload (PBC.cpp:62-82)
-__device__ __forceinline__ double og_mimg(double v, float L) {
-    const float Lh  = L * 0.5f;
-    const float Lhi = 1.0f / Lh;
-    v               = v - (double)(L * (float)(int)(v * (double)Lhi));
-    v               = v - (double)(L * (float)(int)(v * (double)Lhi));
-    return v;
-}
-
 // --------------------------------------------------- leapfrog velocities --
 // SubBox::cpy_vel_prev + update_velocities (SubBox.cpp:1427-1454)
 __global__ void __launch_bounds__(OG_THREADS) k_leapfrog_vel(const og_dev_sys *__restrict__ S,

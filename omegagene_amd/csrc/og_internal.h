@@ -88,6 +88,7 @@ struct og_dev_sys {
     int n_quads;           // SHAKE four-atom groups
     int n_free;            // atoms outside SETTLE waters (fused Langevin step)
     int n_water_blocks;    // fused Langevin step: blocks of OG_WATERS_PER_BLOCK waters; 0 = not fusable
+    int pair_split;        // pair kernel: at most this many of the last cells run as two half-cell waves each
     int n_com_groups;
     int integrator;        // 0 leapfrog-presto, 1 langevin
     int constraint;        // 0 none, 1 shake, 2 shake+settle
@@ -244,6 +245,15 @@ enum og_map_kind {
 #define OG_THREADS 256
 #define OG_TRIO_THREADS 128
 #define OG_QUAD_THREADS 64
+// Pair kernel work units: in a system of at least 2 * pair_split cells the
+// last pair_split cells are evaluated as two waves each (their cell-pair
+// records halved), so a launch that takes several rounds of waves ends on
+// short waves instead of a tail of whole cells.  Smaller systems (a batch of
+// walkers, where the launch holds many systems) keep one wave per cell.
+#define OG_PAIR_SPLIT 4096
+__host__ __device__ __forceinline__ int og_pair_split_cells(int split, int n_cells) {
+    return n_cells >= 2 * split ? split : 0;
+}
 #define OG_WATER_THREADS 256  // fused Langevin step: threads per block
 #define OG_WATERS_PER_WAVE 21  // fused Langevin step: three lanes per water
 #define OG_WATERS_PER_BLOCK (OG_WATERS_PER_WAVE * (OG_WATER_THREADS / 64))
@@ -273,6 +283,33 @@ __device__ __forceinline__ bool og_energy_step_c(const og_dev_sys &s, const og_d
     return c.force_energy || (c.step % s.log_stride) == 0 || s.mc_mode == OG_MC_VMCMD;
 }
 __device__ __forceinline__ bool og_energy_step(const og_dev_sys &s) { return og_energy_step_c(s, *s.cnt); }
+
+// PBC::diff_crd_minim_image (PBC.cpp:54-80) with PBC's float box lengths:
+// two truncating passes d - L (int)(d / (L/2)).  When d lies within rounding
+// of +-L (two atoms close along an axis, one just inside each face of the
+// box), d / (L/2) rounds to +-2 exactly and the two passes swap -L and +L,
+// leaving |d| ~ L.  A third, nearest-image step, taken only when |d| > 3L/4
+// after the two passes (never for any other input), returns the short
+// vector there.  Double differences (integration, bonds) and float ones
+// (excluded pairs).
+__device__ __forceinline__ double og_mimg(double v, float L) {
+#pragma clang fp contract(off)
+    const float Lh  = L * 0.5f;
+    const float Lhi = 1.0f / Lh;
+    v               = v - (double)(L * (float)(int)(v * (double)Lhi));
+    v               = v - (double)(L * (float)(int)(v * (double)Lhi));
+    if (fabs(v) > 0.75 * (double)L) v -= (double)L * nearbyint(v / (double)L);
+    return v;
+}
+__device__ __forceinline__ float og_mimg_f(float v, float L) {
+#pragma clang fp contract(off)
+    const float Lh  = L * 0.5f;
+    const float Lhi = 1.0f / Lh;
+    v               = v - L * (float)(int)(v * Lhi);
+    v               = v - L * (float)(int)(v * Lhi);
+    if (fabsf(v) > 0.75f * L) v -= L * rintf(v / L);
+    return v;
+}
 
 __device__ __forceinline__ int og_lane() { return __lane_id(); }
 

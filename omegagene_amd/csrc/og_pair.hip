@@ -6,7 +6,8 @@
 // same pairs as the reference -- every unmasked atom pair of every listed
 // cell pair closer than the cutoff -- but schedules them for a wave64:
 //
-// * one wave per cell1; its 8 atoms ("i") sit in scalar registers;
+// * one wave per cell1 (the last cells of a large system: two waves, each
+//   with half of the cell's records); its 8 atoms ("i") sit in scalar registers;
 // * filter: the wave reads its cell pairs 8 at a time (lane = 8*pair + j) and
 //   keeps the cell2 atoms ("rows") within the cutoff of the cell1 bounding
 //   box, appending them (shifted to the pair's periodic image) to a per-wave
@@ -250,11 +251,13 @@ __device__ __forceinline__ float og_keep_add(float a, float b, bool upper, int o
     return keep + __shfl_xor(send, off, 64);
 }
 
-// One cell1 (one wave).  ENERGY = false on steps whose energies are not
-// logged: forces and the pair count only.  LJ_LDS: the cell1's LJ rows are in
+// One cell1 (one wave), its cell-pair records [p0, p1) (all of them, or
+// half of them for a split cell).  ENERGY = false on steps whose energies
+// are not logged: forces and the pair count only.  LJ_LDS: the cell1's LJ rows are in
 // the block's LDS copy ljl of the table's leading rows.
 template <int FORM, bool ENERGY, bool LJ_LDS>
-__device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, const og_f4 *__restrict__ shift,
+__device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, const int p0, const int p1,
+                                             const og_f4 *__restrict__ shift,
                                              const typename og_ljt<FORM>::type *__restrict__ ljl, og_f4 *qp,
                                              int2 *qa) {
     const int lane = threadIdx.x & 63;
@@ -270,8 +273,6 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     OG_GLOBAL(long long) *__restrict__ FX       = (OG_GLOBAL(long long) *)s.fx;
     const int   cap = s.cap_atom_array;
     const int   nt  = s.n_types;
-    const int   p0  = __builtin_amdgcn_readfirstlane(s.cp_head[c1]);
-    const int   p1  = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
     // keep-test radius: the cutoff plus 10 ppm, so float rounding of the box
     // distance can never drop a row that holds an in-range pair
     const float rcb2 = s.cutoff2 * 1.00001f;
@@ -403,7 +404,8 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     }
 }
 
-// WAVES waves (one cell each) per block sharing the image table and an LDS
+// WAVES waves (one work unit each: a cell, or half of one of the last
+// cells) per block sharing the image table and an LDS
 // copy of the leading rows of the LJ table (as many whole rows as fit in
 // OG_LJ_LDS_ENTRIES; the engine numbers types by frequency, so for water the
 // oxygen and hydrogen rows against every type).  A cell1 whose types are all
@@ -432,16 +434,32 @@ __global__ void __launch_bounds__(64 * WAVES) k_pair(const og_dev_sys *__restric
     const ljt *tab = (const ljt *)(FORM == 2 ? (const void *)s.lj_hps : (const void *)s.lj_pair);
     for (int k = threadIdx.x; k < rows * nt; k += 64 * WAVES) ljl[k] = tab[k];
     __syncthreads();
+    // work unit u of the system: cells 0 .. n-m-1 whole, then the last m
+    // cells as two units each (first and second half of their records)
     const int w  = threadIdx.x >> 6;
-    const int c1 = __builtin_amdgcn_readfirstlane(bm.y * WAVES + w);
-    if (c1 >= s.cnt->n_cells) return;
+    const int n  = s.cnt->n_cells;
+    const int m  = og_pair_split_cells(s.pair_split, n);
+    const int u  = __builtin_amdgcn_readfirstlane(bm.y * WAVES + w);
+    if (u >= n + m) return;
+    int c1 = u, half = -1;
+    if (u >= n - m) {
+        c1   = n - m + ((u - (n - m)) >> 1);
+        half = (u - (n - m)) & 1;
+    }
+    int p0 = __builtin_amdgcn_readfirstlane(s.cp_head[c1]);
+    int p1 = __builtin_amdgcn_readfirstlane(s.cp_head[c1 + 1]);
+    if (half >= 0) {
+        const int pm = p0 + ((p1 - p0 + 1) >> 1);
+        if (half == 0) p1 = pm;
+        else p0 = pm;
+    }
     const bool lds = __builtin_amdgcn_readfirstlane(__all(s.gtype[c1 * OG_NAC + (threadIdx.x & 7)] < rows)) != 0;
     if (og_energy_step(s)) {
-        if (lds) og_pair_cell<FORM, true, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
-        else og_pair_cell<FORM, true, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        if (lds) og_pair_cell<FORM, true, true>(s, c1, p0, p1, shift, ljl, qpos[w], qaux[w]);
+        else og_pair_cell<FORM, true, false>(s, c1, p0, p1, shift, ljl, qpos[w], qaux[w]);
     } else {
-        if (lds) og_pair_cell<FORM, false, true>(s, c1, shift, ljl, qpos[w], qaux[w]);
-        else og_pair_cell<FORM, false, false>(s, c1, shift, ljl, qpos[w], qaux[w]);
+        if (lds) og_pair_cell<FORM, false, true>(s, c1, p0, p1, shift, ljl, qpos[w], qaux[w]);
+        else og_pair_cell<FORM, false, false>(s, c1, p0, p1, shift, ljl, qpos[w], qaux[w]);
     }
 }
 

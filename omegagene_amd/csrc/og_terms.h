@@ -80,14 +80,7 @@ __device__ __forceinline__ float og_excess_pair(const og_dev_sys &s, const doubl
                                                 float qb, float *w) {
 #pragma clang fp contract(off)
     float d[3];
-    for (int k = 0; k < 3; k++) {
-        const float Lh  = s.L[k] * 0.5f;
-        const float Lhi = 1.0f / Lh;
-        float       v   = (float)xa[k] - (float)xb[k];
-        v               = v - s.L[k] * (float)(int)(v * Lhi);
-        v               = v - s.L[k] * (float)(int)(v * Lhi);
-        d[k]            = v;
-    }
+    for (int k = 0; k < 3; k++) d[k] = og_mimg_f((float)xa[k] - (float)xb[k], s.L[k]);
     const float r2    = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
     const float r     = sqrtf(r2);
     const float rinv  = 1.0f / r;
@@ -117,14 +110,7 @@ __device__ __forceinline__ double og_bond_pair(const og_dev_sys &s, double bk, d
                                                const double *xb, double *w) {
 #pragma clang fp contract(off)
     double d[3];
-    for (int k = 0; k < 3; k++) {
-        const float Lh  = s.L[k] * 0.5f;
-        const float Lhi = 1.0f / Lh;
-        double      v   = xa[k] - xb[k];
-        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-        v               = v - (double)(s.L[k] * (float)(int)(v * (double)Lhi));
-        d[k]            = v;
-    }
+    for (int k = 0; k < 3; k++) d[k] = og_mimg(xa[k] - xb[k], s.L[k]);
     const double r    = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     const double dif  = r - r0;
     const double coef = bk * dif;

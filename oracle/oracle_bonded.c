@@ -24,6 +24,8 @@ static void mimg(double d[3], const double *x1, const double *x2, const float L[
         d[i]             = x1[i] - x2[i];
         d[i]             = d[i] - (double)L[i] * (int)(d[i] * (double)L_half_inv);
         d[i]             = d[i] - (double)L[i] * (int)(d[i] * (double)L_half_inv);
+        /* within rounding of +-L the two passes swap -L and +L: one nearest-image step more */
+        if (fabs(d[i]) > 0.75 * (double)L[i]) d[i] -= (double)L[i] * nearbyint(d[i] / (double)L[i]);
     }
 }
 

@@ -36,6 +36,8 @@ static void mimg(double d[3], const double *x1, const double *x2, const float L[
         d[i]             = x1[i] - x2[i];
         d[i]             = d[i] - (double)(L[i] * (float)(int)(d[i] * (double)L_half_inv));
         d[i]             = d[i] - (double)(L[i] * (float)(int)(d[i] * (double)L_half_inv));
+        /* within rounding of +-L the two passes swap -L and +L: one nearest-image step more */
+        if (fabs(d[i]) > 0.75 * (double)L[i]) d[i] -= (double)L[i] * nearbyint(d[i] / (double)L[i]);
     }
 }
 
@@ -354,6 +356,7 @@ static double group_diff(const float L[3], int na, const int *atoms_a, int nb, c
         /* PBC::diff_crd_minim_image, double overload (PBC.cpp:62-82) */
         v       = v - L[d] * (float)(int)(v * Lhi);
         v       = v - L[d] * (float)(int)(v * Lhi);
+        if (fabs(v) > 0.75 * (double)L[d]) v -= (double)L[d] * nearbyint(v / (double)L[d]);
         diff[d] = v;
     }
     return sqrt(diff[0] * diff[0] + diff[1] * diff[1] + diff[2] * diff[2]);

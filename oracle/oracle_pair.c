@@ -460,6 +460,8 @@ static void minim_image_f(float d[3], const float *x1, const float *x2, const fl
         d[i]             = x1[i] - x2[i];
         d[i]             = d[i] - L[i] * (float)(int)(d[i] * L_half_inv);
         d[i]             = d[i] - L[i] * (float)(int)(d[i] * L_half_inv);
+        /* within rounding of +-L the two passes swap -L and +L: one nearest-image step more */
+        if (fabsf(d[i]) > 0.75f * L[i]) d[i] -= L[i] * rintf(d[i] / L[i]);
     }
 }
 /* double overload (PBC.cpp:62-82) */
@@ -470,6 +472,8 @@ static void minim_image_d(double d[3], const double *x1, const double *x2, const
         d[i]             = x1[i] - x2[i];
         d[i]             = d[i] - (double)L[i] * (int)(d[i] * (double)L_half_inv);
         d[i]             = d[i] - (double)L[i] * (int)(d[i] * (double)L_half_inv);
+        /* within rounding of +-L the two passes swap -L and +L: one nearest-image step more */
+        if (fabs(d[i]) > 0.75 * (double)L[i]) d[i] -= (double)L[i] * nearbyint(d[i] / (double)L[i]);
     }
 }
 

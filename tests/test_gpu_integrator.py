@@ -176,6 +176,57 @@ def test_settle_step_vs_oracle():
     assert dx <= 1e-12 and dv <= 1e-12
 
 
+def _water_across_the_face(ls, dt=2.0, gap=1e-8):
+    """Water 0 of ls made non-interacting (zero charges, a new LJ type with
+    zero coefficients, so its gradient is exactly zero) and placed rigid with
+    its O 0.01 A below the top y face and H1 0.02 A above the bottom one (0.03
+    A apart in y across the face); H1 moves down at a speed that puts its
+    unconstrained position gap - L from O's in y after one leapfrog step --
+    within float rounding of -L, where the reference's two-pass minimum image
+    returns a vector one box long."""
+    L = float(ls.box[4])
+    th = np.radians(104.52)
+    dOH = gen.D_OH
+    o, h1, h2 = ls.settle_ids[0]
+    O = np.array([0.4 * L, L - 0.01, 0.6 * L])
+    u1 = np.array([np.sqrt(dOH ** 2 - 0.03 ** 2), 0.03, 0.0]) / dOH
+    w = np.array([O, O + dOH * u1, O + dOH * (np.cos(th) * u1 + np.sin(th) * np.array([0.0, 0.0, 1.0]))])
+    w[1, 1] -= L
+    ls.crd[[o, h1, h2]] = w
+    ls.vel[[o, h1, h2]] = 0.0
+    ls.vel[h1, 1] = -(0.03 - gap) / dt
+    nt = ls.n_lj_types
+    ls.lj6 = np.pad(ls.lj6, ((0, 1), (0, 1)))
+    ls.lj12 = np.pad(ls.lj12, ((0, 1), (0, 1)))
+    ls.n_lj_types = nt + 1
+    ls.atom_type[[o, h1, h2]] = nt
+    ls.charge[[o, h1, h2]] = 0.0
+    return (o, h1, h2)
+
+
+def _rigid_error(x, ids, L):
+    d = lambda a, b: np.linalg.norm((x[a] - x[b]) - L * np.round((x[a] - x[b]) / L))
+    o, h1, h2 = ids
+    return max(abs(d(o, h1) - gen.D_OH), abs(d(o, h2) - gen.D_OH), abs(d(h1, h2) - gen.D_HH))
+
+
+def test_settle_across_the_box_face():
+    """A water whose unconstrained H1 - O difference in y lands within float
+    rounding of -L (H1 just below the bottom face, O just below the top one):
+    the device SETTLE keeps it rigid with no constraint failure, and the step
+    equals the oracle's (both take one nearest-image step more where the
+    reference's two-pass minimum image returns a vector one box long)."""
+    ls = _settle(gen.build_system(600, seed=48, L=30.0))
+    ids = _water_across_the_face(ls)
+    L = float(ls.box[0])
+    x, v, x1, v1 = _one_leapfrog_step(ls, dict(BASE, constraint="shake-settle"))
+    dx, dv = np.max(np.abs(x - x1)), np.max(np.abs(v - v1))
+    say("water across the y face: leapfrog step vs oracle max|dx| %.2e max|dv| %.2e; rigid error %.2e; "
+        "O y %.6f H1 y %.6f" % (dx, dv, _rigid_error(x1, ids, L), x1[ids[0], 1], x1[ids[1], 1]))
+    assert dx <= 1e-12 and dv <= 1e-12
+    assert _rigid_error(x1, ids, L) < 1e-10
+
+
 def test_com_cancel_step_vs_oracle():
     """--com-motion cancel on two groups (COMMotion.cpp:30-55) with SHAKE trios:
     one device step against the oracle; each group's momentum is zero after."""

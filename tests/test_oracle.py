@@ -408,6 +408,62 @@ def test_settle_keeps_each_atom_in_its_periodic_frame():
     assert abs(d(0, 1) - dOH) < 1e-12 and abs(d(0, 2) - dOH) < 1e-12 and abs(d(1, 2) - dHH) < 1e-12
 
 
+def ref_two_pass_image(v, Lf):
+    """PBC::diff_crd_minim_image (PBC.cpp:62-82) as written: two truncating
+    passes with float L/2 and its float reciprocal."""
+    L32 = np.float32(Lf)
+    Lhi = np.float32(np.float32(1.0) / np.float32(L32 * np.float32(0.5)))
+    for _ in range(2):
+        v = v - float(L32 * np.float32(int(v * float(Lhi))))
+    return v
+
+
+def straddling_water(Lf, gap, dOH, dHH):
+    """A rigid water in the xz plane (so O, H1, H2 share y up to gap) with O
+    just below the top y face and H1 wrapped just above the bottom one: the
+    H1 - O difference in y is gap - L."""
+    th = np.radians(104.52)
+    O = np.array([0.4 * Lf, Lf - gap / 2, 0.6 * Lf])
+    w = np.array([O, O + [dOH, gap, 0.0], O + [dOH * np.cos(th), 0.0, dOH * np.sin(th)]])
+    w[1, 1] -= Lf
+    assert 0.0 < w[1, 1] < Lf and 0.0 < w[0, 1] < Lf
+    return w
+
+
+def test_minimum_image_at_one_box_length_and_settle_across_the_face():
+    """Two atoms a hair apart in y across the box face (difference within
+    float rounding of -L): the reference's two-pass minimum image swaps -L and
+    +L and returns a vector one box long; the oracle (like the device) takes
+    one nearest-image step more there, so SETTLE with such a water as its
+    reference geometry keeps it rigid and gives the positions of the same water
+    moved off the face, and the excluded-pair and bond terms see the short
+    vector.  Any other difference keeps the reference's result."""
+    Lf, gap = 30.0, 2e-8
+    dOH, dHH = gen.D_OH, gen.D_HH
+    prev = straddling_water(Lf, gap, dOH, dHH)
+    v = prev[1, 1] - prev[0, 1]
+    assert abs(ref_two_pass_image(v, Lf)) > 0.75 * Lf   # the input does hit the reference's cycle
+    for u in (v + 1e-3, 7.3, -14.9, 15.1, -44.0):     # elsewhere the reference's passes already give |d| <= L/2
+        assert abs(ref_two_pass_image(u, Lf)) <= 0.5 * Lf + 1e-6
+    L = np.array([Lf] * 3, np.float32)
+    mass = np.array([15.9994, 1.008, 1.008], np.float32)
+    ids = np.array([[0, 1, 2]], np.int32)
+    rng = np.random.default_rng(5)
+    step = rng.normal(scale=0.01, size=(3, 3))
+    new = oracle.settle(L, ids, dOH, dHH, prev + step, prev, mass)
+    d = lambda x, a, b: np.linalg.norm((x[a] - x[b]) - Lf * np.round((x[a] - x[b]) / Lf))
+    assert abs(d(new, 0, 1) - dOH) < 1e-12 and abs(d(new, 0, 2) - dOH) < 1e-12 and abs(d(new, 1, 2) - dHH) < 1e-12
+    # the same water half a box lower (no atom near a face): the same correction
+    sh = np.array([0.0, Lf / 2, 0.0])
+    moved = oracle.settle(L, ids, dOH, dHH, prev - sh + step, prev - sh, mass)
+    diff = (new - (moved + sh)) - Lf * np.round((new - (moved + sh)) / Lf)
+    assert np.max(np.abs(diff)) < 1e-9
+    # bond and excluded-pair terms across the face: the short vector
+    w = np.zeros((3, 3))
+    eb = oracle.bonds(L, np.array([[0, 1]], np.int32), np.array([300.0]), np.array([dOH]), prev, w)
+    assert eb < 1e-9 and np.max(np.abs(w)) < 1e-6
+
+
 # ------------------------------------------------ reference runs of the step loop
 # Each fixture is a run of the reference all-pairs build (oracle/_ref/omegagene_wons,
 # compiled from the reference sources by oracle/Makefile; tests/golden/make_wons_fixtures.py):
