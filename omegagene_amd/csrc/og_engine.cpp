@@ -195,8 +195,9 @@ template <class T> hipError_t dalloc(T **p, size_t n) {
 }
 
 int cdiv(long long a, long long b) { return (int)((a + b - 1) / b); }
-// pair-kernel work units at the cell capacity (cells plus the split cells' second halves)
-int pair_units(const og_dev_sys &h) { return h.cap_cells + og_pair_split_cells(h.pair_split, h.cap_cells); }
+// pair-kernel work units: cells plus the split cells' second halves, at most (the split depends on the
+// system's cell count, which the capacity bounds)
+int pair_units(const og_dev_sys &h) { return h.cap_cells + (h.cap_cells >= 2 * h.pair_split ? h.pair_split : 0); }
 
 // MmSystem::revise_crd_inbox (MmSystem.cpp:808-816): coordinates into [lb, ub)
 og_status wrap_inbox(const GridGeom &g, int n, double *crd) {

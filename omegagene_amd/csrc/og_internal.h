@@ -245,14 +245,15 @@ enum og_map_kind {
 #define OG_THREADS 256
 #define OG_TRIO_THREADS 128
 #define OG_QUAD_THREADS 64
-// Pair kernel work units: in a system of at least 2 * pair_split cells the
-// last pair_split cells are evaluated as two waves each (their cell-pair
-// records halved), so a launch that takes several rounds of waves ends on
+// Pair kernel work units: in a system of 2 * pair_split to 8 * pair_split
+// cells the last pair_split cells are evaluated as two waves each (their
+// cell-pair records halved), so a launch of a few rounds of waves ends on
 // short waves instead of a tail of whole cells.  Smaller systems (a batch of
-// walkers, where the launch holds many systems) keep one wave per cell.
+// walkers, where the launch holds many systems) and larger ones (many rounds,
+// a short tail) keep one wave per cell.
 #define OG_PAIR_SPLIT 4096
 __host__ __device__ __forceinline__ int og_pair_split_cells(int split, int n_cells) {
-    return n_cells >= 2 * split ? split : 0;
+    return n_cells >= 2 * split && n_cells < 8 * split ? split : 0;
 }
 #define OG_WATER_THREADS 256  // fused Langevin step: threads per block
 #define OG_WATERS_PER_WAVE 21  // fused Langevin step: three lanes per water
