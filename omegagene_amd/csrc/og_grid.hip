@@ -293,6 +293,22 @@ __device__ __forceinline__ bool og_valid_pair(int c1, int c2) {
     return !((c2 < c1 && o2) || (c2 > c1 && !o2));
 }
 
+// Squared z gap between cell2 [z2b, z2t] (image-shifted) and cell1 [z1b, z1t]
+// (MiniCell.cpp:1095-1104)
+__device__ __forceinline__ float og_zgap2(float z2t, float z2b, float z1b, float z1t) {
+    float d2;
+    if (z2t < z1b) {
+        d2 = z1b - z2t;
+        d2 = d2 * d2;
+    } else if (z2b > z1t) {
+        d2 = z2b - z1t;
+        d2 = d2 * d2;
+    } else {
+        d2 = 0.0f;
+    }
+    return d2;
+}
+
 // MODE 1: count the cell's pairs and stage them (cell2 | image << 24, list
 // order) in cp_stage[c1 * OG_ENUM_STAGE ...] as far as they fit;
 // MODE 2: write them at cp_head[c1] (cells whose pairs did not fit).
@@ -333,6 +349,7 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
         const int k     = r0 + lane;
         int       mine  = 0;
         int       c2lo[3], c2hi[3], img2 = 0;
+        uint64_t  kept[3] = {0ull, 0ull, 0ull};
         float     dd01  = 0.0f;
         int       imgxy = 0;
         for (int ii = 0; ii < 3; ii++) c2lo[ii] = 0, c2hi[ii] = -1;
@@ -396,21 +413,12 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
             for (int ii = 0; ii < 3; ii++) {
                 const float sh = (float)(ii - 1) * s.L[2];
                 for (int c2 = c2lo[ii]; c2 <= c2hi[ii]; c2++) {
-                    const float z2t = s.cell_zmax[c2] + sh;
-                    const float z2b = s.cell_zmin[c2] + sh;
-                    float       d2;
-                    if (z2t < z1b) {
-                        d2 = z1b - z2t;
-                        d2 = d2 * d2;
-                    } else if (z2b > z1t) {
-                        d2 = z2b - z1t;
-                        d2 = d2 * d2;
-                    } else {
-                        d2 = 0.0f;
-                    }
-                    if (dd01 + d2 > s.cutoff_pair2) continue;
+                    if (dd01 + og_zgap2(s.cell_zmax[c2] + sh, s.cell_zmin[c2] + sh, z1b, z1t) > s.cutoff_pair2) continue;
                     if (!og_valid_pair(c1, c2)) continue;
                     mine++;
+                    // the kept cells of this image as bits from c2lo (the write below
+                    // walks them instead of testing every cell again)
+                    if (c2 - c2lo[ii] < 64) kept[ii] |= 1ull << (c2 - c2lo[ii]);
                 }
             }
         }
@@ -426,21 +434,7 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
             for (int ii = 0; ii < 3; ii++) {
                 const float sh = (float)(ii - 1) * s.L[2];
                 img2           = ii == 0 ? 16 : ii == 2 ? 32 : 0;
-                for (int c2 = c2lo[ii]; c2 <= c2hi[ii]; c2++) {
-                    const float z2t = s.cell_zmax[c2] + sh;
-                    const float z2b = s.cell_zmin[c2] + sh;
-                    float       d2;
-                    if (z2t < z1b) {
-                        d2 = z1b - z2t;
-                        d2 = d2 * d2;
-                    } else if (z2b > z1t) {
-                        d2 = z2b - z1t;
-                        d2 = d2 * d2;
-                    } else {
-                        d2 = 0.0f;
-                    }
-                    if (dd01 + d2 > s.cutoff_pair2) continue;
-                    if (!og_valid_pair(c1, c2)) continue;
+                auto emit = [&](int c2) {
                     const int rec = c2 | ((imgxy | img2) << 24);
                     if (MODE == 1) {
                         if (p < OG_ENUM_STAGE) s.cp_stage[(size_t)c1 * OG_ENUM_STAGE + p] = rec;
@@ -448,6 +442,17 @@ __device__ void og_enum_cell(const og_dev_sys &s, int c1, int *out_count) {
                         s.cp_c2[p] = rec;
                     }
                     p++;
+                };
+                if (c2hi[ii] - c2lo[ii] < 64) {
+                    // ascending cell order, as the count
+                    for (uint64_t m = kept[ii]; m; m &= m - 1) emit(c2lo[ii] + __builtin_ctzll(m));
+                } else {
+                    for (int c2 = c2lo[ii]; c2 <= c2hi[ii]; c2++) {
+                        if (dd01 + og_zgap2(s.cell_zmax[c2] + sh, s.cell_zmin[c2] + sh, z1b, z1t) > s.cutoff_pair2)
+                            continue;
+                        if (!og_valid_pair(c1, c2)) continue;
+                        emit(c2);
+                    }
                 }
             }
         }
