@@ -271,7 +271,7 @@ int og_alloc_atom_info(int max_n_atoms_exbox, int max_n_cells, int max_n_cell_pa
     CS_TRY(cs_alloc(&h.cp_head, (size_t)h.cap_cells + 1));
     CS_TRY(cs_alloc(&h.cp_c2, max_n_cell_pairs));
     CS_TRY(cs_alloc(&h.cp_mask, max_n_cell_pairs));
-    CS_TRY(cs_alloc(&h.fx, 4 * (size_t)cap));
+    CS_TRY(cs_alloc(&h.fx, OG_FX_WORDS(cap)));
     CS_TRY(cs_alloc(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE));
     CS_TRY(cs_alloc(&h.energy, OG_E_NTERMS));
     g_cs.have_alloc = true;
@@ -611,7 +611,7 @@ int og_pairwise_ljzd(bool flg_mod_15mask) {
     CS_REQUIRE(g_cs.have_pairs && g_cs.have_zd, "og_pairwise_ljzd: no cell-pair list (og_enumerate_cell_pairs)");
     og_dev_sys &h = g_cs.h;
     CS_TRY(cs_push_sys());
-    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 4 * (size_t)h.cap_atom_array, g_cs.stream));
+    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * OG_FX_WORDS(h.cap_atom_array), g_cs.stream));
     og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), OG_FORM_ZD, 4);
     og_launch_energy_reduce(cs_launch(g_cs.map_sys, 1));
     CS_HIP(hipGetLastError());
@@ -627,7 +627,7 @@ int og_pairwise_hps_dh(bool flg_mod_15mask) {
                "og_pairwise_hps_dh: call og_alloc_set_hps_params and og_debye_huckel_constant first");
     og_dev_sys &h = g_cs.h;
     CS_TRY(cs_push_sys());
-    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * 4 * (size_t)h.cap_atom_array, g_cs.stream));
+    CS_HIP(hipMemsetAsync(h.fx, 0, sizeof(long long) * OG_FX_WORDS(h.cap_atom_array), g_cs.stream));
     og_launch_pair(cs_launch(g_cs.map_cellw, g_cs.nb_cellw), OG_FORM_HPS_DH, 4);
     og_launch_energy_reduce(cs_launch(g_cs.map_sys, 1));
     CS_HIP(hipGetLastError());

@@ -563,7 +563,7 @@ og_status check_flags(og_exec &x) {
         if (c.shake_fail) OG_HIP(hipMemset(&s->d_cnt->shake_fail, 0, sizeof(int)));
         if (st == OG_E_CAPACITY || st == OG_E_NAN) {
             // a step stopped part-way: drop its partial sums, rebuild before the next one
-            OG_HIP(hipMemset(s->h.fx, 0, sizeof(long long) * 4 * (size_t)s->h.cap_atom_array));
+            OG_HIP(hipMemset(s->h.fx, 0, sizeof(long long) * OG_FX_WORDS(s->h.cap_atom_array)));
             OG_HIP(hipMemset(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE));
             s->grid_valid = false;
         }
@@ -805,7 +805,7 @@ og_status sys_alloc(og_system *s, const og_system_desc *d, const og_params *p) {
               A(&h.cell_u1, h.cap_cells) && A(&h.u2c_min, (size_t)(h.n_columns + 2) * h.n_uni_z) &&
               A(&h.u2c_max, (size_t)(h.n_columns + 2) * h.n_uni_z) && A(&h.cp_count, h.cap_cells + 1) && A(&h.cp_stage, (size_t)h.cap_cells * OG_ENUM_STAGE) &&
               A(&h.cp_head, h.cap_cells + 1) && A(&h.cp_bsum, h.cap_cells / 1024 + 1) && A(&h.cp_c2, h.cap_cell_pairs) && A(&h.cp_mask, h.cap_cell_pairs) &&
-              A(&h.fx, 4 * (size_t)cap) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
+              A(&h.fx, OG_FX_WORDS(cap)) && A(&h.e_acc, (size_t)OG_EACC_SLOTS * OG_EACC_STRIDE);
     if (!ok) return OG_E_DEVICE;
     const int n_tor     = h.n_torsions + h.n_impros;
     const int n_contrib = 3 * h.n_angles + 4 * n_tor + 2 * h.n_nb14;
@@ -1139,7 +1139,7 @@ og_status exec_autotune(og_exec &x, int trials, int *shape_out, double *ms_out) 
         if (ms / trials < best) best = ms / trials, best_shp = shp;
     }
     for (og_system *s : x.members) {
-        OG_HIP(hipMemsetAsync(s->h.fx, 0, sizeof(long long) * 4 * (size_t)s->h.cap_atom_array, x.stream));
+        OG_HIP(hipMemsetAsync(s->h.fx, 0, sizeof(long long) * OG_FX_WORDS(s->h.cap_atom_array), x.stream));
         OG_HIP(hipMemsetAsync(s->h.e_acc, 0, sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE, x.stream));
     }
     OG_HIP(hipStreamSynchronize(x.stream));

@@ -681,9 +681,8 @@ __device__ __forceinline__ void og_water_lanes(const og_dev_sys &s, const int bl
     og_advance(s, XN[ac], x, gc, f, xw);
     if (!live) return;
     ke += ka * (double)m;
+    og_fx_consume(s.fx, cap, og_fx_slot(g));
     for (int d = 0; d < 3; d++) {
-        s.fx[d * cap + og_fx_slot(g)] = 0;
-        if (d == 0) s.fx[3 * cap + og_fx_slot(g)] = 0;
         s.work[3 * i + d]      = w[d];
         s.crd_prev2[3 * i + d] = xp[d];
         s.crd_prev[3 * i + d]  = x[d];

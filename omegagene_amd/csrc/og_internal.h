@@ -23,6 +23,17 @@
 #define OG_FXJ_SCALE 524288.0
 #define OG_FXJ_INV (1.0 / 524288.0)
 #define OG_FXJ_ROW_LIMIT 2048.0f
+// Words of the fixed-point gradient per system (cap = cap_atom_array, a
+// multiple of 32): x, y, z of the i side and of excluded pairs at 2^32 units
+// (cap each); the j-side (x, y) words (cap); the j-side z of each slot pair
+// (2k, 2k + 1) as two 32-bit fields at 2^19 units in one word (cap / 2), so
+// neighbouring rows of one cell2 add both with one atomic; and the value of
+// that word each slot last took (cap).  The pair word is never cleared: a slot
+// takes the change since its last read, so the two slots of a pair may read it
+// in any order (a slot reads it once per evaluation).
+#define OG_FX_ZJ(cap) (4 * (size_t)(cap))
+#define OG_FX_ZLAST(cap) (OG_FX_ZJ(cap) + (size_t)(cap) / 2)
+#define OG_FX_WORDS(cap) (OG_FX_ZLAST(cap) + (size_t)(cap))
 #define OG_NO_STEP_END 0x7fffffffffffffffLL  // og_dev_counters::step_end of a system with no segment limit
 #define OG_TH_SCALING 1  // og_params::thermostat value of the scaling thermostat (OG_THERMOSTAT_SCALING)
 // og_dev_sys::mc_mode values (the public OG_EXT_VMCMD / OG_EXT_VAUS) and coefficients per state (OG_MC_MAX_ORDER + 1)
@@ -214,7 +225,7 @@ struct og_dev_sys {
     int      *cp_c2;        // [cap_cell_pairs] cell2 | image << 24
     uint64_t *cp_mask;      // [cap_cell_pairs] bit a2*8+a1 set = skip
     // ---- forces / energies ----
-    long long *fx;          // [4*cap_atom_array] fixed-point gradient, slot g at og_fx_slot(g): x | y | z
+    long long *fx;          // [OG_FX_WORDS(cap_atom_array)] fixed-point gradient, slot g at og_fx_slot(g)
                             // (2^32 units) | packed j-side (x, y) (og_fxj_pack)
     long long *e_acc;       // [OG_EACC_SLOTS][OG_EACC_STRIDE] energy accumulators, 2^32 fixed point (the pair
                             // count as an integer): vdw, ele, virial, pairs, excess, bond; a cell / block adds to
@@ -367,11 +378,19 @@ __device__ __forceinline__ void og_fxj_unpack(long long w, double &x, double &y)
 }
 // the pair gradient of grid slot g (index f = og_fx_slot(g)) in kcal/mol/A
 __device__ __forceinline__ void og_fx_read(const long long *fx, int cap, int f, double *w) {
-    double jx, jy;
+    double jx, jy, zlo, zhi;
     og_fxj_unpack(fx[3 * cap + f], jx, jy);
+    // the slot pair's z word changed by both slots' sums since this slot's last read
+    og_fxj_unpack(fx[OG_FX_ZJ(cap) + (f >> 1)] - fx[OG_FX_ZLAST(cap) + f], zlo, zhi);
     w[0] = (double)fx[f] * OG_FX_INV + jx;
     w[1] = (double)fx[cap + f] * OG_FX_INV + jy;
-    w[2] = (double)fx[2 * cap + f] * OG_FX_INV;
+    w[2] = (double)fx[2 * cap + f] * OG_FX_INV + ((f & 1) ? zhi : zlo);
+}
+// after og_fx_read: clear the slot's own words and note the pair word it has taken
+__device__ __forceinline__ void og_fx_consume(long long *fx, int cap, int f) {
+    fx[OG_FX_ZLAST(cap) + f] = fx[OG_FX_ZJ(cap) + (f >> 1)];
+    fx[3 * cap + f]          = 0;
+    for (int k = 0; k < 3; k++) fx[k * cap + f] = 0;
 }
 
 #define OG_GLOBAL(T) __attribute__((address_space(1))) T

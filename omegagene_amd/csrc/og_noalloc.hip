@@ -33,13 +33,13 @@ constexpr size_t kOffEacc   = kOffCnt + na_round(sizeof(og_dev_counters));
 constexpr size_t kOffEnergy = kOffEacc + na_round(sizeof(long long) * OG_EACC_SLOTS * OG_EACC_STRIDE);
 constexpr size_t kOffMap    = kOffEnergy + na_round(sizeof(double) * OG_E_NTERMS);
 constexpr size_t kOffFx     = kOffMap + na_round(sizeof(int2) * kMaxCells);
-constexpr size_t kBytes     = kOffFx + sizeof(long long) * 4 * (size_t)kMaxSlots;
+constexpr size_t kBytes     = kOffFx + sizeof(long long) * OG_FX_WORDS(kMaxSlots);
 constexpr int    kWaves     = 4;  // waves (cells) per pair-kernel block
 static_assert(kMaxSlots % 32 == 0, "the fixed-point layout permutes slots within groups of 32");
 
 }  // namespace
 
-// about 8.2 MiB per GPU at 262,144 slots
+// about 11.3 MiB per GPU at 262,144 slots
 static __device__ __align__(256) unsigned char scratch[kBytes];
 
 namespace {
@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(OG_THREADS) k_na_setup(unsigned char *base, og
     const int g      = blockIdx.x * OG_THREADS + threadIdx.x;
     const int stride = gridDim.x * OG_THREADS;
     long long *fx    = (long long *)(base + kOffFx);
-    for (int k = g; k < 4 * sys.cap_atom_array; k += stride) fx[k] = 0;
+    for (size_t k = g; k < OG_FX_WORDS(sys.cap_atom_array); k += stride) fx[k] = 0;
     long long *eacc = (long long *)(base + kOffEacc);
     for (int k = g; k < OG_EACC_SLOTS * OG_EACC_STRIDE; k += stride) eacc[k] = 0;
     int2 *map = (int2 *)(base + kOffMap);

@@ -224,13 +224,8 @@ __device__ __forceinline__ void og_atom_work_calc(const og_dev_sys &s, const int
 // Stores atom i's work and clears its fixed-point accumulators for the next
 // evaluation.
 __device__ __forceinline__ void og_atom_work_store(const og_dev_sys &s, const int i, const double *w) {
-    const int f   = og_fx_slot(s.rev[i]);
-    const int cap = s.cap_atom_array;
-    s.fx[3 * cap + f] = 0;
-    for (int k = 0; k < 3; k++) {
-        s.fx[k * cap + f] = 0;
-        s.work[3 * i + k] = w[k];
-    }
+    og_fx_consume(s.fx, s.cap_atom_array, og_fx_slot(s.rev[i]));
+    for (int k = 0; k < 3; k++) s.work[3 * i + k] = w[k];
 }
 
 __device__ __forceinline__ void og_atom_work(const og_dev_sys &s, const int i, double &e_ex, double &e_b) {
