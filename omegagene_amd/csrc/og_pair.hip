@@ -128,7 +128,6 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
     float px = pj.x, py = pj.y, pz = pj.z, pw = pj.w;
     // the row's eight LJ pairs {A6, A12} load together, ahead of their use
     typename og_ljt<FORM>::type ljv[OG_NAC];
-    int   npl = 0;  // in-cutoff unmasked pairs of this lane (summed over the wave with the energies)
 #pragma unroll
     for (int i = 0; i < OG_NAC; i++) ljv[i] = LJ_LDS ? ljl[c.row[i] + tj] : LJP[c.row[i] + tj];
     // the tables hold {6 A6, 12 A12}; a caller's table in the reference's {A6, A12} is scaled here
@@ -145,17 +144,26 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         const float dy = c.y[i] - py;
         const float dz = c.z[i] - pz;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const bool  ok = r2 < rc2 && !((m8 >> i) & 1u);
-        npl += ok ? 1 : 0;
+        const bool  in = r2 < rc2, unmasked = !((m8 >> i) & 1u);
+        const bool  ok = in && unmasked;
+        // the pair count on the scalar unit: each ballot is its compare's lane mask as it stands
+        if constexpr (ENERGY)
+            npair += __builtin_popcountll(__builtin_amdgcn_ballot_w64(in) & __builtin_amdgcn_ballot_w64(unmasked));
         const typename og_ljt<FORM>::type lj = ljv[i];  // {6 A6, 12 A12(, HPS switch, HPS lambda)}
         const float rinv  = __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
         // r^2 times the LJ gradient coefficient, 6 A6 r^-6 - 12 A12 r^-12, and (energy steps) the LJ energy
-        const float ljf   = r6inv * fmaf(-lj.y, r6inv, lj.x);
+        const float lji   = fmaf(-lj.y, r6inv, lj.x);
+        const float ljf   = r6inv * lji;
         const float cc    = c.q[i] * pw;
         float       g, eel, elj = 0.0f;
-        if constexpr (ENERGY) elj = r6inv * fmaf(lj.y * (1.0f / 12.0f), r6inv, lj.x * (-1.0f / 6.0f));
+        // LJ energy: forms 0, 1, 3, 4 keep -12 times it, r^-6 (6 A6 + lji) = r^-6 (12 A6 - 12 A12 r^-6), and the
+        // cell's sum is scaled once (two fewer instructions per pair slot); the HPS form keeps the energy itself
+        if constexpr (ENERGY) {
+            if constexpr (FORM == 2) elj = r6inv * fmaf(lj.y * (1.0f / 12.0f), r6inv, lj.x * (-1.0f / 6.0f));
+            else elj = r6inv * (lj.x + lji);
+        }
         if constexpr (FORM == 0) {
             eel = cc * (rinv - zcore + bcoeff * r2);
             g   = r2inv * ljf - cc * (rinv * r2inv - fcoeff);
@@ -211,7 +219,6 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
                      "+v"(px), "+v"(py), "+v"(pz), "+v"(pw));
         if constexpr (ENERGY) asm volatile("" : "+v"(fe), "+v"(fq), "+v"(fv));
     }
-    npair += npl;
     pend.x    = fjx;
     pend.y    = fjy;
     pend.z    = fjz;
@@ -334,8 +341,8 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
     float  fi[3 * OG_NAC];
 #pragma unroll
     for (int k = 0; k < 3 * OG_NAC; k++) fi[k] = 0.0f;
-    double ev = 0.0, ee = 0.0, vir = 0.0;
-    int    npair = 0;                    // in-cutoff unmasked pairs of this cell1 (per lane until the reduction)
+    double ev = 0.0, ee = 0.0, vir = 0.0;  // ev: -12 x the LJ energy except under HPS (scaled at the end)
+    int    npair = 0;                    // in-cutoff unmasked pairs of this cell1 (wave-uniform)
     float  vmax = 0.0f, vsum = 0.0f;     // j-gradient range check
     int    qn = 0, qd = 0;               // rows queued / rows evaluated (wave-uniform)
     og_jpend pend{0.0f, 0.0f, 0.0f, 0, false};
@@ -403,11 +410,10 @@ __device__ __forceinline__ void og_pair_cell(const og_dev_sys &s, const int c1, 
         ev += __shfl_xor(ev, off, 64);
         ee += __shfl_xor(ee, off, 64);
         vir += __shfl_xor(vir, off, 64);
-        npair += __shfl_xor(npair, off, 64);
     }
     if (lane == 0) {
         OG_GLOBAL(long long) *A = (OG_GLOBAL(long long) *)s.e_acc + (c1 % OG_EACC_SLOTS) * OG_EACC_STRIDE;
-        og_atomic_add_fx_g(A + OG_EACC_VDW, og_to_fixed(ev));
+        og_atomic_add_fx_g(A + OG_EACC_VDW, og_to_fixed(FORM == 2 ? ev : ev * (-1.0 / 12.0)));
         og_atomic_add_fx_g(A + OG_EACC_ELE, og_to_fixed(ee));
         og_atomic_add_fx_g(A + OG_EACC_VIR, og_to_fixed(vir));
         og_atomic_add_fx_g(A + OG_EACC_PAIRS, (long long)npair);
