@@ -568,25 +568,34 @@ __device__ __forceinline__ void og_close_step(const og_dev_sys &s, const og_dev_
         for (int j = 0; j < 4; j++)
             if (k + j * OG_STEP_THREADS < s.n_ke_parts) t += v[j];
     }
+    // read with the partials, before the counters are known (one memory
+    // round trip less): the accumulator slots and, in thread 0, the energy
+    // row, whose pair, excluded-pair, bonded and kinetic terms this kernel
+    // replaces (the others come from earlier kernels of the step)
+    long long ev[OG_EACC];
+    og_eacc_load(s, ev);
+    double row[OG_E_NTERMS];
+    if (threadIdx.x == 0)
+        for (int k = 0; k < OG_E_NTERMS; k++) row[k] = s.energy[k];
     if (og_frozen_c(c)) return;
     __shared__ double    sh[OG_STEP_THREADS / 64];
     __shared__ long long she[16 * OG_EACC];
-    if (og_energy_step_c(s, c)) og_take_pair_energies(s, she);
+    if (og_energy_step_c(s, c)) {
+        og_eacc_clear(s);
+        og_eacc_reduce(s, ev, she, row);
+    }
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
     __syncthreads();
     if (threadIdx.x == 0) {
         double ke = 0.0;
         for (int w = 0; w < OG_STEP_THREADS / 64; w++) ke += sh[w];
-        s.energy[OG_E_KIN]  = ke * OG_KINETIC_COEFF;
+        row[OG_E_KIN]       = ke * OG_KINETIC_COEFF;
+        s.energy[OG_E_KIN]  = row[OG_E_KIN];
         const long long stp = c.step;
         if (s.log_cap > 0 && (stp % s.log_stride) == 0) {
-            // all terms loaded before the first store: a term-by-term copy
-            // would wait one memory round trip per term (the row may alias)
-            double v[OG_E_NTERMS];
-            for (int k = 0; k < OG_E_NTERMS; k++) v[k] = s.energy[k];
             double *e = s.elog + (size_t)((stp / s.log_stride) % s.log_cap) * OG_E_NTERMS;
-            for (int k = 0; k < OG_E_NTERMS; k++) e[k] = v[k];
+            for (int k = 0; k < OG_E_NTERMS; k++) e[k] = row[k];
         }
         s.cnt->rebuild_now = 0;
         s.cnt->step        = stp + 1;

@@ -406,16 +406,23 @@ enum { OG_EACC_VDW, OG_EACC_ELE, OG_EACC_VIR, OG_EACC_PAIRS, OG_EACC_EXCESS, OG_
 // order of the contributions), which are cleared for the next evaluation.
 // Block-wide (blockDim.x a multiple of 64, at most 1024); sh: 16 * OG_EACC
 // long longs of shared memory.
-__device__ __forceinline__ void og_take_pair_energies(const og_dev_sys &s, long long *sh) {
-    long long v[OG_EACC];
+__device__ __forceinline__ void og_eacc_load(const og_dev_sys &s, long long (&v)[OG_EACC]) {
     for (int q = 0; q < OG_EACC; q++) v[q] = 0;
     for (int slot = threadIdx.x; slot < OG_EACC_SLOTS; slot += blockDim.x) {
-        long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
-        for (int q = 0; q < OG_EACC; q++) {
-            v[q] += a[q];
-            a[q] = 0;
-        }
+        const long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
+        for (int q = 0; q < OG_EACC; q++) v[q] += a[q];
     }
+}
+__device__ __forceinline__ void og_eacc_clear(const og_dev_sys &s) {
+    for (int slot = threadIdx.x; slot < OG_EACC_SLOTS; slot += blockDim.x) {
+        long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
+        for (int q = 0; q < OG_EACC; q++) a[q] = 0;
+    }
+}
+// the block's loaded slot sums into s.energy (and, when e is given, into
+// thread 0's copy e of the energy row)
+__device__ __forceinline__ void og_eacc_reduce(const og_dev_sys &s, long long (&v)[OG_EACC], long long *sh,
+                                               double *e) {
     for (int off = 32; off > 0; off >>= 1)
         for (int q = 0; q < OG_EACC; q++) v[q] += __shfl_down(v[q], off, 64);
     const int w = threadIdx.x >> 6;
@@ -427,20 +434,33 @@ __device__ __forceinline__ void og_take_pair_energies(const og_dev_sys &s, long 
         for (int q = 0; q < OG_EACC; q++) t[q] = 0;
         for (int k = 0; k < (int)(blockDim.x >> 6); k++)
             for (int q = 0; q < OG_EACC; q++) t[q] += sh[k * OG_EACC + q];
-        s.energy[OG_E_VDW]    = (double)t[OG_EACC_VDW] * OG_FX_INV;
-        s.energy[OG_E_ELE]    = (double)t[OG_EACC_ELE] * OG_FX_INV;
-        s.energy[OG_E_VIRIAL] = (double)t[OG_EACC_VIR] * OG_FX_INV;
-        s.energy[OG_E_NPAIRS] = (double)t[OG_EACC_PAIRS];
-        s.energy[OG_E_EXCESS] = (double)t[OG_EACC_EXCESS] * OG_FX_INV;
-        s.energy[OG_E_BOND]   = (double)t[OG_EACC_BOND] * OG_FX_INV;
-        s.energy[OG_E_ANGLE]   = (double)t[OG_EACC_ANGLE] * OG_FX_INV;
-        s.energy[OG_E_TORSION] = (double)t[OG_EACC_TORSION] * OG_FX_INV;
-        s.energy[OG_E_IMPRO]   = (double)t[OG_EACC_IMPRO] * OG_FX_INV;
-        s.energy[OG_E_VDW14]   = (double)t[OG_EACC_VDW14] * OG_FX_INV;
-        s.energy[OG_E_ELE14]   = (double)t[OG_EACC_ELE14] * OG_FX_INV;
-        s.energy[OG_E_SELF]   = s.e_self;
+        double r[OG_E_NTERMS];
+        r[OG_E_VDW]     = (double)t[OG_EACC_VDW] * OG_FX_INV;
+        r[OG_E_ELE]     = (double)t[OG_EACC_ELE] * OG_FX_INV;
+        r[OG_E_VIRIAL]  = (double)t[OG_EACC_VIR] * OG_FX_INV;
+        r[OG_E_NPAIRS]  = (double)t[OG_EACC_PAIRS];
+        r[OG_E_EXCESS]  = (double)t[OG_EACC_EXCESS] * OG_FX_INV;
+        r[OG_E_BOND]    = (double)t[OG_EACC_BOND] * OG_FX_INV;
+        r[OG_E_ANGLE]   = (double)t[OG_EACC_ANGLE] * OG_FX_INV;
+        r[OG_E_TORSION] = (double)t[OG_EACC_TORSION] * OG_FX_INV;
+        r[OG_E_IMPRO]   = (double)t[OG_EACC_IMPRO] * OG_FX_INV;
+        r[OG_E_VDW14]   = (double)t[OG_EACC_VDW14] * OG_FX_INV;
+        r[OG_E_ELE14]   = (double)t[OG_EACC_ELE14] * OG_FX_INV;
+        r[OG_E_SELF]    = s.e_self;
+        const int terms[] = {OG_E_VDW, OG_E_ELE, OG_E_VIRIAL, OG_E_NPAIRS, OG_E_EXCESS, OG_E_BOND, OG_E_ANGLE,
+                             OG_E_TORSION, OG_E_IMPRO, OG_E_VDW14, OG_E_ELE14, OG_E_SELF};
+        for (int k : terms) {
+            s.energy[k] = r[k];
+            if (e) e[k] = r[k];
+        }
     }
     __syncthreads();
+}
+__device__ __forceinline__ void og_take_pair_energies(const og_dev_sys &s, long long *sh) {
+    long long v[OG_EACC];
+    og_eacc_load(s, v);
+    og_eacc_clear(s);
+    og_eacc_reduce(s, v, sh, nullptr);
 }
 
 
