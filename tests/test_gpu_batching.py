@@ -62,6 +62,35 @@ def test_walkers_at_different_rebuild_phases_share_a_batch():
     sb.close()
 
 
+def test_split_tail_system_in_a_batch_equals_its_run_alone():
+    """A system large enough for the pair kernel's split tail (the last 4,096
+    of its 8,192+ cells as half-record waves) batched with a small walker (one
+    wave per cell): both equal their runs alone bitwise over 12 steps (a grid
+    rebuild included), in every block order of the batch."""
+    big = gen.build_system(23000, seed=33)
+    big.settle_ids, big.settle_d = big.shake_trio_ids, big.shake_trio_d
+    big.shake_trio_ids = np.zeros((0, 3), np.int32)
+    big.shake_trio_d = np.zeros((0, 3))
+    small = _walker(3000, 34)
+    cb, cs = dict(CFG, **{"random-seed": 3}), dict(CFG, **{"random-seed": 4})
+    ref_big = _solo(big, cb, 12)
+    ref_small = _solo(small, cs, 12)
+    say("split tail in a batch", big.n_atoms, small.n_atoms)
+    assert big.n_atoms // 8 >= 2 * 4096
+    for order in ("system", "round-robin", "xcd", "xcd-parts"):
+        sbig, ssmall = System(big, cb), System(small, cs)
+        b = Batch()
+        b.add(sbig)
+        b.add(ssmall)
+        b.set_pair_shape(4, order)
+        b.run(12)
+        for s, ref in ((sbig, ref_big), (ssmall, ref_small)):
+            assert all(np.array_equal(x, y) for x, y in zip(s.state(), ref)), order
+        b.close()
+        sbig.close()
+        ssmall.close()
+
+
 def test_faulted_walker_stands_still_and_the_others_go_on():
     """One member's coordinates put two oxygens 0.05 A apart: its pair rows
     exceed the fixed-point range, it is flagged and stands still (step 0),
