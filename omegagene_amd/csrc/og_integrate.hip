@@ -572,17 +572,16 @@ __device__ __forceinline__ void og_close_step(const og_dev_sys &s, const og_dev_
     // round trip less): the accumulator slots and, in thread 0, the energy
     // row, whose pair, excluded-pair, bonded and kinetic terms this kernel
     // replaces (the others come from earlier kernels of the step)
-    long long ev[OG_EACC];
-    og_eacc_load(s, ev);
+    const long long ev = og_eacc_load<OG_STEP_THREADS>(s);
     double row[OG_E_NTERMS];
     if (threadIdx.x == 0)
         for (int k = 0; k < OG_E_NTERMS; k++) row[k] = s.energy[k];
     if (og_frozen_c(c)) return;
     __shared__ double    sh[OG_STEP_THREADS / 64];
-    __shared__ long long she[16 * OG_EACC];
+    __shared__ long long she[16 * OG_EACC_STRIDE];
     if (og_energy_step_c(s, c)) {
-        og_eacc_clear(s);
-        og_eacc_reduce(s, ev, she, row);
+        og_eacc_clear<OG_STEP_THREADS>(s);
+        og_eacc_reduce<OG_STEP_THREADS>(s, ev, she, row);
     }
     for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;

@@ -404,36 +404,43 @@ enum { OG_EACC_VDW, OG_EACC_ELE, OG_EACC_VIR, OG_EACC_PAIRS, OG_EACC_EXCESS, OG_
 // Pair, excluded-pair and bond energies of one system into s.energy from
 // the fixed-point accumulator slots (integer adds: the same bits whatever the
 // order of the contributions), which are cleared for the next evaluation.
-// Block-wide (blockDim.x a multiple of 64, at most 1024); sh: 16 * OG_EACC
-// long longs of shared memory.
-__device__ __forceinline__ void og_eacc_load(const og_dev_sys &s, long long (&v)[OG_EACC]) {
-    for (int q = 0; q < OG_EACC; q++) v[q] = 0;
-    for (int slot = threadIdx.x; slot < OG_EACC_SLOTS; slot += blockDim.x) {
-        const long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
-        for (int q = 0; q < OG_EACC; q++) v[q] += a[q];
-    }
+// Block-wide, NT threads (a multiple of 64, at most 1024); sh: 16 *
+// OG_EACC_STRIDE long longs of shared memory.  The slots are read as one
+// flat array, thread t taking words t, t + NT, ... (coalesced, all of a
+// thread's loads in flight together), so each thread sums one term
+// (t % OG_EACC_STRIDE) over a share of the slots.
+#define OG_EACC_WORDS (OG_EACC_SLOTS * OG_EACC_STRIDE)
+template <int NT>
+__device__ __forceinline__ long long og_eacc_load(const og_dev_sys &s) {
+    static_assert(NT % 64 == 0 && NT <= 1024 && OG_EACC_WORDS % NT == 0, "block shape");
+    long long w[OG_EACC_WORDS / NT];
+#pragma unroll
+    for (int k = 0; k < OG_EACC_WORDS / NT; k++) w[k] = s.e_acc[threadIdx.x + k * NT];
+    long long v = 0;
+#pragma unroll
+    for (int k = 0; k < OG_EACC_WORDS / NT; k++) v += w[k];
+    return (threadIdx.x & (OG_EACC_STRIDE - 1)) < OG_EACC ? v : 0;  // the padding words of a slot count zero
 }
+template <int NT>
 __device__ __forceinline__ void og_eacc_clear(const og_dev_sys &s) {
-    for (int slot = threadIdx.x; slot < OG_EACC_SLOTS; slot += blockDim.x) {
-        long long *a = s.e_acc + (size_t)slot * OG_EACC_STRIDE;
-        for (int q = 0; q < OG_EACC; q++) a[q] = 0;
-    }
+#pragma unroll
+    for (int k = 0; k < OG_EACC_WORDS / NT; k++) s.e_acc[threadIdx.x + k * NT] = 0;
 }
-// the block's loaded slot sums into s.energy (and, when e is given, into
+// the block's loaded term sums into s.energy (and, when e is given, into
 // thread 0's copy e of the energy row)
-__device__ __forceinline__ void og_eacc_reduce(const og_dev_sys &s, long long (&v)[OG_EACC], long long *sh,
-                                               double *e) {
-    for (int off = 32; off > 0; off >>= 1)
-        for (int q = 0; q < OG_EACC; q++) v[q] += __shfl_down(v[q], off, 64);
+template <int NT>
+__device__ __forceinline__ void og_eacc_reduce(const og_dev_sys &s, long long v, long long *sh, double *e) {
+    static_assert(OG_EACC_STRIDE == 16 && OG_EACC <= OG_EACC_STRIDE, "one term per lane of a 16-lane row");
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
     const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
-        for (int q = 0; q < OG_EACC; q++) sh[w * OG_EACC + q] = v[q];
+    if ((threadIdx.x & 63) < OG_EACC_STRIDE) sh[w * OG_EACC_STRIDE + (threadIdx.x & 63)] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         long long t[OG_EACC];
         for (int q = 0; q < OG_EACC; q++) t[q] = 0;
-        for (int k = 0; k < (int)(blockDim.x >> 6); k++)
-            for (int q = 0; q < OG_EACC; q++) t[q] += sh[k * OG_EACC + q];
+        for (int k = 0; k < NT / 64; k++)
+            for (int q = 0; q < OG_EACC; q++) t[q] += sh[k * OG_EACC_STRIDE + q];
         double r[OG_E_NTERMS];
         r[OG_E_VDW]     = (double)t[OG_EACC_VDW] * OG_FX_INV;
         r[OG_E_ELE]     = (double)t[OG_EACC_ELE] * OG_FX_INV;
@@ -456,11 +463,11 @@ __device__ __forceinline__ void og_eacc_reduce(const og_dev_sys &s, long long (&
     }
     __syncthreads();
 }
+template <int NT>
 __device__ __forceinline__ void og_take_pair_energies(const og_dev_sys &s, long long *sh) {
-    long long v[OG_EACC];
-    og_eacc_load(s, v);
-    og_eacc_clear(s);
-    og_eacc_reduce(s, v, sh, nullptr);
+    const long long v = og_eacc_load<NT>(s);
+    og_eacc_clear<NT>(s);
+    og_eacc_reduce<NT>(s, v, sh, nullptr);
 }
 
 

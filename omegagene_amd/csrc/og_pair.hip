@@ -501,8 +501,8 @@ __global__ void __launch_bounds__(64) k_energy_reduce(const og_dev_sys *__restri
     const og_dev_sys &s = S[map[blockIdx.x].x];
     if (threadIdx.x == 0) s.cnt->rebuild_now = 0;  // a forced rebuild of this evaluation is done
     if (og_frozen(s) || !og_energy_step(s)) return;
-    __shared__ long long sh[16 * OG_EACC];
-    og_take_pair_energies(s, sh);
+    __shared__ long long sh[16 * OG_EACC_STRIDE];
+    og_take_pair_energies<64>(s, sh);
 }
 
 // ------------------------------------------------------------- launchers --
