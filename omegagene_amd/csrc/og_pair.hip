@@ -150,13 +150,18 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         if constexpr (ENERGY)
             npair += __builtin_popcountll(__builtin_amdgcn_ballot_w64(in) & __builtin_amdgcn_ballot_w64(unmasked));
         const typename og_ljt<FORM>::type lj = ljv[i];  // {6 A6, 12 A12(, HPS switch, HPS lambda)}
-        const float rinv  = __builtin_amdgcn_rsqf(r2);
+        // ZSEL (energy steps, forms 0, 1, 3, 4): a pair that is not evaluated
+        // gets 1/r = 0 and charge product 0, which zero its gradient and both
+        // energies (one select instead of three); the HPS form's short-range
+        // constant needs the selects
+        constexpr bool ZSEL = ENERGY && FORM != 2;
+        const float rinv  = ZSEL && !ok ? 0.0f : __builtin_amdgcn_rsqf(r2);
         const float r2inv = rinv * rinv;
         const float r6inv = r2inv * r2inv * r2inv;
         // r^2 times the LJ gradient coefficient, 6 A6 r^-6 - 12 A12 r^-12, and (energy steps) the LJ energy
         const float lji   = fmaf(-lj.y, r6inv, lj.x);
         const float ljf   = r6inv * lji;
-        const float cc    = c.q[i] * pw;
+        const float cc    = ZSEL && !ok ? 0.0f : c.q[i] * pw;
         float       g, eel, elj = 0.0f;
         // LJ energy: forms 0, 1, 3, 4 keep -12 times it, r^-6 (6 A6 + lji) = r^-6 (12 A6 - 12 A12 r^-6), and the
         // cell's sum is scaled once (two fewer instructions per pair slot); the HPS form keeps the energy itself
@@ -201,7 +206,7 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
             g   = r2inv * ljf - cc * (erc * rinv * r2inv + piewald * ex * r2inv - fcoeff);
         }
         // lanes without a pair (masked, out of range, idle) contribute exact zeros
-        g                 = ok ? g : 0.0f;
+        if (!ZSEL) g = ok ? g : 0.0f;
         fjx               = fmaf(g, dx, fjx);
         fjy               = fmaf(g, dy, fjy);
         fjz               = fmaf(g, dz, fjz);
@@ -209,8 +214,8 @@ __device__ __forceinline__ void og_rows(const og_dev_sys &s, const og_cell1 &c, 
         fi[3 * i + 1]     = fmaf(g, dy, fi[3 * i + 1]);
         fi[3 * i + 2]     = fmaf(g, dz, fi[3 * i + 2]);
         if (ENERGY) {
-            fe += ok ? elj : 0.0f;
-            fq += ok ? eel : 0.0f;
+            fe += ZSEL || ok ? elj : 0.0f;
+            fq += ZSEL || ok ? eel : 0.0f;
             fv = fmaf(g, r2, fv);
         }
         // finish atom i here: without this the compiler defers the
